@@ -1,0 +1,235 @@
+/*
+ * gridhip.h — C ABI of the MI355X ring-arithmetic library (libgridhip.so).
+ *
+ * This is the drop-in boundary for Gridland's word-size ring path. Each entry
+ * point names the reference interface it replaces (paths relative to the
+ * reference checkout). A Rust caller binds these through a thin `extern "C"`
+ * crate; the backend arms sit inside the reference's existing seams:
+ *   NTTRing / Ring slice hooks (algebra/src/arith/ntt.rs:341-384,
+ *     algebra/src/arith/ring.rs:75-135), dispatched today by GRID_SIMD
+ *     (algebra/src/simd/dispatch.rs:5-66);
+ *   PreparedLinearOps (commit/src/linear.rs:576-631, impl :690-939), whose
+ *     type-erased cache (linear.rs:31) holds a gh_key;
+ *   the LaBRADOR hot calls inner_commit_slice (labrador/src/crs.rs:211-216),
+ *     compute_garbage_g (labrador/src/main_protocol/garbage.rs:76-93) and the
+ *     JL projection (labrador/src/jl.rs:388-508).
+ *
+ * Conventions
+ *  - Values are canonical residues in [0, q). A ring's word is uint32_t when
+ *    q < 2^32 and uint64_t otherwise (gh_ring_query().word_bytes); arrays are
+ *    packed words, polynomial-major, `d` words per polynomial.
+ *  - Polynomial matrices are row-major of polynomials, as RingMat
+ *    (algebra/src/lattice/types.rs:230-234).
+ *  - NTT-domain values use the reference's slot order: slot k holds
+ *    a(psi^(2k+1)) (TwistedNttPoly, algebra/src/poly/twisted_ntt.rs:22-27).
+ *  - `mem` says where caller buffers live: GH_MEM_HOST (copied in/out, the
+ *    call returns when results are in host memory) or GH_MEM_DEVICE (device
+ *    pointers from gh_alloc or any HIP allocation on the context's device;
+ *    work is enqueued on the context stream and the call returns without
+ *    waiting — gh_ctx_synchronize() waits).
+ *  - Return value: GH_OK (0) or a negative gh_status. Errors that the
+ *    reference reports map 1:1 onto NttError (arith/ntt.rs:40-49) and
+ *    CommitmentError (commit/src/error.rs:7-18).
+ *  - Device objects are library-owned; a context is used from one host thread
+ *    at a time.
+ */
+#ifndef GRIDHIP_H
+#define GRIDHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GH_API __attribute__((visibility("default")))
+
+typedef enum gh_status {
+  GH_OK = 0,
+  /* NttError (algebra/src/arith/ntt.rs:40-49) */
+  GH_E_LENGTH_MISMATCH = -1,
+  GH_E_LENGTH_NOT_POW2 = -2,
+  GH_E_UNSUPPORTED_SIZE = -3,
+  GH_E_UNSUPPORTED_TWIST = -4,
+  /* CommitmentError (commit/src/error.rs:7-18) */
+  GH_E_INVALID_PARAMETERS = -10,
+  GH_E_DIMENSION_MISMATCH = -11,
+  GH_E_OPENING_NORM_EXCEEDED = -12,
+  GH_E_INVALID_MESSAGE_ENCODING = -13,
+  GH_E_INVALID_OPENING = -14,
+  /* boundary / device */
+  GH_E_INVALID_ARGUMENT = -20, /* null handle or pointer, bad flag */
+  GH_E_NOT_CANONICAL = -21,    /* an input word >= q */
+  GH_E_UNSUPPORTED_MODULUS = -22, /* q has a root of the right order but no GPU field path */
+  GH_E_DEVICE = -23,           /* HIP runtime error */
+  GH_E_OOM = -24,              /* device allocation failed */
+  GH_E_BUSY = -25,             /* server queue full */
+  GH_E_NO_DEVICE = -26
+} gh_status;
+
+#define GH_MEM_HOST 0
+#define GH_MEM_DEVICE 1
+
+#define GH_DOMAIN_COEFF 0 /* coefficient domain (CyclotomicPolyRing) */
+#define GH_DOMAIN_NTT 1   /* twisted-NTT evaluations (TwistedNttPoly) */
+
+#define GH_RING_Q32 1
+#define GH_RING_GOLDILOCKS 2
+
+typedef struct gh_ctx gh_ctx;
+typedef struct gh_ring gh_ring;
+typedef struct gh_key gh_key;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+GH_API const char *gh_status_string(int status);
+GH_API int gh_version(void);
+GH_API int gh_device_count(int *count);
+GH_API int gh_ctx_create(int device, gh_ctx **out);
+GH_API int gh_ctx_destroy(gh_ctx *ctx);
+GH_API int gh_ctx_synchronize(gh_ctx *ctx);
+/* number of compute units of the context's device (for batch sizing) */
+GH_API int gh_ctx_num_cus(gh_ctx *ctx, int *out);
+
+/* Kernel timing with HIP events recorded on the context stream around the
+ * phases of each commitment (observability; the reference has none). While
+ * enabled, every gh_commit records events around its witness transform
+ * (phase 0), the streaming MAC (phase 1) and the finalize/inverse (phase 2).
+ * gh_ctx_profile_read waits for the stream and returns the summed elapsed
+ * milliseconds and launch count of a phase since the last read. */
+GH_API int gh_ctx_profile_enable(gh_ctx *ctx, int on);
+GH_API int gh_ctx_profile_read(gh_ctx *ctx, int phase, double *total_ms, uint64_t *count);
+
+/* device memory, so callers without a HIP runtime of their own can keep
+ * inputs resident in HBM */
+GH_API int gh_alloc(gh_ctx *ctx, size_t bytes, void **dev_ptr);
+GH_API int gh_free(gh_ctx *ctx, void *dev_ptr);
+GH_API int gh_upload(gh_ctx *ctx, void *dev_dst, const void *host_src, size_t bytes);
+GH_API int gh_download(gh_ctx *ctx, void *host_dst, const void *dev_src, size_t bytes);
+
+/* Checks that `words` ring words are canonical residues (< q). Device-memory
+ * calls trust their buffers, like the reference's *_trusted paths
+ * (commit/src/linear.rs:476-529); host-memory calls check before upload.
+ * Counterpart of PrimeField's Valid impl (algebra/src/arith/prime.rs:723-728).
+ * Returns GH_OK or GH_E_NOT_CANONICAL. */
+GH_API int gh_check_canonical(gh_ring *ring, const void *values, size_t words, int mem);
+
+/* ---- ring plan ----------------------------------------------------------
+ * Replaces NttPlan::build / TwistedNttPlan::build and their plan caches
+ * (algebra/src/arith/ntt.rs:124-171, algebra/src/poly/ntt.rs:140-178) with the
+ * root selection of PrimeField (algebra/src/arith/prime.rs:566-607, :645-687).
+ * Errors: LENGTH_NOT_POW2, UNSUPPORTED_SIZE (no d-th root, or d != 64 on the
+ * GPU), UNSUPPORTED_TWIST (no 2d-th root), UNSUPPORTED_MODULUS (q >= 2^32 and
+ * not Goldilocks). */
+typedef struct gh_ring_info {
+  uint64_t q;
+  uint32_t d;
+  uint32_t word_bytes; /* 4 or 8 */
+  uint32_t kind;       /* GH_RING_Q32 / GH_RING_GOLDILOCKS */
+  uint32_t reserved;
+  uint64_t generator;  /* find_generator */
+  uint64_t psi;        /* root_of_unity(2d): the twist */
+  uint64_t omega;      /* root_of_unity(d) = psi^2 */
+  uint64_t d_inv;      /* inverse_ntt_scale(d) */
+} gh_ring_info;
+
+GH_API int gh_ring_create(gh_ctx *ctx, uint64_t q, uint32_t d, gh_ring **out);
+GH_API int gh_ring_destroy(gh_ring *ring);
+GH_API int gh_ring_query(const gh_ring *ring, gh_ring_info *out);
+
+/* ---- transforms: NTTRing seam and TwistedNttPlan -------------------------
+ * In place on `count` polynomials of d words.
+ * gh_ntt_forward / gh_ntt_inverse: NTTRing::ntt_forward_with_plan /
+ *   ntt_inverse_with_plan, cyclic length-d NTT (arith/ntt.rs:292-335;
+ *   simd/montgomery.rs:636-681).
+ * gh_twisted_forward / gh_twisted_inverse: TwistedNttPlan::forward_in_place /
+ *   inverse_in_place, negacyclic (poly/ntt.rs:55-104). */
+GH_API int gh_ntt_forward(gh_ring *ring, void *values, size_t count, int mem);
+GH_API int gh_ntt_inverse(gh_ring *ring, void *values, size_t count, int mem);
+GH_API int gh_twisted_forward(gh_ring *ring, void *values, size_t count, int mem);
+GH_API int gh_twisted_inverse(gh_ring *ring, void *values, size_t count, int mem);
+
+/* out[i] = a[i] * b[i] mod (X^d + 1) for i < count: poly_mul_ntt /
+ * NegacyclicMulRing::negacyclic_mul_coeffs (poly/ntt.rs:305-319,
+ * poly/ring.rs:83-101), one fused forward/forward/pointwise/inverse pass.
+ * `out` may alias `a` or `b`. */
+GH_API int gh_negacyclic_mul(gh_ring *ring, const void *a, const void *b, void *out, size_t count,
+                             int mem);
+/* Pointwise product of NTT-domain polynomials, `Mul for TwistedNttPoly`
+ * (poly/twisted_ntt.rs:570-581) / Ring::pointwise_mul_assign_slice. */
+GH_API int gh_pointwise_mul(gh_ring *ring, const void *a, const void *b, void *out, size_t count,
+                            int mem);
+
+/* ---- resident commitment key ---------------------------------------------
+ * PreparedLinearOps::build_linear_key_cache (commit/src/linear.rs:701-708,
+ * prepare_linear_key :377-395): A = [A_msg | A_open] kept NTT-domain in HBM in
+ * the library's streaming layout. a_msg is rows x msg_len polynomials and
+ * a_open rows x open_len (open_len may be 0), both row-major, in `domain`.
+ * The key is what the Rust cache box holds; its Drop calls gh_key_destroy. */
+GH_API int gh_key_create(gh_ring *ring, uint32_t rows, const void *a_msg, uint32_t msg_len,
+                         const void *a_open, uint32_t open_len, int domain, int mem,
+                         gh_key **out);
+GH_API int gh_key_destroy(gh_key *key);
+GH_API int gh_key_dims(const gh_key *key, uint32_t *rows, uint32_t *msg_len, uint32_t *open_len);
+
+/* ---- plain commitment (the drop-in) --------------------------------------
+ * PreparedLinearOps::recompute_linear_commitment_runtime with a prepared key
+ * (commit/src/linear.rs:717-770 -> :509-529 -> :476-506, finish :361-375):
+ *   t = A_msg * m + A_open * r over R_q, t coefficient-domain, rows polys.
+ * message: msg_len polys, randomness: open_len polys (may be NULL when the
+ * key has open_len 0), both in `domain` (GH_DOMAIN_NTT is the
+ * commit_with_opening_ntt path, commit/src/ajtai/mod.rs:256-269).
+ * DIMENSION_MISMATCH when lengths differ from the key (linear.rs:270-327). */
+GH_API int gh_commit(gh_key *key, const void *message, uint32_t msg_len, const void *randomness,
+                     uint32_t open_len, int domain, void *t_out, int mem);
+
+/* ---- batched commitments ---------------------------------------------------
+ * Many independent commitments against one shared key in a single launch.
+ * Item i commits witness_i (len_i polys, coefficient domain) against the first
+ * len_i columns of the key: t_i = A[:, :len_i] * w_i. All pointers in one call
+ * share `mem`. The per-item result equals gh_commit on the same inputs with a
+ * key of len_i columns. No reference counterpart: the reference commits one
+ * message per call (commit/src/ajtai/mod.rs:443-452). */
+typedef struct gh_commit_item {
+  const void *witness;
+  uint32_t len;
+  uint32_t reserved;
+  void *t_out;
+} gh_commit_item;
+GH_API int gh_commit_batch(gh_key *key, const gh_commit_item *items, size_t count, int mem);
+
+/* ---- LaBRADOR Gram matrix --------------------------------------------------
+ * compute_garbage_g (labrador/src/main_protocol/garbage.rs:76-93):
+ *   g_ij = <s_i, s_j> = sum_k s_i[k] * s_j[k] over R_q, 0 <= i <= j < r,
+ * written row-major over the upper triangle (garbage_index,
+ * main_protocol/mod.rs:664-670): r(r+1)/2 polynomials, coefficient domain.
+ * s: r parts x n polys, in `domain`. */
+GH_API int gh_gram(gh_ring *ring, const void *s, uint32_t r, uint32_t n, int domain, void *g_out,
+                   int mem);
+
+/* ---- LaBRADOR inner commitments --------------------------------------------
+ * CommitKey::inner_commit_slice (labrador/src/crs.rs:211-216) for r parts:
+ * t_i = A * s_i, s: r x n polys (key msg_len == n, open_len == 0),
+ * t_out: r x rows polys, coefficient domain. */
+GH_API int gh_inner_commit(gh_key *key, const void *s, uint32_t r, uint32_t n, int domain,
+                           void *t_out, int mem);
+
+/* ---- JL projection -----------------------------------------------------------
+ * JLMatrix::project_and_extract_jl_rows (labrador/src/jl.rs:267-303 ->
+ * project_and_extract_seq :388-508). Part p's matrix is the ChaCha20 keystream
+ * (RFC 8439, key = seed, nonce = LE u32(p) || 0^8, counter from 0), ceil(rows/4)
+ * bytes per column, 2 bits per row: 00 -> +1, 01 -> -1, else 0.
+ * parts: num_parts x cols residues (col = poly*d + coeff).
+ * p_modq: rows ring words, the reference's p = sum mod q.
+ * p_exact (nullable): rows int64, the same sum over centred lifts
+ *   (x > q/2 -> x - q), exact in 64-bit integers.
+ * flat (nullable): rows*num_parts*cols int8 entries in the reference's flat
+ *   layout [m][part][coeff][poly] (jl.rs:262-264). */
+GH_API int gh_jl_project(gh_ring *ring, const uint8_t seed[32], uint32_t rows, const void *parts,
+                         uint32_t num_parts, uint64_t cols, void *p_modq, int64_t *p_exact,
+                         int8_t *flat, int mem);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GRIDHIP_H */

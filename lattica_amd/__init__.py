@@ -1,0 +1,368 @@
+"""lattica_amd — MI355X ring-arithmetic path of Gridland behind a C ABI.
+
+The product is libgridhip.so (HIP kernels for gfx950 + the extern "C" API in
+include/gridhip.h). This module is a thin ctypes binding over that ABI with the
+reference's vocabulary (rings, twisted NTT, prepared commitment keys, Gram,
+JL), used by the tests and the benchmark. It never computes anything itself:
+if the shared library is missing, importing `lib()` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = [
+    "GridHipError", "lib", "library_path", "Context", "Ring", "Key", "DeviceBuffer",
+    "Q32", "GOLDILOCKS", "MEM_HOST", "MEM_DEVICE", "DOMAIN_COEFF", "DOMAIN_NTT", "STATUS",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libgridhip.so")
+
+# the 32-bit modulus of the benchmark configurations: largest prime < 2^32 with q = 1 mod 128
+Q32 = 4294966657
+GOLDILOCKS = 0xFFFFFFFF00000001
+
+MEM_HOST, MEM_DEVICE = 0, 1
+DOMAIN_COEFF, DOMAIN_NTT = 0, 1
+
+STATUS = {
+    0: "OK",
+    -1: "LENGTH_MISMATCH", -2: "LENGTH_NOT_POW2", -3: "UNSUPPORTED_SIZE", -4: "UNSUPPORTED_TWIST",
+    -10: "INVALID_PARAMETERS", -11: "DIMENSION_MISMATCH", -12: "OPENING_NORM_EXCEEDED",
+    -13: "INVALID_MESSAGE_ENCODING", -14: "INVALID_OPENING",
+    -20: "INVALID_ARGUMENT", -21: "NOT_CANONICAL", -22: "UNSUPPORTED_MODULUS", -23: "DEVICE",
+    -24: "OOM", -25: "BUSY", -26: "NO_DEVICE",
+}
+
+# every symbol include/gridhip.h declares
+EXPORTS = (
+    "gh_status_string", "gh_version", "gh_device_count", "gh_ctx_create", "gh_ctx_destroy",
+    "gh_ctx_synchronize", "gh_ctx_num_cus", "gh_ctx_profile_enable", "gh_ctx_profile_read", "gh_alloc", "gh_free", "gh_upload", "gh_download",
+    "gh_check_canonical", "gh_ring_create", "gh_ring_destroy", "gh_ring_query", "gh_ntt_forward",
+    "gh_ntt_inverse", "gh_twisted_forward", "gh_twisted_inverse", "gh_negacyclic_mul",
+    "gh_pointwise_mul", "gh_key_create", "gh_key_destroy", "gh_key_dims", "gh_commit",
+    "gh_commit_batch", "gh_gram", "gh_inner_commit", "gh_jl_project",
+)
+
+
+class GridHipError(RuntimeError):
+    def __init__(self, code: int, what: str = ""):
+        name = STATUS.get(code, str(code))
+        super().__init__(f"{what}: GH_E_{name} ({code})" if what else f"GH_E_{name} ({code})")
+        self.code = code
+        self.name = name
+
+
+class RingInfo(ctypes.Structure):
+    _fields_ = [
+        ("q", ctypes.c_uint64), ("d", ctypes.c_uint32), ("word_bytes", ctypes.c_uint32),
+        ("kind", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("generator", ctypes.c_uint64),
+        ("psi", ctypes.c_uint64), ("omega", ctypes.c_uint64), ("d_inv", ctypes.c_uint64),
+    ]
+
+
+class CommitItem(ctypes.Structure):
+    _fields_ = [("witness", ctypes.c_void_p), ("len", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("t_out", ctypes.c_void_p)]
+
+
+_lib = None
+
+
+def library_path() -> str:
+    return _LIB_PATH
+
+
+def lib():
+    """Load libgridhip.so. Raises if it has not been built: there is no fallback."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise RuntimeError(
+            f"{_LIB_PATH} is missing: build it with `python -m lattica_amd.build` "
+            "(no CPU fallback exists by design)")
+    L = ctypes.CDLL(_LIB_PATH)
+    vp, i32, u32, u64, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_size_t
+    sig = {
+        "gh_status_string": (ctypes.c_char_p, [i32]),
+        "gh_version": (i32, []),
+        "gh_device_count": (i32, [ctypes.POINTER(i32)]),
+        "gh_ctx_create": (i32, [i32, ctypes.POINTER(vp)]),
+        "gh_ctx_destroy": (i32, [vp]),
+        "gh_ctx_synchronize": (i32, [vp]),
+        "gh_ctx_num_cus": (i32, [vp, ctypes.POINTER(i32)]),
+        "gh_ctx_profile_enable": (i32, [vp, i32]),
+        "gh_ctx_profile_read": (i32, [vp, i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64)]),
+        "gh_alloc": (i32, [vp, sz, ctypes.POINTER(vp)]),
+        "gh_free": (i32, [vp, vp]),
+        "gh_upload": (i32, [vp, vp, vp, sz]),
+        "gh_download": (i32, [vp, vp, vp, sz]),
+        "gh_check_canonical": (i32, [vp, vp, sz, i32]),
+        "gh_ring_create": (i32, [vp, u64, u32, ctypes.POINTER(vp)]),
+        "gh_ring_destroy": (i32, [vp]),
+        "gh_ring_query": (i32, [vp, ctypes.POINTER(RingInfo)]),
+        "gh_ntt_forward": (i32, [vp, vp, sz, i32]),
+        "gh_ntt_inverse": (i32, [vp, vp, sz, i32]),
+        "gh_twisted_forward": (i32, [vp, vp, sz, i32]),
+        "gh_twisted_inverse": (i32, [vp, vp, sz, i32]),
+        "gh_negacyclic_mul": (i32, [vp, vp, vp, vp, sz, i32]),
+        "gh_pointwise_mul": (i32, [vp, vp, vp, vp, sz, i32]),
+        "gh_key_create": (i32, [vp, u32, vp, u32, vp, u32, i32, i32, ctypes.POINTER(vp)]),
+        "gh_key_destroy": (i32, [vp]),
+        "gh_key_dims": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
+        "gh_commit": (i32, [vp, vp, u32, vp, u32, i32, vp, i32]),
+        "gh_commit_batch": (i32, [vp, ctypes.POINTER(CommitItem), sz, i32]),
+        "gh_gram": (i32, [vp, vp, u32, u32, i32, vp, i32]),
+        "gh_inner_commit": (i32, [vp, vp, u32, u32, i32, vp, i32]),
+        "gh_jl_project": (i32, [vp, vp, u32, vp, u32, u64, vp, vp, vp, i32]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(code: int, what: str):
+    if code != 0:
+        raise GridHipError(code, what)
+
+
+def _ptr(a: np.ndarray):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    _check(lib().gh_device_count(ctypes.byref(n)), "gh_device_count")
+    return n.value
+
+
+class Context:
+    """gh_ctx: one device, one stream, grow-only workspace."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib().gh_ctx_create(device, ctypes.byref(h)), "gh_ctx_create")
+        self.h = h
+        self.device = device
+
+    @property
+    def num_cus(self) -> int:
+        n = ctypes.c_int()
+        _check(lib().gh_ctx_num_cus(self.h, ctypes.byref(n)), "gh_ctx_num_cus")
+        return n.value
+
+    def synchronize(self):
+        _check(lib().gh_ctx_synchronize(self.h), "gh_ctx_synchronize")
+
+    def profile(self, on: bool):
+        _check(lib().gh_ctx_profile_enable(self.h, 1 if on else 0), "gh_ctx_profile_enable")
+
+    def profile_read(self):
+        """{phase: (total_ms, count)} for phases 0 (witness NTT), 1 (MAC), 2 (finalize)."""
+        out = {}
+        for ph in (0, 1, 2):
+            ms, n = ctypes.c_double(), ctypes.c_uint64()
+            _check(lib().gh_ctx_profile_read(self.h, ph, ctypes.byref(ms), ctypes.byref(n)),
+                   "gh_ctx_profile_read")
+            out[ph] = (ms.value, n.value)
+        return out
+
+    def alloc(self, nbytes: int) -> "DeviceBuffer":
+        return DeviceBuffer(self, nbytes)
+
+    def to_device(self, arr: np.ndarray) -> "DeviceBuffer":
+        arr = np.ascontiguousarray(arr)
+        buf = DeviceBuffer(self, arr.nbytes)
+        buf.upload(arr)
+        return buf
+
+    def close(self):
+        if self.h:
+            lib().gh_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class DeviceBuffer:
+    def __init__(self, ctx: Context, nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _check(lib().gh_alloc(ctx.h, self.nbytes, ctypes.byref(p)), "gh_alloc")
+        self.ptr = p
+
+    def upload(self, arr: np.ndarray):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        _check(lib().gh_upload(self.ctx.h, self.ptr, _ptr(arr), arr.nbytes), "gh_upload")
+
+    def download(self, dtype, shape) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        _check(lib().gh_download(self.ctx.h, _ptr(out), self.ptr, out.nbytes), "gh_download")
+        return out
+
+    def offset(self, nbytes: int) -> ctypes.c_void_p:
+        return ctypes.c_void_p(self.ptr.value + int(nbytes))
+
+    def free(self):
+        if self.ptr:
+            lib().gh_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+
+class Ring:
+    """gh_ring: Z_q[X]/(X^d + 1) with the reference's root selection."""
+
+    def __init__(self, ctx: Context, q: int, d: int = 64):
+        h = ctypes.c_void_p()
+        _check(lib().gh_ring_create(ctx.h, q, d, ctypes.byref(h)), "gh_ring_create")
+        self.h = h
+        self.ctx = ctx
+        info = RingInfo()
+        _check(lib().gh_ring_query(h, ctypes.byref(info)), "gh_ring_query")
+        self.info = info
+        self.q, self.d = int(info.q), int(info.d)
+        self.dtype = np.uint32 if info.word_bytes == 4 else np.uint64
+
+    def words(self, a) -> np.ndarray:
+        return np.ascontiguousarray(np.asarray(a).astype(self.dtype, copy=False))
+
+    def _inplace(self, fn, values):
+        v = self.words(values).copy()
+        _check(getattr(lib(), fn)(self.h, _ptr(v), v.size // self.d, MEM_HOST), fn)
+        return v
+
+    def ntt_forward(self, values):
+        return self._inplace("gh_ntt_forward", values)
+
+    def ntt_inverse(self, values):
+        return self._inplace("gh_ntt_inverse", values)
+
+    def twisted_forward(self, values):
+        return self._inplace("gh_twisted_forward", values)
+
+    def twisted_inverse(self, values):
+        return self._inplace("gh_twisted_inverse", values)
+
+    def negacyclic_mul(self, a, b):
+        a, b = self.words(a), self.words(b)
+        out = np.empty_like(a)
+        _check(lib().gh_negacyclic_mul(self.h, _ptr(a), _ptr(b), _ptr(out), a.size // self.d,
+                                       MEM_HOST), "gh_negacyclic_mul")
+        return out
+
+    def pointwise_mul(self, a, b):
+        a, b = self.words(a), self.words(b)
+        out = np.empty_like(a)
+        _check(lib().gh_pointwise_mul(self.h, _ptr(a), _ptr(b), _ptr(out), a.size // self.d,
+                                      MEM_HOST), "gh_pointwise_mul")
+        return out
+
+    def gram(self, s, domain=DOMAIN_COEFF):
+        s = self.words(s)
+        r, n = s.shape[0], s.shape[1]
+        out = np.zeros((r * (r + 1) // 2, self.d), dtype=self.dtype)
+        _check(lib().gh_gram(self.h, _ptr(s), r, n, domain, _ptr(out), MEM_HOST), "gh_gram")
+        return out
+
+    def jl_project(self, seed: bytes, rows: int, parts, want_exact=True, want_flat=False):
+        parts = self.words(parts)
+        num_parts, cols = parts.shape
+        sd = np.frombuffer(bytes(seed), dtype=np.uint8).copy()
+        pm = np.zeros(rows, dtype=self.dtype)
+        pe = np.zeros(rows, dtype=np.int64) if want_exact else None
+        fl = np.zeros(rows * num_parts * cols, dtype=np.int8) if want_flat else None
+        _check(lib().gh_jl_project(self.h, _ptr(sd), rows, _ptr(parts), num_parts, cols, _ptr(pm),
+                                   _ptr(pe) if pe is not None else None,
+                                   _ptr(fl) if fl is not None else None, MEM_HOST),
+               "gh_jl_project")
+        return pm, pe, fl
+
+    def close(self):
+        if self.h:
+            lib().gh_ring_destroy(self.h)
+            self.h = None
+
+
+class Key:
+    """gh_key: resident NTT-domain commitment key A = [A_msg | A_open]."""
+
+    def __init__(self, ring: Ring, a_msg, a_open=None, domain=DOMAIN_COEFF):
+        a_msg = ring.words(a_msg)
+        rows, msg_len = a_msg.shape[0], a_msg.shape[1]
+        if a_open is None:
+            a_open_arr, open_len = None, 0
+        else:
+            a_open_arr = ring.words(a_open)
+            open_len = a_open_arr.shape[1]
+        h = ctypes.c_void_p()
+        _check(lib().gh_key_create(ring.h, rows, _ptr(a_msg), msg_len,
+                                   _ptr(a_open_arr) if a_open_arr is not None else None, open_len,
+                                   domain, MEM_HOST, ctypes.byref(h)), "gh_key_create")
+        self.h = h
+        self.ring = ring
+        self.rows, self.msg_len, self.open_len = rows, msg_len, open_len
+
+    @classmethod
+    def from_device(cls, ring: Ring, rows: int, a_msg_dev: DeviceBuffer, msg_len: int,
+                    domain=DOMAIN_NTT):
+        self = cls.__new__(cls)
+        h = ctypes.c_void_p()
+        _check(lib().gh_key_create(ring.h, rows, a_msg_dev.ptr, msg_len, None, 0, domain,
+                                   MEM_DEVICE, ctypes.byref(h)), "gh_key_create")
+        self.h, self.ring = h, ring
+        self.rows, self.msg_len, self.open_len = rows, msg_len, 0
+        return self
+
+    def commit(self, message, randomness=None, domain=DOMAIN_COEFF):
+        ring = self.ring
+        m = ring.words(message)
+        r = ring.words(randomness) if randomness is not None else None
+        out = np.zeros((self.rows, ring.d), dtype=ring.dtype)
+        _check(lib().gh_commit(self.h, _ptr(m), m.shape[0], _ptr(r) if r is not None else None,
+                               r.shape[0] if r is not None else 0, domain, _ptr(out), MEM_HOST),
+               "gh_commit")
+        return out
+
+    def commit_device(self, s_dev: DeviceBuffer, n: int, t_dev: DeviceBuffer, domain=DOMAIN_COEFF):
+        _check(lib().gh_commit(self.h, s_dev.ptr, n, None, 0, domain, t_dev.ptr, MEM_DEVICE),
+               "gh_commit")
+
+    def commit_batch(self, witnesses):
+        ring = self.ring
+        ws = [ring.words(w) for w in witnesses]
+        outs = [np.zeros((self.rows, ring.d), dtype=ring.dtype) for _ in ws]
+        items = (CommitItem * max(1, len(ws)))()
+        for i, (w, o) in enumerate(zip(ws, outs)):
+            items[i].witness = w.ctypes.data
+            items[i].len = w.shape[0]
+            items[i].t_out = o.ctypes.data
+        _check(lib().gh_commit_batch(self.h, items, len(ws), MEM_HOST), "gh_commit_batch")
+        return outs
+
+    def inner_commit(self, s, domain=DOMAIN_COEFF):
+        ring = self.ring
+        s = ring.words(s)
+        r, n = s.shape[0], s.shape[1]
+        out = np.zeros((r, self.rows, ring.d), dtype=ring.dtype)
+        _check(lib().gh_inner_commit(self.h, _ptr(s), r, n, domain, _ptr(out), MEM_HOST),
+               "gh_inner_commit")
+        return out
+
+    def close(self):
+        if self.h:
+            lib().gh_key_destroy(self.h)
+            self.h = None

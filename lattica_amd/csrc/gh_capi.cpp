@@ -1,0 +1,666 @@
+// extern "C" entry points of libgridhip.so (declared in include/gridhip.h).
+//
+// Host-memory calls stage buffers through a grow-only device staging area,
+// enqueue the kernels on the context stream and wait; device-memory calls only
+// enqueue. Validation mirrors the reference: dimension checks of
+// commit/src/linear.rs:256-327 and the canonical-value check of
+// PrimeField's Valid impl (algebra/src/arith/prime.rs:723-728).
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "gh_internal.h"
+
+using namespace gh;
+
+namespace gh {
+
+static int grow(void** buf, size_t* have, size_t need, hipStream_t stream) {
+  if (*have >= need) return GH_OK;
+  if (*buf) {
+    GH_HIP(hipStreamSynchronize(stream));
+    GH_HIP(hipFree(*buf));
+    *buf = nullptr;
+    *have = 0;
+  }
+  size_t sz = need < (1u << 20) ? (1u << 20) : need;
+  sz = (sz + 4095) & ~(size_t)4095;
+  GH_HIP(hipMalloc(buf, sz));
+  *have = sz;
+  return GH_OK;
+}
+
+int ctx_workspace(gh_ctx* ctx, size_t bytes, void** out) {
+  int st = grow(&ctx->work, &ctx->work_bytes, bytes, ctx->stream);
+  if (st) return st;
+  *out = ctx->work;
+  return GH_OK;
+}
+
+int ctx_staging(gh_ctx* ctx, size_t bytes, void** out) {
+  int st = grow(&ctx->stage, &ctx->stage_bytes, bytes, ctx->stream);
+  if (st) return st;
+  *out = ctx->stage;
+  return GH_OK;
+}
+
+hipEvent_t prof_mark(gh_ctx* ctx) {
+  if (!ctx->prof_on) return nullptr;
+  if (ctx->prof_used == ctx->prof_pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    ctx->prof_pool.push_back(e);
+  }
+  hipEvent_t e = ctx->prof_pool[ctx->prof_used++];
+  if (hipEventRecord(e, ctx->stream) != hipSuccess) return nullptr;
+  return e;
+}
+
+void prof_span(gh_ctx* ctx, int phase, hipEvent_t a, hipEvent_t b) {
+  if (ctx->prof_on && a && b) ctx->prof_recs.push_back({phase, a, b});
+}
+
+}  // namespace gh
+
+namespace {
+
+constexpr int kVersion = 1;
+
+int set_device(gh_ctx* ctx) {
+  GH_HIP(hipSetDevice(ctx->device));
+  return GH_OK;
+}
+
+int check_mem(int mem) { return (mem == GH_MEM_HOST || mem == GH_MEM_DEVICE) ? GH_OK : GH_E_INVALID_ARGUMENT; }
+
+// Rejects device data holding a word >= q (gh_check_canonical).
+int check_canonical(gh_ring* ring, const void* dev, size_t words) {
+  if (words == 0) return GH_OK;
+  gh_ctx* ctx = ring->ctx;
+  int* flag = (int*)ctx->misc;
+  GH_HIP(hipMemsetAsync(flag, 0, sizeof(int), ctx->stream));
+  int st = launch_check_canonical(ring, dev, words, flag);
+  if (st) return st;
+  int host = 0;
+  GH_HIP(hipMemcpyAsync(&host, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  GH_HIP(hipStreamSynchronize(ctx->stream));
+  return host ? GH_E_NOT_CANONICAL : GH_OK;
+}
+
+// Host inputs are checked on the host before upload.
+int check_canonical_host(const gh_ring* ring, const void* p, size_t words) {
+  if (ring->word_bytes == 4) {
+    const uint32_t* w = (const uint32_t*)p;
+    const uint32_t q = (uint32_t)ring->q;
+    for (size_t i = 0; i < words; i++)
+      if (w[i] >= q) return GH_E_NOT_CANONICAL;
+  } else {
+    const uint64_t* w = (const uint64_t*)p;
+    for (size_t i = 0; i < words; i++)
+      if (w[i] >= ring->q) return GH_E_NOT_CANONICAL;
+  }
+  return GH_OK;
+}
+
+int h2d(gh_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (bytes) GH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  return GH_OK;
+}
+int d2h(gh_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (bytes) GH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  return GH_OK;
+}
+int sync(gh_ctx* ctx) {
+  GH_HIP(hipStreamSynchronize(ctx->stream));
+  return GH_OK;
+}
+
+#define TRY(x)              \
+  do {                      \
+    int _s = (x);           \
+    if (_s != GH_OK) return _s; \
+  } while (0)
+
+}  // namespace
+
+extern "C" {
+
+GH_API const char* gh_status_string(int s) {
+  switch (s) {
+    case GH_OK: return "ok";
+    case GH_E_LENGTH_MISMATCH: return "length mismatch";
+    case GH_E_LENGTH_NOT_POW2: return "length is not a power of two";
+    case GH_E_UNSUPPORTED_SIZE: return "unsupported NTT size";
+    case GH_E_UNSUPPORTED_TWIST: return "unsupported twist (no 2d-th root of unity)";
+    case GH_E_INVALID_PARAMETERS: return "invalid commitment parameters";
+    case GH_E_DIMENSION_MISMATCH: return "dimension mismatch";
+    case GH_E_OPENING_NORM_EXCEEDED: return "opening norm exceeded configured bound";
+    case GH_E_INVALID_MESSAGE_ENCODING: return "invalid message encoding";
+    case GH_E_INVALID_OPENING: return "invalid opening";
+    case GH_E_INVALID_ARGUMENT: return "invalid argument";
+    case GH_E_NOT_CANONICAL: return "input value not a canonical residue";
+    case GH_E_UNSUPPORTED_MODULUS: return "modulus has no GPU field implementation";
+    case GH_E_DEVICE: return "HIP runtime error";
+    case GH_E_OOM: return "device out of memory";
+    case GH_E_BUSY: return "busy";
+    case GH_E_NO_DEVICE: return "no HIP device";
+  }
+  return "unknown status";
+}
+
+GH_API int gh_version(void) { return kVersion; }
+
+GH_API int gh_device_count(int* count) {
+  if (!count) return GH_E_INVALID_ARGUMENT;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) n = 0;
+  *count = n;
+  return GH_OK;
+}
+
+GH_API int gh_ctx_create(int device, gh_ctx** out) {
+  if (!out) return GH_E_INVALID_ARGUMENT;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return GH_E_NO_DEVICE;
+  if (device < 0 || device >= n) return GH_E_INVALID_ARGUMENT;
+  GH_HIP(hipSetDevice(device));
+  gh_ctx* ctx = new (std::nothrow) gh_ctx();
+  if (!ctx) return GH_E_OOM;
+  ctx->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return GH_E_DEVICE;
+  }
+  ctx->num_cus = prop.multiProcessorCount;
+  if (hipMalloc(&ctx->misc, 4096) != hipSuccess) {
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return GH_E_OOM;
+  }
+  *out = ctx;
+  return GH_OK;
+}
+
+GH_API int gh_ctx_destroy(gh_ctx* ctx) {
+  if (!ctx) return GH_E_INVALID_ARGUMENT;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  if (ctx->work) hipFree(ctx->work);
+  if (ctx->stage) hipFree(ctx->stage);
+  if (ctx->misc) hipFree(ctx->misc);
+  for (hipEvent_t e : ctx->prof_pool) hipEventDestroy(e);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return GH_OK;
+}
+
+GH_API int gh_ctx_synchronize(gh_ctx* ctx) {
+  if (!ctx) return GH_E_INVALID_ARGUMENT;
+  TRY(set_device(ctx));
+  return sync(ctx);
+}
+
+GH_API int gh_ctx_profile_enable(gh_ctx* ctx, int on) {
+  if (!ctx) return GH_E_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->prof_on = on != 0;
+  return GH_OK;
+}
+
+GH_API int gh_ctx_profile_read(gh_ctx* ctx, int phase, double* total_ms, uint64_t* count) {
+  if (!ctx || !total_ms || !count) return GH_E_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  TRY(sync(ctx));
+  double sum = 0;
+  uint64_t n = 0;
+  for (const auto& r : ctx->prof_recs) {
+    if (r.phase != phase) continue;
+    float ms = 0;
+    GH_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    sum += ms;
+    n++;
+  }
+  *total_ms = sum;
+  *count = n;
+  // every phase has been read out once the caller asks for the last one
+  if (phase == 2) {
+    ctx->prof_recs.clear();
+    ctx->prof_used = 0;
+  }
+  return GH_OK;
+}
+
+GH_API int gh_ctx_num_cus(gh_ctx* ctx, int* out) {
+  if (!ctx || !out) return GH_E_INVALID_ARGUMENT;
+  *out = ctx->num_cus;
+  return GH_OK;
+}
+
+GH_API int gh_alloc(gh_ctx* ctx, size_t bytes, void** dev) {
+  if (!ctx || !dev) return GH_E_INVALID_ARGUMENT;
+  TRY(set_device(ctx));
+  *dev = nullptr;
+  GH_HIP(hipMalloc(dev, bytes ? bytes : 1));
+  return GH_OK;
+}
+
+GH_API int gh_free(gh_ctx* ctx, void* dev) {
+  if (!ctx) return GH_E_INVALID_ARGUMENT;
+  if (!dev) return GH_OK;
+  TRY(set_device(ctx));
+  GH_HIP(hipStreamSynchronize(ctx->stream));
+  GH_HIP(hipFree(dev));
+  return GH_OK;
+}
+
+GH_API int gh_upload(gh_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) return GH_E_INVALID_ARGUMENT;
+  TRY(set_device(ctx));
+  TRY(h2d(ctx, dst, src, bytes));
+  return sync(ctx);
+}
+
+GH_API int gh_download(gh_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) return GH_E_INVALID_ARGUMENT;
+  TRY(set_device(ctx));
+  TRY(d2h(ctx, dst, src, bytes));
+  return sync(ctx);
+}
+
+GH_API int gh_ring_create(gh_ctx* ctx, uint64_t q, uint32_t d, gh_ring** out) {
+  if (!ctx || !out) return GH_E_INVALID_ARGUMENT;
+  *out = nullptr;
+  gh_ring* ring = new (std::nothrow) gh_ring();
+  if (!ring) return GH_E_OOM;
+  int st = plan_ring(q, d, ring);
+  if (st != GH_OK) {
+    delete ring;
+    return st;
+  }
+  ring->ctx = ctx;
+  if ((st = set_device(ctx)) != GH_OK) { delete ring; return st; }
+  const size_t tab_bytes = ring->host_tab.size() * sizeof(uint64_t);
+  if (hipMalloc(&ring->dev_tab, tab_bytes) != hipSuccess) {
+    delete ring;
+    return GH_E_OOM;
+  }
+  if (hipMemcpy(ring->dev_tab, ring->host_tab.data(), tab_bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    hipFree(ring->dev_tab);
+    delete ring;
+    return GH_E_DEVICE;
+  }
+  *out = ring;
+  return GH_OK;
+}
+
+GH_API int gh_ring_destroy(gh_ring* ring) {
+  if (!ring) return GH_E_INVALID_ARGUMENT;
+  hipSetDevice(ring->ctx->device);
+  hipStreamSynchronize(ring->ctx->stream);
+  if (ring->dev_tab) hipFree(ring->dev_tab);
+  delete ring;
+  return GH_OK;
+}
+
+GH_API int gh_ring_query(const gh_ring* ring, gh_ring_info* out) {
+  if (!ring || !out) return GH_E_INVALID_ARGUMENT;
+  std::memset(out, 0, sizeof *out);
+  out->q = ring->q;
+  out->d = ring->d;
+  out->word_bytes = ring->word_bytes;
+  out->kind = (uint32_t)ring->kind;
+  out->generator = ring->generator;
+  out->psi = ring->psi;
+  out->omega = ring->omega;
+  out->d_inv = ring->d_inv;
+  return GH_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int xform_entry(gh_ring* ring, void* values, size_t count, int mem, int mode) {
+  if (!ring || (count && !values)) return GH_E_INVALID_ARGUMENT;
+  TRY(check_mem(mem));
+  if (count == 0) return GH_OK;
+  gh_ctx* ctx = ring->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  const size_t bytes = count * ring->d * ring->word_bytes;
+  if (mem == GH_MEM_DEVICE) return launch_xform(ring, mode, values, values, count);
+  TRY(check_canonical_host(ring, values, count * ring->d));
+  void* dev;
+  TRY(ctx_staging(ctx, bytes, &dev));
+  TRY(h2d(ctx, dev, values, bytes));
+  TRY(launch_xform(ring, mode, dev, dev, count));
+  TRY(d2h(ctx, values, dev, bytes));
+  return sync(ctx);
+}
+
+int binary_entry(gh_ring* ring, const void* a, const void* b, void* out, size_t count, int mem,
+                 bool negacyclic) {
+  if (!ring || (count && (!a || !b || !out))) return GH_E_INVALID_ARGUMENT;
+  TRY(check_mem(mem));
+  if (count == 0) return GH_OK;
+  gh_ctx* ctx = ring->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  const size_t words = count * ring->d;
+  const size_t bytes = words * ring->word_bytes;
+  if (mem == GH_MEM_DEVICE)
+    return negacyclic ? launch_negacyclic_mul(ring, a, b, out, count)
+                      : launch_pointwise_mul(ring, a, b, out, count);
+  TRY(check_canonical_host(ring, a, words));
+  TRY(check_canonical_host(ring, b, words));
+  void* dev;
+  TRY(ctx_staging(ctx, 2 * bytes, &dev));
+  void* da = dev;
+  void* db = (char*)dev + bytes;
+  TRY(h2d(ctx, da, a, bytes));
+  TRY(h2d(ctx, db, b, bytes));
+  TRY(negacyclic ? launch_negacyclic_mul(ring, da, db, da, count)
+                 : launch_pointwise_mul(ring, da, db, da, count));
+  TRY(d2h(ctx, out, da, bytes));
+  return sync(ctx);
+}
+
+}  // namespace
+
+extern "C" {
+
+GH_API int gh_ntt_forward(gh_ring* r, void* v, size_t c, int mem) { return xform_entry(r, v, c, mem, kCyclicFwd); }
+GH_API int gh_ntt_inverse(gh_ring* r, void* v, size_t c, int mem) { return xform_entry(r, v, c, mem, kCyclicInv); }
+GH_API int gh_twisted_forward(gh_ring* r, void* v, size_t c, int mem) { return xform_entry(r, v, c, mem, kTwistedFwd); }
+GH_API int gh_twisted_inverse(gh_ring* r, void* v, size_t c, int mem) { return xform_entry(r, v, c, mem, kTwistedInv); }
+
+GH_API int gh_negacyclic_mul(gh_ring* ring, const void* a, const void* b, void* out, size_t count, int mem) {
+  return binary_entry(ring, a, b, out, count, mem, true);
+}
+GH_API int gh_pointwise_mul(gh_ring* ring, const void* a, const void* b, void* out, size_t count, int mem) {
+  return binary_entry(ring, a, b, out, count, mem, false);
+}
+
+GH_API int gh_key_create(gh_ring* ring, uint32_t rows, const void* a_msg, uint32_t msg_len,
+                         const void* a_open, uint32_t open_len, int domain, int mem, gh_key** out) {
+  if (!ring || !out) return GH_E_INVALID_ARGUMENT;
+  *out = nullptr;
+  TRY(check_mem(mem));
+  if (domain != GH_DOMAIN_COEFF && domain != GH_DOMAIN_NTT) return GH_E_INVALID_ARGUMENT;
+  // CommitmentDimensions::validate (linear.rs:45-59): message and commitment
+  // lengths must be non-zero; the opening block may be empty here.
+  if (rows == 0 || msg_len == 0) return GH_E_INVALID_PARAMETERS;
+  if (!a_msg || (open_len && !a_open)) return GH_E_INVALID_ARGUMENT;
+  gh_ctx* ctx = ring->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  const size_t wb = ring->word_bytes, d = ring->d;
+  const uint32_t cols = msg_len + open_len;
+  if (cols < msg_len) return GH_E_INVALID_PARAMETERS;
+  const size_t key_bytes = (size_t)rows * cols * d * wb;
+  const size_t msg_bytes = (size_t)rows * msg_len * d * wb;
+  const size_t open_bytes = (size_t)rows * open_len * d * wb;
+  if (mem == GH_MEM_HOST) {
+    TRY(check_canonical_host(ring, a_msg, (size_t)rows * msg_len * d));
+    if (open_len) TRY(check_canonical_host(ring, a_open, (size_t)rows * open_len * d));
+  }
+  gh_key* key = new (std::nothrow) gh_key();
+  if (!key) return GH_E_OOM;
+  key->ring = ring;
+  key->rows = rows;
+  key->msg_len = msg_len;
+  key->open_len = open_len;
+  if (hipMalloc(&key->dev_a, key_bytes) != hipSuccess) {
+    delete key;
+    return GH_E_OOM;
+  }
+  int st = GH_OK;
+  const void* src_msg = a_msg;
+  const void* src_open = a_open;
+  if (mem == GH_MEM_HOST) {
+    void* dev;
+    st = ctx_staging(ctx, msg_bytes + open_bytes, &dev);
+    if (st == GH_OK) st = h2d(ctx, dev, a_msg, msg_bytes);
+    if (st == GH_OK && open_len) st = h2d(ctx, (char*)dev + msg_bytes, a_open, open_bytes);
+    src_msg = dev;
+    src_open = (char*)dev + msg_bytes;
+  }
+  if (st == GH_OK) st = launch_key_layout(ring, src_msg, rows, msg_len, 0, cols, key->dev_a, domain);
+  if (st == GH_OK && open_len)
+    st = launch_key_layout(ring, src_open, rows, open_len, msg_len, cols, key->dev_a, domain);
+  // prepare_poly_matrix (linear.rs:345-359): a coefficient-domain key is
+  // transformed once, here; the layout only permutes whole polynomials.
+  if (st == GH_OK && domain == GH_DOMAIN_COEFF)
+    st = launch_xform(ring, kTwistedFwd, key->dev_a, key->dev_a, (size_t)rows * cols);
+  if (st == GH_OK) st = sync(ctx);
+  if (st != GH_OK) {
+    hipFree(key->dev_a);
+    delete key;
+    return st;
+  }
+  *out = key;
+  return GH_OK;
+}
+
+GH_API int gh_key_destroy(gh_key* key) {
+  if (!key) return GH_E_INVALID_ARGUMENT;
+  hipSetDevice(key->ring->ctx->device);
+  hipStreamSynchronize(key->ring->ctx->stream);
+  if (key->dev_a) hipFree(key->dev_a);
+  delete key;
+  return GH_OK;
+}
+
+GH_API int gh_key_dims(const gh_key* key, uint32_t* rows, uint32_t* msg_len, uint32_t* open_len) {
+  if (!key) return GH_E_INVALID_ARGUMENT;
+  if (rows) *rows = key->rows;
+  if (msg_len) *msg_len = key->msg_len;
+  if (open_len) *open_len = key->open_len;
+  return GH_OK;
+}
+
+GH_API int gh_commit(gh_key* key, const void* message, uint32_t msg_len, const void* randomness,
+                     uint32_t open_len, int domain, void* t_out, int mem) {
+  if (!key || !message || !t_out || (open_len && !randomness)) return GH_E_INVALID_ARGUMENT;
+  TRY(check_mem(mem));
+  if (domain != GH_DOMAIN_COEFF && domain != GH_DOMAIN_NTT) return GH_E_INVALID_ARGUMENT;
+  // validate_message / validate_opening_randomness (linear.rs:270-293)
+  if (msg_len != key->msg_len || open_len != key->open_len) return GH_E_DIMENSION_MISMATCH;
+  gh_ring* ring = key->ring;
+  gh_ctx* ctx = ring->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  const size_t wb = ring->word_bytes, d = ring->d;
+  const uint32_t cols = msg_len + open_len;
+  const size_t msg_bytes = (size_t)msg_len * d * wb, open_bytes = (size_t)open_len * d * wb;
+  const size_t t_bytes = (size_t)key->rows * d * wb;
+  if (mem == GH_MEM_DEVICE && open_len == 0) return launch_commit(key, message, cols, domain, t_out);
+  void* dev;
+  TRY(ctx_staging(ctx, msg_bytes + open_bytes + t_bytes, &dev));
+  void* ds = dev;
+  void* dt = (char*)dev + msg_bytes + open_bytes;
+  if (mem == GH_MEM_HOST) {
+    TRY(check_canonical_host(ring, message, (size_t)msg_len * d));
+    if (open_len) TRY(check_canonical_host(ring, randomness, (size_t)open_len * d));
+    TRY(h2d(ctx, ds, message, msg_bytes));
+    if (open_len) TRY(h2d(ctx, (char*)ds + msg_bytes, randomness, open_bytes));
+  } else {
+    GH_HIP(hipMemcpyAsync(ds, message, msg_bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    GH_HIP(hipMemcpyAsync((char*)ds + msg_bytes, randomness, open_bytes, hipMemcpyDeviceToDevice,
+                          ctx->stream));
+  }
+  TRY(launch_commit(key, ds, cols, domain, mem == GH_MEM_HOST ? dt : t_out));
+  if (mem == GH_MEM_HOST) {
+    TRY(d2h(ctx, t_out, dt, t_bytes));
+    return sync(ctx);
+  }
+  return GH_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+GH_API int gh_check_canonical(gh_ring* ring, const void* values, size_t words, int mem) {
+  if (!ring || (words && !values)) return GH_E_INVALID_ARGUMENT;
+  TRY(check_mem(mem));
+  if (mem == GH_MEM_HOST) return check_canonical_host(ring, values, words);
+  gh_ctx* ctx = ring->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  return check_canonical(ring, values, words);
+}
+
+GH_API int gh_inner_commit(gh_key* key, const void* s, uint32_t r, uint32_t n, int domain,
+                           void* t_out, int mem) {
+  if (!key || (r && (!s || !t_out))) return GH_E_INVALID_ARGUMENT;
+  TRY(check_mem(mem));
+  if (domain != GH_DOMAIN_COEFF && domain != GH_DOMAIN_NTT) return GH_E_INVALID_ARGUMENT;
+  if (n != key->msg_len || key->open_len != 0) return GH_E_DIMENSION_MISMATCH;
+  if (r == 0) return GH_OK;
+  gh_ring* ring = key->ring;
+  gh_ctx* ctx = ring->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  const size_t wb = ring->word_bytes, d = ring->d;
+  const size_t s_bytes = (size_t)r * n * d * wb, t_bytes = (size_t)r * key->rows * d * wb;
+  const char* sd = (const char*)s;
+  char* td = (char*)t_out;
+  if (mem == GH_MEM_HOST) {
+    TRY(check_canonical_host(ring, s, (size_t)r * n * d));
+    void* dev;
+    TRY(ctx_staging(ctx, s_bytes + t_bytes, &dev));
+    TRY(h2d(ctx, dev, s, s_bytes));
+    sd = (const char*)dev;
+    td = (char*)dev + s_bytes;
+  }
+  for (uint32_t i = 0; i < r; i++)
+    TRY(launch_commit(key, sd + (size_t)i * n * d * wb, n, domain,
+                      td + (size_t)i * key->rows * d * wb));
+  if (mem == GH_MEM_HOST) {
+    TRY(d2h(ctx, t_out, td, t_bytes));
+    return sync(ctx);
+  }
+  return GH_OK;
+}
+
+GH_API int gh_commit_batch(gh_key* key, const gh_commit_item* items, size_t count, int mem) {
+  if (!key || (count && !items)) return GH_E_INVALID_ARGUMENT;
+  TRY(check_mem(mem));
+  const uint32_t cols = key->msg_len + key->open_len;
+  for (size_t i = 0; i < count; i++) {
+    if (!items[i].witness || !items[i].t_out) return GH_E_INVALID_ARGUMENT;
+    if (items[i].len == 0 || items[i].len > cols) return GH_E_DIMENSION_MISMATCH;
+  }
+  if (count == 0) return GH_OK;
+  gh_ring* ring = key->ring;
+  gh_ctx* ctx = ring->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  const size_t wb = ring->word_bytes, d = ring->d;
+  const size_t t_bytes = (size_t)key->rows * d * wb;
+  if (mem == GH_MEM_DEVICE) {
+    for (size_t i = 0; i < count; i++)
+      TRY(launch_commit(key, items[i].witness, items[i].len, GH_DOMAIN_COEFF, items[i].t_out));
+    return GH_OK;
+  }
+  size_t total = 0;
+  for (size_t i = 0; i < count; i++) {
+    TRY(check_canonical_host(ring, items[i].witness, (size_t)items[i].len * d));
+    total += (size_t)items[i].len * d * wb;
+  }
+  void* dev;
+  TRY(ctx_staging(ctx, total + count * t_bytes, &dev));
+  char* sp = (char*)dev;
+  char* tp = (char*)dev + total;
+  for (size_t i = 0; i < count; i++) {
+    const size_t b = (size_t)items[i].len * d * wb;
+    TRY(h2d(ctx, sp, items[i].witness, b));
+    TRY(launch_commit(key, sp, items[i].len, GH_DOMAIN_COEFF, tp + i * t_bytes));
+    sp += b;
+  }
+  for (size_t i = 0; i < count; i++) TRY(d2h(ctx, items[i].t_out, tp + i * t_bytes, t_bytes));
+  return sync(ctx);
+}
+
+GH_API int gh_gram(gh_ring* ring, const void* s, uint32_t r, uint32_t n, int domain, void* g_out,
+                   int mem) {
+  if (!ring || (r && (!s || !g_out))) return GH_E_INVALID_ARGUMENT;
+  TRY(check_mem(mem));
+  if (domain != GH_DOMAIN_COEFF && domain != GH_DOMAIN_NTT) return GH_E_INVALID_ARGUMENT;
+  if (r == 0) return GH_OK;
+  gh_ctx* ctx = ring->ctx;
+  const size_t wb = ring->word_bytes, d = ring->d;
+  const size_t npairs = (size_t)r * (r + 1) / 2;
+  const size_t g_bytes = npairs * d * wb;
+  if (n == 0) {  // empty witness: every g_ij is zero
+    if (mem == GH_MEM_HOST) { std::memset(g_out, 0, g_bytes); return GH_OK; }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    TRY(set_device(ctx));
+    GH_HIP(hipMemsetAsync(g_out, 0, g_bytes, ctx->stream));
+    return GH_OK;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  if (mem == GH_MEM_DEVICE) return launch_gram(ring, s, r, n, domain, g_out);
+  const size_t s_bytes = (size_t)r * n * d * wb;
+  TRY(check_canonical_host(ring, s, (size_t)r * n * d));
+  void* dev;
+  TRY(ctx_staging(ctx, s_bytes + g_bytes, &dev));
+  TRY(h2d(ctx, dev, s, s_bytes));
+  TRY(launch_gram(ring, dev, r, n, domain, (char*)dev + s_bytes));
+  TRY(d2h(ctx, g_out, (char*)dev + s_bytes, g_bytes));
+  return sync(ctx);
+}
+
+GH_API int gh_jl_project(gh_ring* ring, const uint8_t seed[32], uint32_t rows, const void* parts,
+                         uint32_t num_parts, uint64_t cols, void* p_modq, int64_t* p_exact,
+                         int8_t* flat, int mem) {
+  if (!ring || !seed || !p_modq || (num_parts && cols && !parts)) return GH_E_INVALID_ARGUMENT;
+  TRY(check_mem(mem));
+  // rows > 256 would need more than one keystream block per column per row
+  // byte in LDS; the reference profile uses 256 (labrador/src/params/jl_profile.rs:9-41).
+  if (rows == 0 || rows > 256) return GH_E_INVALID_ARGUMENT;
+  if (cols % ring->d != 0) return GH_E_INVALID_ARGUMENT;
+  if (p_exact && ring->kind != GH_RING_Q32) return GH_E_INVALID_ARGUMENT;
+  if (cols > 0xffffffffull / ((rows + 3) / 4)) return GH_E_INVALID_ARGUMENT;  // 32-bit block counter
+  gh_ctx* ctx = ring->ctx;
+  const size_t wb = ring->word_bytes;
+  const size_t in_bytes = (size_t)num_parts * cols * wb;
+  const size_t p_bytes = (size_t)rows * wb, e_bytes = (size_t)rows * sizeof(int64_t);
+  const size_t f_bytes = flat ? (size_t)rows * num_parts * cols : 0;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  if (num_parts == 0 || cols == 0) {
+    if (mem == GH_MEM_HOST) {
+      std::memset(p_modq, 0, p_bytes);
+      if (p_exact) std::memset(p_exact, 0, e_bytes);
+      return GH_OK;
+    }
+    GH_HIP(hipMemsetAsync(p_modq, 0, p_bytes, ctx->stream));
+    if (p_exact) GH_HIP(hipMemsetAsync(p_exact, 0, e_bytes, ctx->stream));
+    return GH_OK;
+  }
+  if (mem == GH_MEM_DEVICE)
+    return launch_jl(ring, seed, rows, parts, num_parts, cols, p_modq, p_exact, flat);
+  TRY(check_canonical_host(ring, parts, (size_t)num_parts * cols));
+  void* dev;
+  TRY(ctx_staging(ctx, in_bytes + p_bytes + e_bytes + f_bytes + 64, &dev));
+  char* base = (char*)dev;
+  char* dp = base + in_bytes;
+  int64_t* de = (int64_t*)(((uintptr_t)(dp + p_bytes) + 7) & ~(uintptr_t)7);
+  int8_t* df = flat ? (int8_t*)(de + rows) : nullptr;
+  TRY(h2d(ctx, base, parts, in_bytes));
+  TRY(launch_jl(ring, seed, rows, base, num_parts, cols, dp, p_exact ? de : nullptr, df));
+  TRY(d2h(ctx, p_modq, dp, p_bytes));
+  if (p_exact) TRY(d2h(ctx, p_exact, de, e_bytes));
+  if (flat) TRY(d2h(ctx, flat, df, f_bytes));
+  return sync(ctx);
+}
+
+}  // extern "C"

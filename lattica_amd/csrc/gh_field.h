@@ -1,0 +1,116 @@
+// Device arithmetic over Z_q for the two modulus families the GPU path runs:
+//
+//   FQ32  — any odd prime q < 2^32. Montgomery radix 2^32 with the signed-
+//           difference REDC: for t = a*b (64-bit) and m = lo(t)*q^-1 mod 2^32,
+//           (t - m*q)/2^32 = hi(t) - hi(m*q) exactly (the low words cancel), a
+//           value in (-q, q) fixed by one conditional +q. This works for
+//           q > 2^31 (the 32-bit q = 2^32 - 639 has no lazy headroom) and
+//           needs three 32-bit multiplies when the multiplier is a fixed
+//           twiddle with its w*q^-1 precomputed.
+//   FGold — the Goldilocks prime p = 2^64 - 2^32 + 1 with the special-form
+//           reduction 2^64 = 2^32 - 1, 2^96 = -1 (mod p); no Montgomery form.
+//
+// Values crossing kernels are canonical residues in [0, q). The reference does
+// its arithmetic in Montgomery form with R = 2^64 (algebra/src/arith/prime.rs:
+// 141-226); canonical outputs are identical because the ring operations are
+// exact.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gh {
+
+constexpr uint64_t kGoldilocks = 0xffffffff00000001ull;
+constexpr uint64_t kGoldEps = 0xffffffffull;  // 2^64 mod p
+
+struct FQ32 {
+  using W = uint32_t;
+  static constexpr int kTwWords = 2;  // twiddle entry: (w*2^32 mod q, that * q^-1 mod 2^32)
+  uint32_t q;
+  uint32_t qinv;  // q^-1 mod 2^32
+
+  __device__ __forceinline__ W add(W a, W b) const {
+    W nb = q - b;
+    W r = a - nb;
+    return a < nb ? r + q : r;
+  }
+  __device__ __forceinline__ W sub(W a, W b) const {
+    W r = a - b;
+    return a < b ? r + q : r;
+  }
+  // a * b * 2^-32 mod q, for a < 2^32 and b < q
+  __device__ __forceinline__ W mont(W a, W b) const {
+    W lo = a * b;
+    W hi = __umulhi(a, b);
+    W m = lo * qinv;
+    W mh = __umulhi(m, q);
+    W r = hi - mh;
+    return hi < mh ? r + q : r;
+  }
+  // a * v mod q where tw = (v*2^32 mod q, (v*2^32 mod q) * q^-1 mod 2^32)
+  __device__ __forceinline__ W mul_tw(W a, const W* tw) const {
+    W m = a * tw[1];
+    W hi = __umulhi(a, tw[0]);
+    W mh = __umulhi(m, q);
+    W r = hi - mh;
+    return hi < mh ? r + q : r;
+  }
+  // REDC of a 64-bit value t < q * 2^32: t * 2^-32 mod q
+  __device__ __forceinline__ W redc64(uint64_t t) const {
+    W lo = (W)t, hi = (W)(t >> 32);
+    W m = lo * qinv;
+    W mh = __umulhi(m, q);
+    W r = hi - mh;
+    return hi < mh ? r + q : r;
+  }
+  // (h2*2^64 + h1*2^32 + h0) * 2^-64 mod q, for h2 < q
+  __device__ __forceinline__ W reduce96_r2(W h0, W h1, W h2) const {
+    W y = redc64((uint64_t)h0);
+    W z = redc64((uint64_t)h1 + y);
+    return add(h2, z);
+  }
+};
+
+struct FGold {
+  using W = uint64_t;
+  static constexpr int kTwWords = 1;  // twiddles are canonical values
+
+  __device__ __forceinline__ W add(W a, W b) const {
+    W s = a + b;
+    if (s < a) return s + kGoldEps;  // wrapped: a+b-2^64 < p - 2^32, + eps stays < p
+    return s >= kGoldilocks ? s - kGoldilocks : s;
+  }
+  __device__ __forceinline__ W sub(W a, W b) const {
+    W r = a - b;
+    return a < b ? r - kGoldEps : r;  // a-b+2^64 - eps = a-b+p
+  }
+  // (hi*2^64 + lo) mod p, canonical
+  __device__ __forceinline__ W reduce128(W lo, W hi) const {
+    W hh = hi >> 32, hl = hi & kGoldEps;
+    W t0 = lo - hh;
+    if (lo < hh) t0 -= kGoldEps;
+    W t1 = (hl << 32) - hl;
+    W t2 = t0 + t1;
+    if (t2 < t1) t2 += kGoldEps;
+    return t2 >= kGoldilocks ? t2 - kGoldilocks : t2;
+  }
+  __device__ __forceinline__ W mul(W a, W b) const { return reduce128(a * b, __umul64hi(a, b)); }
+  __device__ __forceinline__ W mont(W a, W b) const { return mul(a, b); }
+  __device__ __forceinline__ W mul_tw(W a, const W* tw) const { return mul(a, tw[0]); }
+};
+
+// Twiddle table layout shared by host (plan) and device (kernels), in entries
+// of F::kTwWords words each. One table per ring, resident in HBM.
+enum TabOffset : int {
+  kTabTwist = 0,          // psi^i                       (twisted forward)
+  kTabFwd = 64,           // forward DIT stage twiddles, stage s at kTabFwd + 2^s - 1
+  kTabInv = 128,          // inverse DIF stage twiddles, stage s at kTabInv + 64 - (64 >> s)
+  kTabUntwist = 192,      // psi^-i * d^-1                (twisted inverse)
+  kTabUntwistMul = 256,   // psi^-i * d^-1 * R            (after a Montgomery pointwise product)
+  kTabUnscale = 320,      // d^-1                         (cyclic inverse)
+  kTabUntwistMac = 384,   // psi^-i * d^-1 * R^2          (after a reduce96_r2 accumulation)
+  kTabEntries = 448,
+};
+
+}  // namespace gh

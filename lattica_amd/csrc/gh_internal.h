@@ -1,0 +1,86 @@
+// Library-internal host structures shared by the C ABI and the launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <vector>
+
+#include "../../include/gridhip.h"
+#include "gh_field.h"
+
+struct gh_ctx {
+  int device = 0;
+  int num_cus = 0;
+  hipStream_t stream = nullptr;
+  // grow-only device workspace used by the calls of this context
+  void* work = nullptr;
+  size_t work_bytes = 0;
+  // staging for host-memory calls
+  void* stage = nullptr;
+  size_t stage_bytes = 0;
+  // small fixed device buffer (flags)
+  void* misc = nullptr;
+  // phase timing (gh_ctx_profile_*): event pairs recorded per commit
+  bool prof_on = false;
+  std::vector<hipEvent_t> prof_pool;
+  size_t prof_used = 0;
+  struct ProfRec { int phase; hipEvent_t a, b; };
+  std::vector<ProfRec> prof_recs;
+  std::mutex mu;
+};
+
+struct gh_ring {
+  gh_ctx* ctx = nullptr;
+  uint64_t q = 0;
+  uint32_t d = 0;
+  int kind = 0;
+  uint32_t word_bytes = 0;
+  uint64_t generator = 0, psi = 0, omega = 0, d_inv = 0;
+  uint32_t qinv32 = 0;
+  std::vector<uint64_t> host_tab;  // twiddle table image (layout in gh_field.h)
+  void* dev_tab = nullptr;
+};
+
+struct gh_key {
+  gh_ring* ring = nullptr;
+  uint32_t rows = 0, msg_len = 0, open_len = 0;
+  // A in NTT domain, layout [col][row][slot], col over msg then open columns
+  void* dev_a = nullptr;
+};
+
+namespace gh {
+
+uint64_t powmod(uint64_t a, uint64_t e, uint64_t q);
+int plan_ring(uint64_t q, uint32_t d, gh_ring* ring);
+
+// device workspace of at least `bytes` bytes on the context (grow-only)
+int ctx_workspace(gh_ctx* ctx, size_t bytes, void** out);
+int ctx_staging(gh_ctx* ctx, size_t bytes, void** out);
+// records an event on the context stream when profiling is on (else nullptr)
+hipEvent_t prof_mark(gh_ctx* ctx);
+void prof_span(gh_ctx* ctx, int phase, hipEvent_t a, hipEvent_t b);
+
+// launchers (device pointers; enqueue on ctx->stream)
+enum XformMode { kTwistedFwd = 0, kTwistedInv = 1, kCyclicFwd = 2, kCyclicInv = 3 };
+int launch_xform(gh_ring* ring, int mode, const void* src, void* dst, size_t count);
+int launch_negacyclic_mul(gh_ring* ring, const void* a, const void* b, void* out, size_t count);
+int launch_pointwise_mul(gh_ring* ring, const void* a, const void* b, void* out, size_t count);
+int launch_key_layout(gh_ring* ring, const void* src_rowmajor, uint32_t rows, uint32_t cols,
+                      uint32_t col_offset, uint32_t total_cols, void* dst, int domain);
+int launch_commit(gh_key* key, const void* s_dev, uint32_t cols, int domain, void* t_dev);
+int launch_check_canonical(gh_ring* ring, const void* p, size_t words, int* bad_flag_dev);
+int launch_gram(gh_ring* ring, const void* s_dev, uint32_t r, uint32_t n, int domain, void* g_dev);
+int launch_jl(gh_ring* ring, const uint8_t seed[32], uint32_t rows, const void* parts,
+              uint32_t num_parts, uint64_t cols, void* p_modq, int64_t* p_exact, int8_t* flat);
+
+}  // namespace gh
+
+#define GH_HIP(call)                                  \
+  do {                                                \
+    hipError_t _e = (call);                           \
+    if (_e != hipSuccess) {                           \
+      return _e == hipErrorOutOfMemory ? GH_E_OOM : GH_E_DEVICE; \
+    }                                                 \
+  } while (0)

@@ -1,0 +1,357 @@
+// LaBRADOR hot calls on the GPU: Gram matrix g_ij = <s_i, s_j> and the
+// ternary JL projection p = sum_part Pi_part * s_part.
+//
+// Gram (labrador/src/main_protocol/garbage.rs:76-93): every s_i is transformed
+// once, then a SYRK-like sweep over the NTT domain: one wave per (4 x 4 tile of
+// (i, j) pairs, column chunk), lane = slot, exact 96-bit accumulation for
+// q < 2^32; per-chunk partials are summed and inverse-transformed by
+// k_finalize (gh_kernels.hip). The reference's coefficient-domain dot product
+// does one NTT multiply per term; the result is the same ring element.
+//
+// JL (labrador/src/jl.rs:388-508): column col of part p is ChaCha20 keystream
+// bytes [col*pk, (col+1)*pk), pk = ceil(rows/4), key = seed, nonce = LE u32(p)
+// || 0^8, counter from 0 (RFC 8439 block function below). A block of threads
+// generates the keystream of 256 columns into LDS (one column per thread),
+// then thread m accumulates row m over those columns. For q < 2^32 the
+// accumulation is exact in int64 over centred lifts (|sum| <= cols * q/2 <
+// 2^63 for cols < 2^32) and p mod q is derived from it; for Goldilocks it is
+// modular.
+#include "gh_internal.h"
+#include "gh_ntt.h"
+#include "gh_common_kernels.h"
+
+namespace gh {
+
+// ---------------------------------------------------------------------------
+// Gram
+// ---------------------------------------------------------------------------
+
+constexpr int kGramTile = 4;
+
+__device__ __forceinline__ uint32_t pair_index(uint32_t r, uint32_t i, uint32_t j) {
+  // garbage_index (main_protocol/mod.rs:664-670): row i starts at r*i - i*(i-1)/2
+  return r * i - (i * (i - 1)) / 2 + (j - i);
+}
+
+// tile t of the upper triangle of ntile x ntile tiles -> (a, b), a <= b
+__device__ __forceinline__ void tile_coords(uint32_t t, uint32_t ntile, uint32_t& a, uint32_t& b) {
+  a = 0;
+  uint32_t rem = t;
+  while (rem >= ntile - a) { rem -= ntile - a; a++; }
+  b = a + rem;
+}
+
+__global__ __launch_bounds__(64) void k_gram_q32(FQ32 f, const uint32_t* __restrict__ s_ntt,
+                                                 uint32_t r, uint32_t n, uint32_t chunk,
+                                                 uint32_t* __restrict__ partial) {
+  const int lane = threadIdx.x;
+  const uint32_t ntile = (r + kGramTile - 1) / kGramTile;
+  uint32_t ta, tb;
+  tile_coords(blockIdx.x, ntile, ta, tb);
+  const uint32_t c0 = blockIdx.y * chunk;
+  uint32_t c1 = c0 + chunk;
+  if (c1 > n) c1 = n;
+  uint64_t lo[kGramTile][kGramTile];
+  uint32_t hi[kGramTile][kGramTile];
+#pragma unroll
+  for (int x = 0; x < kGramTile; x++)
+#pragma unroll
+    for (int y = 0; y < kGramTile; y++) { lo[x][y] = 0; hi[x][y] = 0; }
+  uint32_t ii[kGramTile], jj[kGramTile];
+#pragma unroll
+  for (int x = 0; x < kGramTile; x++) {
+    uint32_t i = ta * kGramTile + x, j = tb * kGramTile + x;
+    ii[x] = i < r ? i : r - 1;  // clamp: out-of-range rows are computed but not stored
+    jj[x] = j < r ? j : r - 1;
+  }
+  for (uint32_t c = c0; c < c1; c++) {
+    uint32_t si[kGramTile], sj[kGramTile];
+#pragma unroll
+    for (int x = 0; x < kGramTile; x++) si[x] = s_ntt[((size_t)ii[x] * n + c) * 64 + lane];
+#pragma unroll
+    for (int x = 0; x < kGramTile; x++) sj[x] = s_ntt[((size_t)jj[x] * n + c) * 64 + lane];
+#pragma unroll
+    for (int x = 0; x < kGramTile; x++)
+#pragma unroll
+      for (int y = 0; y < kGramTile; y++) mac96(lo[x][y], hi[x][y], si[x], sj[y]);
+  }
+  const uint32_t npairs = r * (r + 1) / 2;
+#pragma unroll
+  for (int x = 0; x < kGramTile; x++)
+#pragma unroll
+    for (int y = 0; y < kGramTile; y++) {
+      const uint32_t i = ta * kGramTile + x, j = tb * kGramTile + y;
+      if (i < r && j < r && i <= j) {
+        uint32_t v = f.reduce96_r2((uint32_t)lo[x][y], (uint32_t)(lo[x][y] >> 32), hi[x][y]);
+        partial[((size_t)blockIdx.y * npairs + pair_index(r, i, j)) * 64 + lane] = v;
+      }
+    }
+}
+
+__global__ __launch_bounds__(64) void k_gram_gold(FGold f, const uint64_t* __restrict__ s_ntt,
+                                                  uint32_t r, uint32_t n, uint32_t chunk,
+                                                  uint64_t* __restrict__ partial) {
+  const int lane = threadIdx.x;
+  const uint32_t ntile = (r + kGramTile - 1) / kGramTile;
+  uint32_t ta, tb;
+  tile_coords(blockIdx.x, ntile, ta, tb);
+  const uint32_t c0 = blockIdx.y * chunk;
+  uint32_t c1 = c0 + chunk;
+  if (c1 > n) c1 = n;
+  uint64_t acc[kGramTile][kGramTile];
+#pragma unroll
+  for (int x = 0; x < kGramTile; x++)
+#pragma unroll
+    for (int y = 0; y < kGramTile; y++) acc[x][y] = 0;
+  uint32_t ii[kGramTile], jj[kGramTile];
+#pragma unroll
+  for (int x = 0; x < kGramTile; x++) {
+    uint32_t i = ta * kGramTile + x, j = tb * kGramTile + x;
+    ii[x] = i < r ? i : r - 1;
+    jj[x] = j < r ? j : r - 1;
+  }
+  for (uint32_t c = c0; c < c1; c++) {
+    uint64_t si[kGramTile], sj[kGramTile];
+#pragma unroll
+    for (int x = 0; x < kGramTile; x++) si[x] = s_ntt[((size_t)ii[x] * n + c) * 64 + lane];
+#pragma unroll
+    for (int x = 0; x < kGramTile; x++) sj[x] = s_ntt[((size_t)jj[x] * n + c) * 64 + lane];
+#pragma unroll
+    for (int x = 0; x < kGramTile; x++)
+#pragma unroll
+      for (int y = 0; y < kGramTile; y++) acc[x][y] = f.add(acc[x][y], f.mul(si[x], sj[y]));
+  }
+  const uint32_t npairs = r * (r + 1) / 2;
+#pragma unroll
+  for (int x = 0; x < kGramTile; x++)
+#pragma unroll
+    for (int y = 0; y < kGramTile; y++) {
+      const uint32_t i = ta * kGramTile + x, j = tb * kGramTile + y;
+      if (i < r && j < r && i <= j)
+        partial[((size_t)blockIdx.y * npairs + pair_index(r, i, j)) * 64 + lane] = acc[x][y];
+    }
+}
+
+int launch_gram(gh_ring* ring, const void* s_dev, uint32_t r, uint32_t n, int domain,
+                void* g_dev) {
+  gh_ctx* ctx = ring->ctx;
+  const size_t wb = ring->word_bytes;
+  const uint32_t ntile = (r + kGramTile - 1) / kGramTile;
+  const uint32_t tiles = ntile * (ntile + 1) / 2;
+  const uint32_t npairs = r * (r + 1) / 2;
+  // column chunks: enough waves to cover the device a few times
+  uint32_t want = (uint32_t)(4 * ctx->num_cus * 4 / (tiles ? tiles : 1));
+  if (want < 1) want = 1;
+  if (want > n) want = n;
+  uint32_t chunk = (n + want - 1) / want;
+  uint32_t nchunks = (n + chunk - 1) / chunk;
+  const size_t s_bytes = (size_t)r * n * 64 * wb;
+  const size_t p_bytes = (size_t)nchunks * npairs * 64 * wb;
+  void* ws;
+  int st = ctx_workspace(ctx, s_bytes + p_bytes, &ws);
+  if (st) return st;
+  void* s_ntt = ws;
+  void* partial = (char*)ws + s_bytes;
+  if (domain == GH_DOMAIN_COEFF) {
+    st = launch_xform(ring, kTwistedFwd, s_dev, s_ntt, (size_t)r * n);
+    if (st) return st;
+  } else {
+    s_ntt = const_cast<void*>(s_dev);
+  }
+  dim3 grid(tiles, nchunks);
+  if (ring->kind == GH_RING_Q32) {
+    FQ32 f{(uint32_t)ring->q, ring->qinv32};
+    hipLaunchKernelGGL(k_gram_q32, grid, dim3(64), 0, ctx->stream, f, (const uint32_t*)s_ntt, r,
+                       n, chunk, (uint32_t*)partial);
+    GH_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_finalize<FQ32>), dim3(npairs), dim3(1024), 0, ctx->stream, f,
+                       (const uint32_t*)ring->dev_tab, (const uint32_t*)partial, nchunks, npairs,
+                       (int)kTabUntwistMac, (uint32_t*)g_dev);
+  } else {
+    hipLaunchKernelGGL(k_gram_gold, grid, dim3(64), 0, ctx->stream, FGold{},
+                       (const uint64_t*)s_ntt, r, n, chunk, (uint64_t*)partial);
+    GH_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_finalize<FGold>), dim3(npairs), dim3(1024), 0, ctx->stream, FGold{},
+                       (const uint64_t*)ring->dev_tab, (const uint64_t*)partial, nchunks, npairs,
+                       (int)kTabUntwist, (uint64_t*)g_dev);
+  }
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// ChaCha20 (RFC 8439 section 2.3) and JL
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t rotl(uint32_t x, int n) { return __builtin_rotateleft32(x, n); }
+
+#define GH_QR(a, b, c, d)                 \
+  a += b; d ^= a; d = rotl(d, 16);        \
+  c += d; b ^= c; b = rotl(b, 12);        \
+  a += b; d ^= a; d = rotl(d, 8);         \
+  c += d; b ^= c; b = rotl(b, 7);
+
+struct ChaKey {
+  uint32_t k[8];
+  uint32_t n0;  // nonce word 0 = part index; words 1, 2 are zero
+};
+
+__device__ __forceinline__ void chacha_block(const ChaKey& key, uint32_t counter, uint32_t (&out)[16]) {
+  uint32_t x0 = 0x61707865, x1 = 0x3320646e, x2 = 0x79622d32, x3 = 0x6b206574;
+  uint32_t x4 = key.k[0], x5 = key.k[1], x6 = key.k[2], x7 = key.k[3];
+  uint32_t x8 = key.k[4], x9 = key.k[5], x10 = key.k[6], x11 = key.k[7];
+  uint32_t x12 = counter, x13 = key.n0, x14 = 0, x15 = 0;
+#pragma unroll
+  for (int i = 0; i < 10; i++) {
+    GH_QR(x0, x4, x8, x12) GH_QR(x1, x5, x9, x13) GH_QR(x2, x6, x10, x14) GH_QR(x3, x7, x11, x15)
+    GH_QR(x0, x5, x10, x15) GH_QR(x1, x6, x11, x12) GH_QR(x2, x7, x8, x13) GH_QR(x3, x4, x9, x14)
+  }
+  out[0] = x0 + 0x61707865; out[1] = x1 + 0x3320646e; out[2] = x2 + 0x79622d32; out[3] = x3 + 0x6b206574;
+  out[4] = x4 + key.k[0]; out[5] = x5 + key.k[1]; out[6] = x6 + key.k[2]; out[7] = x7 + key.k[3];
+  out[8] = x8 + key.k[4]; out[9] = x9 + key.k[5]; out[10] = x10 + key.k[6]; out[11] = x11 + key.k[7];
+  out[12] = x12 + counter; out[13] = x13 + key.n0; out[14] = x14; out[15] = x15;
+}
+
+constexpr int kJlCols = 256;  // columns per LDS sweep (one per thread)
+
+// grid: (chunks, num_parts); block: 256 threads. Writes per-block row sums to
+// part_acc[(part * chunks + chunk) * rows + m].
+template <class W, bool GOLD>
+__global__ __launch_bounds__(256) void k_jl(ChaKey key0, uint32_t rows, uint64_t q,
+                                            const W* __restrict__ parts, uint64_t cols,
+                                            uint64_t chunk_cols, uint32_t d,
+                                            int64_t* __restrict__ part_acc,
+                                            int8_t* __restrict__ flat, uint32_t num_parts) {
+  __shared__ uint8_t ks[kJlCols][64];
+  __shared__ int64_t sval[kJlCols];
+  __shared__ uint64_t sres[kJlCols];
+  const int t = threadIdx.x;
+  const uint32_t part = blockIdx.y;
+  const uint32_t pk = (rows + 3) / 4;
+  ChaKey key = key0;
+  key.n0 = part;
+  const W* sp = parts + (size_t)part * cols;
+  const uint64_t c_begin = (uint64_t)blockIdx.x * chunk_cols;
+  uint64_t c_end = c_begin + chunk_cols;
+  if (c_end > cols) c_end = cols;
+  int64_t acc = 0;
+  uint64_t accm = 0;  // Goldilocks: modular
+  const uint64_t n = cols / d;
+  for (uint64_t c0 = c_begin; c0 < c_end; c0 += kJlCols) {
+    const uint64_t col = c0 + t;
+    if (col < c_end) {
+      // keystream bytes [col*pk, col*pk + pk): at most two 64-byte blocks
+      const uint64_t off = col * pk;
+      const uint32_t b0 = (uint32_t)(off >> 6);
+      const uint32_t o0 = (uint32_t)(off & 63);
+      uint32_t blk[16];
+      chacha_block(key, b0, blk);
+      const uint8_t* bytes = reinterpret_cast<const uint8_t*>(blk);
+      uint32_t first = 64 - o0 < pk ? 64 - o0 : pk;
+      for (uint32_t i = 0; i < first; i++) ks[t][i] = bytes[o0 + i];
+      if (first < pk) {
+        chacha_block(key, b0 + 1, blk);
+        for (uint32_t i = first; i < pk; i++) ks[t][i] = bytes[i - first];
+      }
+      const W v = sp[col];
+      sres[t] = (uint64_t)v;
+      sval[t] = (uint64_t)v > q / 2 ? (int64_t)((uint64_t)v - q) : (int64_t)(uint64_t)v;
+    }
+    __syncthreads();
+    const uint64_t ncol = c_end - c0 < (uint64_t)kJlCols ? c_end - c0 : (uint64_t)kJlCols;
+    if ((uint32_t)t < rows) {
+      const int byte_i = t >> 2, sh = 2 * (t & 3);
+      for (uint64_t c = 0; c < ncol; c++) {
+        const int code = (ks[c][byte_i] >> sh) & 3;
+        if (GOLD) {
+          const uint64_t v = sres[c];
+          if (code == 0) {
+            uint64_t s = accm + v;
+            accm = s < accm ? s + kGoldEps : (s >= kGoldilocks ? s - kGoldilocks : s);
+          } else if (code == 1) {
+            accm = accm >= v ? accm - v : accm - v - kGoldEps;
+          }
+        } else {
+          const int64_t v = sval[c];
+          acc += code == 0 ? v : (code == 1 ? -v : 0);
+        }
+        if (flat) {
+          const uint64_t cc = c0 + c;
+          const uint64_t poly = cc / d, coeff = cc % d;
+          flat[(uint64_t)t * num_parts * cols + ((uint64_t)part * d + coeff) * n + poly] =
+              (int8_t)(code == 0 ? 1 : (code == 1 ? -1 : 0));
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if ((uint32_t)t < rows)
+    part_acc[((size_t)part * gridDim.x + blockIdx.x) * rows + t] = GOLD ? (int64_t)accm : acc;
+}
+
+// p[m] = sum over blocks; q32: exact int64 then mod q; Goldilocks: modular.
+template <bool GOLD>
+__global__ void k_jl_reduce(const int64_t* __restrict__ part_acc, uint32_t nblk, uint32_t rows,
+                            uint64_t q, int64_t* __restrict__ p_exact, void* __restrict__ p_modq) {
+  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= rows) return;
+  if (GOLD) {
+    uint64_t a = 0;
+    for (uint32_t b = 0; b < nblk; b++) {
+      uint64_t v = (uint64_t)part_acc[(size_t)b * rows + m];
+      uint64_t s = a + v;
+      a = s < a ? s + kGoldEps : (s >= kGoldilocks ? s - kGoldilocks : s);
+    }
+    ((uint64_t*)p_modq)[m] = a;
+  } else {
+    int64_t a = 0;
+    for (uint32_t b = 0; b < nblk; b++) a += part_acc[(size_t)b * rows + m];
+    if (p_exact) p_exact[m] = a;
+    int64_t r = a % (int64_t)q;
+    if (r < 0) r += (int64_t)q;
+    ((uint32_t*)p_modq)[m] = (uint32_t)r;
+  }
+}
+
+int launch_jl(gh_ring* ring, const uint8_t seed[32], uint32_t rows, const void* parts,
+              uint32_t num_parts, uint64_t cols, void* p_modq, int64_t* p_exact, int8_t* flat) {
+  gh_ctx* ctx = ring->ctx;
+  ChaKey key;
+  for (int i = 0; i < 8; i++)
+    key.k[i] = (uint32_t)seed[4 * i] | ((uint32_t)seed[4 * i + 1] << 8) |
+               ((uint32_t)seed[4 * i + 2] << 16) | ((uint32_t)seed[4 * i + 3] << 24);
+  key.n0 = 0;
+  // chunks per part so that the device holds a few blocks per CU
+  uint64_t want = (uint64_t)ctx->num_cus * 4 / (num_parts ? num_parts : 1);
+  if (want < 1) want = 1;
+  uint64_t max_chunks = (cols + kJlCols - 1) / kJlCols;
+  if (want > max_chunks) want = max_chunks;
+  uint64_t chunk_cols = (cols + want - 1) / want;
+  chunk_cols = (chunk_cols + kJlCols - 1) / kJlCols * kJlCols;
+  uint32_t chunks = (uint32_t)((cols + chunk_cols - 1) / chunk_cols);
+  const uint32_t nblk = chunks * num_parts;
+  void* ws;
+  int st = ctx_workspace(ctx, (size_t)nblk * rows * sizeof(int64_t), &ws);
+  if (st) return st;
+  dim3 grid(chunks, num_parts);
+  const bool gold = ring->kind == GH_RING_GOLDILOCKS;
+  if (!gold)
+    hipLaunchKernelGGL((k_jl<uint32_t, false>), grid, dim3(256), 0, ctx->stream, key, rows,
+                       ring->q, (const uint32_t*)parts, cols, chunk_cols, ring->d, (int64_t*)ws,
+                       flat, num_parts);
+  else
+    hipLaunchKernelGGL((k_jl<uint64_t, true>), grid, dim3(256), 0, ctx->stream, key, rows,
+                       ring->q, (const uint64_t*)parts, cols, chunk_cols, ring->d, (int64_t*)ws,
+                       flat, num_parts);
+  GH_HIP(hipGetLastError());
+  if (!gold)
+    hipLaunchKernelGGL((k_jl_reduce<false>), dim3((rows + 255) / 256), dim3(256), 0, ctx->stream,
+                       (const int64_t*)ws, nblk, rows, ring->q, p_exact, p_modq);
+  else
+    hipLaunchKernelGGL((k_jl_reduce<true>), dim3((rows + 255) / 256), dim3(256), 0, ctx->stream,
+                       (const int64_t*)ws, nblk, rows, ring->q, p_exact, p_modq);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+}  // namespace gh

@@ -1,0 +1,148 @@
+// Register-resident length-64 transforms for one polynomial per thread, and a
+// wave-cooperative variant with one slot per lane.
+//
+// Thread-per-polynomial: all 64 coefficients live in VGPRs; every index below
+// is a compile-time constant after unrolling, so the bit-reversal permutations
+// cost nothing and every twiddle index is wave-uniform (the table pointer is a
+// kernel argument), which lets the compiler fetch twiddles with scalar loads
+// into SGPRs — one twiddle set per wavefront.
+//
+// Output order matches the reference's twisted NTT (poly/ntt.rs:75-87): slot k
+// holds a(psi^(2k+1)) in natural k order, because the forward path is the same
+// twist + bit-reverse + DIT sequence (arith/ntt.rs:292-312). The inverse is the
+// DIF + bit-reverse sequence of arith/ntt.rs:314-335 with d^-1 folded into the
+// untwist table.
+#pragma once
+
+#include "gh_field.h"
+
+namespace gh {
+
+__host__ __device__ constexpr int bitrev6(int i) {
+  return ((i & 1) << 5) | ((i & 2) << 3) | ((i & 4) << 1) | ((i & 8) >> 1) | ((i & 16) >> 3) |
+         ((i & 32) >> 5);
+}
+
+// Forward transform in place. TWIST: multiply by psi^i first (negacyclic);
+// otherwise the plain cyclic NTT of the NTTRing seam (arith/ntt.rs:341-384).
+template <bool TWIST, class F>
+__device__ __forceinline__ void fwd64(const F& f, typename F::W (&x)[64],
+                                      const typename F::W* __restrict__ tab) {
+  using W = typename F::W;
+  constexpr int TW = F::kTwWords;
+  W y[64];
+#pragma unroll
+  for (int i = 0; i < 64; i++) {
+    W v = x[i];
+    if (TWIST && i != 0) v = f.mul_tw(v, tab + (kTabTwist + i) * TW);
+    y[bitrev6(i)] = v;
+  }
+#pragma unroll
+  for (int s = 0; s < 6; s++) {
+    const int half = 1 << s, len = 2 << s;
+#pragma unroll
+    for (int st = 0; st < 64; st += len) {
+#pragma unroll
+      for (int j = 0; j < half; j++) {
+        W u = y[st + j];
+        W t = y[st + half + j];
+        if (j != 0) t = f.mul_tw(t, tab + (kTabFwd + half - 1 + j) * TW);
+        y[st + j] = f.add(u, t);
+        y[st + half + j] = f.sub(u, t);
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 64; i++) x[i] = y[i];
+}
+
+// Inverse transform in place; the final per-slot multiplier comes from the
+// table at `post` (untwist, untwist*R, d^-1, ...).
+template <class F>
+__device__ __forceinline__ void inv64(const F& f, typename F::W (&x)[64],
+                                      const typename F::W* __restrict__ tab, int post) {
+  using W = typename F::W;
+  constexpr int TW = F::kTwWords;
+#pragma unroll
+  for (int s = 0; s < 6; s++) {
+    const int len = 64 >> s, half = len / 2, off = 64 - (64 >> s);
+#pragma unroll
+    for (int st = 0; st < 64; st += len) {
+#pragma unroll
+      for (int j = 0; j < half; j++) {
+        W e = x[st + j], o = x[st + half + j];
+        x[st + j] = f.add(e, o);
+        W dlt = f.sub(e, o);
+        if (j != 0) dlt = f.mul_tw(dlt, tab + (kTabInv + off + j) * TW);
+        x[st + half + j] = dlt;
+      }
+    }
+  }
+  W y[64];
+#pragma unroll
+  for (int i = 0; i < 64; i++) y[i] = f.mul_tw(x[bitrev6(i)], tab + (post + i) * TW);
+#pragma unroll
+  for (int i = 0; i < 64; i++) x[i] = y[i];
+}
+
+// 64-bit safe cross-lane exchange (ds_bpermute works on 32-bit words).
+__device__ __forceinline__ uint32_t lane_read(uint32_t v, int src_lane) {
+  return (uint32_t)__shfl((int)v, src_lane, 64);
+}
+__device__ __forceinline__ uint64_t lane_read(uint64_t v, int src_lane) {
+  uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src_lane, 64);
+  uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src_lane, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Wave-cooperative inverse: lane k holds slot k of one polynomial (natural
+// order); returns coefficient k. Same DIF stages as inv64, with the butterfly
+// partner fetched across lanes; twiddles are per-lane table reads.
+template <class F>
+__device__ __forceinline__ typename F::W wave_inv64(const F& f, typename F::W x,
+                                                    const typename F::W* __restrict__ tab,
+                                                    int post, int lane) {
+  using W = typename F::W;
+  constexpr int TW = F::kTwWords;
+#pragma unroll
+  for (int s = 0; s < 6; s++) {
+    const int half = 32 >> s, off = 64 - (64 >> s);
+    W other = lane_read(x, lane ^ half);
+    const int j = lane & (half - 1);
+    if ((lane & half) == 0) {
+      x = f.add(x, other);
+    } else {
+      W dlt = f.sub(other, x);
+      x = j ? f.mul_tw(dlt, tab + (kTabInv + off + j) * TW) : dlt;
+    }
+  }
+  W y = lane_read(x, bitrev6(lane));
+  return f.mul_tw(y, tab + (post + lane) * TW);
+}
+
+// Wave-cooperative forward twisted transform (lane k holds coefficient k).
+template <bool TWIST, class F>
+__device__ __forceinline__ typename F::W wave_fwd64(const F& f, typename F::W x,
+                                                    const typename F::W* __restrict__ tab,
+                                                    int lane) {
+  using W = typename F::W;
+  constexpr int TW = F::kTwWords;
+  if (TWIST && lane) x = f.mul_tw(x, tab + (kTabTwist + lane) * TW);
+  x = lane_read(x, bitrev6(lane));
+#pragma unroll
+  for (int s = 0; s < 6; s++) {
+    const int half = 1 << s;
+    W other = lane_read(x, lane ^ half);
+    const int j = lane & (half - 1);
+    if ((lane & half) == 0) {
+      W t = j ? f.mul_tw(other, tab + (kTabFwd + half - 1 + j) * TW) : other;
+      x = f.add(x, t);
+    } else {
+      W t = j ? f.mul_tw(x, tab + (kTabFwd + half - 1 + j) * TW) : x;
+      x = f.sub(other, t);
+    }
+  }
+  return x;
+}
+
+}  // namespace gh

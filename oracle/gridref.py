@@ -1,0 +1,268 @@
+"""ctypes binding of the oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY. The oracle is the CPU restatement of the reference
+path that the HIP library is checked against; only tests/, the smoke check and
+bench.py's cpu_baseline leg load it. Arrays are numpy uint64 canonical residues.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+U64P = ctypes.POINTER(ctypes.c_uint64)
+I64P = ctypes.POINTER(ctypes.c_int64)
+I8P = ctypes.POINTER(ctypes.c_int8)
+U8P = ctypes.POINTER(ctypes.c_uint8)
+
+
+def build() -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        u64, i32, sz = ctypes.c_uint64, ctypes.c_int, ctypes.c_size_t
+        for name in ("gr_add", "gr_sub", "gr_mul", "gr_pow"):
+            getattr(L, name).restype = u64
+            getattr(L, name).argtypes = [u64, u64, u64]
+        for name in ("gr_inv", "gr_to_montgomery", "gr_from_montgomery"):
+            getattr(L, name).restype = u64
+            getattr(L, name).argtypes = [u64, u64]
+        L.gr_q_inv_neg.restype = u64
+        L.gr_q_inv_neg.argtypes = [u64]
+        L.gr_max_ntt_size.restype = u64
+        L.gr_max_ntt_size.argtypes = [u64]
+        L.gr_find_generator.restype = i32
+        L.gr_find_generator.argtypes = [u64, U64P]
+        L.gr_root_of_unity.restype = i32
+        L.gr_root_of_unity.argtypes = [u64, u64, U64P]
+        for name in ("gr_ntt_forward", "gr_ntt_inverse", "gr_twisted_forward", "gr_twisted_inverse"):
+            getattr(L, name).restype = i32
+            getattr(L, name).argtypes = [u64, u64, U64P, sz]
+        for name in ("gr_negacyclic_mul", "gr_negacyclic_mul_naive"):
+            getattr(L, name).restype = i32
+            getattr(L, name).argtypes = [u64, u64, U64P, U64P, U64P, sz]
+        L.gr_commit_prepared.restype = i32
+        L.gr_commit_prepared.argtypes = [u64, u64, ctypes.c_uint32, U64P, ctypes.c_uint32, U64P,
+                                         ctypes.c_uint32, i32, U64P, U64P, U64P]
+        L.gr_commit_coeff.restype = i32
+        L.gr_commit_coeff.argtypes = [u64, u64, ctypes.c_uint32, U64P, ctypes.c_uint32, U64P,
+                                      ctypes.c_uint32, U64P, U64P, U64P]
+        L.gr_gram.restype = i32
+        L.gr_gram.argtypes = [u64, u64, ctypes.c_uint32, ctypes.c_uint32, U64P, U64P, i32]
+        L.gr_jl_project.restype = i32
+        L.gr_jl_project.argtypes = [u64, U8P, ctypes.c_uint32, ctypes.c_uint32, u64, u64, U64P,
+                                    U64P, I64P, I8P]
+        L.gr_chacha20_block.restype = None
+        L.gr_chacha20_block.argtypes = [U8P, ctypes.c_uint32, U8P, U8P]
+        L.gr_avx2_available.restype = i32
+        L.gr_avx2_commit.restype = i32
+        L.gr_avx2_commit.argtypes = [u64, i32, ctypes.c_uint32, ctypes.c_uint32, U64P, U64P, U64P, i32]
+        L.gr_avx2_prepare.restype = ctypes.c_void_p
+        L.gr_avx2_prepare.argtypes = [u64, i32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, U64P]
+        L.gr_avx2_load_witness.restype = None
+        L.gr_avx2_load_witness.argtypes = [ctypes.c_void_p, U64P]
+        L.gr_avx2_commit_run.restype = None
+        L.gr_avx2_commit_run.argtypes = [ctypes.c_void_p, i32]
+        L.gr_avx2_result.restype = None
+        L.gr_avx2_result.argtypes = [ctypes.c_void_p, U64P]
+        L.gr_avx2_free.restype = None
+        L.gr_avx2_free.argtypes = [ctypes.c_void_p]
+        _lib = L
+    return _lib
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        super().__init__(f"{what}: status {code}")
+        self.code = code
+
+
+def _p(a: np.ndarray, t=U64P):
+    return a.ctypes.data_as(t)
+
+
+def _u64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint64))
+
+
+def _check(code: int, what: str):
+    if code != 0:
+        raise OracleError(code, what)
+
+
+def add(q, a, b):
+    return int(lib().gr_add(q, a, b))
+
+
+def sub(q, a, b):
+    return int(lib().gr_sub(q, a, b))
+
+
+def mul(q, a, b):
+    return int(lib().gr_mul(q, a, b))
+
+
+def pow_(q, a, e):
+    return int(lib().gr_pow(q, a, e))
+
+
+def inv(q, a):
+    return int(lib().gr_inv(q, a))
+
+
+def to_montgomery(q, a):
+    return int(lib().gr_to_montgomery(q, a))
+
+
+def from_montgomery(q, a):
+    return int(lib().gr_from_montgomery(q, a))
+
+
+def q_inv_neg(q):
+    return int(lib().gr_q_inv_neg(q))
+
+
+def max_ntt_size(q):
+    return int(lib().gr_max_ntt_size(q))
+
+
+def find_generator(q) -> int:
+    out = ctypes.c_uint64()
+    _check(lib().gr_find_generator(q, ctypes.byref(out)), "find_generator")
+    return out.value
+
+
+def root_of_unity(q, n) -> int:
+    out = ctypes.c_uint64()
+    _check(lib().gr_root_of_unity(q, n, ctypes.byref(out)), "root_of_unity")
+    return out.value
+
+
+def _xform(name, q, d, values):
+    v = _u64(values).copy()
+    count = v.size // d if d else 0
+    _check(getattr(lib(), name)(q, d, _p(v), count), name)
+    return v
+
+
+def ntt_forward(q, d, values):
+    return _xform("gr_ntt_forward", q, d, values)
+
+
+def ntt_inverse(q, d, values):
+    return _xform("gr_ntt_inverse", q, d, values)
+
+
+def twisted_forward(q, d, values):
+    return _xform("gr_twisted_forward", q, d, values)
+
+
+def twisted_inverse(q, d, values):
+    return _xform("gr_twisted_inverse", q, d, values)
+
+
+def xform_status(name, q, d, values) -> int:
+    v = _u64(values).copy()
+    return int(getattr(lib(), "gr_" + name)(q, d, _p(v), max(1, v.size // max(d, 1))))
+
+
+def negacyclic_mul(q, d, a, b, naive=False):
+    a, b = _u64(a), _u64(b)
+    out = np.zeros_like(a)
+    fn = lib().gr_negacyclic_mul_naive if naive else lib().gr_negacyclic_mul
+    _check(fn(q, d, _p(a), _p(b), _p(out), a.size // d), "negacyclic_mul")
+    return out
+
+
+def commit(q, d, a_msg, a_open, message, randomness, key_domain=0, path="prepared"):
+    """t = A_msg*m + A_open*r. a_msg: (rows, msg_len, d), a_open: (rows, open_len, d)."""
+    a_msg, a_open = _u64(a_msg), _u64(a_open)
+    m, r = _u64(message), _u64(randomness)
+    rows, msg_len = a_msg.shape[0], a_msg.shape[1]
+    open_len = a_open.shape[1]
+    out = np.zeros((rows, d), dtype=np.uint64)
+    if path == "prepared":
+        st = lib().gr_commit_prepared(q, d, rows, _p(a_msg), msg_len, _p(a_open), open_len,
+                                      key_domain, _p(m), _p(r), _p(out))
+    else:
+        st = lib().gr_commit_coeff(q, d, rows, _p(a_msg), msg_len, _p(a_open), open_len,
+                                   _p(m), _p(r), _p(out))
+    _check(st, "commit")
+    return out
+
+
+def gram(q, d, s, ntt_once=True):
+    """s: (r, n, d) -> (r(r+1)/2, d) upper triangle, row-major."""
+    s = _u64(s)
+    r, n = s.shape[0], s.shape[1]
+    out = np.zeros((r * (r + 1) // 2, d), dtype=np.uint64)
+    _check(lib().gr_gram(q, d, r, n, _p(s), _p(out), 1 if ntt_once else 0), "gram")
+    return out
+
+
+def jl_project(q, seed: bytes, rows, parts, d, want_flat=False):
+    """parts: (num_parts, cols) residues. Returns (p_modq, p_exact, flat|None)."""
+    parts = _u64(parts)
+    num_parts, cols = parts.shape
+    sd = np.frombuffer(bytes(seed), dtype=np.uint8).copy()
+    pm = np.zeros(rows, dtype=np.uint64)
+    pe = np.zeros(rows, dtype=np.int64)
+    flat = np.zeros(rows * num_parts * cols, dtype=np.int8) if want_flat else None
+    _check(lib().gr_jl_project(q, _p(sd, U8P), rows, num_parts, cols, d, _p(parts), _p(pm),
+                               _p(pe, I64P), _p(flat, I8P) if want_flat else None), "jl_project")
+    return pm, pe, flat
+
+
+def chacha20_block(key: bytes, counter: int, nonce: bytes) -> bytes:
+    k = np.frombuffer(bytes(key), dtype=np.uint8).copy()
+    n = np.frombuffer(bytes(nonce), dtype=np.uint8).copy()
+    out = np.zeros(64, dtype=np.uint8)
+    lib().gr_chacha20_block(_p(k, U8P), counter, _p(n, U8P), _p(out, U8P))
+    return out.tobytes()
+
+
+class Avx2Commit:
+    """The reference's AVX2 prepared-commit path, restated (CPU baseline)."""
+
+    def __init__(self, q, limb_bits, a_ntt, d=64):
+        a_ntt = _u64(a_ntt)
+        self.rows, self.n = a_ntt.shape[0], a_ntt.shape[1]
+        self.d = d
+        self._keep = a_ntt
+        self.h = lib().gr_avx2_prepare(q, limb_bits, self.rows, self.n, d, _p(a_ntt))
+        if not self.h:
+            raise OracleError(-3, "avx2 prepare")
+
+    def load(self, s):
+        s = _u64(s)
+        lib().gr_avx2_load_witness(self.h, _p(s))
+
+    def run(self, threads=1):
+        lib().gr_avx2_commit_run(self.h, threads)
+
+    def result(self):
+        out = np.zeros((self.rows, self.d), dtype=np.uint64)
+        lib().gr_avx2_result(self.h, _p(out))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().gr_avx2_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
