@@ -109,6 +109,11 @@ def main():
     value = args.steps / elapsed
     log(f"plain: {value:,.0f} commits/s ({1e6 / value:.1f} us/commit)")
 
+    tuning = ctx.get_tuning()
+    if tuning["fused"]:
+        mac_kernel = "k_commit_fused_q32"  # witness transform + MAC in one kernel
+    else:
+        mac_kernel = "k_mac4_q32" if tuning["mac_variant"] == 1 else "k_mac_q32"
     # ---- roofline: MAC kernel duration from HIP events on the library stream ----
     ctx.profile(True)
     for i in range(args.steps):
@@ -140,6 +145,10 @@ def main():
             step(i, 0)
         ctx.synchronize()
         extra["single_key_commits_per_s"] = round(args.steps / (time.perf_counter() - a), 1)
+        # measured HBM ceiling: streaming read / copy over 1 GiB (beyond the Infinity Cache)
+        rd, cp = ctx.measure_bandwidth(1 << 30, 5)
+        extra["hbm_ceiling_GBps"] = {"stream_read": round(rd, 1), "stream_copy": round(cp, 1)}
+        log(f"measured HBM ceiling: read {rd:,.0f} GB/s, copy {cp:,.0f} GB/s")
         # batched negacyclic product, 2^18 pairs (device buffers)
         pairs = 1 << 18
         pa = ctx.to_device(rng.integers(0, Q, size=(pairs, D), dtype=np.uint32))
@@ -209,8 +218,12 @@ def main():
                        "p99": round(float(np.percentile(lat, 99)), 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_mac_q32", "kernel_us": round(mac_ms * 1e3, 2),
-                     "algorithmic_bytes": ALGO_BYTES},
+                     "kernel": mac_kernel, "kernel_us": round(mac_ms * 1e3, 2),
+                     "algorithmic_bytes": ALGO_BYTES,
+                     "frac_of_measured_read_ceiling":
+                         round(achieved / extra["hbm_ceiling_GBps"]["stream_read"], 4)
+                         if "hbm_ceiling_GBps" in extra else None},
+        "tuning": tuning,
         "phases_us": {"witness_ntt": round(ntt_ms * 1e3, 2), "mac": round(mac_ms * 1e3, 2),
                       "finalize": round(fin_ms * 1e3, 2)},
         "cpu_baseline": cpu,

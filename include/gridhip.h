@@ -100,6 +100,25 @@ GH_API int gh_ctx_num_cus(gh_ctx *ctx, int *out);
 GH_API int gh_ctx_profile_enable(gh_ctx *ctx, int on);
 GH_API int gh_ctx_profile_read(gh_ctx *ctx, int phase, double *total_ms, uint64_t *count);
 
+/* Launch-shape tuning of a context (written by the autotuner, read back for
+ * reports). Keys: "mac_variant" (0 dword / 1 16-byte loads, q32 MAC),
+ * "mac_waves" (4, 8, 16), "mac_blocks_per_cu" (1..8), "quad_max" (largest
+ * witness batch transformed four lanes per polynomial), "fused" (1: q32
+ * commitments transform the witness inside the MAC kernel and pipeline the
+ * finalize on a second stream; 0: separate kernels). Values are checked;
+ * an unknown key or value gives GH_E_INVALID_ARGUMENT. Every setting yields
+ * identical results; only the speed changes. */
+GH_API int gh_ctx_set_tuning(gh_ctx *ctx, const char *key, int64_t value);
+GH_API int gh_ctx_get_tuning(gh_ctx *ctx, const char *key, int64_t *value);
+
+/* Achievable HBM bandwidth of the context's device: a streaming read and a
+ * streaming copy over `bytes` (rounded down to 16 B) of freshly allocated
+ * memory, best of `reps` timed kernels each, in GB/s (1e9 B/s; a copy counts
+ * its read and its write). Measurement support for the roofline report; no
+ * reference counterpart. */
+GH_API int gh_measure_bandwidth(gh_ctx *ctx, size_t bytes, int reps, double *read_gbps,
+                                double *copy_gbps);
+
 /* device memory, so callers without a HIP runtime of their own can keep
  * inputs resident in HBM */
 GH_API int gh_alloc(gh_ctx *ctx, size_t bytes, void **dev_ptr);

@@ -40,7 +40,9 @@ STATUS = {
 # every symbol include/gridhip.h declares
 EXPORTS = (
     "gh_status_string", "gh_version", "gh_device_count", "gh_ctx_create", "gh_ctx_destroy",
-    "gh_ctx_synchronize", "gh_ctx_num_cus", "gh_ctx_profile_enable", "gh_ctx_profile_read", "gh_alloc", "gh_free", "gh_upload", "gh_download",
+    "gh_ctx_synchronize", "gh_ctx_num_cus", "gh_ctx_profile_enable", "gh_ctx_profile_read",
+    "gh_ctx_set_tuning", "gh_ctx_get_tuning", "gh_measure_bandwidth", "gh_alloc", "gh_free",
+    "gh_upload", "gh_download",
     "gh_check_canonical", "gh_ring_create", "gh_ring_destroy", "gh_ring_query", "gh_ntt_forward",
     "gh_ntt_inverse", "gh_twisted_forward", "gh_twisted_inverse", "gh_negacyclic_mul",
     "gh_pointwise_mul", "gh_key_create", "gh_key_destroy", "gh_key_dims", "gh_commit",
@@ -96,6 +98,10 @@ def lib():
         "gh_ctx_synchronize": (i32, [vp]),
         "gh_ctx_num_cus": (i32, [vp, ctypes.POINTER(i32)]),
         "gh_ctx_profile_enable": (i32, [vp, i32]),
+        "gh_ctx_set_tuning": (i32, [vp, ctypes.c_char_p, ctypes.c_int64]),
+        "gh_measure_bandwidth": (i32, [vp, sz, i32, ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_double)]),
+        "gh_ctx_get_tuning": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]),
         "gh_ctx_profile_read": (i32, [vp, i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64)]),
         "gh_alloc": (i32, [vp, sz, ctypes.POINTER(vp)]),
         "gh_free": (i32, [vp, vp]),
@@ -146,11 +152,27 @@ def device_count() -> int:
 class Context:
     """gh_ctx: one device, one stream, grow-only workspace."""
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, apply_tuning: bool = True):
         h = ctypes.c_void_p()
         _check(lib().gh_ctx_create(device, ctypes.byref(h)), "gh_ctx_create")
         self.h = h
         self.device = device
+        self.tuning_source = None
+        if apply_tuning:
+            self._apply_tuning_file()
+
+    def _apply_tuning_file(self):
+        """Apply lattica_amd/tuning.json (written by lattica_amd.autotune) if present."""
+        import json
+        path = os.path.join(_HERE, "tuning.json")
+        if not os.path.exists(path):
+            return
+        with open(path) as fh:
+            doc = json.load(fh)
+        if doc.get("device_cus") not in (None, self.num_cus):
+            return  # tuned on a different device
+        self.set_tuning(**doc.get("best", {}))
+        self.tuning_source = path
 
     @property
     def num_cus(self) -> int:
@@ -160,6 +182,27 @@ class Context:
 
     def synchronize(self):
         _check(lib().gh_ctx_synchronize(self.h), "gh_ctx_synchronize")
+
+    TUNING_KEYS = ("mac_variant", "mac_waves", "mac_blocks_per_cu", "quad_max", "fused")
+
+    def set_tuning(self, **kv):
+        for k, v in kv.items():
+            _check(lib().gh_ctx_set_tuning(self.h, k.encode(), int(v)), f"gh_ctx_set_tuning({k})")
+
+    def get_tuning(self) -> dict:
+        out = {}
+        for k in self.TUNING_KEYS:
+            v = ctypes.c_int64()
+            _check(lib().gh_ctx_get_tuning(self.h, k.encode(), ctypes.byref(v)), "gh_ctx_get_tuning")
+            out[k] = v.value
+        return out
+
+    def measure_bandwidth(self, nbytes: int = 1 << 30, reps: int = 5):
+        """(read GB/s, copy GB/s) of streaming kernels over nbytes of HBM."""
+        r, c = ctypes.c_double(), ctypes.c_double()
+        _check(lib().gh_measure_bandwidth(self.h, nbytes, reps, ctypes.byref(r), ctypes.byref(c)),
+               "gh_measure_bandwidth")
+        return r.value, c.value
 
     def profile(self, on: bool):
         _check(lib().gh_ctx_profile_enable(self.h, 1 if on else 0), "gh_ctx_profile_enable")

@@ -1,0 +1,114 @@
+"""Autotuner for the launch shapes of libgridhip.so.
+
+Sweeps the tunable launch parameters of a context (gh_ctx_set_tuning) on a
+representative workload, times each setting with the library's own event
+timing plus wall clock, checks that every setting gives bit-identical results,
+and writes the fastest setting to tuning.json next to the library. Context()
+applies that file when it exists.
+
+    python -m lattica_amd.autotune [--quick]
+
+Workload: plain Ajtai commitments at kappa = 16, n = 2^15, d = 64, q32, with
+three resident keys in rotation (so the key is read from HBM, not from the
+Infinity Cache), the shape the library is deployed for.
+"""
+from __future__ import annotations
+
+import argparse
+import itertools
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TUNING_FILE = os.path.join(HERE, "tuning.json")
+
+
+def _witness(rng, q, n, d=64):
+    x = rng.integers(-128, 128, size=(n, d), dtype=np.int64)
+    return np.where(x < 0, q + x, x).astype(np.uint32)
+
+
+def tune_commit(ctx, kappa=16, n=1 << 15, reps=60, quick=False, log=print):
+    import lattica_amd as la
+    q = la.Q32
+    ring = la.Ring(ctx, q, 64)
+    rng = np.random.default_rng(7)
+    keys = [la.Key(ring, rng.integers(0, q, size=(kappa, n, 64), dtype=np.uint32),
+                   domain=la.DOMAIN_NTT) for _ in range(3)]
+    ws = [ctx.to_device(_witness(rng, q, n)) for _ in range(4)]
+    ts = [ctx.alloc(kappa * 64 * 4) for _ in range(4)]
+
+    bpcs = [1, 2] if quick else [1, 2, 4]
+    configs = []
+    # fused transform+MAC (16-byte loads): block shape only
+    for waves, bpc in itertools.product([16, 8, 4], bpcs):
+        configs.append({"fused": 1, "mac_variant": 1, "mac_waves": waves,
+                        "mac_blocks_per_cu": bpc, "quad_max": 1 << 17})
+    # separate kernels: MAC load width, block shape, witness transform kernel
+    for variant, waves, bpc, quad in itertools.product([1, 0], [16, 8], bpcs, [1 << 17, 0]):
+        if variant == 0 and waves != 16:
+            continue  # the dword kernel has a fixed 16-wave block
+        configs.append({"fused": 0, "mac_variant": variant, "mac_waves": waves,
+                        "mac_blocks_per_cu": bpc, "quad_max": quad})
+    ref = None
+    results = []
+    for cfg in configs:
+        ctx.set_tuning(**cfg)
+        for i in range(6):
+            keys[i % 3].commit_device(ws[i % 4], n, ts[i % 4])
+        ctx.synchronize()
+        keys[0].commit_device(ws[0], n, ts[0])
+        ctx.synchronize()
+        out = ts[0].download(np.uint32, (kappa, 64))
+        if ref is None:
+            ref = out
+        elif not np.array_equal(ref, out):
+            raise RuntimeError(f"tuning {cfg} changed the result")
+        a = time.perf_counter()
+        for i in range(reps):
+            keys[i % 3].commit_device(ws[i % 4], n, ts[i % 4])
+        ctx.synchronize()
+        us = (time.perf_counter() - a) / reps * 1e6
+        ctx.profile(True)
+        for i in range(reps // 2):
+            keys[i % 3].commit_device(ws[i % 4], n, ts[i % 4])
+        prof = ctx.profile_read()
+        ctx.profile(False)
+        phase_us = {ph: 1e3 * ms / max(c, 1) for ph, (ms, c) in prof.items()}
+        results.append({"config": cfg, "us_per_commit": round(us, 2),
+                        "ntt_us": round(phase_us[0], 2), "mac_us": round(phase_us[1], 2),
+                        "finalize_us": round(phase_us[2], 2)})
+        log(f"{cfg} -> {us:7.2f} us/commit (ntt {phase_us[0]:.1f}, mac {phase_us[1]:.1f}, "
+            f"fin {phase_us[2]:.1f})")
+    for k in keys:
+        k.close()
+    for b in ws + ts:
+        b.free()
+    ring.close()
+    results.sort(key=lambda r: r["us_per_commit"])
+    return results
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--quick", action="store_true")
+    ap.add_argument("--out", default=TUNING_FILE)
+    args = ap.parse_args(argv)
+    import lattica_amd as la
+    ctx = la.Context(0, apply_tuning=False)
+    res = tune_commit(ctx, quick=args.quick, log=lambda m: print(m, flush=True))
+    best = res[0]
+    doc = {"device_cus": ctx.num_cus, "workload": "commit_q32_d64_kappa16_n32768",
+           "best": best["config"], "best_us_per_commit": best["us_per_commit"], "sweep": res}
+    with open(args.out, "w") as fh:
+        json.dump(doc, fh, indent=1)
+    print(json.dumps({"best": best}), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -35,21 +35,29 @@ def _hipcc() -> str:
 
 
 def build(verbose: bool = False) -> str:
+    from concurrent.futures import ThreadPoolExecutor
+
     hipcc = _hipcc()
     with tempfile.TemporaryDirectory(prefix="gridhip_") as tmp:
-        objs = []
+        jobs = []
         for src in SOURCES:
             path = os.path.join(CSRC, src)
             obj = os.path.join(tmp, os.path.splitext(src)[0] + ".o")
             lang = ["-x", "hip"] if src.endswith(".hip") else []
-            cmd = [hipcc, *COMMON, *lang, "-c", path, "-o", obj]
+            jobs.append((src, obj, [hipcc, *COMMON, *lang, "-c", path, "-o", obj]))
+
+        def compile_one(job):
+            src, obj, cmd = job
             if verbose:
                 print(" ".join(cmd), flush=True)
             res = subprocess.run(cmd, capture_output=True, text=True)
             if res.returncode != 0:
                 sys.stderr.write(res.stdout + res.stderr)
                 raise RuntimeError(f"hipcc failed on {src}")
-            objs.append(obj)
+            return obj
+
+        with ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 1)) as ex:
+            objs = list(ex.map(compile_one, jobs))
         tmp_lib = LIB + ".tmp"
         cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp_lib, *objs]
         res = subprocess.run(cmd, capture_output=True, text=True)

@@ -15,10 +15,11 @@ using namespace gh;
 
 namespace gh {
 
-static int grow(void** buf, size_t* have, size_t need, hipStream_t stream) {
+// Grow-only device buffer; the stream is drained before a buffer is freed.
+static int grow(gh_ctx* ctx, void** buf, size_t* have, size_t need) {
   if (*have >= need) return GH_OK;
   if (*buf) {
-    GH_HIP(hipStreamSynchronize(stream));
+    GH_HIP(hipStreamSynchronize(ctx->stream));
     GH_HIP(hipFree(*buf));
     *buf = nullptr;
     *have = 0;
@@ -31,17 +32,30 @@ static int grow(void** buf, size_t* have, size_t need, hipStream_t stream) {
 }
 
 int ctx_workspace(gh_ctx* ctx, size_t bytes, void** out) {
-  int st = grow(&ctx->work, &ctx->work_bytes, bytes, ctx->stream);
+  int st = grow(ctx, &ctx->work, &ctx->work_bytes, bytes);
   if (st) return st;
   *out = ctx->work;
   return GH_OK;
 }
 
 int ctx_staging(gh_ctx* ctx, size_t bytes, void** out) {
-  int st = grow(&ctx->stage, &ctx->stage_bytes, bytes, ctx->stream);
+  int st = grow(ctx, &ctx->stage, &ctx->stage_bytes, bytes);
   if (st) return st;
   *out = ctx->stage;
   return GH_OK;
+}
+
+void prof_pair(gh_ctx* ctx, int phase, hipEvent_t* start, hipEvent_t* stop) {
+  *start = *stop = nullptr;
+  if (!ctx->prof_on) return;
+  while (ctx->prof_used + 2 > ctx->prof_pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    ctx->prof_pool.push_back(e);
+  }
+  *start = ctx->prof_pool[ctx->prof_used++];
+  *stop = ctx->prof_pool[ctx->prof_used++];
+  ctx->prof_recs.push_back({phase, *start, *stop});
 }
 
 hipEvent_t prof_mark(gh_ctx* ctx) {
@@ -177,8 +191,7 @@ GH_API int gh_ctx_create(int device, gh_ctx** out) {
   }
   ctx->num_cus = prop.multiProcessorCount;
   if (hipMalloc(&ctx->misc, 4096) != hipSuccess) {
-    hipStreamDestroy(ctx->stream);
-    delete ctx;
+    gh_ctx_destroy(ctx);
     return GH_E_OOM;
   }
   *out = ctx;
@@ -188,12 +201,12 @@ GH_API int gh_ctx_create(int device, gh_ctx** out) {
 GH_API int gh_ctx_destroy(gh_ctx* ctx) {
   if (!ctx) return GH_E_INVALID_ARGUMENT;
   hipSetDevice(ctx->device);
-  hipStreamSynchronize(ctx->stream);
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
   if (ctx->work) hipFree(ctx->work);
   if (ctx->stage) hipFree(ctx->stage);
   if (ctx->misc) hipFree(ctx->misc);
   for (hipEvent_t e : ctx->prof_pool) hipEventDestroy(e);
-  hipStreamDestroy(ctx->stream);
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
   return GH_OK;
 }
@@ -233,6 +246,81 @@ GH_API int gh_ctx_profile_read(gh_ctx* ctx, int phase, double* total_ms, uint64_
     ctx->prof_used = 0;
   }
   return GH_OK;
+}
+
+GH_API int gh_ctx_set_tuning(gh_ctx* ctx, const char* key, int64_t value) {
+  if (!ctx || !key) return GH_E_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  gh_tuning& t = ctx->tune;
+  if (!std::strcmp(key, "mac_variant")) {
+    if (value != 0 && value != 1) return GH_E_INVALID_ARGUMENT;
+    t.mac_variant = (int)value;
+  } else if (!std::strcmp(key, "mac_waves")) {
+    if (value != 4 && value != 8 && value != 16) return GH_E_INVALID_ARGUMENT;
+    t.mac_waves = (int)value;
+  } else if (!std::strcmp(key, "mac_blocks_per_cu")) {
+    if (value < 1 || value > 8) return GH_E_INVALID_ARGUMENT;
+    t.mac_blocks_per_cu = (int)value;
+  } else if (!std::strcmp(key, "quad_max")) {
+    if (value < 0) return GH_E_INVALID_ARGUMENT;
+    t.quad_max = value;
+  } else if (!std::strcmp(key, "fused")) {
+    if (value != 0 && value != 1) return GH_E_INVALID_ARGUMENT;
+    t.fused = (int)value;
+  } else {
+    return GH_E_INVALID_ARGUMENT;
+  }
+  return GH_OK;
+}
+
+GH_API int gh_ctx_get_tuning(gh_ctx* ctx, const char* key, int64_t* value) {
+  if (!ctx || !key || !value) return GH_E_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  const gh_tuning& t = ctx->tune;
+  if (!std::strcmp(key, "mac_variant")) *value = t.mac_variant;
+  else if (!std::strcmp(key, "mac_waves")) *value = t.mac_waves;
+  else if (!std::strcmp(key, "mac_blocks_per_cu")) *value = t.mac_blocks_per_cu;
+  else if (!std::strcmp(key, "quad_max")) *value = t.quad_max;
+  else if (!std::strcmp(key, "fused")) *value = t.fused;
+  else return GH_E_INVALID_ARGUMENT;
+  return GH_OK;
+}
+
+GH_API int gh_measure_bandwidth(gh_ctx* ctx, size_t bytes, int reps, double* read_gbps,
+                                double* copy_gbps) {
+  if (!ctx || !read_gbps || !copy_gbps || reps < 1 || bytes < 16) return GH_E_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  bytes &= ~(size_t)15;
+  void *src = nullptr, *dst = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int st = GH_OK;
+  if (hipMalloc(&src, bytes) != hipSuccess || hipMalloc(&dst, bytes) != hipSuccess) st = GH_E_OOM;
+  if (st == GH_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    st = GH_E_DEVICE;
+  if (st == GH_OK && hipMemsetAsync(src, 0x5a, bytes, ctx->stream) != hipSuccess) st = GH_E_DEVICE;
+  double best[2] = {0, 0};
+  for (int copy = 0; copy < 2 && st == GH_OK; copy++) {
+    for (int r = 0; r < reps + 1 && st == GH_OK; r++) {  // first run warms up
+      st = launch_stream_probe(ctx, src, dst, bytes, copy, e0, e1);
+      float ms = 0;
+      if (st == GH_OK && (hipEventSynchronize(e1) != hipSuccess ||
+                          hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        st = GH_E_DEVICE;
+      if (st == GH_OK && r > 0 && ms > 0) {
+        const double gbps = (copy ? 2.0 : 1.0) * (double)bytes / (ms * 1e-3) / 1e9;
+        if (gbps > best[copy]) best[copy] = gbps;
+      }
+    }
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  hipStreamSynchronize(ctx->stream);
+  if (src) hipFree(src);
+  if (dst) hipFree(dst);
+  *read_gbps = best[0];
+  *copy_gbps = best[1];
+  return st;
 }
 
 GH_API int gh_ctx_num_cus(gh_ctx* ctx, int* out) {

@@ -24,8 +24,19 @@ __global__ __launch_bounds__(1024) void k_finalize(F f, const typename F::W* __r
   using W = typename F::W;
   const int k = threadIdx.x & 63, g = threadIdx.x >> 6;
   const uint32_t row = blockIdx.x;
+  // up to 32 independent loads in flight per thread: one round trip for the
+  // partial counts a device-wide MAC produces (<= 512)
   W acc = 0;
-  for (uint32_t p = g; p < nparts; p += 16) acc = f.add(acc, partial[((size_t)p * rows + row) * 64 + k]);
+  for (uint32_t p0 = g; p0 < nparts; p0 += 16 * 32) {
+    W v[32];
+#pragma unroll
+    for (int u = 0; u < 32; u++) {
+      const uint32_t p = p0 + 16 * u;
+      v[u] = p < nparts ? partial[((size_t)p * rows + row) * 64 + k] : W(0);
+    }
+#pragma unroll
+    for (int u = 0; u < 32; u++) acc = f.add(acc, v[u]);
+  }
   __shared__ W red[16][64];
   red[g][k] = acc;
   __syncthreads();

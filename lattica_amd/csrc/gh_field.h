@@ -21,6 +21,9 @@
 
 namespace gh {
 
+// native 16-byte vector (accepted by the nontemporal load builtin)
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 constexpr uint64_t kGoldilocks = 0xffffffff00000001ull;
 constexpr uint64_t kGoldEps = 0xffffffffull;  // 2^64 mod p
 
@@ -110,7 +113,8 @@ enum TabOffset : int {
   kTabUntwistMul = 256,   // psi^-i * d^-1 * R            (after a Montgomery pointwise product)
   kTabUnscale = 320,      // d^-1                         (cyclic inverse)
   kTabUntwistMac = 384,   // psi^-i * d^-1 * R^2          (after a reduce96_r2 accumulation)
-  kTabEntries = 448,
+  kTabQuad = 448,         // psi^(t*(2k+1)), t = 0..3, k = 0..15 at kTabQuad + 16t + k (quad NTT)
+  kTabEntries = 512,
 };
 
 }  // namespace gh

@@ -10,8 +10,17 @@
 #include "../../include/gridhip.h"
 #include "gh_field.h"
 
+struct gh_tuning {
+  int mac_variant = 1;        // q32 MAC: 0 = dword loads, 1 = 16-byte loads
+  int mac_waves = 16;         // waves per MAC block (4, 8, 16)
+  int mac_blocks_per_cu = 1;  // MAC blocks (= partial sums) per compute unit
+  int64_t quad_max = 1 << 17; // largest batch transformed four lanes per polynomial
+  int fused = 1;              // q32 commit: witness transform fused into the MAC kernel
+};
+
 struct gh_ctx {
   int device = 0;
+  gh_tuning tune;
   int num_cus = 0;
   hipStream_t stream = nullptr;
   // grow-only device workspace used by the calls of this context
@@ -61,6 +70,9 @@ int ctx_staging(gh_ctx* ctx, size_t bytes, void** out);
 // records an event on the context stream when profiling is on (else nullptr)
 hipEvent_t prof_mark(gh_ctx* ctx);
 void prof_span(gh_ctx* ctx, int phase, hipEvent_t a, hipEvent_t b);
+// a start/stop event pair for a kernel launched with hipExtLaunchKernelGGL,
+// registered under `phase` (both null when profiling is off)
+void prof_pair(gh_ctx* ctx, int phase, hipEvent_t* start, hipEvent_t* stop);
 
 // launchers (device pointers; enqueue on ctx->stream)
 enum XformMode { kTwistedFwd = 0, kTwistedInv = 1, kCyclicFwd = 2, kCyclicInv = 3 };
@@ -72,6 +84,8 @@ int launch_key_layout(gh_ring* ring, const void* src_rowmajor, uint32_t rows, ui
 int launch_commit(gh_key* key, const void* s_dev, uint32_t cols, int domain, void* t_dev);
 int launch_check_canonical(gh_ring* ring, const void* p, size_t words, int* bad_flag_dev);
 int launch_gram(gh_ring* ring, const void* s_dev, uint32_t r, uint32_t n, int domain, void* g_dev);
+int launch_stream_probe(gh_ctx* ctx, const void* src, void* dst, size_t bytes, int copy,
+                        hipEvent_t e0, hipEvent_t e1);
 int launch_jl(gh_ring* ring, const uint8_t seed[32], uint32_t rows, const void* parts,
               uint32_t num_parts, uint64_t cols, void* p_modq, int64_t* p_exact, int8_t* flat);
 

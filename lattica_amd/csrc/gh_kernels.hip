@@ -15,6 +15,8 @@
 //      exactly (96-bit for q < 2^32) and reduced once per block;
 //   3. k_finalize: sums the per-block partials and runs the inverse transform
 //      with one slot per lane.
+#include <hip/hip_ext.h>
+
 #include "gh_internal.h"
 #include "gh_ntt.h"
 #include "gh_common_kernels.h"
@@ -109,6 +111,41 @@ __global__ __launch_bounds__(TILE) void k_xform(F f, const typename F::W* __rest
     __syncthreads();
     tile_store<W, TILE>(dst + p0 * 64, lds, np * 64);
     __syncthreads();
+  }
+}
+
+// Twisted forward NTT, one polynomial per quad of lanes (gh_ntt.h quad_fwd64).
+// Loads are stride-4 words (the quad together reads whole 64-byte segments);
+// each lane stores its 16 contiguous output slots with 16-byte stores.
+template <class F>
+__global__ __launch_bounds__(256) void k_twisted_fwd_quad(F f, const typename F::W* __restrict__ tab,
+                                                          const typename F::W* src,
+                                                          typename F::W* dst, size_t count) {
+  using W = typename F::W;
+  constexpr int TW = F::kTwWords;
+  constexpr int V = 16 / sizeof(W);
+  const int t = threadIdx.x & 3;
+  W ltw[16][TW];
+#pragma unroll
+  for (int k = 0; k < 16; k++)
+#pragma unroll
+    for (int w = 0; w < TW; w++) ltw[k][w] = tab[(kTabQuad + 16 * t + k) * TW + w];
+  const size_t stride = ((size_t)gridDim.x * blockDim.x) >> 2;
+  for (size_t p = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 2; p < count; p += stride) {
+    W x[16];
+    const W* sp = src + p * 64 + t;
+#pragma unroll
+    for (int i = 0; i < 16; i++) x[i] = sp[4 * i];
+    quad_fwd64(f, x, tab, ltw, t);
+    uint4* dp = reinterpret_cast<uint4*>(dst + p * 64 + 16 * bitrev2(t));
+#pragma unroll
+    for (int v = 0; v < 16 / V; v++) {
+      uint4 o;
+      W* ow = reinterpret_cast<W*>(&o);
+#pragma unroll
+      for (int c = 0; c < V; c++) ow[c] = x[v * V + c];
+      dp[v] = o;
+    }
   }
 }
 
@@ -240,6 +277,227 @@ __global__ __launch_bounds__(64 * kMacWaves) void k_mac_q32(
   }
 }
 
+// 16-byte variant: one wave load covers four rows of a column (1 KB), lane l
+// holds rows 4i + l/16 and slots 4(l%16)..+3, so a 16-row column is four
+// global_load_dwordx4 plus one for the witness slots. RB is a multiple of 4.
+template <int RB, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_mac4_q32(
+    FQ32 f, const uint32_t* __restrict__ A, const uint32_t* __restrict__ s_ntt,
+    uint32_t* __restrict__ partial, uint32_t rows, uint32_t cols, uint32_t cols_per_block) {
+  constexpr int NI = RB / 4;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int rsub = lane >> 4, k0 = 4 * (lane & 15);
+  const uint32_t row0 = blockIdx.y * RB;
+  const uint32_t c_begin = blockIdx.x * cols_per_block;
+  uint32_t c_end = c_begin + cols_per_block;
+  if (c_end > cols) c_end = cols;
+  uint64_t lo[NI][4];
+  uint32_t hi[NI][4];
+#pragma unroll
+  for (int i = 0; i < NI; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) { lo[i][j] = 0; hi[i][j] = 0; }
+  const size_t col_stride = (size_t)rows * 64;
+  uint32_t c = c_begin + wave;
+  for (; c + WAVES < c_end; c += 2 * WAVES) {
+    const u32x4* a0 = reinterpret_cast<const u32x4*>(A + (size_t)c * col_stride + (size_t)row0 * 64) + lane;
+    const u32x4* a1 = a0 + (size_t)WAVES * (col_stride / 4);
+    const u32x4 s0 = *reinterpret_cast<const u32x4*>(s_ntt + (size_t)c * 64 + k0);
+    const u32x4 s1 = *reinterpret_cast<const u32x4*>(s_ntt + (size_t)(c + WAVES) * 64 + k0);
+    u32x4 va[NI], vb[NI];
+#pragma unroll
+    for (int i = 0; i < NI; i++) va[i] = __builtin_nontemporal_load(a0 + i * 64);
+#pragma unroll
+    for (int i = 0; i < NI; i++) vb[i] = __builtin_nontemporal_load(a1 + i * 64);
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+      mac96(lo[i][0], hi[i][0], va[i].x, s0.x);
+      mac96(lo[i][1], hi[i][1], va[i].y, s0.y);
+      mac96(lo[i][2], hi[i][2], va[i].z, s0.z);
+      mac96(lo[i][3], hi[i][3], va[i].w, s0.w);
+    }
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+      mac96(lo[i][0], hi[i][0], vb[i].x, s1.x);
+      mac96(lo[i][1], hi[i][1], vb[i].y, s1.y);
+      mac96(lo[i][2], hi[i][2], vb[i].z, s1.z);
+      mac96(lo[i][3], hi[i][3], vb[i].w, s1.w);
+    }
+  }
+  if (c < c_end) {
+    const u32x4* a0 = reinterpret_cast<const u32x4*>(A + (size_t)c * col_stride + (size_t)row0 * 64) + lane;
+    const u32x4 s0 = *reinterpret_cast<const u32x4*>(s_ntt + (size_t)c * 64 + k0);
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+      const u32x4 v = __builtin_nontemporal_load(a0 + i * 64);
+      mac96(lo[i][0], hi[i][0], v.x, s0.x);
+      mac96(lo[i][1], hi[i][1], v.y, s0.y);
+      mac96(lo[i][2], hi[i][2], v.z, s0.z);
+      mac96(lo[i][3], hi[i][3], v.w, s0.w);
+    }
+  }
+  __shared__ uint32_t red[WAVES][RB][64];
+#pragma unroll
+  for (int i = 0; i < NI; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      red[wave][4 * i + rsub][k0 + j] =
+          f.reduce96_r2((uint32_t)lo[i][j], (uint32_t)(lo[i][j] >> 32), hi[i][j]);
+  __syncthreads();
+  for (int o = threadIdx.x; o < RB * 64; o += 64 * WAVES) {
+    const int r = o >> 6, k = o & 63;
+    uint32_t acc = red[0][r][k];
+#pragma unroll
+    for (int w = 1; w < WAVES; w++) acc = f.add(acc, red[w][r][k]);
+    partial[((size_t)blockIdx.x * rows + row0 + r) * 64 + k] = acc;
+  }
+}
+
+// Fused witness transform + MAC (q32). A block owns a column range and walks
+// it in chunks of at most kFuseChunk columns: every wave first issues the key
+// loads of its first column of the chunk, then the block transforms the
+// chunk's witness polynomials into LDS (four lanes per polynomial) while those
+// loads are in flight, and finally streams the rest of the chunk against the
+// LDS copy of s_ntt. The transform's VALU work thus overlaps the HBM stream
+// instead of running as its own kernel; results equal k_mac4_q32 on a
+// separately transformed witness.
+constexpr int kFuseChunk = 128;
+constexpr int kFuseStride = 68;  // LDS row stride of s_ntt (16-byte aligned rows)
+
+template <int RB, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_commit_fused_q32(
+    FQ32 f, const uint32_t* __restrict__ tab, const uint32_t* __restrict__ A,
+    const uint32_t* __restrict__ s_coeff, uint32_t* __restrict__ partial, uint32_t rows,
+    uint32_t cols, uint32_t cols_per_block) {
+  constexpr int NI = RB / 4;
+  constexpr int PF = 16 / WAVES;  // key columns prefetched per wave before the transform
+  extern __shared__ uint32_t lds[];  // max(kFuseChunk * kFuseStride, WAVES * RB * 64) words
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int rsub = lane >> 4, k0 = 4 * (lane & 15);
+  const int t = threadIdx.x & 3;
+  const uint32_t row0 = blockIdx.y * RB;
+  const uint32_t c_begin = blockIdx.x * cols_per_block;
+  uint32_t c_end = c_begin + cols_per_block;
+  if (c_end > cols) c_end = cols;
+  const size_t col_stride = (size_t)rows * 64;
+
+  uint64_t lo[NI][4];
+  uint32_t hi[NI][4];
+#pragma unroll
+  for (int i = 0; i < NI; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) { lo[i][j] = 0; hi[i][j] = 0; }
+
+  for (uint32_t ch = c_begin; ch < c_end; ch += kFuseChunk) {
+    const uint32_t ch_end = ch + kFuseChunk < c_end ? ch + kFuseChunk : c_end;
+    // (a) key loads for this wave's first PF columns of the chunk: with
+    // 2 blocks of WAVES waves per CU that is 32 KB per CU of HBM traffic in
+    // flight per prefetched column, enough to cover the transform below
+    u32x4 pa[PF][NI];
+#pragma unroll
+    for (int q = 0; q < PF; q++) {
+      const uint32_t cq = ch + wave + q * WAVES;
+      if (cq < ch_end) {
+        const u32x4* a0 = reinterpret_cast<const u32x4*>(A + (size_t)cq * col_stride + (size_t)row0 * 64) + lane;
+#pragma unroll
+        for (int i = 0; i < NI; i++) pa[q][i] = __builtin_nontemporal_load(a0 + i * 64);
+      }
+    }
+    // (b) witness transform of the chunk into LDS while (a) is in flight
+    __syncthreads();  // the previous chunk's readers are done with lds
+    {
+      uint32_t ltw[16][2];
+#pragma unroll
+      for (int k = 0; k < 16; k++) {
+        ltw[k][0] = tab[(kTabQuad + 16 * t + k) * 2];
+        ltw[k][1] = tab[(kTabQuad + 16 * t + k) * 2 + 1];
+      }
+      for (uint32_t p = threadIdx.x >> 2; p < ch_end - ch; p += 16 * WAVES) {
+        uint32_t x[16];
+        const uint32_t* sp = s_coeff + (size_t)(ch + p) * 64 + t;
+#pragma unroll
+        for (int i = 0; i < 16; i++) x[i] = sp[4 * i];
+        quad_fwd64(f, x, tab, ltw, t);
+        u32x4* dp = reinterpret_cast<u32x4*>(lds + p * kFuseStride + 16 * bitrev2(t));
+#pragma unroll
+        for (int v = 0; v < 4; v++) dp[v] = u32x4{x[4 * v], x[4 * v + 1], x[4 * v + 2], x[4 * v + 3]};
+      }
+    }
+    __syncthreads();
+    // (c) MAC over the chunk: the prefetched columns first, then two per step
+#pragma unroll
+    for (int q = 0; q < PF; q++) {
+      const uint32_t cq = ch + wave + q * WAVES;
+      if (cq < ch_end) {
+        const u32x4 s0 = *reinterpret_cast<const u32x4*>(lds + (cq - ch) * kFuseStride + k0);
+#pragma unroll
+        for (int i = 0; i < NI; i++) {
+          mac96(lo[i][0], hi[i][0], pa[q][i].x, s0.x);
+          mac96(lo[i][1], hi[i][1], pa[q][i].y, s0.y);
+          mac96(lo[i][2], hi[i][2], pa[q][i].z, s0.z);
+          mac96(lo[i][3], hi[i][3], pa[q][i].w, s0.w);
+        }
+      }
+    }
+    uint32_t c = ch + wave + PF * WAVES;
+    for (; c + WAVES < ch_end; c += 2 * WAVES) {
+      const u32x4* a0 = reinterpret_cast<const u32x4*>(A + (size_t)c * col_stride + (size_t)row0 * 64) + lane;
+      const u32x4* a1 = a0 + (size_t)WAVES * (col_stride / 4);
+      u32x4 va[NI], vb[NI];
+#pragma unroll
+      for (int i = 0; i < NI; i++) va[i] = __builtin_nontemporal_load(a0 + i * 64);
+#pragma unroll
+      for (int i = 0; i < NI; i++) vb[i] = __builtin_nontemporal_load(a1 + i * 64);
+      const u32x4 s0 = *reinterpret_cast<const u32x4*>(lds + (c - ch) * kFuseStride + k0);
+      const u32x4 s1 = *reinterpret_cast<const u32x4*>(lds + (c + WAVES - ch) * kFuseStride + k0);
+#pragma unroll
+      for (int i = 0; i < NI; i++) {
+        mac96(lo[i][0], hi[i][0], va[i].x, s0.x);
+        mac96(lo[i][1], hi[i][1], va[i].y, s0.y);
+        mac96(lo[i][2], hi[i][2], va[i].z, s0.z);
+        mac96(lo[i][3], hi[i][3], va[i].w, s0.w);
+      }
+#pragma unroll
+      for (int i = 0; i < NI; i++) {
+        mac96(lo[i][0], hi[i][0], vb[i].x, s1.x);
+        mac96(lo[i][1], hi[i][1], vb[i].y, s1.y);
+        mac96(lo[i][2], hi[i][2], vb[i].z, s1.z);
+        mac96(lo[i][3], hi[i][3], vb[i].w, s1.w);
+      }
+    }
+    if (c < ch_end) {
+      const u32x4* a0 = reinterpret_cast<const u32x4*>(A + (size_t)c * col_stride + (size_t)row0 * 64) + lane;
+      const u32x4 s0 = *reinterpret_cast<const u32x4*>(lds + (c - ch) * kFuseStride + k0);
+#pragma unroll
+      for (int i = 0; i < NI; i++) {
+        const u32x4 v = __builtin_nontemporal_load(a0 + i * 64);
+        mac96(lo[i][0], hi[i][0], v.x, s0.x);
+        mac96(lo[i][1], hi[i][1], v.y, s0.y);
+        mac96(lo[i][2], hi[i][2], v.z, s0.z);
+        mac96(lo[i][3], hi[i][3], v.w, s0.w);
+      }
+    }
+  }
+  // (d) block reduction through the same LDS
+  __syncthreads();
+  uint32_t* red = lds;  // [WAVES][RB][64]
+#pragma unroll
+  for (int i = 0; i < NI; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      red[(wave * RB + 4 * i + rsub) * 64 + k0 + j] =
+          f.reduce96_r2((uint32_t)lo[i][j], (uint32_t)(lo[i][j] >> 32), hi[i][j]);
+  __syncthreads();
+  for (int o = threadIdx.x; o < RB * 64; o += 64 * WAVES) {
+    uint32_t acc = red[o];
+#pragma unroll
+    for (int w = 1; w < WAVES; w++) acc = f.add(acc, red[w * RB * 64 + o]);
+    partial[((size_t)blockIdx.x * rows + row0) * 64 + o] = acc;
+  }
+}
+
 // Goldilocks: modular multiply-add per term, canonical partials.
 template <int RB>
 __global__ __launch_bounds__(64 * kMacWaves) void k_mac_gold(
@@ -277,6 +535,48 @@ __global__ __launch_bounds__(64 * kMacWaves) void k_mac_gold(
   }
 }
 
+// Bandwidth probes: a streaming read (16-byte nontemporal loads, XOR-folded so
+// the loads stay live) and a streaming copy of known size. They give the
+// achievable HBM ceiling beside the nominal peak, and a reference dispatch for
+// scaling hardware traffic counters.
+__global__ __launch_bounds__(256) void k_stream_read(const u32x4* __restrict__ src, size_t n,
+                                                     uint32_t* __restrict__ sink) {
+  u32x4 acc = {0, 0, 0, 0};
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    u32x4 a = __builtin_nontemporal_load(src + i);
+    u32x4 b = __builtin_nontemporal_load(src + i + stride);
+    u32x4 c = __builtin_nontemporal_load(src + i + 2 * stride);
+    u32x4 d = __builtin_nontemporal_load(src + i + 3 * stride);
+    acc ^= a ^ b ^ c ^ d;
+  }
+  for (; i < n; i += stride) acc ^= __builtin_nontemporal_load(src + i);
+  const uint32_t v = acc.x ^ acc.y ^ acc.z ^ acc.w;
+  if (v == 0x9e3779b9u) sink[blockIdx.x] = v;  // data dependent, practically never taken
+}
+
+__global__ __launch_bounds__(256) void k_stream_copy(const u32x4* __restrict__ src, size_t n,
+                                                     u32x4* __restrict__ dst) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = __builtin_nontemporal_load(src + i);
+}
+
+int launch_stream_probe(gh_ctx* ctx, const void* src, void* dst, size_t bytes, int copy,
+                        hipEvent_t e0, hipEvent_t e1) {
+  const size_t n = bytes / 16;
+  const unsigned grid = (unsigned)ctx->num_cus * 8;
+  if (copy)
+    hipExtLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, ctx->stream, e0, e1, 0,
+                          (const u32x4*)src, n, (u32x4*)dst);
+  else
+    hipExtLaunchKernelGGL(k_stream_read, dim3(grid), dim3(256), 0, ctx->stream, e0, e1, 0,
+                          (const u32x4*)src, n, (uint32_t*)dst);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
 // Flags any word >= q (input validation for canonical residues).
 template <class W>
 __global__ void k_check_canonical(const W* __restrict__ p, size_t words, W q, int* bad) {
@@ -303,7 +603,12 @@ static int xform_impl(gh_ring* ring, F f, const void* src, void* dst, size_t cou
   gh_ctx* ctx = ring->ctx;
   const W* tab = (const W*)ring->dev_tab;
   constexpr int BIG = sizeof(W) == 4 ? 256 : 128;
-  if (count >= (size_t)BIG * ctx->num_cus * 2) {
+  if (MODE == kTwistedFwd && count <= (size_t)ctx->tune.quad_max) {
+    // four lanes per polynomial: 64 polynomials per 256-thread block
+    unsigned grid = grid_for((count + 63) / 64, ctx->num_cus, 8);
+    hipLaunchKernelGGL((k_twisted_fwd_quad<F>), dim3(grid), dim3(256), 0, ctx->stream, f, tab,
+                       (const W*)src, (W*)dst, count);
+  } else if (count >= (size_t)BIG * ctx->num_cus * 2) {
     unsigned grid = grid_for((count + BIG - 1) / BIG, ctx->num_cus, 2);
     hipLaunchKernelGGL((k_xform<F, MODE, BIG>), dim3(grid), dim3(BIG), 0, ctx->stream, f, tab,
                        (const W*)src, (W*)dst, count);
@@ -414,18 +719,87 @@ int launch_check_canonical(gh_ring* ring, const void* p, size_t words, int* bad)
 
 template <int RB>
 static int mac_launch(gh_key* key, const void* s_ntt, uint32_t cols, void* partial,
-                      uint32_t nparts, uint32_t cpb) {
+                      uint32_t nparts, uint32_t cpb, int waves) {
   gh_ring* ring = key->ring;
   gh_ctx* ctx = ring->ctx;
   dim3 grid(nparts, key->rows / RB);
-  if (ring->kind == GH_RING_Q32)
-    hipLaunchKernelGGL((k_mac_q32<RB>), grid, dim3(64 * kMacWaves), 0, ctx->stream,
-                       FQ32{(uint32_t)ring->q, ring->qinv32}, (const uint32_t*)key->dev_a,
-                       (const uint32_t*)s_ntt, (uint32_t*)partial, key->rows, cols, cpb);
-  else
+  if (ring->kind == GH_RING_Q32) {
+    FQ32 f{(uint32_t)ring->q, ring->qinv32};
+    const uint32_t* A = (const uint32_t*)key->dev_a;
+    const uint32_t* S = (const uint32_t*)s_ntt;
+    uint32_t* P = (uint32_t*)partial;
+    if constexpr (RB % 4 == 0) {
+      if (ctx->tune.mac_variant == 1) {
+        if (waves == 4)
+          hipLaunchKernelGGL((k_mac4_q32<RB, 4>), grid, dim3(256), 0, ctx->stream, f, A, S, P, key->rows, cols, cpb);
+        else if (waves == 8)
+          hipLaunchKernelGGL((k_mac4_q32<RB, 8>), grid, dim3(512), 0, ctx->stream, f, A, S, P, key->rows, cols, cpb);
+        else
+          hipLaunchKernelGGL((k_mac4_q32<RB, 16>), grid, dim3(1024), 0, ctx->stream, f, A, S, P, key->rows, cols, cpb);
+        GH_HIP(hipGetLastError());
+        return GH_OK;
+      }
+    }
+    hipLaunchKernelGGL((k_mac_q32<RB>), grid, dim3(64 * kMacWaves), 0, ctx->stream, f, A, S, P,
+                       key->rows, cols, cpb);
+  } else {
     hipLaunchKernelGGL((k_mac_gold<RB>), grid, dim3(64 * kMacWaves), 0, ctx->stream, FGold{},
                        (const uint64_t*)key->dev_a, (const uint64_t*)s_ntt, (uint64_t*)partial,
                        key->rows, cols, cpb);
+  }
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+template <int RB, int WAVES>
+static void fused_launch(gh_ctx* ctx, dim3 grid, size_t lds_bytes, hipEvent_t e0, hipEvent_t e1,
+                         FQ32 f, const uint32_t* tab, const uint32_t* A, const uint32_t* S,
+                         uint32_t* P, uint32_t rows, uint32_t cols, uint32_t cpb) {
+  hipExtLaunchKernelGGL((k_commit_fused_q32<RB, WAVES>), grid, dim3(64 * WAVES), lds_bytes,
+                        ctx->stream, e0, e1, 0, f, tab, A, S, P, rows, cols, cpb);
+}
+
+// q32 commitment in two launches on the context stream: the fused
+// transform+MAC kernel, then the finalize. (A second stream for the finalize
+// was measured: the cross-queue event hand-off costs more than it overlaps.)
+static int launch_commit_fused(gh_key* key, const void* s_dev, uint32_t cols, void* t_dev) {
+  gh_ring* ring = key->ring;
+  gh_ctx* ctx = ring->ctx;
+  const uint32_t rows = key->rows;
+  const int waves = ctx->tune.mac_waves;
+  const int rb = rows % 16 == 0 ? 16 : rows % 8 == 0 ? 8 : 4;
+  uint32_t nparts = (uint32_t)(ctx->num_cus * ctx->tune.mac_blocks_per_cu);
+  const uint32_t max_parts = (cols + waves - 1) / waves;
+  if (nparts > max_parts) nparts = max_parts;
+  if (nparts == 0) nparts = 1;
+  const uint32_t cpb = (cols + nparts - 1) / nparts;
+  nparts = (cols + cpb - 1) / cpb;
+  void* ws;
+  int st = ctx_workspace(ctx, (size_t)nparts * rows * 64 * 4, &ws);
+  if (st) return st;
+
+  FQ32 f{(uint32_t)ring->q, ring->qinv32};
+  const uint32_t* tab = (const uint32_t*)ring->dev_tab;
+  const uint32_t* A = (const uint32_t*)key->dev_a;
+  const uint32_t* S = (const uint32_t*)s_dev;
+  uint32_t* P = (uint32_t*)ws;
+  dim3 grid(nparts, rows / rb);
+  size_t lds_words = (size_t)kFuseChunk * kFuseStride;
+  if ((size_t)waves * rb * 64 > lds_words) lds_words = (size_t)waves * rb * 64;
+  const size_t lds_bytes = lds_words * 4;
+  hipEvent_t e0, e1;
+  prof_pair(ctx, 1, &e0, &e1);
+#define GH_FUSED(R, W) fused_launch<R, W>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb)
+  if (rb == 16) { if (waves == 4) GH_FUSED(16, 4); else if (waves == 8) GH_FUSED(16, 8); else GH_FUSED(16, 16); }
+  else if (rb == 8) { if (waves == 4) GH_FUSED(8, 4); else if (waves == 8) GH_FUSED(8, 8); else GH_FUSED(8, 16); }
+  else { if (waves == 4) GH_FUSED(4, 4); else if (waves == 8) GH_FUSED(4, 8); else GH_FUSED(4, 16); }
+#undef GH_FUSED
+  GH_HIP(hipGetLastError());
+  hipEvent_t f0, f1;
+  prof_pair(ctx, 2, &f0, &f1);
+  hipExtLaunchKernelGGL((k_finalize<FQ32>), dim3(rows), dim3(1024), 0, ctx->stream, f0, f1, 0, f,
+                        tab, (const uint32_t*)P, nparts, rows, (int)kTabUntwistMac,
+                        (uint32_t*)t_dev);
   GH_HIP(hipGetLastError());
   return GH_OK;
 }
@@ -434,11 +808,17 @@ static int mac_launch(gh_key* key, const void* s_ntt, uint32_t cols, void* parti
 int launch_commit(gh_key* key, const void* s_dev, uint32_t cols, int domain, void* t_dev) {
   gh_ring* ring = key->ring;
   gh_ctx* ctx = ring->ctx;
+  if (ring->kind == GH_RING_Q32 && ctx->tune.fused && key->rows % 4 == 0 &&
+      domain == GH_DOMAIN_COEFF)
+    return launch_commit_fused(key, s_dev, cols, t_dev);
   const size_t wb = ring->word_bytes;
   const uint32_t rows = key->rows;
-  // partial count: about one block per CU, at least one column per wave
-  uint32_t nparts = (uint32_t)ctx->num_cus;
-  uint32_t max_parts = (cols + kMacWaves - 1) / kMacWaves;
+  // block shape: the tuned wave count for the 16-byte q32 kernel, else 16
+  const bool mac4 = ring->kind == GH_RING_Q32 && ctx->tune.mac_variant == 1 && rows % 4 == 0;
+  const int waves = mac4 ? ctx->tune.mac_waves : kMacWaves;
+  // partial count: the tuned blocks per CU, at least one column per wave
+  uint32_t nparts = (uint32_t)(ctx->num_cus * ctx->tune.mac_blocks_per_cu);
+  uint32_t max_parts = (cols + waves - 1) / waves;
   if (nparts > max_parts) nparts = max_parts;
   if (nparts == 0) nparts = 1;
   uint32_t cpb = (cols + nparts - 1) / nparts;
@@ -461,11 +841,11 @@ int launch_commit(gh_key* key, const void* s_dev, uint32_t cols, int domain, voi
   hipEvent_t e1 = prof_mark(ctx);
   int rb = rows % 16 == 0 ? 16 : rows % 8 == 0 ? 8 : rows % 4 == 0 ? 4 : rows % 2 == 0 ? 2 : 1;
   switch (rb) {
-    case 16: st = mac_launch<16>(key, s_ntt, cols, partial, nparts, cpb); break;
-    case 8: st = mac_launch<8>(key, s_ntt, cols, partial, nparts, cpb); break;
-    case 4: st = mac_launch<4>(key, s_ntt, cols, partial, nparts, cpb); break;
-    case 2: st = mac_launch<2>(key, s_ntt, cols, partial, nparts, cpb); break;
-    default: st = mac_launch<1>(key, s_ntt, cols, partial, nparts, cpb); break;
+    case 16: st = mac_launch<16>(key, s_ntt, cols, partial, nparts, cpb, waves); break;
+    case 8: st = mac_launch<8>(key, s_ntt, cols, partial, nparts, cpb, waves); break;
+    case 4: st = mac_launch<4>(key, s_ntt, cols, partial, nparts, cpb, waves); break;
+    case 2: st = mac_launch<2>(key, s_ntt, cols, partial, nparts, cpb, waves); break;
+    default: st = mac_launch<1>(key, s_ntt, cols, partial, nparts, cpb, waves); break;
   }
   if (st) return st;
   hipEvent_t e2 = prof_mark(ctx);

@@ -97,27 +97,116 @@ __device__ __forceinline__ uint64_t lane_read(uint64_t v, int src_lane) {
 
 // Wave-cooperative inverse: lane k holds slot k of one polynomial (natural
 // order); returns coefficient k. Same DIF stages as inv64, with the butterfly
-// partner fetched across lanes; twiddles are per-lane table reads.
+// partner fetched across lanes. All seven per-lane twiddles are gathered
+// before the first stage so their load latencies overlap (a j = 0 twiddle is
+// the table's 1, so every upper lane multiplies unconditionally).
 template <class F>
 __device__ __forceinline__ typename F::W wave_inv64(const F& f, typename F::W x,
                                                     const typename F::W* __restrict__ tab,
                                                     int post, int lane) {
   using W = typename F::W;
   constexpr int TW = F::kTwWords;
+  W tw[7][TW];
 #pragma unroll
   for (int s = 0; s < 6; s++) {
     const int half = 32 >> s, off = 64 - (64 >> s);
+    const W* p = tab + (kTabInv + off + (lane & (half - 1))) * TW;
+#pragma unroll
+    for (int w = 0; w < TW; w++) tw[s][w] = p[w];
+  }
+#pragma unroll
+  for (int w = 0; w < TW; w++) tw[6][w] = tab[(post + lane) * TW + w];
+#pragma unroll
+  for (int s = 0; s < 6; s++) {
+    const int half = 32 >> s;
     W other = lane_read(x, lane ^ half);
-    const int j = lane & (half - 1);
-    if ((lane & half) == 0) {
-      x = f.add(x, other);
-    } else {
-      W dlt = f.sub(other, x);
-      x = j ? f.mul_tw(dlt, tab + (kTabInv + off + j) * TW) : dlt;
-    }
+    if ((lane & half) == 0) x = f.add(x, other);
+    else x = f.mul_tw(f.sub(other, x), tw[s]);
   }
   W y = lane_read(x, bitrev6(lane));
-  return f.mul_tw(y, tab + (post + lane) * TW);
+  return f.mul_tw(y, tw[6]);
+}
+
+// ---------------------------------------------------------------------------
+// Quad transform: one polynomial on four adjacent lanes (t = lane & 3).
+//
+// With i = t + 4*i2 and k = k2 + 16*k1,
+//   X_k = sum_i a_i psi^(i(2k+1))
+//       = sum_t iota^(t*k1) * psi^(t(2k2+1)) * sum_i2 a_(t+4i2) (psi^4)^(i2(2k2+1)),
+// iota = psi^32. Each lane runs the inner 16-point twisted NTT on its stride-4
+// subsequence (uniform twiddles: the 64-point tables already hold psi^(4i) and
+// the 16-point stage roots), multiplies by its lane twiddle psi^(t(2k2+1)) and
+// the 4-point DFT over t finishes across the quad with two DPP exchanges.
+// Four lanes per polynomial cut the per-polynomial latency about threefold,
+// which matters when a call has only tens of thousands of polynomials.
+// ---------------------------------------------------------------------------
+
+__host__ __device__ constexpr int bitrev4(int i) {
+  return ((i & 1) << 3) | ((i & 2) << 1) | ((i & 4) >> 1) | ((i & 8) >> 3);
+}
+__host__ __device__ constexpr int bitrev2(int i) { return ((i & 1) << 1) | ((i & 2) >> 1); }
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t quad_dpp(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
+}
+template <int CTRL>
+__device__ __forceinline__ uint64_t quad_dpp(uint64_t v) {
+  uint32_t lo = quad_dpp<CTRL>((uint32_t)v), hi = quad_dpp<CTRL>((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+constexpr int kDppXor1 = 0xB1;  // quad_perm [1, 0, 3, 2]
+constexpr int kDppXor2 = 0x4E;  // quad_perm [2, 3, 0, 1]
+
+// In: x[i2] = a[t + 4*i2]. Out: x[k2] = X[16*bitrev2(t) + k2].
+// ltw: this lane's psi^(t(2k2+1)) table entries, k2 = 0..15.
+template <class F>
+__device__ __forceinline__ void quad_fwd64(const F& f, typename F::W (&x)[16],
+                                           const typename F::W* __restrict__ tab,
+                                           const typename F::W (&ltw)[16][F::kTwWords], int t) {
+  using W = typename F::W;
+  constexpr int TW = F::kTwWords;
+  W y[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    W v = x[i];
+    if (i != 0) v = f.mul_tw(v, tab + (kTabTwist + 4 * i) * TW);
+    y[bitrev4(i)] = v;
+  }
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const int half = 1 << s, len = 2 << s;
+#pragma unroll
+    for (int st = 0; st < 16; st += len) {
+#pragma unroll
+      for (int j = 0; j < half; j++) {
+        W u = y[st + j];
+        W v = y[st + half + j];
+        if (j != 0) v = f.mul_tw(v, tab + (kTabFwd + half - 1 + j) * TW);
+        y[st + j] = f.add(u, v);
+        y[st + half + j] = f.sub(u, v);
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 16; k++) y[k] = f.mul_tw(y[k], ltw[k]);
+  // 4-point DFT over t: E/O sums across lanes t, t^2 ...
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    W o = quad_dpp<kDppXor2>(y[k]);
+    y[k] = (t & 2) ? f.sub(o, y[k]) : f.add(y[k], o);
+  }
+  // ... lane 3 holds C1 - C3 and takes the factor iota ...
+  if (t == 3) {
+#pragma unroll
+    for (int k = 0; k < 16; k++) y[k] = f.mul_tw(y[k], tab + (kTabTwist + 32) * TW);
+  }
+  // ... and t, t^1 combine: lanes 0..3 end with Y_0, Y_2, Y_1, Y_3.
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    W o = quad_dpp<kDppXor1>(y[k]);
+    x[k] = (t & 1) ? f.sub(o, y[k]) : f.add(y[k], o);
+  }
 }
 
 // Wave-cooperative forward twisted transform (lane k holds coefficient k).

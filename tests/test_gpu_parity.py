@@ -237,6 +237,70 @@ def test_north_star_commit_full_size(oracle, ctx, ring_q32):
     s_dev.free(); t_dev.free(); key.close()
 
 
+def test_every_tuning_setting_gives_the_same_commitment(oracle, ctx, ring_q32):
+    import lattica_amd as la
+    r = ring_q32
+    rng = np.random.default_rng(61)
+    rows, n = 16, 3001
+    a = rand_residues(rng, r.q, (rows, n, 64))
+    s = small_signed(rng, r.q, (n, 64), -128, 128)
+    z = np.zeros((rows, 1, 64), dtype=np.uint64)
+    exp = oracle.commit(r.q, 64, a, z, s, np.zeros((1, 64), dtype=np.uint64), key_domain=1)
+    key = la.Key(r, a, domain=la.DOMAIN_NTT)
+    saved = ctx.get_tuning()
+    try:
+        for fused, variant, waves, bpc, quad in [
+                (0, 0, 16, 1, 0), (0, 1, 4, 1, 0), (0, 1, 8, 2, 1 << 17), (0, 1, 16, 4, 1 << 17),
+                (0, 1, 16, 8, 0), (1, 1, 4, 1, 0), (1, 1, 8, 2, 0), (1, 1, 16, 1, 0),
+                (1, 1, 16, 8, 0)]:
+            ctx.set_tuning(fused=fused, mac_variant=variant, mac_waves=waves,
+                           mac_blocks_per_cu=bpc, quad_max=quad)
+            got = key.commit(s).astype(np.uint64)
+            assert np.array_equal(got, exp), (fused, variant, waves, bpc, quad)
+        with pytest.raises(la.GridHipError):
+            ctx.set_tuning(mac_waves=5)
+    finally:
+        ctx.set_tuning(**saved)
+        key.close()
+
+
+def test_back_to_back_device_commits_pipelined(oracle, ctx, ring_q32):
+    """Many device-memory commits enqueued without waiting (the finalize of one
+    overlaps the MAC of the next), two keys alternating, outputs reused, and a
+    non-commit call in between: every result equals its oracle value."""
+    import lattica_amd as la
+    r = ring_q32
+    rng = np.random.default_rng(88)
+    rows, n = 16, 700
+    a = [rand_residues(rng, r.q, (rows, n, 64)) for _ in range(2)]
+    keys = [la.Key(r, x, domain=la.DOMAIN_NTT) for x in a]
+    ws = [small_signed(rng, r.q, (n, 64), -128, 128) for _ in range(5)]
+    wd = [ctx.to_device(w.astype(np.uint32)) for w in ws]
+    outs = [ctx.alloc(rows * 64 * 4) for _ in range(3)]
+    z = np.zeros((rows, 1, 64), dtype=np.uint64)
+    exp = {(k, i): oracle.commit(r.q, 64, a[k], z, ws[i], np.zeros((1, 64), dtype=np.uint64),
+                                 key_domain=1) for k in range(2) for i in range(5)}
+    seq = [(i % 2, i % 5) for i in range(12)]
+    got = {}
+    for j, (k, i) in enumerate(seq):
+        keys[k].commit_device(wd[i], n, outs[j % 3])
+        if j % 3 == 2:
+            # a non-commit call on another buffer, then read the three outputs
+            la.lib().gh_twisted_forward(r.h, wd[4].ptr, n, la.MEM_DEVICE)
+            ws[4] = r.twisted_forward(ws[4]).astype(np.uint64)
+            for jj in range(j - 2, j + 1):
+                got[jj] = outs[jj % 3].download(np.uint32, (rows, 64)).astype(np.uint64)
+    for j, (k, i) in enumerate(seq):
+        # witness 4 was transformed in place after each group of three commits
+        if i == 4:
+            continue
+        assert np.array_equal(got[j], exp[(k, i)]), j
+    for b in wd + outs:
+        b.free()
+    for k in keys:
+        k.close()
+
+
 def test_commit_batch_uneven_lengths(ring_q32, ring_gold):
     import lattica_amd as la
     for r in (ring_q32, ring_gold):
