@@ -39,6 +39,86 @@ def witness(rng, n):
     return np.where(x < 0, Q + x, x).astype(np.uint32)
 
 
+def batched_ns(ctx, keys, s_dev, log, batch=32, reps=6):
+    """The batched form at the same shape: `batch` commitments per gh_commit_batch
+    call against one key (the key is loaded once per group of witnesses), keys
+    in rotation across calls. Item 0 is checked against the plain form."""
+    outs = [ctx.alloc(KAPPA * D * 4) for _ in range(batch)]
+    wit = [s_dev[i % len(s_dev)] for i in range(batch)]
+    lens = [N] * batch
+    keys[0].commit_batch_device(wit, lens, outs)
+    probe = ctx.alloc(KAPPA * D * 4)
+    keys[0].commit_device(s_dev[0], N, probe)
+    ctx.synchronize()
+    same = bool(np.array_equal(outs[0].download(np.uint32, (KAPPA, D)),
+                               probe.download(np.uint32, (KAPPA, D))))
+    a = time.perf_counter()
+    for r in range(reps):
+        keys[r % len(keys)].commit_batch_device(wit, lens, outs)
+    ctx.synchronize()
+    dt = time.perf_counter() - a
+    rate = batch * reps / dt
+    log(f"batched ({batch} per call, same shape): {rate:,.0f} commits/s; item 0 == plain: {same}")
+    for b in outs + [probe]:
+        b.free()
+    return {"commits_per_s": round(rate, 1), "batch": batch, "us_per_batch": round(dt / reps * 1e6, 1),
+            "item0_equals_plain": same}
+
+
+def c4_goldilocks(ctx, la, log, count=2000, kappa=16, max_cols=1 << 14):
+    """2000 Goldilocks commitments (kappa = 16, d = 64) with log-uniform witness
+    lengths in [2^8, 2^14] against one shared key: per-call form (one
+    gh_commit_batch call per commitment) versus one batched call."""
+    p = la.GOLDILOCKS
+    rng = np.random.default_rng(44)
+    ring = la.Ring(ctx, p, D)
+    hi = rng.integers(0, 1 << 32, size=(kappa, max_cols, D), dtype=np.uint64)
+    lo = rng.integers(0, 1 << 32, size=(kappa, max_cols, D), dtype=np.uint64)
+    a = ((hi << np.uint64(32)) | lo) % np.uint64(p)
+    del hi, lo
+    key = la.Key(ring, a, domain=la.DOMAIN_NTT)
+    pool_polys = 1 << 15
+    t = rng.integers(-1, 2, size=(pool_polys, D), dtype=np.int64)
+    pool = np.where(t < 0, np.uint64(p - 1), t.astype(np.uint64))
+    pool_dev = ctx.to_device(pool)
+    lens = np.floor(2.0 ** rng.uniform(8, 14, size=count)).astype(np.int64)
+    offs = rng.integers(0, pool_polys - lens + 1)
+    wptr = [pool_dev.ptr.value + int(o) * D * 8 for o in offs]
+    out = ctx.alloc(count * kappa * D * 8)
+    optr = [out.ptr.value + i * kappa * D * 8 for i in range(count)]
+    # warm-up and parity spot check (two short items against the oracle)
+    key.commit_batch_device(wptr, lens, optr)
+    ctx.synchronize()
+    res = out.download(np.uint64, (count, kappa, D))
+    from oracle import gridref
+    ok = True
+    for i in np.argsort(lens)[:2]:
+        n, o = int(lens[i]), int(offs[i])
+        z = np.zeros((kappa, 1, D), dtype=np.uint64)
+        exp = gridref.commit(p, D, a[:, :n], z, pool[o:o + n], np.zeros((1, D), dtype=np.uint64),
+                             key_domain=1)
+        ok = ok and bool(np.array_equal(res[i], exp))
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        key.commit_batch_device(wptr, lens, optr)
+    ctx.synchronize()
+    batched = count * reps / (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    for i in range(count):
+        key.commit_batch_device([wptr[i]], [lens[i]], [optr[i]])
+    ctx.synchronize()
+    plain = count / (time.perf_counter() - t0)
+    log(f"C4 Goldilocks x{count}: batched {batched:,.0f} commits/s, per-call {plain:,.0f} "
+        f"commits/s; oracle spot check {ok}")
+    key.close()
+    pool_dev.free()
+    out.free()
+    ring.close()
+    return {"batched_commits_per_s": round(batched, 1), "per_call_commits_per_s": round(plain, 1),
+            "mean_len": float(lens.mean()), "oracle_spot_check": ok}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +194,14 @@ def main():
         mac_kernel = "k_commit_fused_q32"  # witness transform + MAC in one kernel
     else:
         mac_kernel = "k_mac4_q32" if tuning["mac_variant"] == 1 else "k_mac_q32"
+    # HBM bytes per dispatch of that kernel from the committed rocprofv3 --pmc summary
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_commit_q32_k16_n32768.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tdoc = json.load(fh)
+        if tdoc.get("kernel", "").startswith(mac_kernel):
+            traffic = tdoc["traffic_bytes_per_dispatch"]
     # ---- roofline: MAC kernel duration from HIP events on the library stream ----
     ctx.profile(True)
     for i in range(args.steps):
@@ -169,6 +257,8 @@ def main():
             "algorithmic_GBps": round(3 * pairs * D * 4 / dt / 1e9, 1)}
         for b in (pa, pb, po):
             b.free()
+        extra["batched_same_shape"] = batched_ns(ctx, keys, s_dev, log)
+        extra["c4_goldilocks_2000"] = c4_goldilocks(ctx, la, log)
 
     cpu = None
     if not args.no_cpu:
@@ -217,7 +307,7 @@ def main():
         "latency_us": {"p50": round(float(np.percentile(lat, 50)), 1),
                        "p99": round(float(np.percentile(lat, 99)), 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": mac_kernel, "kernel_us": round(mac_ms * 1e3, 2),
                      "algorithmic_bytes": ALGO_BYTES,
                      "frac_of_measured_read_ceiling":

@@ -183,7 +183,8 @@ class Context:
     def synchronize(self):
         _check(lib().gh_ctx_synchronize(self.h), "gh_ctx_synchronize")
 
-    TUNING_KEYS = ("mac_variant", "mac_waves", "mac_blocks_per_cu", "quad_max", "fused")
+    TUNING_KEYS = ("mac_variant", "mac_waves", "mac_blocks_per_cu", "quad_max", "fused",
+                   "batch_chunk_cols", "batch_group")
 
     def set_tuning(self, **kv):
         for k, v in kv.items():
@@ -395,6 +396,15 @@ class Key:
             items[i].t_out = o.ctypes.data
         _check(lib().gh_commit_batch(self.h, items, len(ws), MEM_HOST), "gh_commit_batch")
         return outs
+
+    def commit_batch_device(self, witnesses, lens, outs):
+        """Device-memory batch: witnesses/outs are DeviceBuffers (or raw addresses)."""
+        items = (CommitItem * max(1, len(lens)))()
+        for i, (w, n, o) in enumerate(zip(witnesses, lens, outs)):
+            items[i].witness = w.ptr.value if isinstance(w, DeviceBuffer) else int(w)
+            items[i].len = int(n)
+            items[i].t_out = o.ptr.value if isinstance(o, DeviceBuffer) else int(o)
+        _check(lib().gh_commit_batch(self.h, items, len(lens), MEM_DEVICE), "gh_commit_batch")
 
     def inner_commit(self, s, domain=DOMAIN_COEFF):
         ring = self.ring

@@ -1,0 +1,312 @@
+// Batched commitments against one resident key (gh_commit_batch).
+//
+// The plain commitment streams the whole key once per witness and is HBM
+// bound. A batch shares the key: witnesses are sorted by length (descending)
+// and cut into groups of G, and every key column a block loads is multiplied
+// against the G witnesses of its group that reach that column. The key is
+// then read once per group instead of once per witness, and the work becomes
+// VALU bound (the exact 96-bit multiply-accumulate for q < 2^32, the modular
+// one for Goldilocks). Three launches:
+//   k_batch_fwd_quad   twisted NTT of every witness polynomial of the batch
+//                      (four lanes per polynomial, one 64-polynomial segment per
+//                      block, segments listed by the host);
+//   k_mac_batch_*      grid (column chunk, group, row block); per block RB rows
+//                      (one wave per 4 rows), lane = slot; partial sums per
+//                      (chunk, item) reduced mod q;
+//   k_finalize_batch   per (item, row): sum of its chunks, inverse NTT.
+// Every item's result equals gh_commit on the same inputs.
+#include <hip/hip_ext.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "gh_internal.h"
+#include "gh_ntt.h"
+#include "gh_common_kernels.h"
+
+namespace gh {
+
+template <class F>
+__global__ __launch_bounds__(256) void k_batch_fwd_quad(F f, const typename F::W* __restrict__ tab,
+                                                        const BatchDesc* __restrict__ desc,
+                                                        const BatchSeg* __restrict__ seg,
+                                                        uint32_t nseg, typename F::W* s_ntt) {
+  using W = typename F::W;
+  constexpr int TW = F::kTwWords;
+  constexpr int V = 16 / sizeof(W);
+  const int t = threadIdx.x & 3;
+  W ltw[16][TW];
+#pragma unroll
+  for (int k = 0; k < 16; k++)
+#pragma unroll
+    for (int w = 0; w < TW; w++) ltw[k][w] = tab[(kTabQuad + 16 * t + k) * TW + w];
+  for (uint32_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
+    const BatchSeg s = seg[sg];
+    const BatchDesc d = desc[s.item];
+    const uint32_t p = s.first + (threadIdx.x >> 2);
+    if (p >= d.len) continue;  // the quad's lanes share p: whole quads drop out
+    W x[16];
+    const W* sp = reinterpret_cast<const W*>(d.src) + (size_t)p * 64 + t;
+#pragma unroll
+    for (int i = 0; i < 16; i++) x[i] = sp[4 * i];
+    quad_fwd64(f, x, tab, ltw, t);
+    uint4* dp = reinterpret_cast<uint4*>(s_ntt + d.s_off + (size_t)p * 64 + 16 * bitrev2(t));
+#pragma unroll
+    for (int v = 0; v < 16 / V; v++) {
+      uint4 o;
+      W* ow = reinterpret_cast<W*>(&o);
+#pragma unroll
+      for (int c = 0; c < V; c++) ow[c] = x[v * V + c];
+      dp[v] = o;
+    }
+  }
+}
+
+// partial layout: [chunk][item][row][slot]
+template <int G, int RB>
+__global__ __launch_bounds__(16 * RB) void k_mac_batch_q32(
+    FQ32 f, const uint32_t* __restrict__ A, const uint32_t* __restrict__ s_ntt,
+    const BatchDesc* __restrict__ desc, uint32_t count, uint32_t rows, uint32_t chunk_cols,
+    uint32_t* __restrict__ partial) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const uint32_t ch = blockIdx.x, g0 = blockIdx.y * G;
+  const uint32_t row0 = blockIdx.z * RB + 4 * wave;
+  const uint32_t c0 = ch * chunk_cols;
+  const uint32_t maxlen = desc[g0].len;  // sorted: the group's first item is its longest
+  if (c0 >= maxlen) return;
+  const uint32_t c1 = c0 + chunk_cols < maxlen ? c0 + chunk_cols : maxlen;
+  uint32_t len[G];
+  const uint32_t* sp[G];
+#pragma unroll
+  for (int i = 0; i < G; i++) {
+    const uint32_t j = g0 + i;
+    len[i] = j < count ? desc[j].len : 0;
+    sp[i] = s_ntt + (j < count ? desc[j].s_off : 0) + lane;
+  }
+  uint64_t lo[G][4];
+  uint32_t hi[G][4];
+#pragma unroll
+  for (int i = 0; i < G; i++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) { lo[i][r] = 0; hi[i][r] = 0; }
+  const size_t col_stride = (size_t)rows * 64;
+  const uint32_t* ap = A + (size_t)row0 * 64 + lane;
+  for (uint32_t c = c0; c < c1; c++) {
+    uint32_t a[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) a[r] = ap[(size_t)c * col_stride + r * 64];
+#pragma unroll
+    for (int i = 0; i < G; i++) {
+      if (c < len[i]) {  // uniform: lengths are block-wide values
+        const uint32_t s = sp[i][(size_t)c * 64];
+#pragma unroll
+        for (int r = 0; r < 4; r++) mac96(lo[i][r], hi[i][r], a[r], s);
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < G; i++) {
+    const uint32_t j = g0 + i;
+    if (j < count && c0 < len[i]) {
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        partial[(((size_t)ch * count + j) * rows + row0 + r) * 64 + lane] =
+            f.reduce96_r2((uint32_t)lo[i][r], (uint32_t)(lo[i][r] >> 32), hi[i][r]);
+    }
+  }
+}
+
+template <int G, int RB>
+__global__ __launch_bounds__(16 * RB) void k_mac_batch_gold(
+    FGold f, const uint64_t* __restrict__ A, const uint64_t* __restrict__ s_ntt,
+    const BatchDesc* __restrict__ desc, uint32_t count, uint32_t rows, uint32_t chunk_cols,
+    uint64_t* __restrict__ partial) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const uint32_t ch = blockIdx.x, g0 = blockIdx.y * G;
+  const uint32_t row0 = blockIdx.z * RB + 4 * wave;
+  const uint32_t c0 = ch * chunk_cols;
+  const uint32_t maxlen = desc[g0].len;
+  if (c0 >= maxlen) return;
+  const uint32_t c1 = c0 + chunk_cols < maxlen ? c0 + chunk_cols : maxlen;
+  uint32_t len[G];
+  const uint64_t* sp[G];
+#pragma unroll
+  for (int i = 0; i < G; i++) {
+    const uint32_t j = g0 + i;
+    len[i] = j < count ? desc[j].len : 0;
+    sp[i] = s_ntt + (j < count ? desc[j].s_off : 0) + lane;
+  }
+  uint64_t acc[G][4];
+#pragma unroll
+  for (int i = 0; i < G; i++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) acc[i][r] = 0;
+  const size_t col_stride = (size_t)rows * 64;
+  const uint64_t* ap = A + (size_t)row0 * 64 + lane;
+  for (uint32_t c = c0; c < c1; c++) {
+    uint64_t a[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) a[r] = ap[(size_t)c * col_stride + r * 64];
+#pragma unroll
+    for (int i = 0; i < G; i++) {
+      if (c < len[i]) {
+        const uint64_t s = sp[i][(size_t)c * 64];
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc[i][r] = f.add(acc[i][r], f.mul(a[r], s));
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < G; i++) {
+    const uint32_t j = g0 + i;
+    if (j < count && c0 < len[i]) {
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        partial[(((size_t)ch * count + j) * rows + row0 + r) * 64 + lane] = acc[i][r];
+    }
+  }
+}
+
+// grid (count, rows): item j's row `row` = INTT(sum over its chunks)
+template <class F>
+__global__ __launch_bounds__(1024) void k_finalize_batch(F f, const typename F::W* __restrict__ tab,
+                                                         const typename F::W* __restrict__ partial,
+                                                         const BatchDesc* __restrict__ desc,
+                                                         uint32_t count, uint32_t rows,
+                                                         uint32_t chunk_cols, int post) {
+  using W = typename F::W;
+  const int k = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const uint32_t j = blockIdx.x, row = blockIdx.y;
+  const uint32_t nch = (desc[j].len + chunk_cols - 1) / chunk_cols;
+  W acc = 0;
+  for (uint32_t c0 = g; c0 < nch; c0 += 16 * 8) {
+    W v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const uint32_t c = c0 + 16 * u;
+      v[u] = c < nch ? partial[(((size_t)c * count + j) * rows + row) * 64 + k] : W(0);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) acc = f.add(acc, v[u]);
+  }
+  __shared__ W red[16][64];
+  red[g][k] = acc;
+  __syncthreads();
+  if (g == 0) {
+    W s = red[0][k];
+#pragma unroll
+    for (int i = 1; i < 16; i++) s = f.add(s, red[i][k]);
+    reinterpret_cast<W*>(desc[j].dst)[(size_t)row * 64 + k] = wave_inv64(f, s, tab, post, k);
+  }
+}
+
+// ---------------------------------------------------------------------------
+
+template <int G, int RB>
+static void mac_batch_launch(gh_key* key, const void* s_ntt, const BatchDesc* desc, uint32_t count,
+                             uint32_t chunk_cols, uint32_t nchunks, void* partial) {
+  gh_ring* ring = key->ring;
+  gh_ctx* ctx = ring->ctx;
+  dim3 grid(nchunks, (count + G - 1) / G, key->rows / RB);
+  if (ring->kind == GH_RING_Q32)
+    hipLaunchKernelGGL((k_mac_batch_q32<G, RB>), grid, dim3(16 * RB), 0, ctx->stream,
+                       FQ32{(uint32_t)ring->q, ring->qinv32}, (const uint32_t*)key->dev_a,
+                       (const uint32_t*)s_ntt, desc, count, key->rows, chunk_cols,
+                       (uint32_t*)partial);
+  else
+    hipLaunchKernelGGL((k_mac_batch_gold<G, RB>), grid, dim3(16 * RB), 0, ctx->stream, FGold{},
+                       (const uint64_t*)key->dev_a, (const uint64_t*)s_ntt, desc, count,
+                       key->rows, chunk_cols, (uint64_t*)partial);
+}
+
+template <int G>
+static void mac_batch_rb(gh_key* key, const void* s_ntt, const BatchDesc* desc, uint32_t count,
+                         uint32_t chunk_cols, uint32_t nchunks, void* partial) {
+  const uint32_t rows = key->rows;
+  if (rows % 16 == 0) mac_batch_launch<G, 16>(key, s_ntt, desc, count, chunk_cols, nchunks, partial);
+  else if (rows % 8 == 0) mac_batch_launch<G, 8>(key, s_ntt, desc, count, chunk_cols, nchunks, partial);
+  else mac_batch_launch<G, 4>(key, s_ntt, desc, count, chunk_cols, nchunks, partial);
+}
+
+// items: device pointers (witness coefficient domain, t_out); lengths checked by the caller.
+int launch_commit_batch(gh_key* key, const gh_commit_item* items, size_t count) {
+  gh_ring* ring = key->ring;
+  gh_ctx* ctx = ring->ctx;
+  if (count == 0) return GH_OK;
+  if (key->rows % 4 != 0 || count > 0x7fffffff) {
+    for (size_t i = 0; i < count; i++) {
+      int st = launch_commit(key, items[i].witness, items[i].len, GH_DOMAIN_COEFF, items[i].t_out);
+      if (st) return st;
+    }
+    return GH_OK;
+  }
+  const size_t wb = ring->word_bytes;
+  const uint32_t rows = key->rows;
+  const uint32_t n = (uint32_t)count;
+  const uint32_t cc = (uint32_t)ctx->tune.batch_chunk_cols;
+  const int G = ctx->tune.batch_group;
+
+  std::vector<uint32_t> order(n);
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint32_t x, uint32_t y) { return items[x].len > items[y].len; });
+  std::vector<BatchDesc>& hd = ctx->batch_desc;
+  std::vector<BatchSeg>& hs = ctx->batch_seg;
+  hd.resize(n);
+  hs.clear();
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    const gh_commit_item& it = items[order[i]];
+    hd[i] = BatchDesc{total * 64, it.len, 0, (uint64_t)(uintptr_t)it.witness,
+                      (uint64_t)(uintptr_t)it.t_out};
+    for (uint32_t p = 0; p < it.len; p += 64) hs.push_back(BatchSeg{i, p});
+    total += it.len;
+  }
+  const uint32_t maxlen = hd[0].len;
+  const uint32_t nchunks = (maxlen + cc - 1) / cc;
+  const size_t s_bytes = (size_t)total * 64 * wb;
+  const size_t p_bytes = (size_t)nchunks * n * rows * 64 * wb;
+  const size_t d_bytes = (size_t)n * sizeof(BatchDesc);
+  const size_t g_bytes = hs.size() * sizeof(BatchSeg);
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  void* ws;
+  int st = ctx_workspace(ctx, al(s_bytes) + al(p_bytes) + al(d_bytes) + al(g_bytes), &ws);
+  if (st) return st;
+  char* base = (char*)ws;
+  void* s_ntt = base;
+  void* partial = base + al(s_bytes);
+  BatchDesc* ddesc = (BatchDesc*)(base + al(s_bytes) + al(p_bytes));
+  BatchSeg* dseg = (BatchSeg*)(base + al(s_bytes) + al(p_bytes) + al(d_bytes));
+  GH_HIP(hipMemcpyAsync(ddesc, hd.data(), d_bytes, hipMemcpyHostToDevice, ctx->stream));
+  GH_HIP(hipMemcpyAsync(dseg, hs.data(), g_bytes, hipMemcpyHostToDevice, ctx->stream));
+
+  const uint32_t nseg = (uint32_t)hs.size();
+  const unsigned sgrid = nseg < (uint32_t)ctx->num_cus * 16 ? nseg : (unsigned)ctx->num_cus * 16;
+  if (ring->kind == GH_RING_Q32)
+    hipLaunchKernelGGL((k_batch_fwd_quad<FQ32>), dim3(sgrid), dim3(256), 0, ctx->stream,
+                       FQ32{(uint32_t)ring->q, ring->qinv32}, (const uint32_t*)ring->dev_tab,
+                       ddesc, dseg, nseg, (uint32_t*)s_ntt);
+  else
+    hipLaunchKernelGGL((k_batch_fwd_quad<FGold>), dim3(sgrid), dim3(256), 0, ctx->stream, FGold{},
+                       (const uint64_t*)ring->dev_tab, ddesc, dseg, nseg, (uint64_t*)s_ntt);
+  GH_HIP(hipGetLastError());
+  if (G == 4) mac_batch_rb<4>(key, s_ntt, ddesc, n, cc, nchunks, partial);
+  else if (G == 16) mac_batch_rb<16>(key, s_ntt, ddesc, n, cc, nchunks, partial);
+  else mac_batch_rb<8>(key, s_ntt, ddesc, n, cc, nchunks, partial);
+  GH_HIP(hipGetLastError());
+  if (ring->kind == GH_RING_Q32)
+    hipLaunchKernelGGL((k_finalize_batch<FQ32>), dim3(n, rows), dim3(1024), 0, ctx->stream,
+                       FQ32{(uint32_t)ring->q, ring->qinv32}, (const uint32_t*)ring->dev_tab,
+                       (const uint32_t*)partial, ddesc, n, rows, cc, (int)kTabUntwistMac);
+  else
+    hipLaunchKernelGGL((k_finalize_batch<FGold>), dim3(n, rows), dim3(1024), 0, ctx->stream,
+                       FGold{}, (const uint64_t*)ring->dev_tab, (const uint64_t*)partial, ddesc, n,
+                       rows, cc, (int)kTabUntwist);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+}  // namespace gh

@@ -267,6 +267,12 @@ GH_API int gh_ctx_set_tuning(gh_ctx* ctx, const char* key, int64_t value) {
   } else if (!std::strcmp(key, "fused")) {
     if (value != 0 && value != 1) return GH_E_INVALID_ARGUMENT;
     t.fused = (int)value;
+  } else if (!std::strcmp(key, "batch_chunk_cols")) {
+    if (value < 16 || value > 4096) return GH_E_INVALID_ARGUMENT;
+    t.batch_chunk_cols = (int)value;
+  } else if (!std::strcmp(key, "batch_group")) {
+    if (value != 4 && value != 8 && value != 16) return GH_E_INVALID_ARGUMENT;
+    t.batch_group = (int)value;
   } else {
     return GH_E_INVALID_ARGUMENT;
   }
@@ -282,6 +288,8 @@ GH_API int gh_ctx_get_tuning(gh_ctx* ctx, const char* key, int64_t* value) {
   else if (!std::strcmp(key, "mac_blocks_per_cu")) *value = t.mac_blocks_per_cu;
   else if (!std::strcmp(key, "quad_max")) *value = t.quad_max;
   else if (!std::strcmp(key, "fused")) *value = t.fused;
+  else if (!std::strcmp(key, "batch_chunk_cols")) *value = t.batch_chunk_cols;
+  else if (!std::strcmp(key, "batch_group")) *value = t.batch_group;
   else return GH_E_INVALID_ARGUMENT;
   return GH_OK;
 }
@@ -652,11 +660,7 @@ GH_API int gh_commit_batch(gh_key* key, const gh_commit_item* items, size_t coun
   TRY(set_device(ctx));
   const size_t wb = ring->word_bytes, d = ring->d;
   const size_t t_bytes = (size_t)key->rows * d * wb;
-  if (mem == GH_MEM_DEVICE) {
-    for (size_t i = 0; i < count; i++)
-      TRY(launch_commit(key, items[i].witness, items[i].len, GH_DOMAIN_COEFF, items[i].t_out));
-    return GH_OK;
-  }
+  if (mem == GH_MEM_DEVICE) return launch_commit_batch(key, items, count);
   size_t total = 0;
   for (size_t i = 0; i < count; i++) {
     TRY(check_canonical_host(ring, items[i].witness, (size_t)items[i].len * d));
@@ -666,12 +670,14 @@ GH_API int gh_commit_batch(gh_key* key, const gh_commit_item* items, size_t coun
   TRY(ctx_staging(ctx, total + count * t_bytes, &dev));
   char* sp = (char*)dev;
   char* tp = (char*)dev + total;
+  std::vector<gh_commit_item> staged(count);
   for (size_t i = 0; i < count; i++) {
     const size_t b = (size_t)items[i].len * d * wb;
     TRY(h2d(ctx, sp, items[i].witness, b));
-    TRY(launch_commit(key, sp, items[i].len, GH_DOMAIN_COEFF, tp + i * t_bytes));
+    staged[i] = gh_commit_item{sp, items[i].len, 0, tp + i * t_bytes};
     sp += b;
   }
+  TRY(launch_commit_batch(key, staged.data(), count));
   for (size_t i = 0; i < count; i++) TRY(d2h(ctx, items[i].t_out, tp + i * t_bytes, t_bytes));
   return sync(ctx);
 }

@@ -10,12 +10,30 @@
 #include "../../include/gridhip.h"
 #include "gh_field.h"
 
+namespace gh {
+// per-item descriptor of a commitment batch (device copy in the workspace)
+struct BatchDesc {
+  uint64_t s_off;  // word offset of the item's transformed witness in the batch buffer
+  uint32_t len;    // witness polynomials (key columns used)
+  uint32_t pad;
+  uint64_t src;    // device address of the coefficient-domain witness
+  uint64_t dst;    // device address of t_out (rows polynomials)
+};
+// a run of at most 64 polynomials of one item, transformed by one block
+struct BatchSeg {
+  uint32_t item;   // index into the sorted descriptors
+  uint32_t first;  // first polynomial of the segment within the item
+};
+}  // namespace gh
+
 struct gh_tuning {
   int mac_variant = 1;        // q32 MAC: 0 = dword loads, 1 = 16-byte loads
   int mac_waves = 16;         // waves per MAC block (4, 8, 16)
   int mac_blocks_per_cu = 1;  // MAC blocks (= partial sums) per compute unit
   int64_t quad_max = 1 << 17; // largest batch transformed four lanes per polynomial
   int fused = 1;              // q32 commit: witness transform fused into the MAC kernel
+  int batch_chunk_cols = 256; // batched MAC: key columns per block
+  int batch_group = 8;        // batched MAC: witnesses sharing each loaded key column (4, 8, 16)
 };
 
 struct gh_ctx {
@@ -31,6 +49,9 @@ struct gh_ctx {
   size_t stage_bytes = 0;
   // small fixed device buffer (flags)
   void* misc = nullptr;
+  // host images of the last batch's descriptors (kept alive across the async upload)
+  std::vector<gh::BatchDesc> batch_desc;
+  std::vector<gh::BatchSeg> batch_seg;
   // phase timing (gh_ctx_profile_*): event pairs recorded per commit
   bool prof_on = false;
   std::vector<hipEvent_t> prof_pool;
@@ -82,6 +103,8 @@ int launch_pointwise_mul(gh_ring* ring, const void* a, const void* b, void* out,
 int launch_key_layout(gh_ring* ring, const void* src_rowmajor, uint32_t rows, uint32_t cols,
                       uint32_t col_offset, uint32_t total_cols, void* dst, int domain);
 int launch_commit(gh_key* key, const void* s_dev, uint32_t cols, int domain, void* t_dev);
+// items hold device pointers; every len in [1, key columns]
+int launch_commit_batch(gh_key* key, const gh_commit_item* items, size_t count);
 int launch_check_canonical(gh_ring* ring, const void* p, size_t words, int* bad_flag_dev);
 int launch_gram(gh_ring* ring, const void* s_dev, uint32_t r, uint32_t n, int domain, void* g_dev);
 int launch_stream_probe(gh_ctx* ctx, const void* src, void* dst, size_t bytes, int copy,

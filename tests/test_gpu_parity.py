@@ -318,6 +318,40 @@ def test_commit_batch_uneven_lengths(ring_q32, ring_gold):
         key.close()
 
 
+@pytest.mark.parametrize("name", ["q32", "goldilocks"])
+def test_commit_batch_device_many_items_all_tunings(oracle, ctx, ring_q32, ring_gold, name):
+    """Log-uniform lengths (the C4 shape, scaled down), device buffers, every
+    batch tuning: each item equals the oracle commitment with the key prefix."""
+    import lattica_amd as la
+    r = ring_for(name, ring_q32, ring_gold)
+    rng = np.random.default_rng(404)
+    rows, cols, count = 16, 600, 37
+    a = rand_residues(rng, r.q, (rows, cols, 64))
+    key = la.Key(r, a, domain=la.DOMAIN_NTT)
+    lens = np.floor(2.0 ** rng.uniform(0, np.log2(cols), size=count)).astype(int)
+    lens[0], lens[1] = cols, 1
+    ws = [small_signed(rng, r.q, (n, 64), -1, 2) for n in lens]
+    wd = [ctx.to_device(r.words(w)) for w in ws]
+    outs = [ctx.alloc(rows * 64 * r.info.word_bytes) for _ in lens]
+    z = np.zeros((rows, 1, 64), dtype=np.uint64)
+    exp = [oracle.commit(r.q, 64, a[:, :n], z, w, np.zeros((1, 64), dtype=np.uint64), key_domain=1)
+           for n, w in zip(lens, ws)]
+    saved = ctx.get_tuning()
+    try:
+        for group, chunk in [(8, 256), (4, 16), (16, 100), (8, 4096)]:
+            ctx.set_tuning(batch_group=group, batch_chunk_cols=chunk)
+            key.commit_batch_device(wd, lens, outs)
+            ctx.synchronize()
+            for i in range(count):
+                got = outs[i].download(r.dtype, (rows, 64)).astype(np.uint64)
+                assert np.array_equal(got, exp[i]), (group, chunk, i, lens[i])
+    finally:
+        ctx.set_tuning(**saved)
+        for b in wd + outs:
+            b.free()
+        key.close()
+
+
 def test_inner_commit_matches_oracle(oracle, ring_q32):
     import lattica_amd as la
     r = ring_q32
