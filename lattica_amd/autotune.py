@@ -93,6 +93,73 @@ def tune_commit(ctx, kappa=16, n=1 << 15, reps=60, quick=False, log=print):
     return results
 
 
+def tune_batch(ctx, quick=False, log=print):
+    """Sweep batch_chunk_cols x batch_group on two batched workloads and rank by
+    the summed time: (a) 16 q32 commitments at kappa = 16, n = 2^15 per call;
+    (b) 400 Goldilocks commitments, kappa = 16, log-uniform lengths in
+    [2^8, 2^14] (the C4 distribution). Results must not change with the tuning."""
+    import lattica_amd as la
+    rng = np.random.default_rng(9)
+    work = []
+    # (a) q32 same shape
+    rq = la.Ring(ctx, la.Q32, 64)
+    kq = la.Key(rq, rng.integers(0, la.Q32, size=(16, 1 << 15, 64), dtype=np.uint32),
+                domain=la.DOMAIN_NTT)
+    wq = [ctx.to_device(_witness(rng, la.Q32, 1 << 15)) for _ in range(4)]
+    oq = [ctx.alloc(16 * 64 * 4) for _ in range(16)]
+    work.append(("q32_16x32768", kq, [wq[i % 4] for i in range(16)], [1 << 15] * 16, oq, 4 * 16 * 64))
+    # (b) Goldilocks, C4 length distribution
+    p = la.GOLDILOCKS
+    rg = la.Ring(ctx, p, 64)
+    hi = rng.integers(0, 1 << 32, size=(16, 1 << 14, 64), dtype=np.uint64)
+    lo = rng.integers(0, 1 << 32, size=(16, 1 << 14, 64), dtype=np.uint64)
+    kg = la.Key(rg, ((hi << np.uint64(32)) | lo) % np.uint64(p), domain=la.DOMAIN_NTT)
+    del hi, lo
+    t = rng.integers(-1, 2, size=(1 << 15, 64), dtype=np.int64)
+    pool = ctx.to_device(np.where(t < 0, np.uint64(p - 1), t.astype(np.uint64)))
+    n_items = 400
+    lens = np.floor(2.0 ** rng.uniform(8, 14, size=n_items)).astype(np.int64)
+    offs = rng.integers(0, (1 << 15) - lens + 1)
+    og = ctx.alloc(n_items * 16 * 64 * 8)
+    work.append(("gold_c4_400", kg, [pool.ptr.value + int(o) * 512 for o in offs], list(lens),
+                 [og.ptr.value + i * 16 * 64 * 8 for i in range(n_items)], None))
+
+    chunks = [64, 128, 256] if quick else [32, 64, 128, 256, 512]
+    groups = [4, 8, 16]
+    results = []
+    refs = {}
+    for cc, g in itertools.product(chunks, groups):
+        ctx.set_tuning(batch_chunk_cols=cc, batch_group=g)
+        total_us, per = 0.0, {}
+        for name, key, ws, ls, os_, probe in work:
+            key.commit_batch_device(ws, ls, os_)
+            ctx.synchronize()
+            out0 = (os_[0].download(np.uint32, (16, 64)) if probe else
+                    og.download(np.uint64, (4, 16, 64)))
+            if name in refs and not np.array_equal(refs[name], out0):
+                raise RuntimeError(f"batch tuning {cc, g} changed the result of {name}")
+            refs.setdefault(name, out0)
+            reps = 3
+            a = time.perf_counter()
+            for _ in range(reps):
+                key.commit_batch_device(ws, ls, os_)
+            ctx.synchronize()
+            us = (time.perf_counter() - a) / reps * 1e6
+            per[name] = round(us / len(ls), 3)
+            total_us += us
+        results.append({"config": {"batch_chunk_cols": cc, "batch_group": g},
+                        "us_per_commit": per, "total_us": round(total_us, 1)})
+        log(f"batch chunk {cc:4d} group {g:2d} -> {per}")
+    for b in wq + oq + [pool, og]:
+        b.free()
+    kq.close()
+    kg.close()
+    rq.close()
+    rg.close()
+    results.sort(key=lambda r: r["total_us"])
+    return results
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--quick", action="store_true")
@@ -102,8 +169,13 @@ def main(argv=None):
     ctx = la.Context(0, apply_tuning=False)
     res = tune_commit(ctx, quick=args.quick, log=lambda m: print(m, flush=True))
     best = res[0]
+    ctx.set_tuning(**best["config"])
+    bres = tune_batch(ctx, quick=args.quick, log=lambda m: print(m, flush=True))
+    cfg = dict(best["config"])
+    cfg.update(bres[0]["config"])
     doc = {"device_cus": ctx.num_cus, "workload": "commit_q32_d64_kappa16_n32768",
-           "best": best["config"], "best_us_per_commit": best["us_per_commit"], "sweep": res}
+           "best": cfg, "best_us_per_commit": best["us_per_commit"],
+           "best_batch": bres[0], "sweep": res, "batch_sweep": bres}
     with open(args.out, "w") as fh:
         json.dump(doc, fh, indent=1)
     print(json.dumps({"best": best}), flush=True)

@@ -118,6 +118,80 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch_q32(
   }
 }
 
+// 16-byte variant for q32: a wave instruction covers 4 rows x 64 slots of a
+// column (lane l: row 4w + l/16, slots 4(l%16)..+3), each witness contributes
+// one 16-byte load per column, and the next column's loads are issued before
+// the current column's multiply-accumulates (one-column software pipeline).
+template <int G, int RB>
+__global__ __launch_bounds__(16 * RB) void k_mac_batch4_q32(
+    FQ32 f, const uint32_t* __restrict__ A, const uint32_t* __restrict__ s_ntt,
+    const BatchDesc* __restrict__ desc, uint32_t count, uint32_t rows, uint32_t chunk_cols,
+    uint32_t* __restrict__ partial) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int k0 = 4 * (lane & 15);
+  const uint32_t ch = blockIdx.x, g0 = blockIdx.y * G;
+  const uint32_t row_w = blockIdx.z * RB + 4 * wave;  // this wave's 4 rows
+  const uint32_t c0 = ch * chunk_cols;
+  const uint32_t maxlen = desc[g0].len;
+  if (c0 >= maxlen) return;
+  const uint32_t c1 = c0 + chunk_cols < maxlen ? c0 + chunk_cols : maxlen;
+  uint32_t len[G];
+  const u32x4* sp[G];
+#pragma unroll
+  for (int i = 0; i < G; i++) {
+    const uint32_t j = g0 + i;
+    len[i] = j < count ? desc[j].len : 0;
+    sp[i] = reinterpret_cast<const u32x4*>(s_ntt + (j < count ? desc[j].s_off : 0) + k0);
+  }
+  const size_t col_stride4 = (size_t)rows * 16;  // u32x4 per column
+  const u32x4* ap = reinterpret_cast<const u32x4*>(A + (size_t)row_w * 64) + lane;
+  uint64_t lo[G][4];
+  uint32_t hi[G][4];
+#pragma unroll
+  for (int i = 0; i < G; i++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) { lo[i][q] = 0; hi[i][q] = 0; }
+  const u32x4 zero = {0, 0, 0, 0};
+  u32x4 an = ap[(size_t)c0 * col_stride4];
+  u32x4 sn[G];
+#pragma unroll
+  for (int i = 0; i < G; i++) sn[i] = c0 < len[i] ? sp[i][(size_t)c0 * 16] : zero;
+  for (uint32_t c = c0; c < c1; c++) {
+    const u32x4 a = an;
+    u32x4 s[G];
+#pragma unroll
+    for (int i = 0; i < G; i++) s[i] = sn[i];
+    if (c + 1 < c1) {
+      an = ap[(size_t)(c + 1) * col_stride4];
+#pragma unroll
+      for (int i = 0; i < G; i++) sn[i] = c + 1 < len[i] ? sp[i][(size_t)(c + 1) * 16] : zero;
+    }
+#pragma unroll
+    for (int i = 0; i < G; i++) {
+      if (c < len[i]) {
+        mac96(lo[i][0], hi[i][0], a.x, s[i].x);
+        mac96(lo[i][1], hi[i][1], a.y, s[i].y);
+        mac96(lo[i][2], hi[i][2], a.z, s[i].z);
+        mac96(lo[i][3], hi[i][3], a.w, s[i].w);
+      }
+    }
+  }
+  const uint32_t row = row_w + (lane >> 4);
+#pragma unroll
+  for (int i = 0; i < G; i++) {
+    const uint32_t j = g0 + i;
+    if (j < count && c0 < len[i]) {
+      u32x4 o;
+      o.x = f.reduce96_r2((uint32_t)lo[i][0], (uint32_t)(lo[i][0] >> 32), hi[i][0]);
+      o.y = f.reduce96_r2((uint32_t)lo[i][1], (uint32_t)(lo[i][1] >> 32), hi[i][1]);
+      o.z = f.reduce96_r2((uint32_t)lo[i][2], (uint32_t)(lo[i][2] >> 32), hi[i][2]);
+      o.w = f.reduce96_r2((uint32_t)lo[i][3], (uint32_t)(lo[i][3] >> 32), hi[i][3]);
+      *reinterpret_cast<u32x4*>(partial + (((size_t)ch * count + j) * rows + row) * 64 + k0) = o;
+    }
+  }
+}
+
 template <int G, int RB>
 __global__ __launch_bounds__(16 * RB) void k_mac_batch_gold(
     FGold f, const uint64_t* __restrict__ A, const uint64_t* __restrict__ s_ntt,
@@ -146,16 +220,29 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch_gold(
     for (int r = 0; r < 4; r++) acc[i][r] = 0;
   const size_t col_stride = (size_t)rows * 64;
   const uint64_t* ap = A + (size_t)row0 * 64 + lane;
-  for (uint32_t c = c0; c < c1; c++) {
-    uint64_t a[4];
+  // one-column software pipeline: the next column's loads precede this one's math
+  uint64_t an[4], sn[G];
 #pragma unroll
-    for (int r = 0; r < 4; r++) a[r] = ap[(size_t)c * col_stride + r * 64];
+  for (int r = 0; r < 4; r++) an[r] = ap[(size_t)c0 * col_stride + r * 64];
+#pragma unroll
+  for (int i = 0; i < G; i++) sn[i] = c0 < len[i] ? sp[i][(size_t)c0 * 64] : 0;
+  for (uint32_t c = c0; c < c1; c++) {
+    uint64_t a[4], s[G];
+#pragma unroll
+    for (int r = 0; r < 4; r++) a[r] = an[r];
+#pragma unroll
+    for (int i = 0; i < G; i++) s[i] = sn[i];
+    if (c + 1 < c1) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) an[r] = ap[(size_t)(c + 1) * col_stride + r * 64];
+#pragma unroll
+      for (int i = 0; i < G; i++) sn[i] = c + 1 < len[i] ? sp[i][(size_t)(c + 1) * 64] : 0;
+    }
 #pragma unroll
     for (int i = 0; i < G; i++) {
       if (c < len[i]) {
-        const uint64_t s = sp[i][(size_t)c * 64];
 #pragma unroll
-        for (int r = 0; r < 4; r++) acc[i][r] = f.add(acc[i][r], f.mul(a[r], s));
+        for (int r = 0; r < 4; r++) acc[i][r] = f.add(acc[i][r], f.mul(a[r], s[i]));
       }
     }
   }
@@ -211,6 +298,15 @@ static void mac_batch_launch(gh_key* key, const void* s_ntt, const BatchDesc* de
   gh_ring* ring = key->ring;
   gh_ctx* ctx = ring->ctx;
   dim3 grid(nchunks, (count + G - 1) / G, key->rows / RB);
+  if constexpr (G <= 8) {
+    if (ring->kind == GH_RING_Q32 && ctx->tune.mac_variant == 1) {
+      hipLaunchKernelGGL((k_mac_batch4_q32<G, RB>), grid, dim3(16 * RB), 0, ctx->stream,
+                         FQ32{(uint32_t)ring->q, ring->qinv32}, (const uint32_t*)key->dev_a,
+                         (const uint32_t*)s_ntt, desc, count, key->rows, chunk_cols,
+                         (uint32_t*)partial);
+      return;
+    }
+  }
   if (ring->kind == GH_RING_Q32)
     hipLaunchKernelGGL((k_mac_batch_q32<G, RB>), grid, dim3(16 * RB), 0, ctx->stream,
                        FQ32{(uint32_t)ring->q, ring->qinv32}, (const uint32_t*)key->dev_a,
@@ -236,7 +332,8 @@ int launch_commit_batch(gh_key* key, const gh_commit_item* items, size_t count) 
   gh_ring* ring = key->ring;
   gh_ctx* ctx = ring->ctx;
   if (count == 0) return GH_OK;
-  if (key->rows % 4 != 0 || count > 0x7fffffff) {
+  // one item, or a key the grouped kernels do not tile: the plain path per item
+  if (count == 1 || key->rows % 4 != 0 || count > 0x7fffffff) {
     for (size_t i = 0; i < count; i++) {
       int st = launch_commit(key, items[i].witness, items[i].len, GH_DOMAIN_COEFF, items[i].t_out);
       if (st) return st;
