@@ -119,6 +119,177 @@ def c4_goldilocks(ctx, la, log, count=2000, kappa=16, max_cols=1 << 14):
             "mean_len": float(lens.mean()), "oracle_spot_check": ok}
 
 
+def serving_c5(ctx, la, log, count=1000, seed=55):
+    """Serving: a Poisson arrival stream of mixed commit and Gram requests, half
+    q32 and half Goldilocks (d = 64, kappa = 16 keys of 2^15 columns, both
+    resident), witness sizes log-uniform in [2^8, 2^15] polynomials split into
+    r in {1..32} parts, coefficients uniform in [-2^7, 2^7), witnesses resident
+    in HBM. The same trace runs through the plain per-call form (one
+    gh_commit per commitment, one gh_gram per Gram request, each synchronised,
+    served in arrival order) and through gh_server, at two offered rates (half
+    and three times the plain form's saturated request rate) and saturated (all
+    requests at once). Latency = completion - arrival. Every served result is
+    compared with the plain result of the same request."""
+    from lattica_amd import CommitItem, Request
+    import ctypes
+    rng = np.random.default_rng(seed)
+    kappa, cols, pool_polys = 16, 1 << 15, 1 << 15
+    rings = {"q32": la.Ring(ctx, Q, D), "gold": la.Ring(ctx, la.GOLDILOCKS, D)}
+    keys, pools = {}, {}
+    for name, ring in rings.items():
+        if name == "q32":
+            a = rng.integers(0, Q, size=(kappa, cols, D), dtype=np.uint32)
+        else:
+            hi = rng.integers(0, 1 << 32, size=(kappa, cols, D), dtype=np.uint64)
+            a = ((hi << np.uint64(32)) | rng.integers(0, 1 << 32, size=(kappa, cols, D),
+                                                       dtype=np.uint64)) % np.uint64(la.GOLDILOCKS)
+            del hi
+        keys[name] = la.Key(ring, a, domain=la.DOMAIN_NTT)
+        del a
+        x = rng.integers(-128, 128, size=(pool_polys, D), dtype=np.int64)
+        lift = np.where(x < 0, np.uint64(ring.q) - (-x).astype(np.uint64), x.astype(np.uint64))
+        pools[name] = ctx.to_device(lift.astype(ring.dtype))
+    reqs, out_bytes = [], 0
+    for i in range(count):
+        name = "q32" if rng.random() < 0.5 else "gold"
+        ring = rings[name]
+        wb = np.dtype(ring.dtype).itemsize
+        op = la.OP_COMMIT if rng.random() < 0.5 else la.OP_GRAM
+        r = int(rng.integers(1, 33))
+        total = int(np.floor(2.0 ** rng.uniform(8, 15)))
+        n = max(1, total // r)
+        off = int(rng.integers(0, pool_polys - r * n + 1))
+        ob = r * kappa * D * wb if op == la.OP_COMMIT else r * (r + 1) // 2 * D * wb
+        reqs.append({"name": name, "op": op, "r": r, "n": n, "w": pools[name].ptr.value + off * D * wb,
+                     "o": out_bytes, "ob": ob, "wb": wb})
+        out_bytes += (ob + 255) // 256 * 256
+    outs = {"plain": ctx.alloc(out_bytes), "served": ctx.alloc(out_bytes)}
+    n_commits = sum(q["r"] for q in reqs if q["op"] == la.OP_COMMIT)
+    L = la.lib()
+
+    # plain per-call form: prebuilt one-item descriptors, no per-call Python building
+    def plain_calls(dst):
+        base = outs[dst].ptr.value
+        calls = []
+        for q in reqs:
+            ring, key = rings[q["name"]], keys[q["name"]]
+            if q["op"] == la.OP_COMMIT:
+                items = []
+                for i in range(q["r"]):
+                    it = (CommitItem * 1)()
+                    it[0].witness = q["w"] + i * q["n"] * D * q["wb"]
+                    it[0].len = q["n"]
+                    it[0].t_out = base + q["o"] + i * kappa * D * q["wb"]
+                    items.append(it)
+                calls.append(("c", key.h, items))
+            else:
+                calls.append(("g", ring.h, (ctypes.c_void_p(q["w"]), q["r"], q["n"],
+                                            ctypes.c_void_p(base + q["o"]))))
+        return calls
+
+    pcalls = plain_calls("plain")
+
+    def run_plain(arrivals):
+        lat = np.zeros(count)
+        t0 = time.perf_counter()
+        for i, c in enumerate(pcalls):
+            ta = t0 + arrivals[i]
+            while time.perf_counter() < ta:
+                pass
+            if c[0] == "c":
+                for it in c[2]:
+                    L.gh_commit_batch(c[1], it, 1, la.MEM_DEVICE)
+            else:
+                w, r, n, g = c[2]
+                L.gh_gram(c[1], w, r, n, la.DOMAIN_COEFF, g, la.MEM_DEVICE)
+            ctx.synchronize()
+            lat[i] = time.perf_counter() - ta
+        return lat, time.perf_counter() - t0
+
+    sbase = outs["served"].ptr.value
+    sreqs = []
+    for q in reqs:
+        rq = Request()
+        rq.ops, rq.r, rq.n, rq.mem = q["op"], q["r"], q["n"], la.MEM_DEVICE
+        rq.ring = rings[q["name"]].h.value
+        rq.key = keys[q["name"]].h.value if q["op"] == la.OP_COMMIT else None
+        rq.witness = q["w"]
+        if q["op"] == la.OP_COMMIT:
+            rq.t_out = sbase + q["o"]
+        else:
+            rq.g_out = sbase + q["o"]
+        sreqs.append(rq)
+
+    def run_served(srv, arrivals):
+        tks = []
+        tk = ctypes.c_uint64()
+        t0 = time.perf_counter()
+        for i, rq in enumerate(sreqs):
+            ta = t0 + arrivals[i]
+            while time.perf_counter() < ta:
+                pass
+            st = L.gh_server_submit(srv.h, ctypes.byref(rq), ctypes.byref(tk))
+            if st:
+                raise la.GridHipError(st, "gh_server_submit")
+            tks.append(tk.value)
+        lat = np.zeros(count)
+        for i, t in enumerate(tks):
+            status, lat_us, _, _ = srv.wait(t)
+            if status:
+                raise la.GridHipError(status, "served request")
+            lat[i] = lat_us * 1e-6
+        return lat, time.perf_counter() - t0
+
+    def summary(lat, elapsed):
+        return {"p50_us": round(float(np.percentile(lat, 50)) * 1e6, 1),
+                "p99_us": round(float(np.percentile(lat, 99)) * 1e6, 1),
+                "commits_per_s": round(n_commits / elapsed, 1),
+                "requests_per_s": round(count / elapsed, 1)}
+
+    zeros = np.zeros(count)
+    run_plain(zeros)  # warm-up (plans, workspaces)
+    lat, el = run_plain(zeros)
+    sat_plain = summary(lat, el)
+    cap = count / el
+    srv = la.Server(ctx, mem_bytes=8 << 30)
+    run_served(srv, zeros)  # warm-up
+    lat, el = run_served(srv, zeros)
+    sat_served = summary(lat, el)
+    res = {"requests": count, "commits": n_commits,
+           "grams": sum(1 for q in reqs if q["op"] == la.OP_GRAM),
+           "saturated": {"plain": sat_plain, "served": sat_served}}
+    for label, mult in (("offered_0.5x_plain_capacity", 0.5), ("offered_3x_plain_capacity", 3.0)):
+        rate = mult * cap
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=count))
+        lp, ep = run_plain(arrivals)
+        ls, es = run_served(srv, arrivals)
+        res[label] = {"offered_requests_per_s": round(rate, 1), "plain": summary(lp, ep),
+                      "served": summary(ls, es)}
+        log(f"C5 {label} ({rate:,.0f} req/s): plain p50 {res[label]['plain']['p50_us']:,.0f} us "
+            f"p99 {res[label]['plain']['p99_us']:,.0f} us {res[label]['plain']['commits_per_s']:,.0f} "
+            f"commits/s | served p50 {res[label]['served']['p50_us']:,.0f} us p99 "
+            f"{res[label]['served']['p99_us']:,.0f} us {res[label]['served']['commits_per_s']:,.0f} "
+            f"commits/s")
+    res["server"] = {k: v for k, v in srv.stats().items()
+                     if k in ("waves", "max_wave_seen", "mem_peak", "device_ms", "failed")}
+    srv.close()
+    ctx.synchronize()
+    a = outs["plain"].download(np.uint8, (out_bytes,))
+    b = outs["served"].download(np.uint8, (out_bytes,))
+    res["served_equals_plain"] = bool(np.array_equal(a, b))
+    log(f"C5 saturated: plain {sat_plain['commits_per_s']:,.0f} commits/s "
+        f"({sat_plain['requests_per_s']:,.0f} req/s), served {sat_served['commits_per_s']:,.0f} "
+        f"commits/s ({sat_served['requests_per_s']:,.0f} req/s, p99 {sat_served['p99_us']:,.0f} us); "
+        f"served == plain for all {count}: {res['served_equals_plain']}")
+    for b in list(outs.values()) + list(pools.values()):
+        b.free()
+    for k in keys.values():
+        k.close()
+    for r in rings.values():
+        r.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,6 +430,7 @@ def main():
             b.free()
         extra["batched_same_shape"] = batched_ns(ctx, keys, s_dev, log)
         extra["c4_goldilocks_2000"] = c4_goldilocks(ctx, la, log)
+        extra["serving_c5"] = serving_c5(ctx, la, log)
 
     cpu = None
     if not args.no_cpu:

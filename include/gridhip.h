@@ -250,6 +250,83 @@ GH_API int gh_jl_project(gh_ring *ring, const uint8_t seed[32], uint32_t rows, c
                          uint32_t num_parts, uint64_t cols, void *p_modq, int64_t *p_exact,
                          int8_t *flat, int mem);
 
+/* ---- serving loop -------------------------------------------------------------
+ * Continuous batching of commit and Gram requests on one device. No reference
+ * counterpart: the reference computes one commitment per call on the calling
+ * thread (commit/src/ajtai/mod.rs:443-452, labrador/src/crs.rs:211-216,
+ * labrador/src/main_protocol/garbage.rs:76-93).
+ *
+ * Requests of any size, ring and kind are submitted from any thread as they
+ * arrive. A worker thread admits them in arrival order while the server's HBM
+ * arena (mem_bytes) has room for their footprint (NTT-domain witness, partial
+ * sums, device copies of host inputs and outputs) and fewer than max_inflight
+ * are admitted. The requests admitted together form a wave: one witness
+ * transform per ring, one grouped multiply-accumulate and finalize per key,
+ * one Gram sweep and finalize per ring. Up to `slots` waves are on the device
+ * at once, each on its own stream, so a wave starts while the previous one
+ * drains. A request's arena block is released when its wave completes.
+ *
+ * Each result is bit-identical to the plain call on the same inputs, whatever
+ * else shares its wave: t_i = A[:, :n] * s_i as gh_commit with a key of the
+ * first n columns, g as gh_gram.
+ *
+ * Keys and rings used by a request must outlive it. Host buffers of a request
+ * (mem == GH_MEM_HOST) must stay valid until gh_server_wait / gh_server_poll
+ * reports it done; host witnesses are checked canonical at submit, device
+ * witnesses are trusted. */
+typedef struct gh_server gh_server;
+
+typedef struct gh_server_config {
+  uint64_t mem_bytes;    /* HBM arena for admitted requests (0: 4 GiB) */
+  uint32_t max_inflight; /* admitted requests not yet completed (0: 4096) */
+  uint32_t max_wave;     /* requests launched together (0: 1024) */
+  uint32_t slots;        /* waves on the device at once, one stream each (0: 2, at most 4) */
+  uint32_t max_pending;  /* submitted and not yet admitted; more gives GH_E_BUSY (0: 65536) */
+} gh_server_config;
+
+#define GH_OP_COMMIT 1 /* t_i = A[:, :n] * s_i for every part i (inner_commit_slice) */
+#define GH_OP_GRAM 2   /* upper triangle of <s_i, s_j> (compute_garbage_g) */
+
+typedef struct gh_request {
+  uint32_t ops;        /* GH_OP_COMMIT, GH_OP_GRAM, or both (one witness transform) */
+  uint32_t r;          /* witness parts, >= 1 */
+  uint32_t n;          /* polynomials per part, >= 1 */
+  int32_t mem;         /* where witness, t_out and g_out live */
+  gh_ring *ring;       /* ring of the context the server runs on */
+  gh_key *key;         /* GH_OP_COMMIT: key of `ring`, rows % 4 == 0, >= n columns */
+  const void *witness; /* r x n polynomials, coefficient domain */
+  void *t_out;         /* GH_OP_COMMIT: r x rows polynomials */
+  void *g_out;         /* GH_OP_GRAM: r(r+1)/2 polynomials */
+} gh_request;
+
+typedef struct gh_server_stats {
+  uint64_t submitted, completed, failed;
+  uint64_t waves;           /* waves launched */
+  uint64_t commits;         /* commitments completed (r per commit request) */
+  uint64_t grams;           /* Gram matrices completed */
+  uint64_t mem_in_use;      /* arena bytes held by admitted requests */
+  uint64_t mem_peak;
+  uint32_t pending;         /* submitted, not admitted */
+  uint32_t admitted;        /* admitted, not completed */
+  uint32_t max_wave_seen;   /* most requests in one wave */
+  uint32_t reserved;
+  double device_ms;         /* summed device time of completed waves */
+} gh_server_stats;
+
+GH_API int gh_server_create(gh_ctx *ctx, const gh_server_config *cfg, gh_server **out);
+/* completes every submitted request, then frees the server */
+GH_API int gh_server_destroy(gh_server *srv);
+/* validates and queues a request; GH_E_BUSY when max_pending are queued,
+ * GH_E_OOM when its footprint exceeds the whole arena */
+GH_API int gh_server_submit(gh_server *srv, const gh_request *req, uint64_t *ticket);
+/* blocks until the request is done; status is its result, latency_us the time
+ * from submit to completion; the ticket is released */
+GH_API int gh_server_wait(gh_server *srv, uint64_t ticket, int *status, double *latency_us);
+/* non-blocking; when *done is 1 the outputs are written and the ticket is released */
+GH_API int gh_server_poll(gh_server *srv, uint64_t ticket, int *done, int *status,
+                          double *latency_us);
+GH_API int gh_server_query(gh_server *srv, gh_server_stats *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,7 +14,8 @@ import os
 import numpy as np
 
 __all__ = [
-    "GridHipError", "lib", "library_path", "Context", "Ring", "Key", "DeviceBuffer",
+    "GridHipError", "lib", "library_path", "Context", "Ring", "Key", "DeviceBuffer", "Server",
+    "OP_COMMIT", "OP_GRAM",
     "Q32", "GOLDILOCKS", "MEM_HOST", "MEM_DEVICE", "DOMAIN_COEFF", "DOMAIN_NTT", "STATUS",
 ]
 
@@ -26,6 +27,7 @@ Q32 = 4294966657
 GOLDILOCKS = 0xFFFFFFFF00000001
 
 MEM_HOST, MEM_DEVICE = 0, 1
+OP_COMMIT, OP_GRAM = 1, 2
 DOMAIN_COEFF, DOMAIN_NTT = 0, 1
 
 STATUS = {
@@ -47,6 +49,8 @@ EXPORTS = (
     "gh_ntt_inverse", "gh_twisted_forward", "gh_twisted_inverse", "gh_negacyclic_mul",
     "gh_pointwise_mul", "gh_key_create", "gh_key_destroy", "gh_key_dims", "gh_commit",
     "gh_commit_batch", "gh_gram", "gh_inner_commit", "gh_jl_project",
+    "gh_server_create", "gh_server_destroy", "gh_server_submit", "gh_server_wait",
+    "gh_server_poll", "gh_server_query",
 )
 
 
@@ -69,6 +73,29 @@ class RingInfo(ctypes.Structure):
 class CommitItem(ctypes.Structure):
     _fields_ = [("witness", ctypes.c_void_p), ("len", ctypes.c_uint32),
                 ("reserved", ctypes.c_uint32), ("t_out", ctypes.c_void_p)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("mem_bytes", ctypes.c_uint64), ("max_inflight", ctypes.c_uint32),
+                ("max_wave", ctypes.c_uint32), ("slots", ctypes.c_uint32),
+                ("max_pending", ctypes.c_uint32)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("ops", ctypes.c_uint32), ("r", ctypes.c_uint32), ("n", ctypes.c_uint32),
+                ("mem", ctypes.c_int32), ("ring", ctypes.c_void_p), ("key", ctypes.c_void_p),
+                ("witness", ctypes.c_void_p), ("t_out", ctypes.c_void_p),
+                ("g_out", ctypes.c_void_p)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("failed", ctypes.c_uint64), ("waves", ctypes.c_uint64),
+                ("commits", ctypes.c_uint64), ("grams", ctypes.c_uint64),
+                ("mem_in_use", ctypes.c_uint64), ("mem_peak", ctypes.c_uint64),
+                ("pending", ctypes.c_uint32), ("admitted", ctypes.c_uint32),
+                ("max_wave_seen", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("device_ms", ctypes.c_double)]
 
 
 _lib = None
@@ -125,6 +152,13 @@ def lib():
         "gh_gram": (i32, [vp, vp, u32, u32, i32, vp, i32]),
         "gh_inner_commit": (i32, [vp, vp, u32, u32, i32, vp, i32]),
         "gh_jl_project": (i32, [vp, vp, u32, vp, u32, u64, vp, vp, vp, i32]),
+        "gh_server_create": (i32, [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]),
+        "gh_server_destroy": (i32, [vp]),
+        "gh_server_submit": (i32, [vp, ctypes.POINTER(Request), ctypes.POINTER(u64)]),
+        "gh_server_wait": (i32, [vp, u64, ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_double)]),
+        "gh_server_poll": (i32, [vp, u64, ctypes.POINTER(i32), ctypes.POINTER(i32),
+                                 ctypes.POINTER(ctypes.c_double)]),
+        "gh_server_query": (i32, [vp, ctypes.POINTER(ServerStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -419,3 +453,87 @@ class Key:
         if self.h:
             lib().gh_key_destroy(self.h)
             self.h = None
+
+
+def _addr(x) -> int:
+    if isinstance(x, DeviceBuffer):
+        return x.ptr.value
+    if isinstance(x, ctypes.c_void_p):
+        return x.value
+    return int(x)
+
+
+class Server:
+    """gh_server: continuous batching of commit and Gram requests on a context.
+
+    submit() takes host arrays (mem=MEM_HOST; outputs are allocated here and
+    returned by wait()) or device addresses (mem=MEM_DEVICE; the caller passes
+    t_out / g_out addresses)."""
+
+    def __init__(self, ctx: Context, mem_bytes=0, max_inflight=0, max_wave=0, slots=0,
+                 max_pending=0):
+        cfg = ServerConfig(mem_bytes, max_inflight, max_wave, slots, max_pending)
+        h = ctypes.c_void_p()
+        _check(lib().gh_server_create(ctx.h, ctypes.byref(cfg), ctypes.byref(h)),
+               "gh_server_create")
+        self.h, self.ctx = h, ctx
+        self._held = {}
+
+    def submit(self, ops, ring: Ring, witness, r: int, n: int, key: "Key" = None,
+               t_out=None, g_out=None, mem=MEM_HOST) -> int:
+        req = Request()
+        req.ops, req.r, req.n, req.mem = ops, r, n, mem
+        req.ring = ring.h.value
+        req.key = key.h.value if key is not None else None
+        held = None
+        if mem == MEM_HOST:
+            w = ring.words(witness)
+            if w.size < r * n * ring.d:
+                raise ValueError("witness holds fewer than r * n polynomials")
+            t = (np.zeros((r, key.rows, ring.d), dtype=ring.dtype)
+                 if ops & OP_COMMIT else None)
+            g = (np.zeros((r * (r + 1) // 2, ring.d), dtype=ring.dtype)
+                 if ops & OP_GRAM else None)
+            req.witness = w.ctypes.data
+            req.t_out = t.ctypes.data if t is not None else None
+            req.g_out = g.ctypes.data if g is not None else None
+            held = (w, t, g)
+        else:
+            req.witness = _addr(witness)
+            req.t_out = _addr(t_out) if t_out is not None else None
+            req.g_out = _addr(g_out) if g_out is not None else None
+        tk = ctypes.c_uint64()
+        _check(lib().gh_server_submit(self.h, ctypes.byref(req), ctypes.byref(tk)),
+               "gh_server_submit")
+        if held is not None:
+            self._held[tk.value] = held
+        return tk.value
+
+    def wait(self, ticket: int):
+        """-> (status, latency_us, t_out, g_out); outputs are None for device requests."""
+        st, lat = ctypes.c_int(0), ctypes.c_double(0)
+        _check(lib().gh_server_wait(self.h, ticket, ctypes.byref(st), ctypes.byref(lat)),
+               "gh_server_wait")
+        held = self._held.pop(ticket, (None, None, None))
+        return st.value, lat.value, held[1], held[2]
+
+    def poll(self, ticket: int):
+        """-> None while pending, else as wait()."""
+        done, st, lat = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0)
+        _check(lib().gh_server_poll(self.h, ticket, ctypes.byref(done), ctypes.byref(st),
+                                    ctypes.byref(lat)), "gh_server_poll")
+        if not done.value:
+            return None
+        held = self._held.pop(ticket, (None, None, None))
+        return st.value, lat.value, held[1], held[2]
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        _check(lib().gh_server_query(self.h, ctypes.byref(s)), "gh_server_stats")
+        return {name: getattr(s, name) for name, _ in ServerStats._fields_ if name != "reserved"}
+
+    def close(self):
+        if self.h:
+            lib().gh_server_destroy(self.h)
+            self.h = None
+            self._held.clear()

@@ -16,7 +16,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgridhip.so")
 ARCH = os.environ.get("GRIDHIP_ARCH", "gfx950")
 
-SOURCES = ["gh_plan.cpp", "gh_capi.cpp", "gh_kernels.hip", "gh_labrador.hip", "gh_batch.hip"]
+SOURCES = ["gh_plan.cpp", "gh_capi.cpp", "gh_kernels.hip", "gh_labrador.hip", "gh_batch.hip",
+           "gh_server.cpp"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",

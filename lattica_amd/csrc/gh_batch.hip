@@ -31,7 +31,7 @@ template <class F>
 __global__ __launch_bounds__(256) void k_batch_fwd_quad(F f, const typename F::W* __restrict__ tab,
                                                         const BatchDesc* __restrict__ desc,
                                                         const BatchSeg* __restrict__ seg,
-                                                        uint32_t nseg, typename F::W* s_ntt) {
+                                                        uint32_t nseg) {
   using W = typename F::W;
   constexpr int TW = F::kTwWords;
   constexpr int V = 16 / sizeof(W);
@@ -51,7 +51,8 @@ __global__ __launch_bounds__(256) void k_batch_fwd_quad(F f, const typename F::W
 #pragma unroll
     for (int i = 0; i < 16; i++) x[i] = sp[4 * i];
     quad_fwd64(f, x, tab, ltw, t);
-    uint4* dp = reinterpret_cast<uint4*>(s_ntt + d.s_off + (size_t)p * 64 + 16 * bitrev2(t));
+    uint4* dp = reinterpret_cast<uint4*>(reinterpret_cast<W*>(d.s_ntt) + (size_t)p * 64 +
+                                         16 * bitrev2(t));
 #pragma unroll
     for (int v = 0; v < 16 / V; v++) {
       uint4 o;
@@ -66,9 +67,8 @@ __global__ __launch_bounds__(256) void k_batch_fwd_quad(F f, const typename F::W
 // partial layout: [chunk][item][row][slot]
 template <int G, int RB>
 __global__ __launch_bounds__(16 * RB) void k_mac_batch_q32(
-    FQ32 f, const uint32_t* __restrict__ A, const uint32_t* __restrict__ s_ntt,
-    const BatchDesc* __restrict__ desc, uint32_t count, uint32_t rows, uint32_t chunk_cols,
-    uint32_t* __restrict__ partial) {
+    FQ32 f, const uint32_t* __restrict__ A, const BatchDesc* __restrict__ desc, uint32_t count,
+    uint32_t rows, uint32_t chunk_cols) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const uint32_t ch = blockIdx.x, g0 = blockIdx.y * G;
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch_q32(
   for (int i = 0; i < G; i++) {
     const uint32_t j = g0 + i;
     len[i] = j < count ? desc[j].len : 0;
-    sp[i] = s_ntt + (j < count ? desc[j].s_off : 0) + lane;
+    sp[i] = reinterpret_cast<const uint32_t*>(desc[j < count ? j : g0].s_ntt) + lane;
   }
   uint64_t lo[G][4];
   uint32_t hi[G][4];
@@ -110,9 +110,10 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch_q32(
   for (int i = 0; i < G; i++) {
     const uint32_t j = g0 + i;
     if (j < count && c0 < len[i]) {
+      uint32_t* pp = reinterpret_cast<uint32_t*>(desc[j].partial);
 #pragma unroll
       for (int r = 0; r < 4; r++)
-        partial[(((size_t)ch * count + j) * rows + row0 + r) * 64 + lane] =
+        pp[((size_t)ch * rows + row0 + r) * 64 + lane] =
             f.reduce96_r2((uint32_t)lo[i][r], (uint32_t)(lo[i][r] >> 32), hi[i][r]);
     }
   }
@@ -124,9 +125,8 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch_q32(
 // the current column's multiply-accumulates (one-column software pipeline).
 template <int G, int RB>
 __global__ __launch_bounds__(16 * RB) void k_mac_batch4_q32(
-    FQ32 f, const uint32_t* __restrict__ A, const uint32_t* __restrict__ s_ntt,
-    const BatchDesc* __restrict__ desc, uint32_t count, uint32_t rows, uint32_t chunk_cols,
-    uint32_t* __restrict__ partial) {
+    FQ32 f, const uint32_t* __restrict__ A, const BatchDesc* __restrict__ desc, uint32_t count,
+    uint32_t rows, uint32_t chunk_cols) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int k0 = 4 * (lane & 15);
@@ -142,7 +142,8 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch4_q32(
   for (int i = 0; i < G; i++) {
     const uint32_t j = g0 + i;
     len[i] = j < count ? desc[j].len : 0;
-    sp[i] = reinterpret_cast<const u32x4*>(s_ntt + (j < count ? desc[j].s_off : 0) + k0);
+    sp[i] = reinterpret_cast<const u32x4*>(
+        reinterpret_cast<const uint32_t*>(desc[j < count ? j : g0].s_ntt) + k0);
   }
   const size_t col_stride4 = (size_t)rows * 16;  // u32x4 per column
   const u32x4* ap = reinterpret_cast<const u32x4*>(A + (size_t)row_w * 64) + lane;
@@ -187,16 +188,16 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch4_q32(
       o.y = f.reduce96_r2((uint32_t)lo[i][1], (uint32_t)(lo[i][1] >> 32), hi[i][1]);
       o.z = f.reduce96_r2((uint32_t)lo[i][2], (uint32_t)(lo[i][2] >> 32), hi[i][2]);
       o.w = f.reduce96_r2((uint32_t)lo[i][3], (uint32_t)(lo[i][3] >> 32), hi[i][3]);
-      *reinterpret_cast<u32x4*>(partial + (((size_t)ch * count + j) * rows + row) * 64 + k0) = o;
+      *reinterpret_cast<u32x4*>(reinterpret_cast<uint32_t*>(desc[j].partial) +
+                                ((size_t)ch * rows + row) * 64 + k0) = o;
     }
   }
 }
 
 template <int G, int RB>
 __global__ __launch_bounds__(16 * RB) void k_mac_batch_gold(
-    FGold f, const uint64_t* __restrict__ A, const uint64_t* __restrict__ s_ntt,
-    const BatchDesc* __restrict__ desc, uint32_t count, uint32_t rows, uint32_t chunk_cols,
-    uint64_t* __restrict__ partial) {
+    FGold f, const uint64_t* __restrict__ A, const BatchDesc* __restrict__ desc, uint32_t count,
+    uint32_t rows, uint32_t chunk_cols) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const uint32_t ch = blockIdx.x, g0 = blockIdx.y * G;
@@ -211,7 +212,7 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch_gold(
   for (int i = 0; i < G; i++) {
     const uint32_t j = g0 + i;
     len[i] = j < count ? desc[j].len : 0;
-    sp[i] = s_ntt + (j < count ? desc[j].s_off : 0) + lane;
+    sp[i] = reinterpret_cast<const uint64_t*>(desc[j < count ? j : g0].s_ntt) + lane;
   }
   uint64_t acc[G][4];
 #pragma unroll
@@ -250,9 +251,9 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch_gold(
   for (int i = 0; i < G; i++) {
     const uint32_t j = g0 + i;
     if (j < count && c0 < len[i]) {
+      uint64_t* pp = reinterpret_cast<uint64_t*>(desc[j].partial);
 #pragma unroll
-      for (int r = 0; r < 4; r++)
-        partial[(((size_t)ch * count + j) * rows + row0 + r) * 64 + lane] = acc[i][r];
+      for (int r = 0; r < 4; r++) pp[((size_t)ch * rows + row0 + r) * 64 + lane] = acc[i][r];
     }
   }
 }
@@ -260,21 +261,21 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch_gold(
 // grid (count, rows): item j's row `row` = INTT(sum over its chunks)
 template <class F>
 __global__ __launch_bounds__(1024) void k_finalize_batch(F f, const typename F::W* __restrict__ tab,
-                                                         const typename F::W* __restrict__ partial,
                                                          const BatchDesc* __restrict__ desc,
-                                                         uint32_t count, uint32_t rows,
-                                                         uint32_t chunk_cols, int post) {
+                                                         uint32_t rows, uint32_t chunk_cols,
+                                                         int post) {
   using W = typename F::W;
   const int k = threadIdx.x & 63, g = threadIdx.x >> 6;
   const uint32_t j = blockIdx.x, row = blockIdx.y;
   const uint32_t nch = (desc[j].len + chunk_cols - 1) / chunk_cols;
+  const W* partial = reinterpret_cast<const W*>(desc[j].partial);
   W acc = 0;
   for (uint32_t c0 = g; c0 < nch; c0 += 16 * 8) {
     W v[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) {
       const uint32_t c = c0 + 16 * u;
-      v[u] = c < nch ? partial[(((size_t)c * count + j) * rows + row) * 64 + k] : W(0);
+      v[u] = c < nch ? partial[((size_t)c * rows + row) * 64 + k] : W(0);
     }
 #pragma unroll
     for (int u = 0; u < 8; u++) acc = f.add(acc, v[u]);
@@ -292,39 +293,81 @@ __global__ __launch_bounds__(1024) void k_finalize_batch(F f, const typename F::
 
 // ---------------------------------------------------------------------------
 
+int enqueue_batch_xform(gh_ring* ring, hipStream_t stream, const BatchDesc* desc,
+                        const BatchSeg* seg, uint32_t nseg) {
+  if (nseg == 0) return GH_OK;
+  const int cus = ring->ctx->num_cus;
+  const unsigned grid = nseg < (uint32_t)cus * 16 ? nseg : (unsigned)cus * 16;
+  if (ring->kind == GH_RING_Q32)
+    hipLaunchKernelGGL((k_batch_fwd_quad<FQ32>), dim3(grid), dim3(256), 0, stream,
+                       FQ32{(uint32_t)ring->q, ring->qinv32}, (const uint32_t*)ring->dev_tab, desc,
+                       seg, nseg);
+  else
+    hipLaunchKernelGGL((k_batch_fwd_quad<FGold>), dim3(grid), dim3(256), 0, stream, FGold{},
+                       (const uint64_t*)ring->dev_tab, desc, seg, nseg);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
 template <int G, int RB>
-static void mac_batch_launch(gh_key* key, const void* s_ntt, const BatchDesc* desc, uint32_t count,
-                             uint32_t chunk_cols, uint32_t nchunks, void* partial) {
+static void mac_batch_launch(gh_key* key, hipStream_t stream, const BatchDesc* desc,
+                             uint32_t count, uint32_t chunk_cols, uint32_t nchunks, int variant) {
   gh_ring* ring = key->ring;
-  gh_ctx* ctx = ring->ctx;
   dim3 grid(nchunks, (count + G - 1) / G, key->rows / RB);
   if constexpr (G <= 8) {
-    if (ring->kind == GH_RING_Q32 && ctx->tune.mac_variant == 1) {
-      hipLaunchKernelGGL((k_mac_batch4_q32<G, RB>), grid, dim3(16 * RB), 0, ctx->stream,
-                         FQ32{(uint32_t)ring->q, ring->qinv32}, (const uint32_t*)key->dev_a,
-                         (const uint32_t*)s_ntt, desc, count, key->rows, chunk_cols,
-                         (uint32_t*)partial);
+    if (ring->kind == GH_RING_Q32 && variant == 1) {
+      hipLaunchKernelGGL((k_mac_batch4_q32<G, RB>), grid, dim3(16 * RB), 0, stream,
+                         FQ32{(uint32_t)ring->q, ring->qinv32}, (const uint32_t*)key->dev_a, desc,
+                         count, key->rows, chunk_cols);
       return;
     }
   }
   if (ring->kind == GH_RING_Q32)
-    hipLaunchKernelGGL((k_mac_batch_q32<G, RB>), grid, dim3(16 * RB), 0, ctx->stream,
-                       FQ32{(uint32_t)ring->q, ring->qinv32}, (const uint32_t*)key->dev_a,
-                       (const uint32_t*)s_ntt, desc, count, key->rows, chunk_cols,
-                       (uint32_t*)partial);
+    hipLaunchKernelGGL((k_mac_batch_q32<G, RB>), grid, dim3(16 * RB), 0, stream,
+                       FQ32{(uint32_t)ring->q, ring->qinv32}, (const uint32_t*)key->dev_a, desc,
+                       count, key->rows, chunk_cols);
   else
-    hipLaunchKernelGGL((k_mac_batch_gold<G, RB>), grid, dim3(16 * RB), 0, ctx->stream, FGold{},
-                       (const uint64_t*)key->dev_a, (const uint64_t*)s_ntt, desc, count,
-                       key->rows, chunk_cols, (uint64_t*)partial);
+    hipLaunchKernelGGL((k_mac_batch_gold<G, RB>), grid, dim3(16 * RB), 0, stream, FGold{},
+                       (const uint64_t*)key->dev_a, desc, count, key->rows, chunk_cols);
 }
 
 template <int G>
-static void mac_batch_rb(gh_key* key, const void* s_ntt, const BatchDesc* desc, uint32_t count,
-                         uint32_t chunk_cols, uint32_t nchunks, void* partial) {
+static void mac_batch_rb(gh_key* key, hipStream_t stream, const BatchDesc* desc, uint32_t count,
+                         uint32_t chunk_cols, uint32_t nchunks, int variant) {
   const uint32_t rows = key->rows;
-  if (rows % 16 == 0) mac_batch_launch<G, 16>(key, s_ntt, desc, count, chunk_cols, nchunks, partial);
-  else if (rows % 8 == 0) mac_batch_launch<G, 8>(key, s_ntt, desc, count, chunk_cols, nchunks, partial);
-  else mac_batch_launch<G, 4>(key, s_ntt, desc, count, chunk_cols, nchunks, partial);
+  if (rows % 16 == 0)
+    mac_batch_launch<G, 16>(key, stream, desc, count, chunk_cols, nchunks, variant);
+  else if (rows % 8 == 0)
+    mac_batch_launch<G, 8>(key, stream, desc, count, chunk_cols, nchunks, variant);
+  else
+    mac_batch_launch<G, 4>(key, stream, desc, count, chunk_cols, nchunks, variant);
+}
+
+int enqueue_batch_mac(gh_key* key, hipStream_t stream, const BatchDesc* desc, uint32_t count,
+                      uint32_t maxlen, uint32_t chunk_cols, int group, int variant) {
+  if (count == 0 || maxlen == 0) return GH_OK;
+  if (key->rows % 4 != 0) return GH_E_INVALID_ARGUMENT;
+  const uint32_t nchunks = (maxlen + chunk_cols - 1) / chunk_cols;
+  if (group == 4) mac_batch_rb<4>(key, stream, desc, count, chunk_cols, nchunks, variant);
+  else if (group == 16) mac_batch_rb<16>(key, stream, desc, count, chunk_cols, nchunks, variant);
+  else mac_batch_rb<8>(key, stream, desc, count, chunk_cols, nchunks, variant);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+int enqueue_batch_finalize(gh_ring* ring, hipStream_t stream, const BatchDesc* desc,
+                           uint32_t count, uint32_t rows, uint32_t chunk_cols) {
+  if (count == 0) return GH_OK;
+  if (ring->kind == GH_RING_Q32)
+    hipLaunchKernelGGL((k_finalize_batch<FQ32>), dim3(count, rows), dim3(1024), 0, stream,
+                       FQ32{(uint32_t)ring->q, ring->qinv32}, (const uint32_t*)ring->dev_tab, desc,
+                       rows, chunk_cols, (int)kTabUntwistMac);
+  else
+    hipLaunchKernelGGL((k_finalize_batch<FGold>), dim3(count, rows), dim3(1024), 0, stream,
+                       FGold{}, (const uint64_t*)ring->dev_tab, desc, rows, chunk_cols,
+                       (int)kTabUntwist);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
 }
 
 // items: device pointers (witness coefficient domain, t_out); lengths checked by the caller.
@@ -344,66 +387,55 @@ int launch_commit_batch(gh_key* key, const gh_commit_item* items, size_t count) 
   const uint32_t rows = key->rows;
   const uint32_t n = (uint32_t)count;
   const uint32_t cc = (uint32_t)ctx->tune.batch_chunk_cols;
-  const int G = ctx->tune.batch_group;
 
   std::vector<uint32_t> order(n);
   std::iota(order.begin(), order.end(), 0u);
   std::stable_sort(order.begin(), order.end(),
                    [&](uint32_t x, uint32_t y) { return items[x].len > items[y].len; });
+  // workspace: [NTT-domain witnesses][partials][descriptors][segments]
+  uint64_t s_words = 0, p_words = 0;
+  size_t nseg = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    const uint32_t len = items[order[i]].len;
+    s_words += (uint64_t)len * 64;
+    p_words += (uint64_t)((len + cc - 1) / cc) * rows * 64;
+    nseg += (len + 63) / 64;
+  }
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t s_bytes = al(s_words * wb), p_bytes = al(p_words * wb);
+  const size_t d_bytes = al((size_t)n * sizeof(BatchDesc));
+  const size_t g_bytes = al(nseg * sizeof(BatchSeg));
+  void* ws;
+  int st = ctx_workspace(ctx, s_bytes + p_bytes + d_bytes + g_bytes, &ws);
+  if (st) return st;
+  char* base = (char*)ws;
+  BatchDesc* ddesc = (BatchDesc*)(base + s_bytes + p_bytes);
+  BatchSeg* dseg = (BatchSeg*)(base + s_bytes + p_bytes + d_bytes);
+
   std::vector<BatchDesc>& hd = ctx->batch_desc;
   std::vector<BatchSeg>& hs = ctx->batch_seg;
   hd.resize(n);
   hs.clear();
-  uint64_t total = 0;
+  uint64_t s_at = 0, p_at = 0;
   for (uint32_t i = 0; i < n; i++) {
     const gh_commit_item& it = items[order[i]];
-    hd[i] = BatchDesc{total * 64, it.len, 0, (uint64_t)(uintptr_t)it.witness,
-                      (uint64_t)(uintptr_t)it.t_out};
+    hd[i] = BatchDesc{(uint64_t)(uintptr_t)(base + s_at * wb), (uint64_t)(uintptr_t)it.witness,
+                      (uint64_t)(uintptr_t)it.t_out, (uint64_t)(uintptr_t)(base + s_bytes + p_at * wb),
+                      it.len, 0};
     for (uint32_t p = 0; p < it.len; p += 64) hs.push_back(BatchSeg{i, p});
-    total += it.len;
+    s_at += (uint64_t)it.len * 64;
+    p_at += (uint64_t)((it.len + cc - 1) / cc) * rows * 64;
   }
-  const uint32_t maxlen = hd[0].len;
-  const uint32_t nchunks = (maxlen + cc - 1) / cc;
-  const size_t s_bytes = (size_t)total * 64 * wb;
-  const size_t p_bytes = (size_t)nchunks * n * rows * 64 * wb;
-  const size_t d_bytes = (size_t)n * sizeof(BatchDesc);
-  const size_t g_bytes = hs.size() * sizeof(BatchSeg);
-  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-  void* ws;
-  int st = ctx_workspace(ctx, al(s_bytes) + al(p_bytes) + al(d_bytes) + al(g_bytes), &ws);
+  GH_HIP(hipMemcpyAsync(ddesc, hd.data(), n * sizeof(BatchDesc), hipMemcpyHostToDevice,
+                        ctx->stream));
+  GH_HIP(hipMemcpyAsync(dseg, hs.data(), hs.size() * sizeof(BatchSeg), hipMemcpyHostToDevice,
+                        ctx->stream));
+  st = enqueue_batch_xform(ring, ctx->stream, ddesc, dseg, (uint32_t)hs.size());
   if (st) return st;
-  char* base = (char*)ws;
-  void* s_ntt = base;
-  void* partial = base + al(s_bytes);
-  BatchDesc* ddesc = (BatchDesc*)(base + al(s_bytes) + al(p_bytes));
-  BatchSeg* dseg = (BatchSeg*)(base + al(s_bytes) + al(p_bytes) + al(d_bytes));
-  GH_HIP(hipMemcpyAsync(ddesc, hd.data(), d_bytes, hipMemcpyHostToDevice, ctx->stream));
-  GH_HIP(hipMemcpyAsync(dseg, hs.data(), g_bytes, hipMemcpyHostToDevice, ctx->stream));
-
-  const uint32_t nseg = (uint32_t)hs.size();
-  const unsigned sgrid = nseg < (uint32_t)ctx->num_cus * 16 ? nseg : (unsigned)ctx->num_cus * 16;
-  if (ring->kind == GH_RING_Q32)
-    hipLaunchKernelGGL((k_batch_fwd_quad<FQ32>), dim3(sgrid), dim3(256), 0, ctx->stream,
-                       FQ32{(uint32_t)ring->q, ring->qinv32}, (const uint32_t*)ring->dev_tab,
-                       ddesc, dseg, nseg, (uint32_t*)s_ntt);
-  else
-    hipLaunchKernelGGL((k_batch_fwd_quad<FGold>), dim3(sgrid), dim3(256), 0, ctx->stream, FGold{},
-                       (const uint64_t*)ring->dev_tab, ddesc, dseg, nseg, (uint64_t*)s_ntt);
-  GH_HIP(hipGetLastError());
-  if (G == 4) mac_batch_rb<4>(key, s_ntt, ddesc, n, cc, nchunks, partial);
-  else if (G == 16) mac_batch_rb<16>(key, s_ntt, ddesc, n, cc, nchunks, partial);
-  else mac_batch_rb<8>(key, s_ntt, ddesc, n, cc, nchunks, partial);
-  GH_HIP(hipGetLastError());
-  if (ring->kind == GH_RING_Q32)
-    hipLaunchKernelGGL((k_finalize_batch<FQ32>), dim3(n, rows), dim3(1024), 0, ctx->stream,
-                       FQ32{(uint32_t)ring->q, ring->qinv32}, (const uint32_t*)ring->dev_tab,
-                       (const uint32_t*)partial, ddesc, n, rows, cc, (int)kTabUntwistMac);
-  else
-    hipLaunchKernelGGL((k_finalize_batch<FGold>), dim3(n, rows), dim3(1024), 0, ctx->stream,
-                       FGold{}, (const uint64_t*)ring->dev_tab, (const uint64_t*)partial, ddesc, n,
-                       rows, cc, (int)kTabUntwist);
-  GH_HIP(hipGetLastError());
-  return GH_OK;
+  st = enqueue_batch_mac(key, ctx->stream, ddesc, n, hd[0].len, cc, ctx->tune.batch_group,
+                         ctx->tune.mac_variant);
+  if (st) return st;
+  return enqueue_batch_finalize(ring, ctx->stream, ddesc, n, rows, cc);
 }
 
 }  // namespace gh

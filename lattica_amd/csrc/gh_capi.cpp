@@ -58,6 +58,21 @@ void prof_pair(gh_ctx* ctx, int phase, hipEvent_t* start, hipEvent_t* stop) {
   ctx->prof_recs.push_back({phase, *start, *stop});
 }
 
+// Host inputs are checked on the host before upload.
+int check_canonical_host(const gh_ring* ring, const void* p, size_t words) {
+  if (ring->word_bytes == 4) {
+    const uint32_t* w = (const uint32_t*)p;
+    const uint32_t q = (uint32_t)ring->q;
+    for (size_t i = 0; i < words; i++)
+      if (w[i] >= q) return GH_E_NOT_CANONICAL;
+  } else {
+    const uint64_t* w = (const uint64_t*)p;
+    for (size_t i = 0; i < words; i++)
+      if (w[i] >= ring->q) return GH_E_NOT_CANONICAL;
+  }
+  return GH_OK;
+}
+
 hipEvent_t prof_mark(gh_ctx* ctx) {
   if (!ctx->prof_on) return nullptr;
   if (ctx->prof_used == ctx->prof_pool.size()) {
@@ -99,21 +114,6 @@ int check_canonical(gh_ring* ring, const void* dev, size_t words) {
   GH_HIP(hipMemcpyAsync(&host, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   GH_HIP(hipStreamSynchronize(ctx->stream));
   return host ? GH_E_NOT_CANONICAL : GH_OK;
-}
-
-// Host inputs are checked on the host before upload.
-int check_canonical_host(const gh_ring* ring, const void* p, size_t words) {
-  if (ring->word_bytes == 4) {
-    const uint32_t* w = (const uint32_t*)p;
-    const uint32_t q = (uint32_t)ring->q;
-    for (size_t i = 0; i < words; i++)
-      if (w[i] >= q) return GH_E_NOT_CANONICAL;
-  } else {
-    const uint64_t* w = (const uint64_t*)p;
-    for (size_t i = 0; i < words; i++)
-      if (w[i] >= ring->q) return GH_E_NOT_CANONICAL;
-  }
-  return GH_OK;
 }
 
 int h2d(gh_ctx* ctx, void* dst, const void* src, size_t bytes) {

@@ -15,15 +15,15 @@ __device__ __forceinline__ void mac96(uint64_t& lo, uint32_t& hi, uint32_t a, ui
       : "vcc");
 }
 
-// t[row] = INTT(sum_p partial[p][row]) with the untwist table `post`.
+// t[row] = INTT(sum_p partial[p][row]) with the untwist table `post`, by one
+// block of 1024 threads.
 template <class F>
-__global__ __launch_bounds__(1024) void k_finalize(F f, const typename F::W* __restrict__ tab,
-                                                   const typename F::W* __restrict__ partial,
-                                                   uint32_t nparts, uint32_t rows, int post,
-                                                   typename F::W* __restrict__ t_out) {
+__device__ __forceinline__ void finalize_row(F f, const typename F::W* __restrict__ tab,
+                                             const typename F::W* __restrict__ partial,
+                                             uint32_t nparts, uint32_t rows, uint32_t row, int post,
+                                             typename F::W* __restrict__ t_out) {
   using W = typename F::W;
   const int k = threadIdx.x & 63, g = threadIdx.x >> 6;
-  const uint32_t row = blockIdx.x;
   // up to 32 independent loads in flight per thread: one round trip for the
   // partial counts a device-wide MAC produces (<= 512)
   W acc = 0;
@@ -46,6 +46,14 @@ __global__ __launch_bounds__(1024) void k_finalize(F f, const typename F::W* __r
     for (int i = 1; i < 16; i++) s = f.add(s, red[i][k]);
     t_out[(size_t)row * 64 + k] = wave_inv64(f, s, tab, post, k);
   }
+}
+
+template <class F>
+__global__ __launch_bounds__(1024) void k_finalize(F f, const typename F::W* __restrict__ tab,
+                                                   const typename F::W* __restrict__ partial,
+                                                   uint32_t nparts, uint32_t rows, int post,
+                                                   typename F::W* __restrict__ t_out) {
+  finalize_row(f, tab, partial, nparts, rows, blockIdx.x, post, t_out);
 }
 
 }  // namespace gh

@@ -11,18 +11,34 @@
 #include "gh_field.h"
 
 namespace gh {
-// per-item descriptor of a commitment batch (device copy in the workspace)
+// Per-item descriptor of a batch (device copy next to the batch's buffers).
+// All addresses are device addresses, so the items of one batch may live in
+// unrelated allocations.
 struct BatchDesc {
-  uint64_t s_off;  // word offset of the item's transformed witness in the batch buffer
-  uint32_t len;    // witness polynomials (key columns used)
+  uint64_t s_ntt;    // the item's NTT-domain witness, len x 64 words
+  uint64_t src;      // the coefficient-domain witness (transform input)
+  uint64_t dst;      // t_out: rows polynomials
+  uint64_t partial;  // per-chunk partial sums, [chunk][row][slot]
+  uint32_t len;      // witness polynomials (key columns used)
   uint32_t pad;
-  uint64_t src;    // device address of the coefficient-domain witness
-  uint64_t dst;    // device address of t_out (rows polynomials)
 };
 // a run of at most 64 polynomials of one item, transformed by one block
 struct BatchSeg {
-  uint32_t item;   // index into the sorted descriptors
+  uint32_t item;   // index into the descriptors of the transform
   uint32_t first;  // first polynomial of the segment within the item
+};
+// one Gram matrix of a batch: r parts of n NTT-domain polynomials
+struct GramDesc {
+  uint64_t s_ntt;    // r x n x 64 words
+  uint64_t partial;  // [chunk][pair][slot]
+  uint64_t dst;      // r(r+1)/2 polynomials, coefficient domain
+  uint32_t r, n, chunk, nchunks;
+};
+struct GramWork {  // one wave of a Gram batch: (matrix, 4x4 pair tile, column chunk)
+  uint32_t g, tile, chunk;
+};
+struct GramFin {  // one output polynomial of a Gram batch
+  uint32_t g, pair;
 };
 }  // namespace gh
 
@@ -85,6 +101,8 @@ namespace gh {
 uint64_t powmod(uint64_t a, uint64_t e, uint64_t q);
 int plan_ring(uint64_t q, uint32_t d, gh_ring* ring);
 
+// host inputs are checked on the host before upload: GH_OK or GH_E_NOT_CANONICAL
+int check_canonical_host(const gh_ring* ring, const void* p, size_t words);
 // device workspace of at least `bytes` bytes on the context (grow-only)
 int ctx_workspace(gh_ctx* ctx, size_t bytes, void** out);
 int ctx_staging(gh_ctx* ctx, size_t bytes, void** out);
@@ -105,6 +123,21 @@ int launch_key_layout(gh_ring* ring, const void* src_rowmajor, uint32_t rows, ui
 int launch_commit(gh_key* key, const void* s_dev, uint32_t cols, int domain, void* t_dev);
 // items hold device pointers; every len in [1, key columns]
 int launch_commit_batch(gh_key* key, const gh_commit_item* items, size_t count);
+
+// Batch building blocks on an explicit stream. Descriptor arrays are device
+// copies; MAC descriptors are sorted by len, longest first, and the host
+// passes the longest len (the grid's chunk count follows from it).
+int enqueue_batch_xform(gh_ring* ring, hipStream_t stream, const BatchDesc* desc,
+                        const BatchSeg* seg, uint32_t nseg);
+int enqueue_batch_mac(gh_key* key, hipStream_t stream, const BatchDesc* desc, uint32_t count,
+                      uint32_t maxlen, uint32_t chunk_cols, int group, int variant);
+int enqueue_batch_finalize(gh_ring* ring, hipStream_t stream, const BatchDesc* desc,
+                           uint32_t count, uint32_t rows, uint32_t chunk_cols);
+// column chunking of one Gram matrix (partial sums: nchunks x r(r+1)/2 polys)
+void gram_chunking(int num_cus, uint32_t r, uint32_t n, uint32_t* chunk, uint32_t* nchunks);
+uint32_t gram_tiles(uint32_t r);
+int enqueue_gram_batch(gh_ring* ring, hipStream_t stream, const GramDesc* desc,
+                       const GramWork* work, uint32_t nwork, const GramFin* fin, uint32_t nfin);
 int launch_check_canonical(gh_ring* ring, const void* p, size_t words, int* bad_flag_dev);
 int launch_gram(gh_ring* ring, const void* s_dev, uint32_t r, uint32_t n, int domain, void* g_dev);
 int launch_stream_probe(gh_ctx* ctx, const void* src, void* dst, size_t bytes, int copy,

@@ -41,14 +41,15 @@ __device__ __forceinline__ void tile_coords(uint32_t t, uint32_t ntile, uint32_t
   b = a + rem;
 }
 
-__global__ __launch_bounds__(64) void k_gram_q32(FQ32 f, const uint32_t* __restrict__ s_ntt,
-                                                 uint32_t r, uint32_t n, uint32_t chunk,
-                                                 uint32_t* __restrict__ partial) {
+// one wave: pair tile `tile`, column chunk `ci` of the Gram matrix of s_ntt
+__device__ __forceinline__ void gram_tile_q32(FQ32 f, const uint32_t* __restrict__ s_ntt,
+                                              uint32_t r, uint32_t n, uint32_t chunk, uint32_t ci,
+                                              uint32_t tile, uint32_t* __restrict__ partial) {
   const int lane = threadIdx.x;
   const uint32_t ntile = (r + kGramTile - 1) / kGramTile;
   uint32_t ta, tb;
-  tile_coords(blockIdx.x, ntile, ta, tb);
-  const uint32_t c0 = blockIdx.y * chunk;
+  tile_coords(tile, ntile, ta, tb);
+  const uint32_t c0 = ci * chunk;
   uint32_t c1 = c0 + chunk;
   if (c1 > n) c1 = n;
   uint64_t lo[kGramTile][kGramTile];
@@ -83,19 +84,19 @@ __global__ __launch_bounds__(64) void k_gram_q32(FQ32 f, const uint32_t* __restr
       const uint32_t i = ta * kGramTile + x, j = tb * kGramTile + y;
       if (i < r && j < r && i <= j) {
         uint32_t v = f.reduce96_r2((uint32_t)lo[x][y], (uint32_t)(lo[x][y] >> 32), hi[x][y]);
-        partial[((size_t)blockIdx.y * npairs + pair_index(r, i, j)) * 64 + lane] = v;
+        partial[((size_t)ci * npairs + pair_index(r, i, j)) * 64 + lane] = v;
       }
     }
 }
 
-__global__ __launch_bounds__(64) void k_gram_gold(FGold f, const uint64_t* __restrict__ s_ntt,
-                                                  uint32_t r, uint32_t n, uint32_t chunk,
-                                                  uint64_t* __restrict__ partial) {
+__device__ __forceinline__ void gram_tile_gold(FGold f, const uint64_t* __restrict__ s_ntt,
+                                               uint32_t r, uint32_t n, uint32_t chunk, uint32_t ci,
+                                               uint32_t tile, uint64_t* __restrict__ partial) {
   const int lane = threadIdx.x;
   const uint32_t ntile = (r + kGramTile - 1) / kGramTile;
   uint32_t ta, tb;
-  tile_coords(blockIdx.x, ntile, ta, tb);
-  const uint32_t c0 = blockIdx.y * chunk;
+  tile_coords(tile, ntile, ta, tb);
+  const uint32_t c0 = ci * chunk;
   uint32_t c1 = c0 + chunk;
   if (c1 > n) c1 = n;
   uint64_t acc[kGramTile][kGramTile];
@@ -128,8 +129,82 @@ __global__ __launch_bounds__(64) void k_gram_gold(FGold f, const uint64_t* __res
     for (int y = 0; y < kGramTile; y++) {
       const uint32_t i = ta * kGramTile + x, j = tb * kGramTile + y;
       if (i < r && j < r && i <= j)
-        partial[((size_t)blockIdx.y * npairs + pair_index(r, i, j)) * 64 + lane] = acc[x][y];
+        partial[((size_t)ci * npairs + pair_index(r, i, j)) * 64 + lane] = acc[x][y];
     }
+}
+
+__global__ __launch_bounds__(64) void k_gram_q32(FQ32 f, const uint32_t* __restrict__ s_ntt,
+                                                 uint32_t r, uint32_t n, uint32_t chunk,
+                                                 uint32_t* __restrict__ partial) {
+  gram_tile_q32(f, s_ntt, r, n, chunk, blockIdx.y, blockIdx.x, partial);
+}
+
+__global__ __launch_bounds__(64) void k_gram_gold(FGold f, const uint64_t* __restrict__ s_ntt,
+                                                  uint32_t r, uint32_t n, uint32_t chunk,
+                                                  uint64_t* __restrict__ partial) {
+  gram_tile_gold(f, s_ntt, r, n, chunk, blockIdx.y, blockIdx.x, partial);
+}
+
+// Gram matrices of several witnesses in one launch: one wave per work entry
+__global__ __launch_bounds__(64) void k_gram_batch_q32(FQ32 f, const GramDesc* __restrict__ desc,
+                                                       const GramWork* __restrict__ work) {
+  const GramWork w = work[blockIdx.x];
+  const GramDesc d = desc[w.g];
+  gram_tile_q32(f, reinterpret_cast<const uint32_t*>(d.s_ntt), d.r, d.n, d.chunk, w.chunk, w.tile,
+                reinterpret_cast<uint32_t*>(d.partial));
+}
+
+__global__ __launch_bounds__(64) void k_gram_batch_gold(FGold f, const GramDesc* __restrict__ desc,
+                                                        const GramWork* __restrict__ work) {
+  const GramWork w = work[blockIdx.x];
+  const GramDesc d = desc[w.g];
+  gram_tile_gold(f, reinterpret_cast<const uint64_t*>(d.s_ntt), d.r, d.n, d.chunk, w.chunk, w.tile,
+                 reinterpret_cast<uint64_t*>(d.partial));
+}
+
+template <class F>
+__global__ __launch_bounds__(1024) void k_gram_fin_batch(F f, const typename F::W* __restrict__ tab,
+                                                         const GramDesc* __restrict__ desc,
+                                                         const GramFin* __restrict__ fin, int post) {
+  using W = typename F::W;
+  const GramFin e = fin[blockIdx.x];
+  const GramDesc d = desc[e.g];
+  finalize_row(f, tab, reinterpret_cast<const W*>(d.partial), d.nchunks, d.r * (d.r + 1) / 2,
+               e.pair, post, reinterpret_cast<W*>(d.dst));
+}
+
+uint32_t gram_tiles(uint32_t r) {
+  const uint32_t ntile = (r + kGramTile - 1) / kGramTile;
+  return ntile * (ntile + 1) / 2;
+}
+
+// column chunks: enough waves to cover the device a few times
+void gram_chunking(int num_cus, uint32_t r, uint32_t n, uint32_t* chunk, uint32_t* nchunks) {
+  const uint32_t tiles = gram_tiles(r);
+  uint32_t want = (uint32_t)(4 * num_cus * 4 / (tiles ? tiles : 1));
+  if (want < 1) want = 1;
+  if (want > n) want = n;
+  *chunk = (n + want - 1) / want;
+  *nchunks = (n + *chunk - 1) / *chunk;
+}
+
+int enqueue_gram_batch(gh_ring* ring, hipStream_t stream, const GramDesc* desc,
+                       const GramWork* work, uint32_t nwork, const GramFin* fin, uint32_t nfin) {
+  if (nwork == 0) return GH_OK;
+  if (ring->kind == GH_RING_Q32) {
+    FQ32 f{(uint32_t)ring->q, ring->qinv32};
+    hipLaunchKernelGGL(k_gram_batch_q32, dim3(nwork), dim3(64), 0, stream, f, desc, work);
+    GH_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_gram_fin_batch<FQ32>), dim3(nfin), dim3(1024), 0, stream, f,
+                       (const uint32_t*)ring->dev_tab, desc, fin, (int)kTabUntwistMac);
+  } else {
+    hipLaunchKernelGGL(k_gram_batch_gold, dim3(nwork), dim3(64), 0, stream, FGold{}, desc, work);
+    GH_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_gram_fin_batch<FGold>), dim3(nfin), dim3(1024), 0, stream, FGold{},
+                       (const uint64_t*)ring->dev_tab, desc, fin, (int)kTabUntwist);
+  }
+  GH_HIP(hipGetLastError());
+  return GH_OK;
 }
 
 int launch_gram(gh_ring* ring, const void* s_dev, uint32_t r, uint32_t n, int domain,
@@ -139,12 +214,8 @@ int launch_gram(gh_ring* ring, const void* s_dev, uint32_t r, uint32_t n, int do
   const uint32_t ntile = (r + kGramTile - 1) / kGramTile;
   const uint32_t tiles = ntile * (ntile + 1) / 2;
   const uint32_t npairs = r * (r + 1) / 2;
-  // column chunks: enough waves to cover the device a few times
-  uint32_t want = (uint32_t)(4 * ctx->num_cus * 4 / (tiles ? tiles : 1));
-  if (want < 1) want = 1;
-  if (want > n) want = n;
-  uint32_t chunk = (n + want - 1) / want;
-  uint32_t nchunks = (n + chunk - 1) / chunk;
+  uint32_t chunk, nchunks;
+  gram_chunking(ctx->num_cus, r, n, &chunk, &nchunks);
   const size_t s_bytes = (size_t)r * n * 64 * wb;
   const size_t p_bytes = (size_t)nchunks * npairs * 64 * wb;
   void* ws;
