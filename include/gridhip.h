@@ -116,8 +116,9 @@ GH_API int gh_ctx_get_tuning(gh_ctx *ctx, const char *key, int64_t *value);
 /* Achievable HBM bandwidth of the context's device: a streaming read and a
  * streaming copy over `bytes` (rounded down to 16 B) of freshly allocated
  * memory, best of `reps` timed kernels each, in GB/s (1e9 B/s; a copy counts
- * its read and its write). Measurement support for the roofline report; no
- * reference counterpart. */
+ * its read and its write). Up to 512 MiB, three source buffers are read in
+ * turn so that none is served from the Infinity Cache. Measurement support for
+ * the roofline report; no reference counterpart. */
 GH_API int gh_measure_bandwidth(gh_ctx *ctx, size_t bytes, int reps, double *read_gbps,
                                 double *copy_gbps);
 
