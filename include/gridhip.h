@@ -220,6 +220,21 @@ typedef struct gh_commit_item {
 } gh_commit_item;
 GH_API int gh_commit_batch(gh_key *key, const gh_commit_item *items, size_t count, int mem);
 
+/* Batch plans: a fixed batch of device-memory items whose kernels (segmented
+ * witness transform, grouped multiply-accumulate, batched finalize) are
+ * recorded once into a HIP graph over plan-owned device memory. Every
+ * gh_batch_plan_launch replays the graph on the context stream: one launch
+ * per batch, recomputing each t_i from the current contents of its witness
+ * buffer (callers refill witnesses in place between launches). Results equal
+ * gh_commit_batch on the same items. The key must have rows % 4 == 0; the
+ * launch-shape tuning in force at creation is the one recorded. No reference
+ * counterpart (see gh_commit_batch). */
+typedef struct gh_batch_plan gh_batch_plan;
+GH_API int gh_batch_plan_create(gh_key *key, const gh_commit_item *items, size_t count,
+                                gh_batch_plan **out);
+GH_API int gh_batch_plan_launch(gh_batch_plan *plan);
+GH_API int gh_batch_plan_destroy(gh_batch_plan *plan);
+
 /* ---- LaBRADOR Gram matrix --------------------------------------------------
  * compute_garbage_g (labrador/src/main_protocol/garbage.rs:76-93):
  *   g_ij = <s_i, s_j> = sum_k s_i[k] * s_j[k] over R_q, 0 <= i <= j < r,

@@ -15,6 +15,7 @@ import numpy as np
 
 __all__ = [
     "GridHipError", "lib", "library_path", "Context", "Ring", "Key", "DeviceBuffer", "Server",
+    "BatchPlan",
     "OP_COMMIT", "OP_GRAM",
     "Q32", "GOLDILOCKS", "MEM_HOST", "MEM_DEVICE", "DOMAIN_COEFF", "DOMAIN_NTT", "STATUS",
 ]
@@ -50,7 +51,8 @@ EXPORTS = (
     "gh_pointwise_mul", "gh_key_create", "gh_key_destroy", "gh_key_dims", "gh_commit",
     "gh_commit_batch", "gh_gram", "gh_inner_commit", "gh_jl_project",
     "gh_server_create", "gh_server_destroy", "gh_server_submit", "gh_server_wait",
-    "gh_server_poll", "gh_server_query",
+    "gh_server_poll", "gh_server_query", "gh_batch_plan_create", "gh_batch_plan_launch",
+    "gh_batch_plan_destroy",
 )
 
 
@@ -159,6 +161,9 @@ def lib():
         "gh_server_poll": (i32, [vp, u64, ctypes.POINTER(i32), ctypes.POINTER(i32),
                                  ctypes.POINTER(ctypes.c_double)]),
         "gh_server_query": (i32, [vp, ctypes.POINTER(ServerStats)]),
+        "gh_batch_plan_create": (i32, [vp, ctypes.POINTER(CommitItem), sz, ctypes.POINTER(vp)]),
+        "gh_batch_plan_launch": (i32, [vp]),
+        "gh_batch_plan_destroy": (i32, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -440,6 +445,10 @@ class Key:
             items[i].t_out = o.ptr.value if isinstance(o, DeviceBuffer) else int(o)
         _check(lib().gh_commit_batch(self.h, items, len(lens), MEM_DEVICE), "gh_commit_batch")
 
+    def batch_plan(self, witnesses, lens, outs) -> "BatchPlan":
+        """gh_batch_plan over device-memory items (DeviceBuffers or addresses)."""
+        return BatchPlan(self, witnesses, lens, outs)
+
     def inner_commit(self, s, domain=DOMAIN_COEFF):
         ring = self.ring
         s = ring.words(s)
@@ -452,6 +461,29 @@ class Key:
     def close(self):
         if self.h:
             lib().gh_key_destroy(self.h)
+            self.h = None
+
+
+class BatchPlan:
+    """gh_batch_plan: a fixed device-memory batch replayed as one graph launch."""
+
+    def __init__(self, key: "Key", witnesses, lens, outs):
+        items = (CommitItem * max(1, len(lens)))()
+        for i, (w, n, o) in enumerate(zip(witnesses, lens, outs)):
+            items[i].witness = _addr(w)
+            items[i].len = int(n)
+            items[i].t_out = _addr(o)
+        h = ctypes.c_void_p()
+        _check(lib().gh_batch_plan_create(key.h, items, len(lens), ctypes.byref(h)),
+               "gh_batch_plan_create")
+        self.h, self.key = h, key
+
+    def launch(self):
+        _check(lib().gh_batch_plan_launch(self.h), "gh_batch_plan_launch")
+
+    def close(self):
+        if self.h:
+            lib().gh_batch_plan_destroy(self.h)
             self.h = None
 
 

@@ -18,6 +18,7 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <new>
 #include <numeric>
 #include <vector>
 
@@ -370,6 +371,56 @@ int enqueue_batch_finalize(gh_ring* ring, hipStream_t stream, const BatchDesc* d
   return GH_OK;
 }
 
+static void batch_layout(gh_key* key, const gh_commit_item* items, uint32_t n, uint32_t cc,
+                         char* base, BatchLayout& L) {
+  const size_t wb = key->ring->word_bytes;
+  const uint32_t rows = key->rows;
+  std::vector<uint32_t> order(n);
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint32_t x, uint32_t y) { return items[x].len > items[y].len; });
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  uint64_t s_words = 0, p_words = 0;
+  size_t nseg = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    const uint32_t len = items[i].len;
+    s_words += (uint64_t)len * 64;
+    p_words += (uint64_t)((len + cc - 1) / cc) * rows * 64;
+    nseg += (len + 63) / 64;
+  }
+  L.s_bytes = al(s_words * wb);
+  L.p_bytes = al(p_words * wb);
+  L.d_bytes = al((size_t)n * sizeof(BatchDesc));
+  L.g_bytes = al(nseg * sizeof(BatchSeg));
+  L.desc.resize(n);
+  L.seg.clear();
+  L.seg.reserve(nseg);
+  uint64_t s_at = 0, p_at = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    const gh_commit_item& it = items[order[i]];
+    L.desc[i] = BatchDesc{(uint64_t)(uintptr_t)(base + s_at * wb), (uint64_t)(uintptr_t)it.witness,
+                          (uint64_t)(uintptr_t)it.t_out,
+                          (uint64_t)(uintptr_t)(base + L.s_bytes + p_at * wb), it.len, 0};
+    for (uint32_t p = 0; p < it.len; p += 64) L.seg.push_back(BatchSeg{i, p});
+    s_at += (uint64_t)it.len * 64;
+    p_at += (uint64_t)((it.len + cc - 1) / cc) * rows * 64;
+  }
+}
+
+// The batch's kernels on `stream`; descriptors already on the device at
+// base + s_bytes + p_bytes.
+static int batch_enqueue(gh_key* key, hipStream_t stream, const BatchLayout& L, char* base,
+                         uint32_t cc, int group, int variant) {
+  const BatchDesc* ddesc = (const BatchDesc*)(base + L.s_bytes + L.p_bytes);
+  const BatchSeg* dseg = (const BatchSeg*)(base + L.s_bytes + L.p_bytes + L.d_bytes);
+  const uint32_t n = (uint32_t)L.desc.size();
+  int st = enqueue_batch_xform(key->ring, stream, ddesc, dseg, (uint32_t)L.seg.size());
+  if (st) return st;
+  st = enqueue_batch_mac(key, stream, ddesc, n, L.desc[0].len, cc, group, variant);
+  if (st) return st;
+  return enqueue_batch_finalize(key->ring, stream, ddesc, n, key->rows, cc);
+}
+
 // items: device pointers (witness coefficient domain, t_out); lengths checked by the caller.
 int launch_commit_batch(gh_key* key, const gh_commit_item* items, size_t count) {
   gh_ring* ring = key->ring;
@@ -383,59 +434,120 @@ int launch_commit_batch(gh_key* key, const gh_commit_item* items, size_t count) 
     }
     return GH_OK;
   }
-  const size_t wb = ring->word_bytes;
-  const uint32_t rows = key->rows;
-  const uint32_t n = (uint32_t)count;
   const uint32_t cc = (uint32_t)ctx->tune.batch_chunk_cols;
-
-  std::vector<uint32_t> order(n);
-  std::iota(order.begin(), order.end(), 0u);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](uint32_t x, uint32_t y) { return items[x].len > items[y].len; });
-  // workspace: [NTT-domain witnesses][partials][descriptors][segments]
-  uint64_t s_words = 0, p_words = 0;
-  size_t nseg = 0;
-  for (uint32_t i = 0; i < n; i++) {
-    const uint32_t len = items[order[i]].len;
-    s_words += (uint64_t)len * 64;
-    p_words += (uint64_t)((len + cc - 1) / cc) * rows * 64;
-    nseg += (len + 63) / 64;
-  }
-  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-  const size_t s_bytes = al(s_words * wb), p_bytes = al(p_words * wb);
-  const size_t d_bytes = al((size_t)n * sizeof(BatchDesc));
-  const size_t g_bytes = al(nseg * sizeof(BatchSeg));
+  BatchLayout sizes;
+  batch_layout(key, items, (uint32_t)count, cc, nullptr, sizes);
   void* ws;
-  int st = ctx_workspace(ctx, s_bytes + p_bytes + d_bytes + g_bytes, &ws);
+  int st = ctx_workspace(ctx, sizes.total(), &ws);
   if (st) return st;
   char* base = (char*)ws;
-  BatchDesc* ddesc = (BatchDesc*)(base + s_bytes + p_bytes);
-  BatchSeg* dseg = (BatchSeg*)(base + s_bytes + p_bytes + d_bytes);
-
-  std::vector<BatchDesc>& hd = ctx->batch_desc;
-  std::vector<BatchSeg>& hs = ctx->batch_seg;
-  hd.resize(n);
-  hs.clear();
-  uint64_t s_at = 0, p_at = 0;
-  for (uint32_t i = 0; i < n; i++) {
-    const gh_commit_item& it = items[order[i]];
-    hd[i] = BatchDesc{(uint64_t)(uintptr_t)(base + s_at * wb), (uint64_t)(uintptr_t)it.witness,
-                      (uint64_t)(uintptr_t)it.t_out, (uint64_t)(uintptr_t)(base + s_bytes + p_at * wb),
-                      it.len, 0};
-    for (uint32_t p = 0; p < it.len; p += 64) hs.push_back(BatchSeg{i, p});
-    s_at += (uint64_t)it.len * 64;
-    p_at += (uint64_t)((it.len + cc - 1) / cc) * rows * 64;
-  }
-  GH_HIP(hipMemcpyAsync(ddesc, hd.data(), n * sizeof(BatchDesc), hipMemcpyHostToDevice,
-                        ctx->stream));
-  GH_HIP(hipMemcpyAsync(dseg, hs.data(), hs.size() * sizeof(BatchSeg), hipMemcpyHostToDevice,
-                        ctx->stream));
-  st = enqueue_batch_xform(ring, ctx->stream, ddesc, dseg, (uint32_t)hs.size());
-  if (st) return st;
-  st = enqueue_batch_mac(key, ctx->stream, ddesc, n, hd[0].len, cc, ctx->tune.batch_group,
-                         ctx->tune.mac_variant);
-  if (st) return st;
-  return enqueue_batch_finalize(ring, ctx->stream, ddesc, n, rows, cc);
+  // the host images stay in the context until the next batch: the uploads are asynchronous
+  BatchLayout& L = ctx->batch_layout;
+  batch_layout(key, items, (uint32_t)count, cc, base, L);
+  GH_HIP(hipMemcpyAsync(base + L.s_bytes + L.p_bytes, L.desc.data(),
+                        L.desc.size() * sizeof(BatchDesc), hipMemcpyHostToDevice, ctx->stream));
+  GH_HIP(hipMemcpyAsync(base + L.s_bytes + L.p_bytes + L.d_bytes, L.seg.data(),
+                        L.seg.size() * sizeof(BatchSeg), hipMemcpyHostToDevice, ctx->stream));
+  return batch_enqueue(key, ctx->stream, L, base, cc, ctx->tune.batch_group,
+                       ctx->tune.mac_variant);
 }
 
 }  // namespace gh
+
+// ---------------------------------------------------------------------------
+// Batch plans: a batch's kernels recorded once into a HIP graph over
+// plan-owned device memory, replayed by one graph launch per batch.
+// ---------------------------------------------------------------------------
+
+using namespace gh;
+
+struct gh_batch_plan {
+  gh_key* key = nullptr;
+  void* mem = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  size_t count = 0;
+};
+
+static void plan_free(gh_batch_plan* p) {
+  if (p->exec) hipGraphExecDestroy(p->exec);
+  if (p->graph) hipGraphDestroy(p->graph);
+  if (p->mem) hipFree(p->mem);
+  delete p;
+}
+
+extern "C" {
+
+GH_API int gh_batch_plan_create(gh_key* key, const gh_commit_item* items, size_t count,
+                                gh_batch_plan** out) {
+  if (!key || !out || !items || count == 0 || count > 0x7fffffff) return GH_E_INVALID_ARGUMENT;
+  *out = nullptr;
+  if (key->rows % 4 != 0) return GH_E_INVALID_ARGUMENT;
+  const uint32_t cols = key->msg_len + key->open_len;
+  for (size_t i = 0; i < count; i++) {
+    if (!items[i].witness || !items[i].t_out) return GH_E_INVALID_ARGUMENT;
+    if (items[i].len == 0 || items[i].len > cols) return GH_E_DIMENSION_MISMATCH;
+  }
+  gh_ctx* ctx = key->ring->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  GH_HIP(hipSetDevice(ctx->device));
+  gh_batch_plan* p = new (std::nothrow) gh_batch_plan();
+  if (!p) return GH_E_OOM;
+  p->key = key;
+  p->count = count;
+  const uint32_t cc = (uint32_t)ctx->tune.batch_chunk_cols;
+  BatchLayout L;
+  batch_layout(key, items, (uint32_t)count, cc, nullptr, L);
+  if (hipMalloc(&p->mem, L.total()) != hipSuccess) {
+    plan_free(p);
+    return GH_E_OOM;
+  }
+  char* base = (char*)p->mem;
+  batch_layout(key, items, (uint32_t)count, cc, base, L);
+  int st = GH_OK;
+  if (hipMemcpy(base + L.s_bytes + L.p_bytes, L.desc.data(), L.desc.size() * sizeof(BatchDesc),
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(base + L.s_bytes + L.p_bytes + L.d_bytes, L.seg.data(),
+                L.seg.size() * sizeof(BatchSeg), hipMemcpyHostToDevice) != hipSuccess)
+    st = GH_E_DEVICE;
+  if (st == GH_OK &&
+      hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    st = GH_E_DEVICE;
+  if (st == GH_OK) {
+    const int est = batch_enqueue(key, ctx->stream, L, base, cc, ctx->tune.batch_group,
+                                  ctx->tune.mac_variant);
+    const hipError_t ce = hipStreamEndCapture(ctx->stream, &p->graph);
+    st = est ? est : (ce != hipSuccess ? GH_E_DEVICE : GH_OK);
+  }
+  if (st == GH_OK && hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0) != hipSuccess)
+    st = GH_E_DEVICE;
+  if (st) {
+    plan_free(p);
+    return st;
+  }
+  *out = p;
+  return GH_OK;
+}
+
+GH_API int gh_batch_plan_launch(gh_batch_plan* p) {
+  if (!p) return GH_E_INVALID_ARGUMENT;
+  gh_ctx* ctx = p->key->ring->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  GH_HIP(hipSetDevice(ctx->device));
+  GH_HIP(hipGraphLaunch(p->exec, ctx->stream));
+  return GH_OK;
+}
+
+GH_API int gh_batch_plan_destroy(gh_batch_plan* p) {
+  if (!p) return GH_E_INVALID_ARGUMENT;
+  gh_ctx* ctx = p->key->ring->ctx;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+  }
+  plan_free(p);
+  return GH_OK;
+}
+
+}  // extern "C"

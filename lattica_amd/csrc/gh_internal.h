@@ -40,6 +40,15 @@ struct GramWork {  // one wave of a Gram batch: (matrix, 4x4 pair tile, column c
 struct GramFin {  // one output polynomial of a Gram batch
   uint32_t g, pair;
 };
+// Layout of a commitment batch's device memory: [NTT-domain witnesses]
+// [partials][descriptors][segments], each 256-byte aligned; descriptors sorted
+// by witness length, longest first.
+struct BatchLayout {
+  std::vector<BatchDesc> desc;
+  std::vector<BatchSeg> seg;
+  size_t s_bytes = 0, p_bytes = 0, d_bytes = 0, g_bytes = 0;
+  size_t total() const { return s_bytes + p_bytes + d_bytes + g_bytes; }
+};
 }  // namespace gh
 
 struct gh_tuning {
@@ -65,9 +74,8 @@ struct gh_ctx {
   size_t stage_bytes = 0;
   // small fixed device buffer (flags)
   void* misc = nullptr;
-  // host images of the last batch's descriptors (kept alive across the async upload)
-  std::vector<gh::BatchDesc> batch_desc;
-  std::vector<gh::BatchSeg> batch_seg;
+  // host image of the last batch's descriptors (kept alive across the async upload)
+  gh::BatchLayout batch_layout;
   // phase timing (gh_ctx_profile_*): event pairs recorded per commit
   bool prof_on = false;
   std::vector<hipEvent_t> prof_pool;
