@@ -9,10 +9,17 @@ A (134 MB each, 402 MB together) is read from HBM and not from the 256 MB
 Infinity Cache; the single-key (cache-warm) rate is reported beside it.
 
 Prints one JSON line. Extra fields: p50/p99 latency of individually
-synchronised commits, the roofline of the dominant kernel (the MAC, timed with
-HIP events on the library stream), the CPU baseline (the reference's AVX2
-prepared-commit path restated in C, run on this host), and the batched
-negacyclic-product rate (2^18 pairs).
+synchronised commits, the roofline of the dominant kernel (the fused
+transform+MAC kernel, timed with HIP events on the library stream), the CPU
+baseline (the reference's AVX2 prepared-commit path restated in C, run on this
+host), and the side workloads, each checked against the oracle or the plain
+form: the batched form at the same shape, kappa 8 x n 256 commitments
+(`small`), 2^18 negacyclic products (`negacyclic`), LaBRADOR-scale inner
+commitments, Gram matrix and JL projection (`labrador`), 2000 Goldilocks
+commitments batched, graph-replayed and per call (`goldilocks`), and the serving
+loop against the plain per-call form on a Poisson stream of mixed requests
+(`serving`). `--only NAME` runs one side workload alone and prints its JSON
+(for profiling).
 """
 import argparse
 import json
@@ -39,7 +46,7 @@ def witness(rng, n):
     return np.where(x < 0, Q + x, x).astype(np.uint32)
 
 
-def batched_ns(ctx, keys, s_dev, log, batch=32, reps=6):
+def batched_same_shape(ctx, keys, s_dev, log, batch=32, reps=6):
     """The batched form at the same shape: `batch` commitments per gh_commit_batch
     call against one key (the key is loaded once per group of witnesses), keys
     in rotation across calls. Item 0 is checked against the plain form."""
@@ -65,7 +72,7 @@ def batched_ns(ctx, keys, s_dev, log, batch=32, reps=6):
             "item0_equals_plain": same}
 
 
-def c4_goldilocks(ctx, la, log, count=2000, kappa=16, max_cols=1 << 14):
+def goldilocks_batch(ctx, la, log, count=2000, kappa=16, max_cols=1 << 14):
     """2000 Goldilocks commitments (kappa = 16, d = 64) with log-uniform witness
     lengths in [2^8, 2^14] against one shared key: per-call form (one
     gh_commit_batch call per commitment) versus one batched call."""
@@ -120,7 +127,7 @@ def c4_goldilocks(ctx, la, log, count=2000, kappa=16, max_cols=1 << 14):
         key.commit_batch_device([wptr[i]], [lens[i]], [optr[i]])
     ctx.synchronize()
     plain = count / (time.perf_counter() - t0)
-    log(f"C4 Goldilocks x{count}: batched {batched:,.0f} commits/s, graph replay {graph:,.0f} "
+    log(f"Goldilocks batch x{count}: batched {batched:,.0f} commits/s, graph replay {graph:,.0f} "
         f"commits/s (== batched: {graph_same}), per-call {plain:,.0f} commits/s; "
         f"oracle spot check {ok}")
     key.close()
@@ -133,7 +140,138 @@ def c4_goldilocks(ctx, la, log, count=2000, kappa=16, max_cols=1 << 14):
             "mean_len": float(lens.mean()), "oracle_spot_check": ok}
 
 
-def serving_c5(ctx, la, log, count=1000, seed=55):
+def negacyclic_batch(ctx, la, ring, log, pairs=1 << 18, reps=20):
+    """Batched negacyclic products a*b mod (X^64 + 1), 2^18 pairs of uniform
+    residues on resident device buffers (one gh_negacyclic_mul call); a sample
+    is checked against the oracle."""
+    rng = np.random.default_rng(22)
+    ha = rng.integers(0, Q, size=(pairs, D), dtype=np.uint32)
+    hb = rng.integers(0, Q, size=(pairs, D), dtype=np.uint32)
+    pa, pb = ctx.to_device(ha), ctx.to_device(hb)
+    po = ctx.alloc(pairs * D * 4)
+    L = la.lib()
+    for _ in range(3):
+        L.gh_negacyclic_mul(ring.h, pa.ptr, pb.ptr, po.ptr, pairs, la.MEM_DEVICE)
+    ctx.synchronize()
+    a = time.perf_counter()
+    for _ in range(reps):
+        L.gh_negacyclic_mul(ring.h, pa.ptr, pb.ptr, po.ptr, pairs, la.MEM_DEVICE)
+    ctx.synchronize()
+    dt = (time.perf_counter() - a) / reps
+    from oracle import gridref
+    idx = rng.choice(pairs, 64, replace=False)
+    got = po.download(np.uint32, (pairs, D))[idx].astype(np.uint64)
+    ok = bool(np.array_equal(got, gridref.negacyclic_mul(Q, D, ha[idx].astype(np.uint64),
+                                                         hb[idx].astype(np.uint64))))
+    for b in (pa, pb, po):
+        b.free()
+    log(f"negacyclic x{pairs}: {dt * 1e6:,.1f} us ({pairs / dt / 1e9:.2f} G pairs/s, "
+        f"{3 * pairs * D * 4 / dt / 1e9:,.0f} GB/s algorithmic); oracle sample {ok}")
+    return {"pairs_per_s": round(pairs / dt, 1), "us_per_batch": round(dt * 1e6, 1),
+            "algorithmic_GBps": round(3 * pairs * D * 4 / dt / 1e9, 1), "oracle_sample": ok}
+
+
+def commit_small(ctx, la, ring, log, batch=1024, reps=200):
+    """Plain Ajtai commitment at kappa = 8, n = 256 (uniform key, ternary witness):
+    per-call rate (latency-bound: A is 512 KB and stays in cache) and the batched
+    rate (`batch` commitments per gh_commit_batch call)."""
+    rng = np.random.default_rng(11)
+    kappa, n = 8, 256
+    a = rng.integers(0, Q, size=(kappa, n, D), dtype=np.uint32)
+    key = la.Key(ring, a, domain=la.DOMAIN_NTT)
+    t = rng.integers(-1, 2, size=(batch, n, D), dtype=np.int64)
+    s_dev = ctx.to_device(np.where(t < 0, Q - 1, t).astype(np.uint32))
+    o_dev = ctx.alloc(batch * kappa * D * 4)
+    for _ in range(10):
+        key.commit_device(s_dev, n, o_dev)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        key.commit_device(s_dev, n, o_dev)
+    ctx.synchronize()
+    per_call = reps / (time.perf_counter() - t0)
+    wptr = [s_dev.ptr.value + i * n * D * 4 for i in range(batch)]
+    optr = [o_dev.ptr.value + i * kappa * D * 4 for i in range(batch)]
+    key.commit_batch_device(wptr, [n] * batch, optr)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        key.commit_batch_device(wptr, [n] * batch, optr)
+    ctx.synchronize()
+    batched = batch * 10 / (time.perf_counter() - t0)
+    log(f"commit kappa 8 x n 256: per-call {per_call:,.0f} commits/s, batched ({batch}/call) "
+        f"{batched:,.0f} commits/s")
+    key.close()
+    s_dev.free()
+    o_dev.free()
+    return {"per_call_commits_per_s": round(per_call, 1), "batched_commits_per_s": round(batched, 1),
+            "batch": batch}
+
+
+def labrador_calls(ctx, la, ring, log, reps=10):
+    """LaBRADOR scale: r = 32 witness vectors x 1024 polynomials (n = 2^15 in all),
+    coefficients in [-2^7, 2^7), kappa = 16. Times the 32 inner commitments
+    t_i = A*s_i (gh_inner_commit), the full Gram matrix (528 polynomials,
+    gh_gram) and the 256 x 2^21 ternary JL projection (gh_jl_project), each on
+    resident device buffers. The oracle checks one Gram entry and, for the JL,
+    p mod q against the exact sum."""
+    rng = np.random.default_rng(33)
+    r_, n_, rows = 32, 1024, 256
+    a = rng.integers(0, Q, size=(KAPPA, n_, D), dtype=np.uint32)
+    key = la.Key(ring, a, domain=la.DOMAIN_NTT)
+    s = witness(rng, r_ * n_).reshape(r_, n_, D)
+    s_dev = ctx.to_device(s)
+    t_dev = ctx.alloc(r_ * KAPPA * D * 4)
+    npairs = r_ * (r_ + 1) // 2
+    g_dev = ctx.alloc(npairs * D * 4)
+    pm_dev, pe_dev = ctx.alloc(rows * 4), ctx.alloc(rows * 8)
+    seed = np.frombuffer(bytes(range(32)), dtype=np.uint8).copy()
+    L = la.lib()
+    import ctypes
+    calls = {
+        "inner_commit_32x1024": lambda: L.gh_inner_commit(key.h, s_dev.ptr, r_, n_, la.DOMAIN_COEFF,
+                                                          t_dev.ptr, la.MEM_DEVICE),
+        "gram_32x1024": lambda: L.gh_gram(ring.h, s_dev.ptr, r_, n_, la.DOMAIN_COEFF, g_dev.ptr,
+                                          la.MEM_DEVICE),
+        "jl_256x2^21": lambda: L.gh_jl_project(ring.h, ctypes.c_void_p(seed.ctypes.data), rows,
+                                               s_dev.ptr, r_, n_ * D, pm_dev.ptr,
+                                               ctypes.c_void_p(pe_dev.ptr.value), None,
+                                               la.MEM_DEVICE),
+    }
+    res = {}
+    for name, fn in calls.items():
+        for _ in range(2):
+            st = fn()
+            if st:
+                raise la.GridHipError(st, name)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        ctx.synchronize()
+        res[name] = {"us_per_call": round((time.perf_counter() - t0) / reps * 1e6, 1)}
+    res["inner_commit_32x1024"]["commits_per_s"] = round(
+        r_ * 1e6 / res["inner_commit_32x1024"]["us_per_call"], 1)
+    from oracle import gridref
+    g = g_dev.download(np.uint32, (npairs, D))
+    exp = gridref.gram(Q, D, np.stack([s[3], s[17]]).astype(np.uint64))
+    idx = r_ * 3 - 3 * 2 // 2 + (17 - 3)
+    pm = pm_dev.download(np.uint32, (rows,)).astype(np.int64)
+    pe = pe_dev.download(np.int64, (rows,))
+    res["oracle_gram_entry"] = bool(np.array_equal(g[idx].astype(np.uint64), exp[1]))
+    res["jl_modq_matches_exact"] = bool(np.array_equal(pm, pe % Q))
+    log(f"LaBRADOR r 32 x 1024: inner commits {res['inner_commit_32x1024']['us_per_call']:,.0f} us "
+        f"({res['inner_commit_32x1024']['commits_per_s']:,.0f} commits/s), Gram "
+        f"{res['gram_32x1024']['us_per_call']:,.0f} us, JL {res['jl_256x2^21']['us_per_call']:,.0f} us; "
+        f"oracle Gram entry {res['oracle_gram_entry']}, JL mod q == exact mod q "
+        f"{res['jl_modq_matches_exact']}")
+    key.close()
+    for b in (s_dev, t_dev, g_dev, pm_dev, pe_dev):
+        b.free()
+    return res
+
+
+def serving(ctx, la, log, count=1000, seed=55):
     """Serving: a Poisson arrival stream of mixed commit and Gram requests, half
     q32 and half Goldilocks (d = 64, kappa = 16 keys of 2^15 columns, both
     resident), witness sizes log-uniform in [2^8, 2^15] polynomials split into
@@ -279,7 +417,7 @@ def serving_c5(ctx, la, log, count=1000, seed=55):
         ls, es = run_served(srv, arrivals)
         res[label] = {"offered_requests_per_s": round(rate, 1), "plain": summary(lp, ep),
                       "served": summary(ls, es)}
-        log(f"C5 {label} ({rate:,.0f} req/s): plain p50 {res[label]['plain']['p50_us']:,.0f} us "
+        log(f"serving {label} ({rate:,.0f} req/s): plain p50 {res[label]['plain']['p50_us']:,.0f} us "
             f"p99 {res[label]['plain']['p99_us']:,.0f} us {res[label]['plain']['commits_per_s']:,.0f} "
             f"commits/s | served p50 {res[label]['served']['p50_us']:,.0f} us p99 "
             f"{res[label]['served']['p99_us']:,.0f} us {res[label]['served']['commits_per_s']:,.0f} "
@@ -291,7 +429,7 @@ def serving_c5(ctx, la, log, count=1000, seed=55):
     a = outs["plain"].download(np.uint8, (out_bytes,))
     b = outs["served"].download(np.uint8, (out_bytes,))
     res["served_equals_plain"] = bool(np.array_equal(a, b))
-    log(f"C5 saturated: plain {sat_plain['commits_per_s']:,.0f} commits/s "
+    log(f"serving saturated: plain {sat_plain['commits_per_s']:,.0f} commits/s "
         f"({sat_plain['requests_per_s']:,.0f} req/s), served {sat_served['commits_per_s']:,.0f} "
         f"commits/s ({sat_served['requests_per_s']:,.0f} req/s, p99 {sat_served['p99_us']:,.0f} us); "
         f"served == plain for all {count}: {res['served_equals_plain']}")
@@ -314,6 +452,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--only", choices=["small", "negacyclic", "labrador", "goldilocks", "serving"],
+                    help="run one side workload alone (for profiling) and print its JSON")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -328,6 +468,16 @@ def main():
     rng = np.random.default_rng(20260517)
     ctx = la.Context(0)
     ring = la.Ring(ctx, Q, D)
+    if args.only:
+        legs = {"small": lambda: commit_small(ctx, la, ring, log),
+                "negacyclic": lambda: negacyclic_batch(ctx, la, ring, log),
+                "labrador": lambda: labrador_calls(ctx, la, ring, log),
+                "goldilocks": lambda: goldilocks_batch(ctx, la, log),
+                "serving": lambda: serving(ctx, la, log)}
+        print(json.dumps({"workload": args.only, "result": legs[args.only]()}), flush=True)
+        ring.close()
+        ctx.close()
+        return
     log(f"device 0: {ctx.num_cus} CUs; building {args.keys} resident keys "
         f"({KAPPA}x{N} polys, {KAPPA * N * D * 4 / 2**20:.0f} MiB each)")
     keys, a_host0 = [], None
@@ -428,29 +578,12 @@ def main():
                                      "stream_read_commit_size": round(rd_at, 1)}
         log(f"measured HBM ceiling: read {rd:,.0f} GB/s, copy {cp:,.0f} GB/s (1 GiB); "
             f"read {rd_at:,.0f} GB/s at {ALGO_BYTES / 1e6:.1f} MB")
-        # batched negacyclic product, 2^18 pairs (device buffers)
-        pairs = 1 << 18
-        pa = ctx.to_device(rng.integers(0, Q, size=(pairs, D), dtype=np.uint32))
-        pb = ctx.to_device(rng.integers(0, Q, size=(pairs, D), dtype=np.uint32))
-        po = ctx.alloc(pairs * D * 4)
-        L = la.lib()
-        for _ in range(3):
-            L.gh_negacyclic_mul(ring.h, pa.ptr, pb.ptr, po.ptr, pairs, la.MEM_DEVICE)
-        ctx.synchronize()
-        reps = 20
-        a = time.perf_counter()
-        for _ in range(reps):
-            L.gh_negacyclic_mul(ring.h, pa.ptr, pb.ptr, po.ptr, pairs, la.MEM_DEVICE)
-        ctx.synchronize()
-        dt = (time.perf_counter() - a) / reps
-        extra["negacyclic_2^18_pairs"] = {
-            "pairs_per_s": round(pairs / dt, 1), "us_per_batch": round(dt * 1e6, 1),
-            "algorithmic_GBps": round(3 * pairs * D * 4 / dt / 1e9, 1)}
-        for b in (pa, pb, po):
-            b.free()
-        extra["batched_same_shape"] = batched_ns(ctx, keys, s_dev, log)
-        extra["c4_goldilocks_2000"] = c4_goldilocks(ctx, la, log)
-        extra["serving_c5"] = serving_c5(ctx, la, log)
+        extra["negacyclic_q32_2^18_pairs"] = negacyclic_batch(ctx, la, ring, log)
+        extra["batched_same_shape"] = batched_same_shape(ctx, keys, s_dev, log)
+        extra["commit_q32_k8_n256"] = commit_small(ctx, la, ring, log)
+        extra["labrador_q32_r32_n1024"] = labrador_calls(ctx, la, ring, log)
+        extra["goldilocks_batch_2000"] = goldilocks_batch(ctx, la, log)
+        extra["serving_mixed_1000"] = serving(ctx, la, log)
 
     cpu = None
     if not args.no_cpu:

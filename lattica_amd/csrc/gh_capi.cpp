@@ -642,9 +642,18 @@ GH_API int gh_inner_commit(gh_key* key, const void* s, uint32_t r, uint32_t n, i
     sd = (const char*)dev;
     td = (char*)dev + s_bytes;
   }
-  for (uint32_t i = 0; i < r; i++)
-    TRY(launch_commit(key, sd + (size_t)i * n * d * wb, n, domain,
-                      td + (size_t)i * key->rows * d * wb));
+  if (domain == GH_DOMAIN_COEFF && r > 1) {
+    // the r parts share the key: one batched launch reads A once per group of parts
+    std::vector<gh_commit_item> items(r);
+    for (uint32_t i = 0; i < r; i++)
+      items[i] = gh_commit_item{sd + (size_t)i * n * d * wb, n, 0,
+                                td + (size_t)i * key->rows * d * wb};
+    TRY(launch_commit_batch(key, items.data(), r));
+  } else {
+    for (uint32_t i = 0; i < r; i++)
+      TRY(launch_commit(key, sd + (size_t)i * n * d * wb, n, domain,
+                        td + (size_t)i * key->rows * d * wb));
+  }
   if (mem == GH_MEM_HOST) {
     TRY(d2h(ctx, t_out, td, t_bytes));
     return sync(ctx);

@@ -361,24 +361,44 @@ __global__ __launch_bounds__(256) void k_jl(ChaKey key0, uint32_t rows, uint64_t
 }
 
 // p[m] = sum over blocks; q32: exact int64 then mod q; Goldilocks: modular.
+// One block per row: 256 threads sum strided partials, then a tree in LDS.
 template <bool GOLD>
-__global__ void k_jl_reduce(const int64_t* __restrict__ part_acc, uint32_t nblk, uint32_t rows,
-                            uint64_t q, int64_t* __restrict__ p_exact, void* __restrict__ p_modq) {
-  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= rows) return;
-  if (GOLD) {
-    uint64_t a = 0;
-    for (uint32_t b = 0; b < nblk; b++) {
-      uint64_t v = (uint64_t)part_acc[(size_t)b * rows + m];
-      uint64_t s = a + v;
-      a = s < a ? s + kGoldEps : (s >= kGoldilocks ? s - kGoldilocks : s);
+__global__ __launch_bounds__(256) void k_jl_reduce(const int64_t* __restrict__ part_acc,
+                                                   uint32_t nblk, uint32_t rows, uint64_t q,
+                                                   int64_t* __restrict__ p_exact,
+                                                   void* __restrict__ p_modq) {
+  __shared__ int64_t red[256];
+  const uint32_t m = blockIdx.x, t = threadIdx.x;
+  int64_t a = 0;
+  for (uint32_t b = t; b < nblk; b += 256) {
+    const int64_t v = part_acc[(size_t)b * rows + m];
+    if (GOLD) {
+      const uint64_t x = (uint64_t)a, y = (uint64_t)v, s = x + y;
+      a = (int64_t)(s < x ? s + kGoldEps : (s >= kGoldilocks ? s - kGoldilocks : s));
+    } else {
+      a += v;
     }
-    ((uint64_t*)p_modq)[m] = a;
+  }
+  red[t] = a;
+  __syncthreads();
+  for (uint32_t h = 128; h > 0; h >>= 1) {
+    if (t < h) {
+      if (GOLD) {
+        const uint64_t x = (uint64_t)red[t], y = (uint64_t)red[t + h], s = x + y;
+        red[t] = (int64_t)(s < x ? s + kGoldEps : (s >= kGoldilocks ? s - kGoldilocks : s));
+      } else {
+        red[t] += red[t + h];
+      }
+    }
+    __syncthreads();
+  }
+  if (t != 0) return;
+  if (GOLD) {
+    ((uint64_t*)p_modq)[m] = (uint64_t)red[0];
   } else {
-    int64_t a = 0;
-    for (uint32_t b = 0; b < nblk; b++) a += part_acc[(size_t)b * rows + m];
-    if (p_exact) p_exact[m] = a;
-    int64_t r = a % (int64_t)q;
+    const int64_t sum = red[0];
+    if (p_exact) p_exact[m] = sum;
+    int64_t r = sum % (int64_t)q;
     if (r < 0) r += (int64_t)q;
     ((uint32_t*)p_modq)[m] = (uint32_t)r;
   }
@@ -416,10 +436,10 @@ int launch_jl(gh_ring* ring, const uint8_t seed[32], uint32_t rows, const void* 
                        flat, num_parts);
   GH_HIP(hipGetLastError());
   if (!gold)
-    hipLaunchKernelGGL((k_jl_reduce<false>), dim3((rows + 255) / 256), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL((k_jl_reduce<false>), dim3(rows), dim3(256), 0, ctx->stream,
                        (const int64_t*)ws, nblk, rows, ring->q, p_exact, p_modq);
   else
-    hipLaunchKernelGGL((k_jl_reduce<true>), dim3((rows + 255) / 256), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL((k_jl_reduce<true>), dim3(rows), dim3(256), 0, ctx->stream,
                        (const int64_t*)ws, nblk, rows, ring->q, p_exact, p_modq);
   GH_HIP(hipGetLastError());
   return GH_OK;
