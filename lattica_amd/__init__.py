@@ -222,7 +222,7 @@ class Context:
     def synchronize(self):
         _check(lib().gh_ctx_synchronize(self.h), "gh_ctx_synchronize")
 
-    TUNING_KEYS = ("mac_variant", "mac_waves", "mac_blocks_per_cu", "quad_max", "fused",
+    TUNING_KEYS = ("mac_variant", "mac_waves", "mac_blocks_per_cu", "quad_max", "fused", "fused_depth",
                    "batch_chunk_cols", "batch_group")
 
     def set_tuning(self, **kv):

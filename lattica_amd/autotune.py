@@ -42,11 +42,11 @@ def tune_commit(ctx, kappa=16, n=1 << 15, reps=60, quick=False, log=print):
     ws = [ctx.to_device(_witness(rng, q, n)) for _ in range(4)]
     ts = [ctx.alloc(kappa * 64 * 4) for _ in range(4)]
 
-    bpcs = [1, 2] if quick else [1, 2, 4]
+    bpcs = [1, 2] if quick else [1, 2, 3, 4]
     configs = []
-    # fused transform+MAC (16-byte loads): block shape only
-    for waves, bpc in itertools.product([16, 8, 4], bpcs):
-        configs.append({"fused": 1, "mac_variant": 1, "mac_waves": waves,
+    # fused transform+MAC (16-byte loads): block shape and key prefetch depth
+    for waves, bpc, depth in itertools.product([16, 8, 4], bpcs, [2, 1]):
+        configs.append({"fused": 1, "fused_depth": depth, "mac_variant": 1, "mac_waves": waves,
                         "mac_blocks_per_cu": bpc, "quad_max": 1 << 17})
     # separate kernels: MAC load width, block shape, witness transform kernel
     for variant, waves, bpc, quad in itertools.product([1, 0], [16, 8], bpcs, [1 << 17, 0]):

@@ -267,6 +267,9 @@ GH_API int gh_ctx_set_tuning(gh_ctx* ctx, const char* key, int64_t value) {
   } else if (!std::strcmp(key, "fused")) {
     if (value != 0 && value != 1) return GH_E_INVALID_ARGUMENT;
     t.fused = (int)value;
+  } else if (!std::strcmp(key, "fused_depth")) {
+    if (value != 1 && value != 2) return GH_E_INVALID_ARGUMENT;
+    t.fused_depth = (int)value;
   } else if (!std::strcmp(key, "batch_chunk_cols")) {
     if (value < 16 || value > 4096) return GH_E_INVALID_ARGUMENT;
     t.batch_chunk_cols = (int)value;
@@ -288,6 +291,7 @@ GH_API int gh_ctx_get_tuning(gh_ctx* ctx, const char* key, int64_t* value) {
   else if (!std::strcmp(key, "mac_blocks_per_cu")) *value = t.mac_blocks_per_cu;
   else if (!std::strcmp(key, "quad_max")) *value = t.quad_max;
   else if (!std::strcmp(key, "fused")) *value = t.fused;
+  else if (!std::strcmp(key, "fused_depth")) *value = t.fused_depth;
   else if (!std::strcmp(key, "batch_chunk_cols")) *value = t.batch_chunk_cols;
   else if (!std::strcmp(key, "batch_group")) *value = t.batch_group;
   else return GH_E_INVALID_ARGUMENT;

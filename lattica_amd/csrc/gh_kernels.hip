@@ -354,34 +354,37 @@ __global__ __launch_bounds__(64 * WAVES) void k_mac4_q32(
   }
 }
 
-// Fused witness transform + MAC (q32). A block owns a column range and walks
-// it in chunks of at most kFuseChunk columns: every wave first issues the key
-// loads of its first column of the chunk, then the block transforms the
-// chunk's witness polynomials into LDS (four lanes per polynomial) while those
-// loads are in flight, and finally streams the rest of the chunk against the
-// LDS copy of s_ntt. The transform's VALU work thus overlaps the HBM stream
-// instead of running as its own kernel; results equal k_mac4_q32 on a
-// separately transformed witness.
+// Fused witness transform + MAC (q32). A block owns a contiguous range of at
+// most kFuseChunk key columns (the launcher sizes the grid so). In issue order:
+// the lane twiddles and the witness of the block's columns (which the transform
+// waits for), then the key loads of each wave's first PF columns, then the
+// transform of the witness polynomials into LDS (four lanes per polynomial)
+// while those key loads are in flight, then the stream over the remaining
+// columns against the LDS copy. PF = DEPTH * 16 / WAVES columns per wave: at
+// depth 2 with two 4-wave blocks per CU that is 256 KB of key in flight per CU
+// during the transform, enough to keep HBM busy through it. Results equal
+// k_mac4_q32 on a separately transformed witness.
 constexpr int kFuseChunk = 128;
 constexpr int kFuseStride = 68;  // LDS row stride of s_ntt (16-byte aligned rows)
 
-template <int RB, int WAVES>
+template <int RB, int WAVES, int DEPTH>
 __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_q32(
     FQ32 f, const uint32_t* __restrict__ tab, const uint32_t* __restrict__ A,
     const uint32_t* __restrict__ s_coeff, uint32_t* __restrict__ partial, uint32_t rows,
     uint32_t cols, uint32_t cols_per_block) {
   constexpr int NI = RB / 4;
-  constexpr int PF = 16 / WAVES;  // key columns prefetched per wave before the transform
-  extern __shared__ uint32_t lds[];  // max(kFuseChunk * kFuseStride, WAVES * RB * 64) words
+  constexpr int PF = DEPTH * 16 / WAVES;
+  extern __shared__ uint32_t lds[];  // max(cols_per_block * kFuseStride, WAVES * RB * 64) words
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int rsub = lane >> 4, k0 = 4 * (lane & 15);
   const int t = threadIdx.x & 3;
   const uint32_t row0 = blockIdx.y * RB;
-  const uint32_t c_begin = blockIdx.x * cols_per_block;
-  uint32_t c_end = c_begin + cols_per_block;
-  if (c_end > cols) c_end = cols;
+  const uint32_t ch = blockIdx.x * cols_per_block;  // < cols: every block owns a column
+  const uint32_t ncol = ch + cols_per_block < cols ? cols_per_block : cols - ch;
   const size_t col_stride = (size_t)rows * 64;
+  const u32x4* akey = reinterpret_cast<const u32x4*>(A + (size_t)row0 * 64) + lane;
+  const size_t col4 = col_stride / 4;  // one key column, in u32x4
 
   uint64_t lo[NI][4];
   uint32_t hi[NI][4];
@@ -390,75 +393,85 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_q32(
 #pragma unroll
     for (int j = 0; j < 4; j++) { lo[i][j] = 0; hi[i][j] = 0; }
 
-  for (uint32_t ch = c_begin; ch < c_end; ch += kFuseChunk) {
-    const uint32_t ch_end = ch + kFuseChunk < c_end ? ch + kFuseChunk : c_end;
-    // (a) key loads for this wave's first PF columns of the chunk: with
-    // 2 blocks of WAVES waves per CU that is 32 KB per CU of HBM traffic in
-    // flight per prefetched column, enough to cover the transform below
-    u32x4 pa[PF][NI];
+  // (a) loads the transform waits for: lane twiddles, this thread's first
+  // witness polynomial (vector loads complete in issue order)
+  uint32_t ltw[16][2];
 #pragma unroll
-    for (int q = 0; q < PF; q++) {
-      const uint32_t cq = ch + wave + q * WAVES;
-      if (cq < ch_end) {
-        const u32x4* a0 = reinterpret_cast<const u32x4*>(A + (size_t)cq * col_stride + (size_t)row0 * 64) + lane;
+  for (int k = 0; k < 16; k++) {
+    ltw[k][0] = tab[(kTabQuad + 16 * t + k) * 2];
+    ltw[k][1] = tab[(kTabQuad + 16 * t + k) * 2 + 1];
+  }
+  const uint32_t p0 = threadIdx.x >> 2;
+  uint32_t x0[16];
+  if (p0 < ncol) {
+    const uint32_t* sp = s_coeff + (size_t)(ch + p0) * 64 + t;
 #pragma unroll
-        for (int i = 0; i < NI; i++) pa[q][i] = __builtin_nontemporal_load(a0 + i * 64);
-      }
+    for (int i = 0; i < 16; i++) x0[i] = sp[4 * i];
+  }
+  // (b) key prefetch: this wave's first PF columns
+  u32x4 pa[PF][NI];
+#pragma unroll
+  for (int q = 0; q < PF; q++) {
+    const uint32_t cq = wave + q * WAVES;
+    if (cq < ncol) {
+#pragma unroll
+      for (int i = 0; i < NI; i++)
+        pa[q][i] = __builtin_nontemporal_load(akey + (size_t)(ch + cq) * col4 + i * 64);
     }
-    // (b) witness transform of the chunk into LDS while (a) is in flight
-    __syncthreads();  // the previous chunk's readers are done with lds
-    {
-      uint32_t ltw[16][2];
+  }
+  // (c) witness transform into LDS while (b) is in flight
+  for (uint32_t p = p0; p < ncol; p += 16 * WAVES) {
+    uint32_t x[16];
+    if (p == p0) {
 #pragma unroll
-      for (int k = 0; k < 16; k++) {
-        ltw[k][0] = tab[(kTabQuad + 16 * t + k) * 2];
-        ltw[k][1] = tab[(kTabQuad + 16 * t + k) * 2 + 1];
-      }
-      for (uint32_t p = threadIdx.x >> 2; p < ch_end - ch; p += 16 * WAVES) {
-        uint32_t x[16];
-        const uint32_t* sp = s_coeff + (size_t)(ch + p) * 64 + t;
+      for (int i = 0; i < 16; i++) x[i] = x0[i];
+    } else {
+      const uint32_t* sp = s_coeff + (size_t)(ch + p) * 64 + t;
 #pragma unroll
-        for (int i = 0; i < 16; i++) x[i] = sp[4 * i];
-        quad_fwd64(f, x, tab, ltw, t);
-        u32x4* dp = reinterpret_cast<u32x4*>(lds + p * kFuseStride + 16 * bitrev2(t));
-#pragma unroll
-        for (int v = 0; v < 4; v++) dp[v] = u32x4{x[4 * v], x[4 * v + 1], x[4 * v + 2], x[4 * v + 3]};
-      }
+      for (int i = 0; i < 16; i++) x[i] = sp[4 * i];
     }
-    __syncthreads();
-    // (c) MAC over the chunk: the prefetched columns first, then two per step
+    quad_fwd64(f, x, tab, ltw, t);
+    u32x4* dp = reinterpret_cast<u32x4*>(lds + p * kFuseStride + 16 * bitrev2(t));
 #pragma unroll
-    for (int q = 0; q < PF; q++) {
-      const uint32_t cq = ch + wave + q * WAVES;
-      if (cq < ch_end) {
-        const u32x4 s0 = *reinterpret_cast<const u32x4*>(lds + (cq - ch) * kFuseStride + k0);
+    for (int v = 0; v < 4; v++) dp[v] = u32x4{x[4 * v], x[4 * v + 1], x[4 * v + 2], x[4 * v + 3]};
+  }
+  __syncthreads();
+  // (d) MAC: the prefetched columns, then the rest two per step
 #pragma unroll
-        for (int i = 0; i < NI; i++) {
-          mac96(lo[i][0], hi[i][0], pa[q][i].x, s0.x);
-          mac96(lo[i][1], hi[i][1], pa[q][i].y, s0.y);
-          mac96(lo[i][2], hi[i][2], pa[q][i].z, s0.z);
-          mac96(lo[i][3], hi[i][3], pa[q][i].w, s0.w);
-        }
-      }
-    }
-    uint32_t c = ch + wave + PF * WAVES;
-    for (; c + WAVES < ch_end; c += 2 * WAVES) {
-      const u32x4* a0 = reinterpret_cast<const u32x4*>(A + (size_t)c * col_stride + (size_t)row0 * 64) + lane;
-      const u32x4* a1 = a0 + (size_t)WAVES * (col_stride / 4);
-      u32x4 va[NI], vb[NI];
-#pragma unroll
-      for (int i = 0; i < NI; i++) va[i] = __builtin_nontemporal_load(a0 + i * 64);
-#pragma unroll
-      for (int i = 0; i < NI; i++) vb[i] = __builtin_nontemporal_load(a1 + i * 64);
-      const u32x4 s0 = *reinterpret_cast<const u32x4*>(lds + (c - ch) * kFuseStride + k0);
-      const u32x4 s1 = *reinterpret_cast<const u32x4*>(lds + (c + WAVES - ch) * kFuseStride + k0);
+  for (int q = 0; q < PF; q++) {
+    const uint32_t cq = wave + q * WAVES;
+    if (cq < ncol) {
+      const u32x4 s0 = *reinterpret_cast<const u32x4*>(lds + cq * kFuseStride + k0);
 #pragma unroll
       for (int i = 0; i < NI; i++) {
-        mac96(lo[i][0], hi[i][0], va[i].x, s0.x);
-        mac96(lo[i][1], hi[i][1], va[i].y, s0.y);
-        mac96(lo[i][2], hi[i][2], va[i].z, s0.z);
-        mac96(lo[i][3], hi[i][3], va[i].w, s0.w);
+        mac96(lo[i][0], hi[i][0], pa[q][i].x, s0.x);
+        mac96(lo[i][1], hi[i][1], pa[q][i].y, s0.y);
+        mac96(lo[i][2], hi[i][2], pa[q][i].z, s0.z);
+        mac96(lo[i][3], hi[i][3], pa[q][i].w, s0.w);
       }
+    }
+  }
+  for (uint32_t c = wave + PF * WAVES; c < ncol; c += 2 * WAVES) {
+    const bool two = c + WAVES < ncol;
+    u32x4 va[NI], vb[NI];
+#pragma unroll
+    for (int i = 0; i < NI; i++)
+      va[i] = __builtin_nontemporal_load(akey + (size_t)(ch + c) * col4 + i * 64);
+    if (two) {
+#pragma unroll
+      for (int i = 0; i < NI; i++)
+        vb[i] = __builtin_nontemporal_load(akey + (size_t)(ch + c + WAVES) * col4 + i * 64);
+    }
+    const u32x4 s0 = *reinterpret_cast<const u32x4*>(lds + c * kFuseStride + k0);
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+      mac96(lo[i][0], hi[i][0], va[i].x, s0.x);
+      mac96(lo[i][1], hi[i][1], va[i].y, s0.y);
+      mac96(lo[i][2], hi[i][2], va[i].z, s0.z);
+      mac96(lo[i][3], hi[i][3], va[i].w, s0.w);
+    }
+    if (two) {
+      const u32x4 s1 = *reinterpret_cast<const u32x4*>(lds + (c + WAVES) * kFuseStride + k0);
 #pragma unroll
       for (int i = 0; i < NI; i++) {
         mac96(lo[i][0], hi[i][0], vb[i].x, s1.x);
@@ -467,20 +480,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_q32(
         mac96(lo[i][3], hi[i][3], vb[i].w, s1.w);
       }
     }
-    if (c < ch_end) {
-      const u32x4* a0 = reinterpret_cast<const u32x4*>(A + (size_t)c * col_stride + (size_t)row0 * 64) + lane;
-      const u32x4 s0 = *reinterpret_cast<const u32x4*>(lds + (c - ch) * kFuseStride + k0);
-#pragma unroll
-      for (int i = 0; i < NI; i++) {
-        const u32x4 v = __builtin_nontemporal_load(a0 + i * 64);
-        mac96(lo[i][0], hi[i][0], v.x, s0.x);
-        mac96(lo[i][1], hi[i][1], v.y, s0.y);
-        mac96(lo[i][2], hi[i][2], v.z, s0.z);
-        mac96(lo[i][3], hi[i][3], v.w, s0.w);
-      }
-    }
   }
-  // (d) block reduction through the same LDS
+  // (e) block reduction through the same LDS
   __syncthreads();
   uint32_t* red = lds;  // [WAVES][RB][64]
 #pragma unroll
@@ -763,8 +764,12 @@ template <int RB, int WAVES>
 static void fused_launch(gh_ctx* ctx, dim3 grid, size_t lds_bytes, hipEvent_t e0, hipEvent_t e1,
                          FQ32 f, const uint32_t* tab, const uint32_t* A, const uint32_t* S,
                          uint32_t* P, uint32_t rows, uint32_t cols, uint32_t cpb) {
-  hipExtLaunchKernelGGL((k_commit_fused_q32<RB, WAVES>), grid, dim3(64 * WAVES), lds_bytes,
-                        ctx->stream, e0, e1, 0, f, tab, A, S, P, rows, cols, cpb);
+  if (ctx->tune.fused_depth == 2)
+    hipExtLaunchKernelGGL((k_commit_fused_q32<RB, WAVES, 2>), grid, dim3(64 * WAVES), lds_bytes,
+                          ctx->stream, e0, e1, 0, f, tab, A, S, P, rows, cols, cpb);
+  else
+    hipExtLaunchKernelGGL((k_commit_fused_q32<RB, WAVES, 1>), grid, dim3(64 * WAVES), lds_bytes,
+                          ctx->stream, e0, e1, 0, f, tab, A, S, P, rows, cols, cpb);
 }
 
 // q32 commitment in two launches on the context stream: the fused
@@ -776,7 +781,12 @@ static int launch_commit_fused(gh_key* key, const void* s_dev, uint32_t cols, vo
   const uint32_t rows = key->rows;
   const int waves = ctx->tune.mac_waves;
   const int rb = rows % 16 == 0 ? 16 : rows % 8 == 0 ? 8 : 4;
+  // blocks: the tuned count per CU, at least one per kFuseChunk columns (a
+  // block's witness fits its LDS), at most one per `waves` columns; every block
+  // owns at least one column
   uint32_t nparts = (uint32_t)(ctx->num_cus * ctx->tune.mac_blocks_per_cu);
+  const uint32_t min_parts = (cols + kFuseChunk - 1) / kFuseChunk;
+  if (nparts < min_parts) nparts = min_parts;
   const uint32_t max_parts = (cols + waves - 1) / waves;
   if (nparts > max_parts) nparts = max_parts;
   if (nparts == 0) nparts = 1;
@@ -792,7 +802,7 @@ static int launch_commit_fused(gh_key* key, const void* s_dev, uint32_t cols, vo
   const uint32_t* S = (const uint32_t*)s_dev;
   uint32_t* P = (uint32_t*)ws;
   dim3 grid(nparts, rows / rb);
-  size_t lds_words = (size_t)kFuseChunk * kFuseStride;
+  size_t lds_words = (size_t)cpb * kFuseStride;
   if ((size_t)waves * rb * 64 > lds_words) lds_words = (size_t)waves * rb * 64;
   const size_t lds_bytes = lds_words * 4;
   hipEvent_t e0, e1;

@@ -249,14 +249,16 @@ def test_every_tuning_setting_gives_the_same_commitment(oracle, ctx, ring_q32):
     key = la.Key(r, a, domain=la.DOMAIN_NTT)
     saved = ctx.get_tuning()
     try:
-        for fused, variant, waves, bpc, quad in [
-                (0, 0, 16, 1, 0), (0, 1, 4, 1, 0), (0, 1, 8, 2, 1 << 17), (0, 1, 16, 4, 1 << 17),
-                (0, 1, 16, 8, 0), (1, 1, 4, 1, 0), (1, 1, 8, 2, 0), (1, 1, 16, 1, 0),
-                (1, 1, 16, 8, 0)]:
+        for fused, variant, waves, bpc, quad, depth in [
+                (0, 0, 16, 1, 0, 1), (0, 1, 4, 1, 0, 1), (0, 1, 8, 2, 1 << 17, 1),
+                (0, 1, 16, 4, 1 << 17, 1), (0, 1, 16, 8, 0, 1), (1, 1, 4, 1, 0, 1),
+                (1, 1, 8, 2, 0, 1), (1, 1, 16, 1, 0, 1), (1, 1, 16, 8, 0, 1), (1, 1, 4, 3, 0, 1),
+                (1, 1, 8, 3, 0, 1), (1, 1, 4, 1, 0, 2), (1, 1, 4, 2, 0, 2), (1, 1, 4, 3, 0, 2),
+                (1, 1, 8, 1, 0, 2), (1, 1, 16, 2, 0, 2)]:
             ctx.set_tuning(fused=fused, mac_variant=variant, mac_waves=waves,
-                           mac_blocks_per_cu=bpc, quad_max=quad)
+                           mac_blocks_per_cu=bpc, quad_max=quad, fused_depth=depth)
             got = key.commit(s).astype(np.uint64)
-            assert np.array_equal(got, exp), (fused, variant, waves, bpc, quad)
+            assert np.array_equal(got, exp), (fused, variant, waves, bpc, quad, depth)
         with pytest.raises(la.GridHipError):
             ctx.set_tuning(mac_waves=5)
     finally:
