@@ -215,9 +215,14 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch_gold(
     len[i] = j < count ? desc[j].len : 0;
     sp[i] = reinterpret_cast<const uint64_t*>(desc[j < count ? j : g0].s_ntt) + lane;
   }
-  uint64_t acc[G][4];
+  // groups of up to 4 accumulate lazily (exact 96-bit partial products, one
+  // reduction per chunk); larger groups keep one modular sum per accumulator,
+  // which needs fewer registers
+  constexpr bool kLazy = G <= 4;
+  GoldAcc lz[kLazy ? G : 1][4];
+  uint64_t acc[kLazy ? 1 : G][4];
 #pragma unroll
-  for (int i = 0; i < G; i++)
+  for (int i = 0; i < (kLazy ? 1 : G); i++)
 #pragma unroll
     for (int r = 0; r < 4; r++) acc[i][r] = 0;
   const size_t col_stride = (size_t)rows * 64;
@@ -244,7 +249,10 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch_gold(
     for (int i = 0; i < G; i++) {
       if (c < len[i]) {
 #pragma unroll
-        for (int r = 0; r < 4; r++) acc[i][r] = f.add(acc[i][r], f.mul(a[r], s[i]));
+        for (int r = 0; r < 4; r++) {
+          if constexpr (kLazy) gold_mac(lz[i][r], a[r], s[i]);
+          else acc[i][r] = f.add(acc[i][r], f.mul(a[r], s[i]));
+        }
       }
     }
   }
@@ -254,7 +262,12 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch_gold(
     if (j < count && c0 < len[i]) {
       uint64_t* pp = reinterpret_cast<uint64_t*>(desc[j].partial);
 #pragma unroll
-      for (int r = 0; r < 4; r++) pp[((size_t)ch * rows + row0 + r) * 64 + lane] = acc[i][r];
+      for (int r = 0; r < 4; r++) {
+        uint64_t v;
+        if constexpr (kLazy) v = gold_reduce(f, lz[i][r]);
+        else v = acc[i][r];
+        pp[((size_t)ch * rows + row0 + r) * 64 + lane] = v;
+      }
     }
   }
 }

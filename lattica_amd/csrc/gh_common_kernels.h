@@ -15,6 +15,30 @@ __device__ __forceinline__ void mac96(uint64_t& lo, uint32_t& hi, uint32_t a, ui
       : "vcc");
 }
 
+// Exact lazy accumulation of Goldilocks products: x*y split into 32-bit halves,
+// x0*y0 into a (weight 1), x0*y1 + x1*y0 into b (weight 2^32), x1*y1 into d
+// (weight 2^64), each a 96-bit (lo64, hi32) sum. Four multiply-accumulates
+// per term instead of a 128-bit product, a reduction and a modular add; exact
+// for fewer than 2^32 terms.
+struct GoldAcc {
+  uint64_t a = 0, b = 0, d = 0;
+  uint32_t ah = 0, bh = 0, dh = 0;
+};
+__device__ __forceinline__ void gold_mac(GoldAcc& s, uint64_t x, uint64_t y) {
+  const uint32_t x0 = (uint32_t)x, x1 = (uint32_t)(x >> 32);
+  const uint32_t y0 = (uint32_t)y, y1 = (uint32_t)(y >> 32);
+  mac96(s.a, s.ah, x0, y0);
+  mac96(s.b, s.bh, x0, y1);
+  mac96(s.b, s.bh, x1, y0);
+  mac96(s.d, s.dh, x1, y1);
+}
+// canonical a + b*2^32 + d*2^64 mod p, with 2^64 = 2^32 - 1 mod p
+__device__ __forceinline__ uint64_t gold_reduce(const FGold& f, const GoldAcc& s) {
+  const uint64_t ra = f.reduce128(s.a, s.ah), rb = f.reduce128(s.b, s.bh);
+  const uint64_t rd = f.reduce128(s.d, s.dh);
+  return f.add(f.add(ra, f.mul(rb, 1ull << 32)), f.mul(rd, 0xffffffffull));
+}
+
 // t[row] = INTT(sum_p partial[p][row]) with the untwist table `post`, by one
 // block of 1024 threads.
 template <class F>

@@ -536,14 +536,13 @@ __global__ __launch_bounds__(64 * kMacWaves) void k_mac_gold(
   }
 }
 
-// Bandwidth probes: a streaming read (16-byte nontemporal loads, XOR-folded so
-// the loads stay live) and a streaming copy of known size. They give the
-// achievable HBM ceiling beside the nominal peak, and a reference dispatch for
-// scaling hardware traffic counters.
-// Streaming probes: every block reads (copies) one contiguous span of the
-// buffer with nontemporal 16-byte loads, four in flight per thread. Per-block
-// contiguous spans stream faster on MI355X than a grid-wide interleave, whose
-// waves each touch a new region every iteration.
+// Bandwidth probes: a streaming read (XOR-folded so the loads stay live) and a
+// streaming copy of known size. They give the achievable HBM ceiling beside the
+// nominal peak, and a reference dispatch for scaling hardware traffic
+// counters. Every block reads (copies) one contiguous span of the buffer with
+// nontemporal 16-byte loads, four in flight per thread: per-block contiguous
+// spans stream faster on MI355X than a grid-wide interleave, whose waves each
+// touch a new region every iteration.
 __global__ __launch_bounds__(512) void k_stream_read(const u32x4* __restrict__ src, size_t n,
                                                      size_t span, uint32_t* __restrict__ sink) {
   u32x4 acc = {0, 0, 0, 0};

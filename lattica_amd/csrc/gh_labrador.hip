@@ -99,11 +99,7 @@ __device__ __forceinline__ void gram_tile_gold(FGold f, const uint64_t* __restri
   const uint32_t c0 = ci * chunk;
   uint32_t c1 = c0 + chunk;
   if (c1 > n) c1 = n;
-  uint64_t acc[kGramTile][kGramTile];
-#pragma unroll
-  for (int x = 0; x < kGramTile; x++)
-#pragma unroll
-    for (int y = 0; y < kGramTile; y++) acc[x][y] = 0;
+  GoldAcc acc[kGramTile][kGramTile];  // exact lazy sums, reduced once per chunk
   uint32_t ii[kGramTile], jj[kGramTile];
 #pragma unroll
   for (int x = 0; x < kGramTile; x++) {
@@ -120,7 +116,7 @@ __device__ __forceinline__ void gram_tile_gold(FGold f, const uint64_t* __restri
 #pragma unroll
     for (int x = 0; x < kGramTile; x++)
 #pragma unroll
-      for (int y = 0; y < kGramTile; y++) acc[x][y] = f.add(acc[x][y], f.mul(si[x], sj[y]));
+      for (int y = 0; y < kGramTile; y++) gold_mac(acc[x][y], si[x], sj[y]);
   }
   const uint32_t npairs = r * (r + 1) / 2;
 #pragma unroll
@@ -129,7 +125,7 @@ __device__ __forceinline__ void gram_tile_gold(FGold f, const uint64_t* __restri
     for (int y = 0; y < kGramTile; y++) {
       const uint32_t i = ta * kGramTile + x, j = tb * kGramTile + y;
       if (i < r && j < r && i <= j)
-        partial[((size_t)ci * npairs + pair_index(r, i, j)) * 64 + lane] = acc[x][y];
+        partial[((size_t)ci * npairs + pair_index(r, i, j)) * 64 + lane] = gold_reduce(f, acc[x][y]);
     }
 }
 
