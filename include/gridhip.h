@@ -207,6 +207,20 @@ GH_API int gh_key_dims(const gh_key *key, uint32_t *rows, uint32_t *msg_len, uin
 GH_API int gh_commit(gh_key *key, const void *message, uint32_t msg_len, const void *randomness,
                      uint32_t open_len, int domain, void *t_out, int mem);
 
+/* ---- BDLOP commitment ------------------------------------------------------
+ * BdlopCommitmentScheme::commit / verify recompute (commit/src/bdlop/mod.rs:34-54)
+ * through PreparedLinearOps::recompute_bdlop_parts_runtime
+ * (commit/src/linear.rs:800-938):
+ *   u = A_mask * r (mask->rows polys), v = A_msg * m + A_open * r (key->rows polys),
+ * coefficient domain. `mask` holds A_mask as a key of open_len message columns
+ * (gh_key_create with msg_len = open_len, open_len = 0); `key` holds
+ * [A_msg | A_open]. message: msg_len polys, randomness: open_len polys, both in
+ * `domain`. DIMENSION_MISMATCH when the lengths disagree with the keys
+ * (validate_bdlop_key, bdlop/mod.rs:56-64). */
+GH_API int gh_bdlop_commit(gh_key *mask, gh_key *key, const void *message, uint32_t msg_len,
+                           const void *randomness, uint32_t open_len, int domain, void *u_out,
+                           void *v_out, int mem);
+
 /* ---- batched commitments ---------------------------------------------------
  * Many independent commitments against one shared key in a single launch.
  * Item i commits witness_i (len_i polys, coefficient domain) against the first
@@ -252,6 +266,29 @@ GH_API int gh_gram(gh_ring *ring, const void *s, uint32_t r, uint32_t n, int dom
  * t_out: r x rows polys, coefficient domain. */
 GH_API int gh_inner_commit(gh_key *key, const void *s, uint32_t r, uint32_t n, int domain,
                            void *t_out, int mem);
+
+/* ---- LaBRADOR step around the commitment -------------------------------------
+ * gh_decompose_balanced: decompose_polys / decompose_poly_balanced
+ *   (labrador/src/main_protocol/mod.rs:258-320, :731-745): each coefficient,
+ *   centred (x > q/2 -> x - q), is split into `limbs` base-`base` digits in
+ *   [-base/2, base/2] (centered_mod, truncating division), stored mod q;
+ *   out: count x limbs x d words, [poly][limb][coeff]. Digits beyond `limbs`
+ *   are dropped, as in the reference. base in [2, 2^32], limbs >= 1, else
+ *   GH_E_INVALID_PARAMETERS.
+ * gh_garbage_h: compute_garbage_h (labrador/src/main_protocol/garbage.rs:101-133):
+ *   h_ij = (<phi_i, s_j> + <phi_j, s_i>) * 2^-1, 0 <= i <= j < r, row-major
+ *   upper triangle as gh_gram; phi, s: r x n polys in `domain`.
+ * gh_amortize: compute_amortized_witness (main_protocol/amortization.rs:61-84):
+ *   z[k] = sum_i c_i * s_i[k]; c: r polys, s: r x n polys, both in `domain`;
+ *   z: n polys, coefficient domain.
+ * The outer commitments u1 = B*t + C*g and u2 = D*h (labrador/src/crs.rs:273-304)
+ * are gh_commit with a key [B | C] (message t, randomness g) and a key D. */
+GH_API int gh_decompose_balanced(gh_ring *ring, const void *values, size_t count, uint64_t base,
+                                 uint32_t limbs, void *out, int mem);
+GH_API int gh_garbage_h(gh_ring *ring, const void *phi, const void *s, uint32_t r, uint32_t n,
+                        int domain, void *h_out, int mem);
+GH_API int gh_amortize(gh_ring *ring, const void *c, const void *s, uint32_t r, uint32_t n,
+                       int domain, void *z_out, int mem);
 
 /* ---- JL projection -----------------------------------------------------------
  * JLMatrix::project_and_extract_jl_rows (labrador/src/jl.rs:267-303 ->

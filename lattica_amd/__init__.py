@@ -52,7 +52,8 @@ EXPORTS = (
     "gh_commit_batch", "gh_gram", "gh_inner_commit", "gh_jl_project",
     "gh_server_create", "gh_server_destroy", "gh_server_submit", "gh_server_wait",
     "gh_server_poll", "gh_server_query", "gh_batch_plan_create", "gh_batch_plan_launch",
-    "gh_batch_plan_destroy",
+    "gh_batch_plan_destroy", "gh_bdlop_commit", "gh_decompose_balanced", "gh_garbage_h",
+    "gh_amortize",
 )
 
 
@@ -164,6 +165,10 @@ def lib():
         "gh_batch_plan_create": (i32, [vp, ctypes.POINTER(CommitItem), sz, ctypes.POINTER(vp)]),
         "gh_batch_plan_launch": (i32, [vp]),
         "gh_batch_plan_destroy": (i32, [vp]),
+        "gh_bdlop_commit": (i32, [vp, vp, vp, u32, vp, u32, i32, vp, vp, i32]),
+        "gh_decompose_balanced": (i32, [vp, vp, sz, u64, u32, vp, i32]),
+        "gh_garbage_h": (i32, [vp, vp, vp, u32, u32, i32, vp, i32]),
+        "gh_amortize": (i32, [vp, vp, vp, u32, u32, i32, vp, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -361,6 +366,32 @@ class Ring:
         _check(lib().gh_gram(self.h, _ptr(s), r, n, domain, _ptr(out), MEM_HOST), "gh_gram")
         return out
 
+    def decompose_balanced(self, polys, base: int, limbs: int):
+        """(m, d) -> (m, limbs, d): gh_decompose_balanced."""
+        v = self.words(polys)
+        out = np.zeros((v.shape[0], limbs, self.d), dtype=self.dtype)
+        _check(lib().gh_decompose_balanced(self.h, _ptr(v), v.shape[0], base, limbs, _ptr(out),
+                                           MEM_HOST), "gh_decompose_balanced")
+        return out
+
+    def garbage_h(self, phi, s, domain=DOMAIN_COEFF):
+        """(r, n, d) x (r, n, d) -> (r(r+1)/2, d): gh_garbage_h."""
+        phi, s = self.words(phi), self.words(s)
+        r, n = s.shape[0], s.shape[1]
+        out = np.zeros((r * (r + 1) // 2, self.d), dtype=self.dtype)
+        _check(lib().gh_garbage_h(self.h, _ptr(phi), _ptr(s), r, n, domain, _ptr(out), MEM_HOST),
+               "gh_garbage_h")
+        return out
+
+    def amortize(self, c, s, domain=DOMAIN_COEFF):
+        """c: (r, d), s: (r, n, d) -> z: (n, d) = sum_i c_i * s_i: gh_amortize."""
+        c, s = self.words(c), self.words(s)
+        r, n = s.shape[0], s.shape[1]
+        out = np.zeros((n, self.d), dtype=self.dtype)
+        _check(lib().gh_amortize(self.h, _ptr(c), _ptr(s), r, n, domain, _ptr(out), MEM_HOST),
+               "gh_amortize")
+        return out
+
     def jl_project(self, seed: bytes, rows: int, parts, want_exact=True, want_flat=False):
         parts = self.words(parts)
         num_parts, cols = parts.shape
@@ -444,6 +475,16 @@ class Key:
             items[i].len = int(n)
             items[i].t_out = o.ptr.value if isinstance(o, DeviceBuffer) else int(o)
         _check(lib().gh_commit_batch(self.h, items, len(lens), MEM_DEVICE), "gh_commit_batch")
+
+    def bdlop(self, mask: "Key", message, randomness, domain=DOMAIN_COEFF):
+        """gh_bdlop_commit with this key as [A_msg | A_open]: returns (u, v)."""
+        ring = self.ring
+        m, r = ring.words(message), ring.words(randomness)
+        u = np.zeros((mask.rows, ring.d), dtype=ring.dtype)
+        v = np.zeros((self.rows, ring.d), dtype=ring.dtype)
+        _check(lib().gh_bdlop_commit(mask.h, self.h, _ptr(m), m.shape[0], _ptr(r), r.shape[0],
+                                     domain, _ptr(u), _ptr(v), MEM_HOST), "gh_bdlop_commit")
+        return u, v
 
     def batch_plan(self, witnesses, lens, outs) -> "BatchPlan":
         """gh_batch_plan over device-memory items (DeviceBuffers or addresses)."""

@@ -609,6 +609,141 @@ GH_API int gh_commit(gh_key* key, const void* message, uint32_t msg_len, const v
   return GH_OK;
 }
 
+GH_API int gh_decompose_balanced(gh_ring* ring, const void* values, size_t count, uint64_t base,
+                                 uint32_t limbs, void* out, int mem) {
+  if (!ring || (count && (!values || !out))) return GH_E_INVALID_ARGUMENT;
+  TRY(check_mem(mem));
+  if (base < 2 || base > (1ull << 32) || limbs == 0) return GH_E_INVALID_PARAMETERS;
+  if (count == 0) return GH_OK;
+  gh_ctx* ctx = ring->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  const size_t wb = ring->word_bytes, d = ring->d;
+  if (mem == GH_MEM_DEVICE) return launch_decompose(ring, values, count, base, limbs, out);
+  TRY(check_canonical_host(ring, values, count * d));
+  const size_t in_bytes = count * d * wb, out_bytes = in_bytes * limbs;
+  void* dev;
+  TRY(ctx_staging(ctx, in_bytes + out_bytes, &dev));
+  TRY(h2d(ctx, dev, values, in_bytes));
+  TRY(launch_decompose(ring, dev, count, base, limbs, (char*)dev + in_bytes));
+  TRY(d2h(ctx, out, (char*)dev + in_bytes, out_bytes));
+  return sync(ctx);
+}
+
+GH_API int gh_garbage_h(gh_ring* ring, const void* phi, const void* s, uint32_t r, uint32_t n,
+                        int domain, void* h_out, int mem) {
+  if (!ring || (r && (!phi || !s || !h_out))) return GH_E_INVALID_ARGUMENT;
+  TRY(check_mem(mem));
+  if (domain != GH_DOMAIN_COEFF && domain != GH_DOMAIN_NTT) return GH_E_INVALID_ARGUMENT;
+  if (r == 0) return GH_OK;
+  if (r > 0xffff || (uint64_t)r * n > 0xffffffffull) return GH_E_INVALID_ARGUMENT;
+  gh_ctx* ctx = ring->ctx;
+  const size_t wb = ring->word_bytes, d = ring->d;
+  const size_t h_bytes = (size_t)r * (r + 1) / 2 * d * wb;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  if (n == 0) {  // empty witness: every h_ij is zero
+    if (mem == GH_MEM_HOST) { std::memset(h_out, 0, h_bytes); return GH_OK; }
+    GH_HIP(hipMemsetAsync(h_out, 0, h_bytes, ctx->stream));
+    return GH_OK;
+  }
+  if (mem == GH_MEM_DEVICE) return launch_garbage_h(ring, phi, s, r, n, domain, h_out);
+  const size_t in_bytes = (size_t)r * n * d * wb;
+  TRY(check_canonical_host(ring, phi, (size_t)r * n * d));
+  TRY(check_canonical_host(ring, s, (size_t)r * n * d));
+  void* dev;
+  TRY(ctx_staging(ctx, 2 * in_bytes + h_bytes, &dev));
+  char* b = (char*)dev;
+  TRY(h2d(ctx, b, phi, in_bytes));
+  TRY(h2d(ctx, b + in_bytes, s, in_bytes));
+  TRY(launch_garbage_h(ring, b, b + in_bytes, r, n, domain, b + 2 * in_bytes));
+  TRY(d2h(ctx, h_out, b + 2 * in_bytes, h_bytes));
+  return sync(ctx);
+}
+
+GH_API int gh_amortize(gh_ring* ring, const void* c, const void* s, uint32_t r, uint32_t n,
+                       int domain, void* z_out, int mem) {
+  if (!ring || (n && (!z_out || (r && (!c || !s))))) return GH_E_INVALID_ARGUMENT;
+  TRY(check_mem(mem));
+  if (domain != GH_DOMAIN_COEFF && domain != GH_DOMAIN_NTT) return GH_E_INVALID_ARGUMENT;
+  if (n == 0) return GH_OK;
+  if ((uint64_t)r * n > 0xffffffffull) return GH_E_INVALID_ARGUMENT;
+  gh_ctx* ctx = ring->ctx;
+  const size_t wb = ring->word_bytes, d = ring->d;
+  const size_t z_bytes = (size_t)n * d * wb;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  if (r == 0) {  // empty sum
+    if (mem == GH_MEM_HOST) { std::memset(z_out, 0, z_bytes); return GH_OK; }
+    GH_HIP(hipMemsetAsync(z_out, 0, z_bytes, ctx->stream));
+    return GH_OK;
+  }
+  const size_t c_bytes = (size_t)r * d * wb, s_bytes = (size_t)r * n * d * wb;
+  const size_t scratch = domain == GH_DOMAIN_COEFF ? s_bytes : 0;
+  void* dev;
+  if (mem == GH_MEM_DEVICE) {
+    void* scr = nullptr;
+    if (scratch) TRY(ctx_staging(ctx, scratch, &scr));
+    return launch_amortize(ring, c, s, r, n, domain, z_out, scr);
+  }
+  TRY(check_canonical_host(ring, c, (size_t)r * d));
+  TRY(check_canonical_host(ring, s, (size_t)r * n * d));
+  TRY(ctx_staging(ctx, c_bytes + s_bytes + z_bytes + scratch, &dev));
+  char* b = (char*)dev;
+  TRY(h2d(ctx, b, c, c_bytes));
+  TRY(h2d(ctx, b + c_bytes, s, s_bytes));
+  char* zd = b + c_bytes + s_bytes;
+  TRY(launch_amortize(ring, b, b + c_bytes, r, n, domain, zd, scratch ? zd + z_bytes : nullptr));
+  TRY(d2h(ctx, z_out, zd, z_bytes));
+  return sync(ctx);
+}
+
+GH_API int gh_bdlop_commit(gh_key* mask, gh_key* key, const void* message, uint32_t msg_len,
+                           const void* randomness, uint32_t open_len, int domain, void* u_out,
+                           void* v_out, int mem) {
+  if (!mask || !key || !message || !randomness || !u_out || !v_out) return GH_E_INVALID_ARGUMENT;
+  TRY(check_mem(mem));
+  if (domain != GH_DOMAIN_COEFF && domain != GH_DOMAIN_NTT) return GH_E_INVALID_ARGUMENT;
+  if (mask->ring != key->ring) return GH_E_INVALID_ARGUMENT;
+  // validate_bdlop_key / validate_message / validate_opening_randomness
+  // (commit/src/bdlop/mod.rs:56-64, commit/src/linear.rs:270-293)
+  if (open_len == 0 || msg_len != key->msg_len || open_len != key->open_len ||
+      mask->msg_len != open_len || mask->open_len != 0)
+    return GH_E_DIMENSION_MISMATCH;
+  gh_ring* ring = key->ring;
+  gh_ctx* ctx = ring->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  const size_t wb = ring->word_bytes, d = ring->d;
+  const size_t msg_bytes = (size_t)msg_len * d * wb, open_bytes = (size_t)open_len * d * wb;
+  const size_t u_bytes = (size_t)mask->rows * d * wb, v_bytes = (size_t)key->rows * d * wb;
+  // [m | r] contiguous: v streams [A_msg | A_open] over it, u streams A_mask over r
+  void* dev;
+  TRY(ctx_staging(ctx, msg_bytes + open_bytes + u_bytes + v_bytes, &dev));
+  char* ds = (char*)dev;
+  char* du = ds + msg_bytes + open_bytes;
+  char* dv = du + u_bytes;
+  if (mem == GH_MEM_HOST) {
+    TRY(check_canonical_host(ring, message, (size_t)msg_len * d));
+    TRY(check_canonical_host(ring, randomness, (size_t)open_len * d));
+    TRY(h2d(ctx, ds, message, msg_bytes));
+    TRY(h2d(ctx, ds + msg_bytes, randomness, open_bytes));
+  } else {
+    GH_HIP(hipMemcpyAsync(ds, message, msg_bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    GH_HIP(hipMemcpyAsync(ds + msg_bytes, randomness, open_bytes, hipMemcpyDeviceToDevice,
+                          ctx->stream));
+  }
+  const bool host = mem == GH_MEM_HOST;
+  TRY(launch_commit(key, ds, msg_len + open_len, domain, host ? dv : v_out));
+  TRY(launch_commit(mask, ds + msg_bytes, open_len, domain, host ? du : u_out));
+  if (host) {
+    TRY(d2h(ctx, u_out, du, u_bytes));
+    TRY(d2h(ctx, v_out, dv, v_bytes));
+    return sync(ctx);
+  }
+  return GH_OK;
+}
+
 }  // extern "C"
 
 extern "C" {

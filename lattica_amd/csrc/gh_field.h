@@ -114,7 +114,8 @@ enum TabOffset : int {
   kTabUnscale = 320,      // d^-1                         (cyclic inverse)
   kTabUntwistMac = 384,   // psi^-i * d^-1 * R^2          (after a reduce96_r2 accumulation)
   kTabQuad = 448,         // psi^(t*(2k+1)), t = 0..3, k = 0..15 at kTabQuad + 16t + k (quad NTT)
-  kTabEntries = 512,
+  kTabUntwistMacHalf = 512,  // psi^-i * d^-1 * R^2 * 2^-1  (accumulated sums halved: garbage h)
+  kTabEntries = 576,
 };
 
 }  // namespace gh

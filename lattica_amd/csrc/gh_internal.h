@@ -149,6 +149,13 @@ int enqueue_gram_batch(gh_ring* ring, hipStream_t stream, const GramDesc* desc,
                        const GramWork* work, uint32_t nwork, const GramFin* fin, uint32_t nfin);
 int launch_check_canonical(gh_ring* ring, const void* p, size_t words, int* bad_flag_dev);
 int launch_gram(gh_ring* ring, const void* s_dev, uint32_t r, uint32_t n, int domain, void* g_dev);
+int launch_decompose(gh_ring* ring, const void* in, size_t polys, uint64_t base, uint32_t limbs,
+                     void* out);
+int launch_garbage_h(gh_ring* ring, const void* phi, const void* s, uint32_t r, uint32_t n,
+                     int domain, void* h_dev);
+// s_scratch: r x n polys of device memory for the NTT-domain witness (domain COEFF)
+int launch_amortize(gh_ring* ring, const void* c, const void* s, uint32_t r, uint32_t n,
+                    int domain, void* z_dev, void* s_scratch);
 int launch_stream_probe(gh_ctx* ctx, const void* src, void* dst, size_t bytes, int copy,
                         hipEvent_t e0, hipEvent_t e1);
 int launch_jl(gh_ring* ring, const uint8_t seed[32], uint32_t rows, const void* parts,

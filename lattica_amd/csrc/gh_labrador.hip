@@ -247,6 +247,236 @@ int launch_gram(gh_ring* ring, const void* s_dev, uint32_t r, uint32_t n, int do
 }
 
 // ---------------------------------------------------------------------------
+// LaBRADOR step around the commitment: balanced decomposition, garbage h,
+// amortization
+// ---------------------------------------------------------------------------
+
+// decompose_coeff_balanced (labrador/src/main_protocol/mod.rs:258-289), one
+// thread per coefficient; out[(poly * limbs + l) * 64 + k] as decompose_polys
+// (mod.rs:731-745). |carry| <= q/2 and base <= 2^32 keep every step in int64;
+// % and / truncate toward zero as the reference's i128 arithmetic does.
+template <class W>
+__global__ __launch_bounds__(256) void k_decompose(const W* __restrict__ in, size_t words,
+                                                   uint64_t q, int64_t base, uint32_t limbs,
+                                                   W* __restrict__ out) {
+  const int64_t half = base / 2;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t p = i >> 6, k = i & 63;
+    const uint64_t c = (uint64_t)in[i];
+    int64_t carry = c > q / 2 ? -(int64_t)(q - c) : (int64_t)c;
+    for (uint32_t l = 0; l < limbs; l++) {
+      int64_t r = carry % base;
+      if (r > half) r -= base;
+      else if (r < -half) r += base;
+      out[(p * limbs + l) * 64 + k] = (W)(r >= 0 ? (uint64_t)r : q - (uint64_t)(-r));
+      carry = (carry - r) / base;
+    }
+  }
+}
+
+// h_ij = (<phi_i, s_j> + <phi_j, s_i>) / 2 (garbage.rs:101-133): the Gram
+// sweep's 4x4 pair tiles with two inputs, both NTT domain; the halving is
+// folded into the inverse transform's table.
+__device__ __forceinline__ void garbage_tile_q32(FQ32 f, const uint32_t* __restrict__ phi,
+                                                 const uint32_t* __restrict__ s, uint32_t r,
+                                                 uint32_t n, uint32_t chunk, uint32_t ci,
+                                                 uint32_t tile, uint32_t* __restrict__ partial) {
+  const int lane = threadIdx.x;
+  const uint32_t ntile = (r + kGramTile - 1) / kGramTile;
+  uint32_t ta, tb;
+  tile_coords(tile, ntile, ta, tb);
+  const uint32_t c0 = ci * chunk;
+  uint32_t c1 = c0 + chunk;
+  if (c1 > n) c1 = n;
+  uint64_t lo[kGramTile][kGramTile];
+  uint32_t hi[kGramTile][kGramTile];
+#pragma unroll
+  for (int x = 0; x < kGramTile; x++)
+#pragma unroll
+    for (int y = 0; y < kGramTile; y++) { lo[x][y] = 0; hi[x][y] = 0; }
+  uint32_t ii[kGramTile], jj[kGramTile];
+#pragma unroll
+  for (int x = 0; x < kGramTile; x++) {
+    const uint32_t i = ta * kGramTile + x, j = tb * kGramTile + x;
+    ii[x] = i < r ? i : r - 1;
+    jj[x] = j < r ? j : r - 1;
+  }
+  for (uint32_t c = c0; c < c1; c++) {
+    uint32_t pi[kGramTile], si[kGramTile], pj[kGramTile], sj[kGramTile];
+#pragma unroll
+    for (int x = 0; x < kGramTile; x++) {
+      pi[x] = phi[((size_t)ii[x] * n + c) * 64 + lane];
+      si[x] = s[((size_t)ii[x] * n + c) * 64 + lane];
+      pj[x] = phi[((size_t)jj[x] * n + c) * 64 + lane];
+      sj[x] = s[((size_t)jj[x] * n + c) * 64 + lane];
+    }
+#pragma unroll
+    for (int x = 0; x < kGramTile; x++)
+#pragma unroll
+      for (int y = 0; y < kGramTile; y++) {
+        mac96(lo[x][y], hi[x][y], pi[x], sj[y]);
+        mac96(lo[x][y], hi[x][y], pj[y], si[x]);
+      }
+  }
+  const uint32_t npairs = r * (r + 1) / 2;
+#pragma unroll
+  for (int x = 0; x < kGramTile; x++)
+#pragma unroll
+    for (int y = 0; y < kGramTile; y++) {
+      const uint32_t i = ta * kGramTile + x, j = tb * kGramTile + y;
+      if (i < r && j < r && i <= j)
+        partial[((size_t)ci * npairs + pair_index(r, i, j)) * 64 + lane] =
+            f.reduce96_r2((uint32_t)lo[x][y], (uint32_t)(lo[x][y] >> 32), hi[x][y]);
+    }
+}
+
+__device__ __forceinline__ void garbage_tile_gold(FGold f, const uint64_t* __restrict__ phi,
+                                                  const uint64_t* __restrict__ s, uint32_t r,
+                                                  uint32_t n, uint32_t chunk, uint32_t ci,
+                                                  uint32_t tile, uint64_t* __restrict__ partial) {
+  const int lane = threadIdx.x;
+  const uint32_t ntile = (r + kGramTile - 1) / kGramTile;
+  uint32_t ta, tb;
+  tile_coords(tile, ntile, ta, tb);
+  const uint32_t c0 = ci * chunk;
+  uint32_t c1 = c0 + chunk;
+  if (c1 > n) c1 = n;
+  GoldAcc acc[kGramTile][kGramTile];
+  uint32_t ii[kGramTile], jj[kGramTile];
+#pragma unroll
+  for (int x = 0; x < kGramTile; x++) {
+    const uint32_t i = ta * kGramTile + x, j = tb * kGramTile + x;
+    ii[x] = i < r ? i : r - 1;
+    jj[x] = j < r ? j : r - 1;
+  }
+  for (uint32_t c = c0; c < c1; c++) {
+    uint64_t pi[kGramTile], si[kGramTile], pj[kGramTile], sj[kGramTile];
+#pragma unroll
+    for (int x = 0; x < kGramTile; x++) {
+      pi[x] = phi[((size_t)ii[x] * n + c) * 64 + lane];
+      si[x] = s[((size_t)ii[x] * n + c) * 64 + lane];
+      pj[x] = phi[((size_t)jj[x] * n + c) * 64 + lane];
+      sj[x] = s[((size_t)jj[x] * n + c) * 64 + lane];
+    }
+#pragma unroll
+    for (int x = 0; x < kGramTile; x++)
+#pragma unroll
+      for (int y = 0; y < kGramTile; y++) {
+        gold_mac(acc[x][y], pi[x], sj[y]);
+        gold_mac(acc[x][y], pj[y], si[x]);
+      }
+  }
+  const uint32_t npairs = r * (r + 1) / 2;
+#pragma unroll
+  for (int x = 0; x < kGramTile; x++)
+#pragma unroll
+    for (int y = 0; y < kGramTile; y++) {
+      const uint32_t i = ta * kGramTile + x, j = tb * kGramTile + y;
+      if (i < r && j < r && i <= j)
+        partial[((size_t)ci * npairs + pair_index(r, i, j)) * 64 + lane] = gold_reduce(f, acc[x][y]);
+    }
+}
+
+__global__ __launch_bounds__(64) void k_garbage_q32(FQ32 f, const uint32_t* __restrict__ phi,
+                                                    const uint32_t* __restrict__ s, uint32_t r,
+                                                    uint32_t n, uint32_t chunk,
+                                                    uint32_t* __restrict__ partial) {
+  garbage_tile_q32(f, phi, s, r, n, chunk, blockIdx.y, blockIdx.x, partial);
+}
+
+__global__ __launch_bounds__(64) void k_garbage_gold(FGold f, const uint64_t* __restrict__ phi,
+                                                     const uint64_t* __restrict__ s, uint32_t r,
+                                                     uint32_t n, uint32_t chunk,
+                                                     uint64_t* __restrict__ partial) {
+  garbage_tile_gold(f, phi, s, r, n, chunk, blockIdx.y, blockIdx.x, partial);
+}
+
+int launch_decompose(gh_ring* ring, const void* in, size_t polys, uint64_t base, uint32_t limbs,
+                     void* out) {
+  gh_ctx* ctx = ring->ctx;
+  const size_t words = polys * 64;
+  if (words == 0) return GH_OK;
+  const size_t want = (words + 255) / 256;
+  const unsigned grid = (unsigned)(want < (size_t)ctx->num_cus * 8 ? want : (size_t)ctx->num_cus * 8);
+  if (ring->kind == GH_RING_Q32)
+    hipLaunchKernelGGL((k_decompose<uint32_t>), dim3(grid), dim3(256), 0, ctx->stream,
+                       (const uint32_t*)in, words, ring->q, (int64_t)base, limbs, (uint32_t*)out);
+  else
+    hipLaunchKernelGGL((k_decompose<uint64_t>), dim3(grid), dim3(256), 0, ctx->stream,
+                       (const uint64_t*)in, words, ring->q, (int64_t)base, limbs, (uint64_t*)out);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+// phi, s: r x n polys each in `domain`; h_dev: r(r+1)/2 polys
+int launch_garbage_h(gh_ring* ring, const void* phi, const void* s, uint32_t r, uint32_t n,
+                     int domain, void* h_dev) {
+  gh_ctx* ctx = ring->ctx;
+  const size_t wb = ring->word_bytes;
+  const uint32_t tiles = gram_tiles(r);
+  const uint32_t npairs = r * (r + 1) / 2;
+  uint32_t chunk, nchunks;
+  gram_chunking(ctx->num_cus, r, n, &chunk, &nchunks);
+  const size_t in_bytes = (size_t)r * n * 64 * wb;
+  const size_t p_bytes = (size_t)nchunks * npairs * 64 * wb;
+  void* ws;
+  int st = ctx_workspace(ctx, (domain == GH_DOMAIN_COEFF ? 2 * in_bytes : 0) + p_bytes, &ws);
+  if (st) return st;
+  const void* pn = phi;
+  const void* sn = s;
+  void* partial = ws;
+  if (domain == GH_DOMAIN_COEFF) {
+    st = launch_xform(ring, kTwistedFwd, phi, ws, (size_t)r * n);
+    if (st) return st;
+    st = launch_xform(ring, kTwistedFwd, s, (char*)ws + in_bytes, (size_t)r * n);
+    if (st) return st;
+    pn = ws;
+    sn = (char*)ws + in_bytes;
+    partial = (char*)ws + 2 * in_bytes;
+  }
+  dim3 grid(tiles, nchunks);
+  if (ring->kind == GH_RING_Q32) {
+    FQ32 f{(uint32_t)ring->q, ring->qinv32};
+    hipLaunchKernelGGL(k_garbage_q32, grid, dim3(64), 0, ctx->stream, f, (const uint32_t*)pn,
+                       (const uint32_t*)sn, r, n, chunk, (uint32_t*)partial);
+    GH_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_finalize<FQ32>), dim3(npairs), dim3(1024), 0, ctx->stream, f,
+                       (const uint32_t*)ring->dev_tab, (const uint32_t*)partial, nchunks, npairs,
+                       (int)kTabUntwistMacHalf, (uint32_t*)h_dev);
+  } else {
+    hipLaunchKernelGGL(k_garbage_gold, grid, dim3(64), 0, ctx->stream, FGold{},
+                       (const uint64_t*)pn, (const uint64_t*)sn, r, n, chunk, (uint64_t*)partial);
+    GH_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_finalize<FGold>), dim3(npairs), dim3(1024), 0, ctx->stream, FGold{},
+                       (const uint64_t*)ring->dev_tab, (const uint64_t*)partial, nchunks, npairs,
+                       (int)kTabUntwistMacHalf, (uint64_t*)h_dev);
+  }
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+// z[k] = sum_i c_i * s_i[k]. The NTT-domain witness laid out [i][k][slot] is a
+// commitment key's [column][row][slot]: z = "A" c with A[k][i] = s_i[k], one
+// launch of the commitment path with c as the witness.
+int launch_amortize(gh_ring* ring, const void* c, const void* s, uint32_t r, uint32_t n,
+                    int domain, void* z_dev, void* s_scratch) {
+  const void* sn = s;
+  if (domain == GH_DOMAIN_COEFF) {
+    int st = launch_xform(ring, kTwistedFwd, s, s_scratch, (size_t)r * n);
+    if (st) return st;
+    sn = s_scratch;
+  }
+  gh_key view;
+  view.ring = ring;
+  view.rows = n;
+  view.msg_len = r;
+  view.open_len = 0;
+  view.dev_a = const_cast<void*>(sn);
+  return launch_commit(&view, c, r, domain, z_dev);
+}
+
+// ---------------------------------------------------------------------------
 // ChaCha20 (RFC 8439 section 2.3) and JL
 // ---------------------------------------------------------------------------
 

@@ -126,12 +126,14 @@ int plan_ring(uint64_t q, uint32_t d, gh_ring* ring) {
   // Montgomery radix of the device pointwise product / accumulation
   uint64_t R = kind == GH_RING_Q32 ? (1ull << 32) % q : 1;
   uint64_t R2 = mulmod(R, R, q);
+  const uint64_t inv2 = q / 2 + 1;  // q odd
   for (uint32_t i = 0; i < 64; i++) {
     uint64_t u = mulmod(powmod(ipsi, i, q), d_inv, q);
     v[kTabUntwist + i] = u;
     v[kTabUntwistMul + i] = mulmod(u, R, q);
     v[kTabUnscale + i] = d_inv;
     v[kTabUntwistMac + i] = mulmod(u, R2, q);
+    v[kTabUntwistMacHalf + i] = mulmod(mulmod(u, R2, q), inv2, q);
   }
   for (uint32_t t = 0; t < 4; t++)
     for (uint32_t k = 0; k < 16; k++) v[kTabQuad + 16 * t + k] = powmod(psi, t * (2 * k + 1), q);

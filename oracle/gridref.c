@@ -530,6 +530,90 @@ int gr_gram(uint64_t q, uint64_t len, uint32_t r, uint32_t n, const uint64_t *s,
 }
 
 /* ------------------------------------------------------------------------ */
+/* LaBRADOR step around the commitment                                        */
+/* ------------------------------------------------------------------------ */
+
+/* decompose_coeff_balanced / centered_mod / decompose_poly_balanced
+ * (labrador/src/main_protocol/mod.rs:258-320), flattened as decompose_polys
+ * (mod.rs:731-745): out[(poly * limbs + l) * len + k]. */
+int gr_decompose_balanced(uint64_t q, uint64_t len, size_t npolys, const uint64_t *in,
+                          uint64_t base, uint32_t limbs, uint64_t *out) {
+    if (base < 2 || limbs == 0) return GR_E_INVALID_PARAMETERS;
+    const __int128 half_q = (__int128)(q / 2), b = (__int128)base, half = b / 2;
+    for (size_t p = 0; p < npolys; p++) {
+        for (uint64_t k = 0; k < len; k++) {
+            __int128 carry = (__int128)in[p * len + k];
+            if (carry > half_q) carry -= (__int128)q;
+            for (uint32_t l = 0; l < limbs; l++) {
+                __int128 r = carry % b;  /* truncating, like i128 % in Rust */
+                if (r > half) r -= b;
+                else if (r < -half) r += b;
+                out[(p * limbs + l) * len + k] = r >= 0 ? (uint64_t)r : q - (uint64_t)(-r);
+                carry = (carry - r) / b;
+            }
+        }
+    }
+    return GR_OK;
+}
+
+/* compute_garbage_h (labrador/src/main_protocol/garbage.rs:101-133):
+ * h_ij = (<phi_i, s_j> + <phi_j, s_i>) * 2^-1 for 0 <= i <= j < r, row-major
+ * upper triangle; phi and s: r parts x n polys x len, coefficient domain. */
+int gr_garbage_h(uint64_t q, uint64_t len, uint32_t r, uint32_t n, const uint64_t *phi,
+                 const uint64_t *s, uint64_t *h_out) {
+    plan_t p;
+    int st = plan_build(&p, q, len, 1);
+    if (st != GR_OK) return st;
+    const field_t *f = &p.f;
+    const size_t part = (size_t)n * len;
+    uint64_t *pm = prepare(&p, phi, (size_t)r * n, 1);
+    uint64_t *sm = prepare(&p, s, (size_t)r * n, 1);
+    uint64_t *buf = (uint64_t *)malloc(sizeof(uint64_t) * len * 4);
+    uint64_t *t1 = buf, *t2 = buf + len, *scratch = buf + 2 * len;
+    const uint64_t two_inv = f_to(f, q / 2 + 1);
+    size_t idx = 0;
+    for (uint32_t i = 0; i < r; i++) {
+        for (uint32_t j = i; j < r; j++) {
+            coeff_dot(&p, pm + i * part, sm + j * part, n, t1, scratch);
+            coeff_dot(&p, pm + j * part, sm + i * part, n, t2, scratch);
+            for (uint64_t k = 0; k < len; k++)
+                h_out[idx * len + k] = f_from(f, f_mul(f, f_add(f, t1[k], t2[k]), two_inv));
+            idx++;
+        }
+    }
+    free(buf); free(pm); free(sm);
+    plan_free(&p);
+    return GR_OK;
+}
+
+/* compute_amortized_witness (labrador/src/main_protocol/amortization.rs:61-84):
+ * z[k] = sum_i c_i * s_i[k]; c: r polys, s: r parts x n polys, z: n polys. */
+int gr_amortize(uint64_t q, uint64_t len, uint32_t r, uint32_t n, const uint64_t *c,
+                const uint64_t *s, uint64_t *z_out) {
+    plan_t p;
+    int st = plan_build(&p, q, len, 1);
+    if (st != GR_OK) return st;
+    const field_t *f = &p.f;
+    uint64_t *cm = prepare(&p, c, r, 1);
+    uint64_t *sm = prepare(&p, s, (size_t)r * n, 1);
+    uint64_t *buf = (uint64_t *)malloc(sizeof(uint64_t) * len * 3);
+    uint64_t *acc = buf, *x = buf + len, *y = buf + 2 * len;
+    for (uint32_t k = 0; k < n; k++) {
+        for (uint64_t e = 0; e < len; e++) acc[e] = 0;
+        for (uint32_t i = 0; i < r; i++) {
+            memcpy(x, cm + (size_t)i * len, sizeof(uint64_t) * len);
+            memcpy(y, sm + ((size_t)i * n + k) * len, sizeof(uint64_t) * len);
+            plan_multiply(&p, x, y);
+            for (uint64_t e = 0; e < len; e++) acc[e] = f_add(f, acc[e], x[e]);
+        }
+        for (uint64_t e = 0; e < len; e++) z_out[(size_t)k * len + e] = f_from(f, acc[e]);
+    }
+    free(buf); free(cm); free(sm);
+    plan_free(&p);
+    return GR_OK;
+}
+
+/* ------------------------------------------------------------------------ */
 /* ChaCha20 (RFC 8439 section 2.3) and the JL projection                      */
 /* ------------------------------------------------------------------------ */
 

@@ -89,6 +89,21 @@ int gr_commit_coeff(uint64_t q, uint64_t len, uint32_t rows, const uint64_t *a_m
 int gr_gram(uint64_t q, uint64_t len, uint32_t r, uint32_t n, const uint64_t *s,
             uint64_t *g_out, int ntt_once);
 
+/* ---- LaBRADOR step around the commitment ---------------------------------
+ * gr_decompose_balanced: decompose_polys / decompose_poly_balanced
+ *   (main_protocol/mod.rs:258-320, :731-745): out[(poly*limbs + l)*len + k],
+ *   limbs centred in [-base/2, base/2] and stored mod q.
+ * gr_garbage_h: compute_garbage_h (main_protocol/garbage.rs:101-133),
+ *   h_ij = (<phi_i, s_j> + <phi_j, s_i>) / 2, i <= j, row-major upper triangle.
+ * gr_amortize: compute_amortized_witness (main_protocol/amortization.rs:61-84),
+ *   z[k] = sum_i c_i * s_i[k].                                                 */
+int gr_decompose_balanced(uint64_t q, uint64_t len, size_t npolys, const uint64_t *in,
+                          uint64_t base, uint32_t limbs, uint64_t *out);
+int gr_garbage_h(uint64_t q, uint64_t len, uint32_t r, uint32_t n, const uint64_t *phi,
+                 const uint64_t *s, uint64_t *h_out);
+int gr_amortize(uint64_t q, uint64_t len, uint32_t r, uint32_t n, const uint64_t *c,
+                const uint64_t *s, uint64_t *z_out);
+
 /* ---- JL projection (labrador/src/jl.rs:388-508) ---------------------------
  * parts: num_parts x cols canonical residues, col = poly*len + coeff.
  * p_modq[rows]: the reference output (sum mod q).

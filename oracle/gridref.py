@@ -63,6 +63,12 @@ def lib():
                                       ctypes.c_uint32, U64P, U64P, U64P]
         L.gr_gram.restype = i32
         L.gr_gram.argtypes = [u64, u64, ctypes.c_uint32, ctypes.c_uint32, U64P, U64P, i32]
+        L.gr_decompose_balanced.restype = i32
+        L.gr_decompose_balanced.argtypes = [u64, u64, sz, U64P, u64, ctypes.c_uint32, U64P]
+        L.gr_garbage_h.restype = i32
+        L.gr_garbage_h.argtypes = [u64, u64, ctypes.c_uint32, ctypes.c_uint32, U64P, U64P, U64P]
+        L.gr_amortize.restype = i32
+        L.gr_amortize.argtypes = [u64, u64, ctypes.c_uint32, ctypes.c_uint32, U64P, U64P, U64P]
         L.gr_jl_project.restype = i32
         L.gr_jl_project.argtypes = [u64, U8P, ctypes.c_uint32, ctypes.c_uint32, u64, u64, U64P,
                                     U64P, I64P, I8P]
@@ -211,6 +217,34 @@ def gram(q, d, s, ntt_once=True):
     r, n = s.shape[0], s.shape[1]
     out = np.zeros((r * (r + 1) // 2, d), dtype=np.uint64)
     _check(lib().gr_gram(q, d, r, n, _p(s), _p(out), 1 if ntt_once else 0), "gram")
+    return out
+
+
+def decompose_balanced(q, d, polys, base, limbs):
+    """polys: (m, d) -> (m, limbs, d) balanced base-`base` limbs stored mod q."""
+    polys = _u64(polys)
+    m = polys.shape[0]
+    out = np.zeros((m, limbs, d), dtype=np.uint64)
+    _check(lib().gr_decompose_balanced(q, d, m, _p(polys), base, limbs, _p(out)),
+           "decompose_balanced")
+    return out
+
+
+def garbage_h(q, d, phi, s):
+    """phi, s: (r, n, d) -> (r(r+1)/2, d): (<phi_i, s_j> + <phi_j, s_i>) / 2."""
+    phi, s = _u64(phi), _u64(s)
+    r, n = s.shape[0], s.shape[1]
+    out = np.zeros((r * (r + 1) // 2, d), dtype=np.uint64)
+    _check(lib().gr_garbage_h(q, d, r, n, _p(phi), _p(s), _p(out)), "garbage_h")
+    return out
+
+
+def amortize(q, d, c, s):
+    """c: (r, d), s: (r, n, d) -> z: (n, d) = sum_i c_i * s_i."""
+    c, s = _u64(c), _u64(s)
+    r, n = s.shape[0], s.shape[1]
+    out = np.zeros((n, d), dtype=np.uint64)
+    _check(lib().gr_amortize(q, d, r, n, _p(c), _p(s), _p(out)), "amortize")
     return out
 
 

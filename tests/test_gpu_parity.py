@@ -354,6 +354,60 @@ def test_commit_batch_device_many_items_all_tunings(oracle, ctx, ring_q32, ring_
         key.close()
 
 
+@pytest.mark.parametrize("name", ["q32", "goldilocks"])
+@pytest.mark.parametrize("rows,msg,opn", [(4, 3, 2), (16, 100, 37), (8, 1, 1)])
+def test_bdlop_matches_oracle(oracle, ring_q32, ring_gold, name, rows, msg, opn):
+    """u = A_mask r, v = A_msg m + A_open r (commit/src/bdlop/mod.rs:34-54), both
+    domains, against the oracle's linear commitment applied to each part."""
+    import lattica_amd as la
+    r = ring_for(name, ring_q32, ring_gold)
+    rng = np.random.default_rng(rows * 100 + msg + opn)
+    a_mask = rand_residues(rng, r.q, (rows, opn, 64))
+    a_msg = rand_residues(rng, r.q, (rows, msg, 64))
+    a_open = rand_residues(rng, r.q, (rows, opn, 64))
+    m = small_signed(rng, r.q, (msg, 64), -128, 128)
+    rr = small_signed(rng, r.q, (opn, 64), -1, 2)
+    mask = la.Key(r, a_mask)
+    key = la.Key(r, a_msg, a_open)
+    u, v = key.bdlop(mask, m, rr)
+    z = np.zeros((rows, 1, 64), dtype=np.uint64)
+    z1 = np.zeros((1, 64), dtype=np.uint64)
+    assert np.array_equal(u.astype(np.uint64), oracle.commit(r.q, 64, a_mask, z, rr, z1))
+    assert np.array_equal(v.astype(np.uint64), oracle.commit(r.q, 64, a_msg, a_open, m, rr))
+    # NTT-domain inputs give the same commitment (commit_with_opening_ntt)
+    u2, v2 = key.bdlop(mask, r.twisted_forward(m), r.twisted_forward(rr), domain=la.DOMAIN_NTT)
+    assert np.array_equal(u2, u) and np.array_equal(v2, v)
+    mask.close()
+    key.close()
+
+
+def test_bdlop_additive_homomorphism_and_errors(ring_q32):
+    """commit(m1, r1) + commit(m2, r2) = commit(m1 + m2, r1 + r2)
+    (bdlop/tests.rs:498-528), and the dimension rules."""
+    import lattica_amd as la
+    r = ring_q32
+    q = r.q
+    rng = np.random.default_rng(5)
+    mask = la.Key(r, rand_residues(rng, q, (8, 6, 64)))
+    key = la.Key(r, rand_residues(rng, q, (8, 10, 64)), rand_residues(rng, q, (8, 6, 64)))
+    m1, m2 = rand_residues(rng, q, (10, 64)), rand_residues(rng, q, (10, 64))
+    r1, r2 = rand_residues(rng, q, (6, 64)), rand_residues(rng, q, (6, 64))
+    u1, v1 = key.bdlop(mask, m1, r1)
+    u2, v2 = key.bdlop(mask, m2, r2)
+    u3, v3 = key.bdlop(mask, (m1 + m2) % np.uint64(q), (r1 + r2) % np.uint64(q))
+    add = lambda x, y: ((x.astype(np.uint64) + y.astype(np.uint64)) % np.uint64(q))  # noqa: E731
+    assert np.array_equal(add(u1, u2), u3.astype(np.uint64))
+    assert np.array_equal(add(v1, v2), v3.astype(np.uint64))
+    with pytest.raises(la.GridHipError) as e:
+        key.bdlop(mask, m1[:9], r1)
+    assert e.value.name == "DIMENSION_MISMATCH"
+    with pytest.raises(la.GridHipError) as e:
+        key.bdlop(mask, m1, r1[:5])
+    assert e.value.name == "DIMENSION_MISMATCH"
+    mask.close()
+    key.close()
+
+
 def test_inner_commit_matches_oracle(oracle, ring_q32):
     import lattica_amd as la
     r = ring_q32

@@ -272,3 +272,102 @@ def test_avx2_commit_matches_oracle(oracle, limb):
     ref = oracle.commit(q, 64, a, z, s, np.zeros((1, 64), dtype=np.uint64), key_domain=1)
     assert np.array_equal(c.result(), ref)
     c.close()
+
+
+# ---- LaBRADOR step around the commitment ---------------------------------------
+
+def _schoolbook(q, a, b):
+    """negacyclic product mod (X^len + 1), by definition."""
+    n = len(a)
+    out = [0] * n
+    for i in range(n):
+        for j in range(n):
+            k = i + j
+            if k < n:
+                out[k] = (out[k] + a[i] * b[j]) % q
+            else:
+                out[k - n] = (out[k - n] - a[i] * b[j]) % q
+    return out
+
+
+def test_decompose_balanced_reference_cases(oracle):
+    # decompose_coeff_balanced round trips (main_protocol/mod.rs:1156-1188):
+    # q = 12289, base 8, 8 limbs, all v < 256 and the near-q values
+    q, base, limbs = 12289, 8, 8
+    vals = list(range(256)) + [q - 10, q - 1, q // 2, q // 2 + 1, 0, 1]
+    vals += [0] * (-len(vals) % 4)
+    polys = np.array(vals, dtype=np.uint64).reshape(-1, 4)
+    out = oracle.decompose_balanced(q, 4, polys, base, limbs)
+    for p in range(polys.shape[0]):
+        for k in range(4):
+            centred = [int(x) if int(x) <= q // 2 else int(x) - q for x in out[p, :, k]]
+            assert all(abs(c) <= base // 2 for c in centred)
+            assert sum(c * base ** l for l, c in enumerate(centred)) % q == int(polys[p, k])
+    # the poly case of mod.rs:1190-1236: c_i = (193 i + 7) mod 12289, 8 limbs
+    poly = np.array([[(193 * i + 7) % q for i in range(64)]], dtype=np.uint64)
+    out = oracle.decompose_balanced(q, 64, poly, base, limbs)
+    for i in range(64):
+        centred = [int(x) if int(x) <= q // 2 else int(x) - q for x in out[0, :, i]]
+        assert sum(c * base ** l for l, c in enumerate(centred)) % q == int(poly[0, i])
+
+
+def test_decompose_balanced_boundaries(oracle):
+    # centered_mod (mod.rs:277-289) keeps +b/2 for a non-negative carry and -b/2
+    # for a negative one; truncating division carries the sign
+    q = 4294966657
+    cases = {100: [4, 6, 0], 15: [-1, 1, 0], q - 1: [-1, 0, 0], 8: [8, 0, 0], q - 8: [-8, 0, 0],
+             24: [8, 1, 0], q - 24: [-8, -1, 0], 0: [0, 0, 0]}
+    vals = list(cases)
+    out = oracle.decompose_balanced(q, 8, np.array([vals], dtype=np.uint64), 16, 3)
+    for k, v in enumerate(vals):
+        got = [int(x) if int(x) <= q // 2 else int(x) - q for x in out[0, :, k]]
+        assert got == cases[v], (v, got)
+
+
+def test_garbage_h_pins(oracle):
+    # zero phi -> zero (garbage.rs:218-231); phi = s -> h_ij = <s_i, s_j> (the
+    # Gram matrix from the NTT-once path); schoolbook brute force over F17, len 4
+    rng = np.random.default_rng(3)
+    q = 4294966657
+    s = rng.integers(0, q, size=(3, 5, 64), dtype=np.uint64)
+    assert not oracle.garbage_h(q, 64, np.zeros_like(s), s).any()
+    assert np.array_equal(oracle.garbage_h(q, 64, s, s), oracle.gram(q, 64, s, ntt_once=True))
+    q, d, r, n = 17, 4, 2, 3
+    phi = rng.integers(0, q, size=(r, n, d), dtype=np.uint64)
+    s = rng.integers(0, q, size=(r, n, d), dtype=np.uint64)
+    inv2 = (q + 1) // 2
+    exp = []
+    for i in range(r):
+        for j in range(i, r):
+            acc = [0] * d
+            for k in range(n):
+                t1 = _schoolbook(q, [int(x) for x in phi[i, k]], [int(x) for x in s[j, k]])
+                t2 = _schoolbook(q, [int(x) for x in phi[j, k]], [int(x) for x in s[i, k]])
+                acc = [(a + b + c) % q for a, b, c in zip(acc, t1, t2)]
+            exp.append([a * inv2 % q for a in acc])
+    assert oracle.garbage_h(q, d, phi, s).tolist() == exp
+
+
+def test_amortize_pins(oracle):
+    # zero witness -> zero (amortization.rs:172-196); c_i = 1 -> z = sum_i s_i;
+    # schoolbook brute force over F17, len 4
+    rng = np.random.default_rng(4)
+    q = 4294966657
+    s = rng.integers(0, q, size=(4, 6, 64), dtype=np.uint64)
+    c = rng.integers(0, q, size=(4, 64), dtype=np.uint64)
+    assert not oracle.amortize(q, 64, c, np.zeros_like(s)).any()
+    one = np.zeros((4, 64), dtype=np.uint64)
+    one[:, 0] = 1
+    exp = [[sum(int(s[i, k, e]) for i in range(4)) % q for e in range(64)] for k in range(6)]
+    assert oracle.amortize(q, 64, one, s).tolist() == exp
+    q, d, r, n = 17, 4, 3, 2
+    s = rng.integers(0, q, size=(r, n, d), dtype=np.uint64)
+    c = rng.integers(0, q, size=(r, d), dtype=np.uint64)
+    exp = []
+    for k in range(n):
+        acc = [0] * d
+        for i in range(r):
+            t = _schoolbook(q, [int(x) for x in c[i]], [int(x) for x in s[i, k]])
+            acc = [(a + b) % q for a, b in zip(acc, t)]
+        exp.append(acc)
+    assert oracle.amortize(q, d, c, s).tolist() == exp
