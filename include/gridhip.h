@@ -65,7 +65,10 @@ typedef enum gh_status {
   GH_E_DEVICE = -23,           /* HIP runtime error */
   GH_E_OOM = -24,              /* device allocation failed */
   GH_E_BUSY = -25,             /* server queue full */
-  GH_E_NO_DEVICE = -26
+  GH_E_NO_DEVICE = -26,
+  /* SerializationError (serialize/src/lib.rs:28-34) */
+  GH_E_UNEXPECTED_END = -30, /* encoded data ends before what its headers announce */
+  GH_E_INVALID_DATA = -31    /* element >= q, shape overflow, trailing bytes, inconsistent key */
 } gh_status;
 
 #define GH_MEM_HOST 0
@@ -150,7 +153,7 @@ typedef struct gh_ring_info {
   uint32_t d;
   uint32_t word_bytes; /* 4 or 8 */
   uint32_t kind;       /* GH_RING_Q32 / GH_RING_GOLDILOCKS */
-  uint32_t reserved;
+  uint32_t wire_bytes; /* bytes of one encoded element: ceil(bits(q - 1) / 8) */
   uint64_t generator;  /* find_generator */
   uint64_t psi;        /* root_of_unity(2d): the twist */
   uint64_t omega;      /* root_of_unity(d) = psi^2 */
@@ -305,6 +308,77 @@ GH_API int gh_jl_project(gh_ring *ring, const uint8_t seed[32], uint32_t rows, c
                          uint32_t num_parts, uint64_t cols, void *p_modq, int64_t *p_exact,
                          int8_t *flat, int mem);
 
+/* ---- canonical wire formats ----------------------------------------------------
+ * The reference's canonical encodings (serialize/src/lib.rs:1-7), byte for byte:
+ *   element     PrimeField: gh_ring_info.wire_bytes = ceil(bits(q - 1) / 8) bytes,
+ *               little-endian, value < q (algebra/src/arith/prime.rs:54-58, :690-721);
+ *   polynomial  d elements, no header: CyclotomicPolyRing coefficients
+ *               (poly/ring.rs:469-505) or TwistedNttPoly evaluations in the
+ *               library's slot order (poly/twisted_ntt.rs:647-685).
+ * Containers, by `kind`; an item is a polynomial except for GH_WIRE_ELEMS:
+ *   GH_WIRE_POLYS    `rows` polynomials back to back ([T; N], struct fields);
+ *   GH_WIRE_RINGVEC  RingVec: u64 count, then the items (lattice/types.rs:183-219);
+ *   GH_WIRE_RINGMAT  RingMat: u64 rows, u64 cols, then rows x cols items row-major
+ *                    (lattice/types.rs:440-490);
+ *   GH_WIRE_VEC      Vec<polynomial>: u32 count, then the items (serialize/src/lib.rs:210-245);
+ *   GH_WIRE_ELEMS    Vec<element>: u32 count, then bare field elements.
+ * Shapes are rows x cols items; cols is 1 for every kind but GH_WIRE_RINGMAT.
+ * Items in memory are ring words (rows*cols*d of them, or rows for ELEMS) in
+ * `mem`; encoded bytes are always host memory. Elements are packed and unpacked
+ * by device kernels; headers are written and read on the host.
+ *
+ * gh_wire_size:   encoded bytes of a shape; INVALID_DATA when it overflows or a
+ *                 Vec length exceeds u32.
+ * gh_wire_encode: items at `src` -> `out` (at most out_cap bytes); *written is
+ *                 the encoded size (out NULL: size only). Device items are
+ *                 trusted canonical; host items are checked (NOT_CANONICAL).
+ * gh_wire_peek:   the header of `bytes`: rows, cols and the total encoded size
+ *                 (GH_WIRE_POLYS has no header: *rows is read as the count).
+ * gh_wire_decode: `bytes` -> items at `dst`, for the shape in the header, which
+ *                 must be rows x cols (else DIMENSION_MISMATCH; for
+ *                 GH_WIRE_POLYS rows is the count). Errors in the reference's
+ *                 order: INVALID_DATA for an element >= q or a shape that
+ *                 overflows; UNEXPECTED_END when the bytes end early, after the
+ *                 elements present were checked (every whole element in the
+ *                 coefficient domain, as CyclotomicPolyRing decodes element by
+ *                 element; whole polynomials only in the NTT domain, as
+ *                 TwistedNttPoly checks its length first). `domain` names what
+ *                 the polynomials hold (ignored for ELEMS). consumed NULL asks
+ *                 for exact consumption (deserialize_exact, serialize/src/lib.rs:79-83):
+ *                 trailing bytes give INVALID_DATA; else *consumed is the bytes
+ *                 used. On error a device `dst` may be partly written; a host
+ *                 `dst` is untouched. */
+#define GH_WIRE_POLYS 0
+#define GH_WIRE_RINGVEC 1
+#define GH_WIRE_RINGMAT 2
+#define GH_WIRE_VEC 3
+#define GH_WIRE_ELEMS 4
+GH_API int gh_wire_size(const gh_ring *ring, int kind, uint64_t rows, uint64_t cols,
+                        uint64_t *bytes);
+GH_API int gh_wire_encode(gh_ring *ring, int kind, uint64_t rows, uint64_t cols, const void *src,
+                          int mem, void *out, uint64_t out_cap, uint64_t *written);
+GH_API int gh_wire_peek(const gh_ring *ring, int kind, const void *bytes, uint64_t len,
+                        uint64_t *rows, uint64_t *cols, uint64_t *total);
+GH_API int gh_wire_decode(gh_ring *ring, int kind, int domain, const void *bytes, uint64_t len,
+                          uint64_t rows, uint64_t cols, void *dst, int mem, uint64_t *consumed);
+
+/* ---- prepared-key persistence ---------------------------------------------------
+ * The encoding of LinearCommitmentKey (commit/src/linear.rs:169-186): RingMat
+ * a_msg (rows x msg_len) then RingMat a_open (rows x open_len).
+ * gh_key_save writes the resident key in `domain`: GH_DOMAIN_NTT gives the
+ * TwistedNttPoly matrices a prepared key holds (PreparedLinearCommitmentKey,
+ * linear.rs:162-167, :377-395), GH_DOMAIN_COEFF the coefficient-domain key
+ * (finish_twisted_ring_mat, poly/twisted_ntt.rs:318-330). out NULL: size only.
+ * gh_key_load rebuilds a resident key from either form, as gh_key_create
+ * would from the matrices (a coefficient-domain key is transformed once).
+ * Decoding errors as gh_wire_decode; then row counts that differ give
+ * INVALID_DATA (deserialize_and_validate, linear.rs:188-192, serialize/src/lib.rs:106-110),
+ * and an empty message block or row count INVALID_PARAMETERS
+ * (CommitmentDimensions::validate, linear.rs:45-59). */
+GH_API int gh_key_save(gh_key *key, int domain, void *out, uint64_t out_cap, uint64_t *written);
+GH_API int gh_key_load(gh_ring *ring, const void *bytes, uint64_t len, int domain,
+                       uint64_t *consumed, gh_key **out);
+
 /* ---- serving loop -------------------------------------------------------------
  * Continuous batching of commit and Gram requests on one device. No reference
  * counterpart: the reference computes one commitment per call on the calling
@@ -352,7 +426,32 @@ typedef struct gh_request {
   const void *witness; /* r x n polynomials, coefficient domain */
   void *t_out;         /* GH_OP_COMMIT: r x rows polynomials */
   void *g_out;         /* GH_OP_GRAM: r(r+1)/2 polynomials */
+  uint32_t wire;       /* 1: witness, t_out and g_out hold canonical encodings */
+  uint32_t reserved;
+  uint64_t witness_len; /* wire: bytes at witness */
+  uint64_t t_cap;       /* wire: bytes available at t_out */
+  uint64_t g_cap;       /* wire: bytes available at g_out */
 } gh_request;
+
+/* Wire requests (wire = 1, mem = GH_MEM_HOST) carry the canonical encodings
+ * of the LaBRADOR step's objects, so a server front end can pass network bytes
+ * through unchanged:
+ *   witness  LabradorWitness::parts, a Vec<Vec<poly>> (labrador/src/relation.rs:113-115):
+ *            u32 r, then per part u32 n and n coefficient-domain polynomials;
+ *            every part has the same n; r and n of the request are 0 (read
+ *            from the witness) or must match it;
+ *   t_out    the inner commitments t_vecs, a Vec<RingVec<poly>>
+ *            (labrador/src/main_protocol/mod.rs:702-726): u32 r, then per part
+ *            u64 rows and rows polynomials: 4 + r * (8 + rows * d * wire_bytes) bytes;
+ *   g_out    the Gram polynomials, a Vec<poly> (garbage.rs:76-93): u32 r(r+1)/2,
+ *            then the polynomials: 4 + r(r+1)/2 * d * wire_bytes bytes.
+ * The ring's element width must equal its word width (the 32-bit prime
+ * 4294966657 and Goldilocks do), else GH_E_INVALID_PARAMETERS. Submit checks
+ * the structure: UNEXPECTED_END, INVALID_DATA for trailing bytes,
+ * DIMENSION_MISMATCH for parts of different lengths, INVALID_ARGUMENT for
+ * outputs too small. Elements are unpacked and checked on the device: one >= q
+ * completes the request with GH_E_INVALID_DATA and leaves its outputs
+ * unwritten. */
 
 typedef struct gh_server_stats {
   uint64_t submitted, completed, failed;

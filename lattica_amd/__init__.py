@@ -17,6 +17,7 @@ __all__ = [
     "GridHipError", "lib", "library_path", "Context", "Ring", "Key", "DeviceBuffer", "Server",
     "BatchPlan",
     "OP_COMMIT", "OP_GRAM",
+    "WIRE_POLYS", "WIRE_RINGVEC", "WIRE_RINGMAT", "WIRE_VEC", "WIRE_ELEMS",
     "Q32", "GOLDILOCKS", "MEM_HOST", "MEM_DEVICE", "DOMAIN_COEFF", "DOMAIN_NTT", "STATUS",
 ]
 
@@ -30,6 +31,7 @@ GOLDILOCKS = 0xFFFFFFFF00000001
 MEM_HOST, MEM_DEVICE = 0, 1
 OP_COMMIT, OP_GRAM = 1, 2
 DOMAIN_COEFF, DOMAIN_NTT = 0, 1
+WIRE_POLYS, WIRE_RINGVEC, WIRE_RINGMAT, WIRE_VEC, WIRE_ELEMS = 0, 1, 2, 3, 4
 
 STATUS = {
     0: "OK",
@@ -37,7 +39,7 @@ STATUS = {
     -10: "INVALID_PARAMETERS", -11: "DIMENSION_MISMATCH", -12: "OPENING_NORM_EXCEEDED",
     -13: "INVALID_MESSAGE_ENCODING", -14: "INVALID_OPENING",
     -20: "INVALID_ARGUMENT", -21: "NOT_CANONICAL", -22: "UNSUPPORTED_MODULUS", -23: "DEVICE",
-    -24: "OOM", -25: "BUSY", -26: "NO_DEVICE",
+    -24: "OOM", -25: "BUSY", -26: "NO_DEVICE", -30: "UNEXPECTED_END", -31: "INVALID_DATA",
 }
 
 # every symbol include/gridhip.h declares
@@ -53,7 +55,8 @@ EXPORTS = (
     "gh_server_create", "gh_server_destroy", "gh_server_submit", "gh_server_wait",
     "gh_server_poll", "gh_server_query", "gh_batch_plan_create", "gh_batch_plan_launch",
     "gh_batch_plan_destroy", "gh_bdlop_commit", "gh_decompose_balanced", "gh_garbage_h",
-    "gh_amortize",
+    "gh_amortize", "gh_wire_size", "gh_wire_encode", "gh_wire_peek", "gh_wire_decode",
+    "gh_key_save", "gh_key_load",
 )
 
 
@@ -68,7 +71,7 @@ class GridHipError(RuntimeError):
 class RingInfo(ctypes.Structure):
     _fields_ = [
         ("q", ctypes.c_uint64), ("d", ctypes.c_uint32), ("word_bytes", ctypes.c_uint32),
-        ("kind", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("generator", ctypes.c_uint64),
+        ("kind", ctypes.c_uint32), ("wire_bytes", ctypes.c_uint32), ("generator", ctypes.c_uint64),
         ("psi", ctypes.c_uint64), ("omega", ctypes.c_uint64), ("d_inv", ctypes.c_uint64),
     ]
 
@@ -88,7 +91,9 @@ class Request(ctypes.Structure):
     _fields_ = [("ops", ctypes.c_uint32), ("r", ctypes.c_uint32), ("n", ctypes.c_uint32),
                 ("mem", ctypes.c_int32), ("ring", ctypes.c_void_p), ("key", ctypes.c_void_p),
                 ("witness", ctypes.c_void_p), ("t_out", ctypes.c_void_p),
-                ("g_out", ctypes.c_void_p)]
+                ("g_out", ctypes.c_void_p), ("wire", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("witness_len", ctypes.c_uint64),
+                ("t_cap", ctypes.c_uint64), ("g_cap", ctypes.c_uint64)]
 
 
 class ServerStats(ctypes.Structure):
@@ -169,6 +174,13 @@ def lib():
         "gh_decompose_balanced": (i32, [vp, vp, sz, u64, u32, vp, i32]),
         "gh_garbage_h": (i32, [vp, vp, vp, u32, u32, i32, vp, i32]),
         "gh_amortize": (i32, [vp, vp, vp, u32, u32, i32, vp, i32]),
+        "gh_wire_size": (i32, [vp, i32, u64, u64, ctypes.POINTER(u64)]),
+        "gh_wire_encode": (i32, [vp, i32, u64, u64, vp, i32, vp, u64, ctypes.POINTER(u64)]),
+        "gh_wire_peek": (i32, [vp, i32, vp, u64, ctypes.POINTER(u64), ctypes.POINTER(u64),
+                               ctypes.POINTER(u64)]),
+        "gh_wire_decode": (i32, [vp, i32, i32, vp, u64, u64, u64, vp, i32, ctypes.POINTER(u64)]),
+        "gh_key_save": (i32, [vp, i32, vp, u64, ctypes.POINTER(u64)]),
+        "gh_key_load": (i32, [vp, vp, u64, i32, ctypes.POINTER(u64), ctypes.POINTER(vp)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -405,6 +417,55 @@ class Ring:
                "gh_jl_project")
         return pm, pe, fl
 
+    # ---- canonical wire formats (gh_wire_*) ----
+    def wire_size(self, kind: int, rows: int, cols: int = 1) -> int:
+        out = ctypes.c_uint64()
+        _check(lib().gh_wire_size(self.h, kind, rows, cols, ctypes.byref(out)), "gh_wire_size")
+        return out.value
+
+    def wire_encode(self, kind: int, items, rows: int = None, cols: int = 1) -> bytes:
+        """Ring words (..., d) or, for WIRE_ELEMS, (rows,) -> encoded bytes."""
+        v = self.words(items)
+        if rows is None:
+            rows = v.size // (1 if kind == WIRE_ELEMS else self.d) // cols
+        size = self.wire_size(kind, rows, cols)
+        out = np.zeros(max(1, size), dtype=np.uint8)
+        wr = ctypes.c_uint64()
+        _check(lib().gh_wire_encode(self.h, kind, rows, cols, _ptr(v) if v.size else None,
+                                    MEM_HOST, _ptr(out), size, ctypes.byref(wr)),
+               "gh_wire_encode")
+        return out[:wr.value].tobytes()
+
+    def wire_peek(self, kind: int, data: bytes, rows: int = 0):
+        """-> (rows, cols, total_bytes) from the header."""
+        buf = np.frombuffer(data, dtype=np.uint8) if len(data) else np.zeros(1, np.uint8)
+        r, c, t = ctypes.c_uint64(rows), ctypes.c_uint64(1), ctypes.c_uint64()
+        _check(lib().gh_wire_peek(self.h, kind, _ptr(buf), len(data), ctypes.byref(r),
+                                  ctypes.byref(c), ctypes.byref(t)), "gh_wire_peek")
+        return r.value, c.value, t.value
+
+    def wire_decode(self, kind: int, data: bytes, domain=DOMAIN_COEFF, rows: int = None,
+                    cols: int = None, exact: bool = True):
+        """Encoded bytes -> ring words, shape (rows*cols, d) or (rows,) for WIRE_ELEMS.
+        Returns (words, consumed); exact=True asks for deserialize_exact."""
+        if rows is None or cols is None:
+            pr, pc, _ = self.wire_peek(kind, data, rows or 0)
+            rows = pr if rows is None else rows
+            cols = pc if cols is None else cols
+        per = 1 if kind == WIRE_ELEMS else self.d
+        out = np.zeros((rows * cols * per,), dtype=self.dtype)
+        buf = np.frombuffer(data, dtype=np.uint8) if len(data) else np.zeros(1, np.uint8)
+        used = ctypes.c_uint64()
+        _check(lib().gh_wire_decode(self.h, kind, domain, _ptr(buf), len(data), rows, cols,
+                                    _ptr(out) if out.size else None, MEM_HOST,
+                                    None if exact else ctypes.byref(used)), "gh_wire_decode")
+        shape = (rows,) if kind == WIRE_ELEMS else (rows * cols, self.d)
+        return out.reshape(shape), (len(data) if exact else used.value)
+
+    def load_key(self, data: bytes, domain=DOMAIN_NTT, exact: bool = True) -> "Key":
+        """gh_key_load: a resident key from LinearCommitmentKey bytes."""
+        return Key.load(self, data, domain, exact)
+
     def close(self):
         if self.h:
             lib().gh_ring_destroy(self.h)
@@ -489,6 +550,32 @@ class Key:
     def batch_plan(self, witnesses, lens, outs) -> "BatchPlan":
         """gh_batch_plan over device-memory items (DeviceBuffers or addresses)."""
         return BatchPlan(self, witnesses, lens, outs)
+
+    def save(self, domain=DOMAIN_NTT) -> bytes:
+        """gh_key_save: LinearCommitmentKey bytes of this key in `domain`."""
+        size = ctypes.c_uint64()
+        _check(lib().gh_key_save(self.h, domain, None, 0, ctypes.byref(size)), "gh_key_save")
+        out = np.zeros(max(1, size.value), dtype=np.uint8)
+        _check(lib().gh_key_save(self.h, domain, _ptr(out), size.value, ctypes.byref(size)),
+               "gh_key_save")
+        return out[:size.value].tobytes()
+
+    @classmethod
+    def load(cls, ring: Ring, data: bytes, domain=DOMAIN_NTT, exact: bool = True) -> "Key":
+        buf = np.frombuffer(data, dtype=np.uint8) if len(data) else np.zeros(1, np.uint8)
+        h = ctypes.c_void_p()
+        used = ctypes.c_uint64()
+        _check(lib().gh_key_load(ring.h, _ptr(buf), len(data), domain,
+                                 None if exact else ctypes.byref(used), ctypes.byref(h)),
+               "gh_key_load")
+        self = cls.__new__(cls)
+        self.h, self.ring = h, ring
+        rows, ml, ol = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib().gh_key_dims(h, ctypes.byref(rows), ctypes.byref(ml), ctypes.byref(ol)),
+               "gh_key_dims")
+        self.rows, self.msg_len, self.open_len = rows.value, ml.value, ol.value
+        self.consumed = len(data) if exact else used.value
+        return self
 
     def inner_commit(self, s, domain=DOMAIN_COEFF):
         ring = self.ring
@@ -582,13 +669,51 @@ class Server:
             self._held[tk.value] = held
         return tk.value
 
+    def submit_wire(self, ops, ring: Ring, witness: bytes, key: "Key" = None, r: int = 0,
+                    n: int = 0) -> int:
+        """A wire request: witness is LabradorWitness::parts encoded as Vec<Vec<poly>>;
+        wait() returns the encoded t_vecs / Gram polynomials as bytes."""
+        req = Request()
+        req.ops, req.r, req.n, req.mem = ops, r, n, MEM_HOST
+        req.ring = ring.h.value
+        req.key = key.h.value if key is not None else None
+        w = np.frombuffer(witness, dtype=np.uint8) if len(witness) else np.zeros(1, np.uint8)
+        if len(witness) >= 4:  # output sizes from the part count; the library checks the rest
+            parts = int.from_bytes(witness[:4], "little")
+        else:
+            parts = 0
+        eb = ring.info.wire_bytes
+        t_cap = 4 + parts * (8 + key.rows * ring.d * eb) if (ops & OP_COMMIT and key) else 0
+        g_cap = 4 + parts * (parts + 1) // 2 * ring.d * eb if ops & OP_GRAM else 0
+        t = np.zeros(max(1, t_cap), dtype=np.uint8) if ops & OP_COMMIT else None
+        g = np.zeros(max(1, g_cap), dtype=np.uint8) if ops & OP_GRAM else None
+        req.witness = w.ctypes.data
+        req.witness_len = len(witness)
+        req.wire = 1
+        req.t_out = t.ctypes.data if t is not None else None
+        req.g_out = g.ctypes.data if g is not None else None
+        req.t_cap, req.g_cap = t_cap, g_cap
+        tk = ctypes.c_uint64()
+        _check(lib().gh_server_submit(self.h, ctypes.byref(req), ctypes.byref(tk)),
+               "gh_server_submit")
+        self._held[tk.value] = (w, t[:t_cap].view() if t is not None else None,
+                                g[:g_cap].view() if g is not None else None, True)
+        return tk.value
+
+    def _outputs(self, ticket):
+        held = self._held.pop(ticket, (None, None, None))
+        if len(held) == 4:  # wire request: encoded bytes
+            return (held[1].tobytes() if held[1] is not None else None,
+                    held[2].tobytes() if held[2] is not None else None)
+        return held[1], held[2]
+
     def wait(self, ticket: int):
         """-> (status, latency_us, t_out, g_out); outputs are None for device requests."""
         st, lat = ctypes.c_int(0), ctypes.c_double(0)
         _check(lib().gh_server_wait(self.h, ticket, ctypes.byref(st), ctypes.byref(lat)),
                "gh_server_wait")
-        held = self._held.pop(ticket, (None, None, None))
-        return st.value, lat.value, held[1], held[2]
+        t, g = self._outputs(ticket)
+        return st.value, lat.value, t, g
 
     def poll(self, ticket: int):
         """-> None while pending, else as wait()."""
@@ -597,8 +722,8 @@ class Server:
                                     ctypes.byref(lat)), "gh_server_poll")
         if not done.value:
             return None
-        held = self._held.pop(ticket, (None, None, None))
-        return st.value, lat.value, held[1], held[2]
+        t, g = self._outputs(ticket)
+        return st.value, lat.value, t, g
 
     def stats(self) -> dict:
         s = ServerStats()

@@ -421,6 +421,7 @@ GH_API int gh_ring_query(const gh_ring* ring, gh_ring_info* out) {
   out->d = ring->d;
   out->word_bytes = ring->word_bytes;
   out->kind = (uint32_t)ring->kind;
+  out->wire_bytes = ring->wire_bytes;
   out->generator = ring->generator;
   out->psi = ring->psi;
   out->omega = ring->omega;
@@ -493,6 +494,48 @@ GH_API int gh_pointwise_mul(gh_ring* ring, const void* a, const void* b, void* o
   return binary_entry(ring, a, b, out, count, mem, false);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Builds a resident key from device sources (context lock held): the
+// [col][row][slot] layout of both blocks, then, for a coefficient-domain key,
+// prepare_poly_matrix (linear.rs:345-359): one forward transform; the layout
+// only permutes whole polynomials.
+int key_create_dev(gh_ring* ring, uint32_t rows, const void* src_msg, uint32_t msg_len,
+                   const void* src_open, uint32_t open_len, int domain, gh_key** out) {
+  gh_ctx* ctx = ring->ctx;
+  const uint32_t cols = msg_len + open_len;
+  const size_t key_bytes = (size_t)rows * cols * ring->d * ring->word_bytes;
+  gh_key* key = new (std::nothrow) gh_key();
+  if (!key) return GH_E_OOM;
+  key->ring = ring;
+  key->rows = rows;
+  key->msg_len = msg_len;
+  key->open_len = open_len;
+  if (hipMalloc(&key->dev_a, key_bytes) != hipSuccess) {
+    delete key;
+    return GH_E_OOM;
+  }
+  int st = launch_key_layout(ring, src_msg, rows, msg_len, 0, cols, key->dev_a, domain);
+  if (st == GH_OK && open_len)
+    st = launch_key_layout(ring, src_open, rows, open_len, msg_len, cols, key->dev_a, domain);
+  if (st == GH_OK && domain == GH_DOMAIN_COEFF)
+    st = launch_xform(ring, kTwistedFwd, key->dev_a, key->dev_a, (size_t)rows * cols);
+  if (st == GH_OK) st = sync(ctx);
+  if (st != GH_OK) {
+    hipFree(key->dev_a);
+    delete key;
+    return st;
+  }
+  *out = key;
+  return GH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 GH_API int gh_key_create(gh_ring* ring, uint32_t rows, const void* a_msg, uint32_t msg_len,
                          const void* a_open, uint32_t open_len, int domain, int mem, gh_key** out) {
   if (!ring || !out) return GH_E_INVALID_ARGUMENT;
@@ -509,49 +552,21 @@ GH_API int gh_key_create(gh_ring* ring, uint32_t rows, const void* a_msg, uint32
   const size_t wb = ring->word_bytes, d = ring->d;
   const uint32_t cols = msg_len + open_len;
   if (cols < msg_len) return GH_E_INVALID_PARAMETERS;
-  const size_t key_bytes = (size_t)rows * cols * d * wb;
   const size_t msg_bytes = (size_t)rows * msg_len * d * wb;
   const size_t open_bytes = (size_t)rows * open_len * d * wb;
-  if (mem == GH_MEM_HOST) {
-    TRY(check_canonical_host(ring, a_msg, (size_t)rows * msg_len * d));
-    if (open_len) TRY(check_canonical_host(ring, a_open, (size_t)rows * open_len * d));
-  }
-  gh_key* key = new (std::nothrow) gh_key();
-  if (!key) return GH_E_OOM;
-  key->ring = ring;
-  key->rows = rows;
-  key->msg_len = msg_len;
-  key->open_len = open_len;
-  if (hipMalloc(&key->dev_a, key_bytes) != hipSuccess) {
-    delete key;
-    return GH_E_OOM;
-  }
-  int st = GH_OK;
   const void* src_msg = a_msg;
   const void* src_open = a_open;
   if (mem == GH_MEM_HOST) {
+    TRY(check_canonical_host(ring, a_msg, (size_t)rows * msg_len * d));
+    if (open_len) TRY(check_canonical_host(ring, a_open, (size_t)rows * open_len * d));
     void* dev;
-    st = ctx_staging(ctx, msg_bytes + open_bytes, &dev);
-    if (st == GH_OK) st = h2d(ctx, dev, a_msg, msg_bytes);
-    if (st == GH_OK && open_len) st = h2d(ctx, (char*)dev + msg_bytes, a_open, open_bytes);
+    TRY(ctx_staging(ctx, msg_bytes + open_bytes, &dev));
+    TRY(h2d(ctx, dev, a_msg, msg_bytes));
+    if (open_len) TRY(h2d(ctx, (char*)dev + msg_bytes, a_open, open_bytes));
     src_msg = dev;
     src_open = (char*)dev + msg_bytes;
   }
-  if (st == GH_OK) st = launch_key_layout(ring, src_msg, rows, msg_len, 0, cols, key->dev_a, domain);
-  if (st == GH_OK && open_len)
-    st = launch_key_layout(ring, src_open, rows, open_len, msg_len, cols, key->dev_a, domain);
-  // prepare_poly_matrix (linear.rs:345-359): a coefficient-domain key is
-  // transformed once, here; the layout only permutes whole polynomials.
-  if (st == GH_OK && domain == GH_DOMAIN_COEFF)
-    st = launch_xform(ring, kTwistedFwd, key->dev_a, key->dev_a, (size_t)rows * cols);
-  if (st == GH_OK) st = sync(ctx);
-  if (st != GH_OK) {
-    hipFree(key->dev_a);
-    delete key;
-    return st;
-  }
-  *out = key;
-  return GH_OK;
+  return key_create_dev(ring, rows, src_msg, msg_len, src_open, open_len, domain, out);
 }
 
 GH_API int gh_key_destroy(gh_key* key) {
@@ -910,6 +925,347 @@ GH_API int gh_jl_project(gh_ring* ring, const uint8_t seed[32], uint32_t rows, c
   if (p_exact) TRY(d2h(ctx, p_exact, de, e_bytes));
   if (flat) TRY(d2h(ctx, flat, df, f_bytes));
   return sync(ctx);
+}
+
+}  // extern "C"
+
+// ---- canonical wire formats -------------------------------------------------
+// Headers are read and written here; element runs move through the device
+// pack/unpack kernels (gh_wire.hip).
+
+namespace {
+
+size_t al(size_t b) { return (b + 255) & ~(size_t)255; }
+
+struct WireKind {
+  uint32_t hdr;   // header bytes
+  bool per_poly;  // items are polynomials (else bare field elements)
+};
+
+int wire_kind(int kind, WireKind* k) {
+  switch (kind) {
+    case GH_WIRE_POLYS: *k = {0, true}; return GH_OK;
+    case GH_WIRE_RINGVEC: *k = {8, true}; return GH_OK;
+    case GH_WIRE_RINGMAT: *k = {16, true}; return GH_OK;
+    case GH_WIRE_VEC: *k = {4, true}; return GH_OK;
+    case GH_WIRE_ELEMS: *k = {4, false}; return GH_OK;
+    default: return GH_E_INVALID_ARGUMENT;
+  }
+}
+
+uint64_t get_le(const uint8_t* p, int n) {
+  uint64_t v = 0;
+  for (int i = 0; i < n; i++) v |= (uint64_t)p[i] << (8 * i);
+  return v;
+}
+void put_le(uint8_t* p, uint64_t v, int n) {
+  for (int i = 0; i < n; i++) p[i] = (uint8_t)(v >> (8 * i));
+}
+
+// elements and payload bytes of rows x cols items; GH_E_INVALID_DATA when the
+// shape overflows (RingMat's checked_mul, lattice/types.rs:464-466), as a
+// 63-bit byte count bounds what the reference could reserve
+int wire_shape(const gh_ring* ring, const WireKind& k, uint64_t rows, uint64_t cols,
+               uint64_t* elems, uint64_t* bytes) {
+  const unsigned __int128 items = (unsigned __int128)rows * cols;
+  const unsigned __int128 e = items * (k.per_poly ? ring->d : 1);
+  const unsigned __int128 b = e * ring->wire_bytes + k.hdr;
+  if (b >> 63) return GH_E_INVALID_DATA;
+  *elems = (uint64_t)e;
+  *bytes = (uint64_t)b - k.hdr;
+  return GH_OK;
+}
+
+void write_header(int kind, uint8_t* out, uint64_t rows, uint64_t cols) {
+  switch (kind) {
+    case GH_WIRE_RINGVEC: put_le(out, rows, 8); break;
+    case GH_WIRE_RINGMAT: put_le(out, rows, 8); put_le(out + 8, cols, 8); break;
+    case GH_WIRE_VEC:
+    case GH_WIRE_ELEMS: put_le(out, rows, 4); break;
+    default: break;
+  }
+}
+
+// Reads the header at p; rows and cols come back (GH_WIRE_POLYS keeps *rows).
+int read_header(int kind, const uint8_t* p, uint64_t len, uint64_t* rows, uint64_t* cols) {
+  WireKind k;
+  TRY(wire_kind(kind, &k));
+  if (len < k.hdr) return GH_E_UNEXPECTED_END;
+  switch (kind) {
+    case GH_WIRE_RINGVEC: *rows = get_le(p, 8); *cols = 1; break;
+    case GH_WIRE_RINGMAT: *rows = get_le(p, 8); *cols = get_le(p + 8, 8); break;
+    case GH_WIRE_VEC:
+    case GH_WIRE_ELEMS: *rows = get_le(p, 4); *cols = 1; break;
+    default: *cols = 1; break;
+  }
+  return GH_OK;
+}
+
+// A run of `count` elements at byte offset `off` of an encoded image, to or
+// from `words` (device; null: check only).
+struct Run {
+  uint64_t off, count;
+  void* words;
+};
+
+// Elements a truncated payload of `avail` bytes still checks before the
+// reference reports UnexpectedEnd: every whole element of a coefficient-domain
+// polynomial (CyclotomicPolyRing decodes element by element, poly/ring.rs:488-504)
+// or field-element vector, but only whole polynomials in the NTT domain
+// (TwistedNttPoly checks its length first, poly/twisted_ntt.rs:663-684).
+uint64_t truncated_prefix(const gh_ring* ring, const WireKind& k, int domain, uint64_t avail) {
+  const uint64_t n = avail / ring->wire_bytes;
+  return (k.per_poly && domain == GH_DOMAIN_NTT) ? n / ring->d * ring->d : n;
+}
+
+// Unpacks runs of `bytes` (host; the first `nbytes` are copied) on the device;
+// *bad reports an element >= q. Context lock held.
+int unpack_runs(gh_ring* ring, const uint8_t* bytes, uint64_t nbytes, const std::vector<Run>& runs,
+                bool* bad) {
+  gh_ctx* ctx = ring->ctx;
+  *bad = false;
+  std::vector<WireSeg> segs;
+  for (const Run& r : runs)
+    if (r.count) segs.push_back(WireSeg{r.off, (uint64_t)(uintptr_t)r.words, r.count, 0, 0});
+  if (segs.empty()) return GH_OK;
+  std::vector<WireChunk> chunks;
+  wire_chunks(segs, &chunks);
+  if (chunks.size() > 0xffffffffull) return GH_E_INVALID_ARGUMENT;
+  const size_t o_seg = 0, o_chk = al(segs.size() * sizeof(WireSeg));
+  const size_t o_in = o_chk + al(chunks.size() * sizeof(WireChunk));
+  void* ws;
+  TRY(ctx_workspace(ctx, o_in + nbytes, &ws));
+  char* w = (char*)ws;
+  for (WireSeg& s : segs) s.bytes += (uint64_t)(uintptr_t)(w + o_in);
+  uint32_t* flag = (uint32_t*)ctx->misc;
+  TRY(h2d(ctx, w + o_in, bytes, nbytes));
+  TRY(h2d(ctx, w + o_seg, segs.data(), segs.size() * sizeof(WireSeg)));
+  TRY(h2d(ctx, w + o_chk, chunks.data(), chunks.size() * sizeof(WireChunk)));
+  GH_HIP(hipMemsetAsync(flag, 0, sizeof(uint32_t), ctx->stream));
+  TRY(enqueue_wire_unpack(ring, ctx->stream, (const WireSeg*)(w + o_seg),
+                          (const WireChunk*)(w + o_chk), (uint32_t)chunks.size(), flag));
+  uint32_t host = 0;
+  TRY(d2h(ctx, &host, flag, sizeof host));
+  TRY(sync(ctx));
+  *bad = host != 0;
+  return GH_OK;
+}
+
+// Packs runs (device words) into an image of `nbytes` on the device and copies
+// it to `out`; headers are written by the caller afterwards. Context lock held.
+int pack_runs(gh_ring* ring, const std::vector<Run>& runs, uint64_t nbytes, uint8_t* out) {
+  gh_ctx* ctx = ring->ctx;
+  std::vector<WireSeg> segs;
+  for (const Run& r : runs)
+    if (r.count) segs.push_back(WireSeg{r.off, (uint64_t)(uintptr_t)r.words, r.count, 0, 0});
+  if (segs.empty()) return GH_OK;
+  std::vector<WireChunk> chunks;
+  wire_chunks(segs, &chunks);
+  if (chunks.size() > 0xffffffffull) return GH_E_INVALID_ARGUMENT;
+  const size_t o_seg = 0, o_chk = al(segs.size() * sizeof(WireSeg));
+  const size_t o_out = o_chk + al(chunks.size() * sizeof(WireChunk));
+  void* ws;
+  TRY(ctx_workspace(ctx, o_out + nbytes, &ws));
+  char* w = (char*)ws;
+  for (WireSeg& s : segs) s.bytes += (uint64_t)(uintptr_t)(w + o_out);
+  TRY(h2d(ctx, w + o_seg, segs.data(), segs.size() * sizeof(WireSeg)));
+  TRY(h2d(ctx, w + o_chk, chunks.data(), chunks.size() * sizeof(WireChunk)));
+  TRY(enqueue_wire_pack(ring, ctx->stream, (const WireSeg*)(w + o_seg),
+                        (const WireChunk*)(w + o_chk), (uint32_t)chunks.size()));
+  TRY(d2h(ctx, out, w + o_out, nbytes));
+  return sync(ctx);
+}
+
+}  // namespace
+
+extern "C" {
+
+GH_API int gh_wire_size(const gh_ring* ring, int kind, uint64_t rows, uint64_t cols,
+                        uint64_t* bytes) {
+  if (!ring || !bytes) return GH_E_INVALID_ARGUMENT;
+  WireKind k;
+  TRY(wire_kind(kind, &k));
+  if (kind != GH_WIRE_RINGMAT && cols != 1) return GH_E_INVALID_ARGUMENT;
+  // Vec lengths are u32 (serialize/src/lib.rs:217-218)
+  if ((kind == GH_WIRE_VEC || kind == GH_WIRE_ELEMS) && rows > 0xffffffffull)
+    return GH_E_INVALID_DATA;
+  uint64_t elems, payload;
+  TRY(wire_shape(ring, k, rows, cols, &elems, &payload));
+  *bytes = k.hdr + payload;
+  return GH_OK;
+}
+
+GH_API int gh_wire_encode(gh_ring* ring, int kind, uint64_t rows, uint64_t cols, const void* src,
+                          int mem, void* out, uint64_t out_cap, uint64_t* written) {
+  if (!ring || !written) return GH_E_INVALID_ARGUMENT;
+  TRY(check_mem(mem));
+  uint64_t size;
+  TRY(gh_wire_size(ring, kind, rows, cols, &size));
+  *written = size;
+  if (!out) return GH_OK;
+  if (out_cap < size) return GH_E_INVALID_ARGUMENT;
+  WireKind k;
+  TRY(wire_kind(kind, &k));
+  uint64_t elems, payload;
+  TRY(wire_shape(ring, k, rows, cols, &elems, &payload));
+  if (elems && !src) return GH_E_INVALID_ARGUMENT;
+  gh_ctx* ctx = ring->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  uint8_t* o = (uint8_t*)out;
+  if (elems) {
+    const void* words = src;
+    if (mem == GH_MEM_HOST) {
+      TRY(check_canonical_host(ring, src, (size_t)elems));
+      void* dev;
+      const size_t wbytes = (size_t)elems * ring->word_bytes;
+      TRY(ctx_staging(ctx, wbytes, &dev));
+      TRY(h2d(ctx, dev, src, wbytes));
+      words = dev;
+    }
+    TRY(pack_runs(ring, {Run{0, elems, const_cast<void*>(words)}}, payload, o + k.hdr));
+  }
+  write_header(kind, o, rows, cols);
+  return GH_OK;
+}
+
+GH_API int gh_wire_peek(const gh_ring* ring, int kind, const void* bytes, uint64_t len,
+                        uint64_t* rows, uint64_t* cols, uint64_t* total) {
+  if (!ring || !rows || !cols || (len && !bytes)) return GH_E_INVALID_ARGUMENT;
+  WireKind k;
+  TRY(wire_kind(kind, &k));
+  uint64_t r = *rows, c = 1;
+  TRY(read_header(kind, (const uint8_t*)bytes, len, &r, &c));
+  uint64_t elems, payload;
+  TRY(wire_shape(ring, k, r, c, &elems, &payload));
+  *rows = r;
+  *cols = c;
+  if (total) *total = k.hdr + payload;
+  return GH_OK;
+}
+
+GH_API int gh_wire_decode(gh_ring* ring, int kind, int domain, const void* bytes, uint64_t len,
+                          uint64_t rows, uint64_t cols, void* dst, int mem, uint64_t* consumed) {
+  if (!ring || (len && !bytes)) return GH_E_INVALID_ARGUMENT;
+  TRY(check_mem(mem));
+  if (domain != GH_DOMAIN_COEFF && domain != GH_DOMAIN_NTT) return GH_E_INVALID_ARGUMENT;
+  WireKind k;
+  TRY(wire_kind(kind, &k));
+  const uint8_t* in = (const uint8_t*)bytes;
+  uint64_t r = rows, c = 1;
+  TRY(read_header(kind, in, len, &r, &c));
+  uint64_t elems, payload;
+  TRY(wire_shape(ring, k, r, c, &elems, &payload));
+  if (r != rows || c != cols) return GH_E_DIMENSION_MISMATCH;
+  if (elems && !dst) return GH_E_INVALID_ARGUMENT;
+  gh_ctx* ctx = ring->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  const uint64_t avail = len - k.hdr;
+  bool bad = false;
+  if (avail < payload) {
+    const uint64_t n = truncated_prefix(ring, k, domain, avail);
+    TRY(unpack_runs(ring, in, k.hdr + n * ring->wire_bytes, {Run{k.hdr, n, nullptr}}, &bad));
+    return bad ? GH_E_INVALID_DATA : GH_E_UNEXPECTED_END;
+  }
+  const size_t wbytes = (size_t)elems * ring->word_bytes;
+  void* words = dst;
+  if (mem == GH_MEM_HOST && elems) TRY(ctx_staging(ctx, wbytes, &words));
+  TRY(unpack_runs(ring, in, k.hdr + payload, {Run{k.hdr, elems, words}}, &bad));
+  if (bad) return GH_E_INVALID_DATA;
+  // deserialize_exact (serialize/src/lib.rs:79-83)
+  if (!consumed && len != k.hdr + payload) return GH_E_INVALID_DATA;
+  if (mem == GH_MEM_HOST && elems) {
+    TRY(d2h(ctx, dst, words, wbytes));
+    TRY(sync(ctx));
+  }
+  if (consumed) *consumed = k.hdr + payload;
+  return GH_OK;
+}
+
+GH_API int gh_key_save(gh_key* key, int domain, void* out, uint64_t out_cap, uint64_t* written) {
+  if (!key || !written) return GH_E_INVALID_ARGUMENT;
+  if (domain != GH_DOMAIN_COEFF && domain != GH_DOMAIN_NTT) return GH_E_INVALID_ARGUMENT;
+  gh_ring* ring = key->ring;
+  const uint64_t d = ring->d, eb = ring->wire_bytes, wb = ring->word_bytes;
+  const uint64_t m1 = (uint64_t)key->rows * key->msg_len, m2 = (uint64_t)key->rows * key->open_len;
+  const uint64_t b1 = m1 * d * eb, b2 = m2 * d * eb;
+  const uint64_t size = 16 + b1 + 16 + b2;
+  *written = size;
+  if (!out) return GH_OK;
+  if (out_cap < size) return GH_E_INVALID_ARGUMENT;
+  gh_ctx* ctx = ring->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  void* dev;
+  TRY(ctx_staging(ctx, (size_t)(m1 + m2) * d * wb, &dev));
+  char* msg = (char*)dev;
+  char* open = msg + m1 * d * wb;
+  TRY(launch_key_rowmajor(ring, key->dev_a, key->rows, key->msg_len, 0, msg));
+  TRY(launch_key_rowmajor(ring, key->dev_a, key->rows, key->open_len, key->msg_len, open));
+  // finish_twisted_ring_mat (poly/twisted_ntt.rs:318-330): back to coefficients
+  if (domain == GH_DOMAIN_COEFF) TRY(launch_xform(ring, kTwistedInv, dev, dev, (size_t)(m1 + m2)));
+  uint8_t* o = (uint8_t*)out;
+  TRY(pack_runs(ring, {Run{16, m1 * d, msg}, Run{16 + b1 + 16, m2 * d, open}}, size, o));
+  write_header(GH_WIRE_RINGMAT, o, key->rows, key->msg_len);
+  write_header(GH_WIRE_RINGMAT, o + 16 + b1, key->rows, key->open_len);
+  return GH_OK;
+}
+
+GH_API int gh_key_load(gh_ring* ring, const void* bytes, uint64_t len, int domain,
+                       uint64_t* consumed, gh_key** out) {
+  if (!ring || !out || (len && !bytes)) return GH_E_INVALID_ARGUMENT;
+  *out = nullptr;
+  if (domain != GH_DOMAIN_COEFF && domain != GH_DOMAIN_NTT) return GH_E_INVALID_ARGUMENT;
+  const WireKind k{16, true};
+  const uint8_t* in = (const uint8_t*)bytes;
+  const uint64_t eb = ring->wire_bytes, wb = ring->word_bytes;
+  gh_ctx* ctx = ring->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  // LinearCommitmentKey::deserialize (commit/src/linear.rs:181-186): a_msg then
+  // a_open, each RingMat::deserialize; a truncated or invalid stream reports
+  // what the first failing element or header would.
+  uint64_t rows1 = 0, cols1 = 1;
+  TRY(read_header(GH_WIRE_RINGMAT, in, len, &rows1, &cols1));
+  uint64_t e1, b1;
+  TRY(wire_shape(ring, k, rows1, cols1, &e1, &b1));
+  bool bad = false;
+  if (len - 16 < b1) {
+    const uint64_t n = truncated_prefix(ring, k, domain, len - 16);
+    TRY(unpack_runs(ring, in, 16 + n * eb, {Run{16, n, nullptr}}, &bad));
+    return bad ? GH_E_INVALID_DATA : GH_E_UNEXPECTED_END;
+  }
+  const uint64_t off2 = 16 + b1;
+  uint64_t rows2 = 0, cols2 = 1, e2 = 0, b2 = 0;
+  int st2 = read_header(GH_WIRE_RINGMAT, in + off2, len - off2, &rows2, &cols2);
+  if (st2 == GH_OK) st2 = wire_shape(ring, k, rows2, cols2, &e2, &b2);
+  if (st2 != GH_OK || len - off2 - 16 < b2) {
+    uint64_t n2 = 0;
+    if (st2 == GH_OK) n2 = truncated_prefix(ring, k, domain, len - off2 - 16);
+    TRY(unpack_runs(ring, in, off2 + (n2 ? 16 + n2 * eb : 0),
+                    {Run{16, e1, nullptr}, Run{off2 + 16, n2, nullptr}}, &bad));
+    if (bad) return GH_E_INVALID_DATA;
+    return st2 != GH_OK ? st2 : GH_E_UNEXPECTED_END;
+  }
+  const uint64_t total = off2 + 16 + b2;
+  void* dev = nullptr;
+  const size_t w1 = (size_t)e1 * wb, w2 = (size_t)e2 * wb;
+  if (e1 + e2) TRY(ctx_staging(ctx, w1 + w2, &dev));
+  char* msg = (char*)dev;
+  char* open = msg ? msg + w1 : nullptr;
+  TRY(unpack_runs(ring, in, total, {Run{16, e1, msg}, Run{off2 + 16, e2, open}}, &bad));
+  if (bad) return GH_E_INVALID_DATA;
+  if (!consumed && len != total) return GH_E_INVALID_DATA;
+  // Valid for LinearCommitmentKey (linear.rs:188-192): equal row counts
+  if (rows1 != rows2) return GH_E_INVALID_DATA;
+  // CommitmentDimensions::validate (linear.rs:45-59), and the key's 32-bit dimensions
+  if (rows1 == 0 || cols1 == 0) return GH_E_INVALID_PARAMETERS;
+  if (rows1 > 0xffffffffull || cols1 + cols2 > 0xffffffffull) return GH_E_INVALID_PARAMETERS;
+  TRY(key_create_dev(ring, (uint32_t)rows1, msg, (uint32_t)cols1, open, (uint32_t)cols2, domain,
+                     out));
+  if (consumed) *consumed = total;
+  return GH_OK;
 }
 
 }  // extern "C"

@@ -40,6 +40,22 @@ struct GramWork {  // one wave of a Gram batch: (matrix, 4x4 pair tile, column c
 struct GramFin {  // one output polynomial of a Gram batch
   uint32_t g, pair;
 };
+// `count` field elements between an encoded byte stream and ring words (both
+// device addresses). Unpacking raises flags[flag] on an element >= q; a
+// segment with words == 0 is only checked.
+struct WireSeg {
+  uint64_t bytes;
+  uint64_t words;
+  uint64_t count;
+  uint32_t flag;
+  uint32_t pad;
+};
+constexpr uint32_t kWireChunk = 4096;  // elements of a segment per block
+struct WireChunk {                     // one block's run of a segment
+  uint64_t first;
+  uint32_t seg;
+  uint32_t pad;
+};
 // Layout of a commitment batch's device memory: [NTT-domain witnesses]
 // [partials][descriptors][segments], each 256-byte aligned; descriptors sorted
 // by witness length, longest first.
@@ -92,6 +108,7 @@ struct gh_ring {
   uint32_t d = 0;
   int kind = 0;
   uint32_t word_bytes = 0;
+  uint32_t wire_bytes = 0;  // encoded element: ceil(bits(q - 1) / 8)
   uint64_t generator = 0, psi = 0, omega = 0, d_inv = 0;
   uint32_t qinv32 = 0;
   std::vector<uint64_t> host_tab;  // twiddle table image (layout in gh_field.h)
@@ -158,6 +175,15 @@ int launch_amortize(gh_ring* ring, const void* c, const void* s, uint32_t r, uin
                     int domain, void* z_dev, void* s_scratch);
 int launch_stream_probe(gh_ctx* ctx, const void* src, void* dst, size_t bytes, int copy,
                         hipEvent_t e0, hipEvent_t e1);
+// canonical encodings (gh_wire.hip); chunk lists follow the segments
+void wire_chunks(const std::vector<WireSeg>& segs, std::vector<WireChunk>* out);
+int enqueue_wire_unpack(gh_ring* ring, hipStream_t stream, const WireSeg* segs,
+                        const WireChunk* chunks, uint32_t nchunks, uint32_t* flags);
+int enqueue_wire_pack(gh_ring* ring, hipStream_t stream, const WireSeg* segs,
+                      const WireChunk* chunks, uint32_t nchunks);
+// dst (row-major rows x cols polys) = key columns [col_offset, col_offset + cols)
+int launch_key_rowmajor(gh_ring* ring, const void* key_a, uint32_t rows, uint32_t cols,
+                        uint32_t col_offset, void* dst);
 int launch_jl(gh_ring* ring, const uint8_t seed[32], uint32_t rows, const void* parts,
               uint32_t num_parts, uint64_t cols, void* p_modq, int64_t* p_exact, int8_t* flat);
 

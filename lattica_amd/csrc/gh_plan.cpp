@@ -103,6 +103,10 @@ int plan_ring(uint64_t q, uint32_t d, gh_ring* ring) {
   ring->d = d;
   ring->kind = kind;
   ring->word_bytes = kind == GH_RING_Q32 ? 4 : 8;
+  // PrimeField::SERIALIZED_BYTES (arith/prime.rs:54-58)
+  uint32_t bits = 0;
+  for (uint64_t v = q - 1; v; v >>= 1) bits++;
+  ring->wire_bytes = (bits + 7) / 8;
   ring->generator = g;
   ring->psi = psi;
   ring->omega = omega;

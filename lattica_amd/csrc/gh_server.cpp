@@ -97,6 +97,8 @@ struct SrvReq {
   uint32_t g_chunk = 0, g_nchunks = 0;
   size_t t_bytes = 0, g_bytes = 0;   // output sizes
   size_t stage_t = 0, stage_g = 0;   // host-memory outputs: offsets in the slot's pinned buffer
+  size_t stage_f = 0;                // wire witness: its flag in the slot's pinned buffer
+  bool bad = false;                  // wire witness held an element >= q
   int status = GH_OK;
   bool done = false;
   Clock::time_point t_sub, t_done;
@@ -142,9 +144,9 @@ int plan_request(gh_server* s, SrvReq* q) {
   const size_t wb = r.ring->word_bytes;
   const uint64_t m = (uint64_t)r.r * r.n * 64;  // witness words
   size_t at = 0;
-  if (r.mem == GH_MEM_HOST) {
+  if (r.mem == GH_MEM_HOST) {  // device copy of the host witness (wire: its encoding)
     q->o_in = at;
-    at += al(m * wb);
+    at += al(r.wire ? r.witness_len : m * wb);
   }
   q->o_sntt = at;
   at += al(m * wb);
@@ -174,10 +176,68 @@ int plan_request(gh_server* s, SrvReq* q) {
   return at > s->arena.size() ? GH_E_OOM : GH_OK;
 }
 
-int validate(gh_server* s, const gh_request* r) {
+uint64_t get_le(const uint8_t* p, int n) {
+  uint64_t v = 0;
+  for (int i = 0; i < n; i++) v |= (uint64_t)p[i] << (8 * i);
+  return v;
+}
+void put_le(uint8_t* p, uint64_t v, int n) {
+  for (int i = 0; i < n; i++) p[i] = (uint8_t)(v >> (8 * i));
+}
+
+// bytes of one encoded polynomial
+uint64_t wire_poly(const gh_ring* g) { return (uint64_t)g->d * g->wire_bytes; }
+// byte offset of part i's polynomials in a wire witness of parts of n polynomials
+uint64_t wire_part(const gh_request& r, uint32_t i) {
+  return 4 + (uint64_t)i * (4 + r.n * wire_poly(r.ring)) + 4;
+}
+// t_vecs as Vec<RingVec<poly>> (labrador/src/main_protocol/mod.rs:702-726)
+uint64_t wire_t_bytes(const gh_request& r) {
+  return 4 + (uint64_t)r.r * (8 + (uint64_t)r.key->rows * wire_poly(r.ring));
+}
+// the Gram polynomials as Vec<poly> (labrador/src/main_protocol/garbage.rs:76-93)
+uint64_t wire_g_bytes(const gh_request& r) {
+  return 4 + (uint64_t)r.r * (r.r + 1) / 2 * wire_poly(r.ring);
+}
+
+// Structure of a wire witness: LabradorWitness::parts, a Vec<Vec<poly>>
+// (labrador/src/relation.rs:113-115; serialize/src/lib.rs:210-245), u32 part
+// count, then per part a u32 length and its polynomials. Parts must share one
+// length. Fills r and n (0 in the request) or checks them. Elements are
+// checked on the device.
+int parse_wire_witness(gh_request* r) {
+  if (r->wire != 1 || r->mem != GH_MEM_HOST) return GH_E_INVALID_ARGUMENT;
+  const gh_ring* g = r->ring;
+  if (g->wire_bytes != g->word_bytes) return GH_E_INVALID_PARAMETERS;
+  const uint8_t* p = (const uint8_t*)r->witness;
+  const uint64_t len = r->witness_len, pb = wire_poly(g);
+  if (len < 4) return GH_E_UNEXPECTED_END;
+  const uint32_t parts = (uint32_t)get_le(p, 4);
+  uint64_t off = 4, n = 0;
+  for (uint32_t i = 0; i < parts; i++) {
+    if (len - off < 4) return GH_E_UNEXPECTED_END;
+    const uint32_t ni = (uint32_t)get_le(p + off, 4);
+    if (i == 0) n = ni;
+    else if (ni != n) return GH_E_DIMENSION_MISMATCH;
+    off += 4;
+    if ((len - off) / pb < ni) return GH_E_UNEXPECTED_END;
+    off += ni * pb;
+  }
+  if (off != len) return GH_E_INVALID_DATA;  // deserialize_exact: trailing bytes
+  if ((r->r && r->r != parts) || (r->n && r->n != n)) return GH_E_DIMENSION_MISMATCH;
+  r->r = parts;
+  r->n = (uint32_t)n;
+  return GH_OK;
+}
+
+int validate(gh_server* s, gh_request* r) {
   if (r->ops == 0 || (r->ops & ~(uint32_t)(GH_OP_COMMIT | GH_OP_GRAM))) return GH_E_INVALID_ARGUMENT;
   if (r->mem != GH_MEM_HOST && r->mem != GH_MEM_DEVICE) return GH_E_INVALID_ARGUMENT;
   if (!r->ring || !r->witness || r->ring->ctx != s->ctx) return GH_E_INVALID_ARGUMENT;
+  if (r->wire) {
+    const int st = parse_wire_witness(r);
+    if (st) return st;
+  }
   if (r->r == 0 || r->n == 0) return GH_E_DIMENSION_MISMATCH;
   if ((uint64_t)r->r * r->n > 0xffffffffull || r->r > 0xffff) return GH_E_INVALID_ARGUMENT;
   if (r->ops & GH_OP_COMMIT) {
@@ -186,6 +246,11 @@ int validate(gh_server* s, const gh_request* r) {
     if (r->n > r->key->msg_len + r->key->open_len) return GH_E_DIMENSION_MISMATCH;
   }
   if ((r->ops & GH_OP_GRAM) && !r->g_out) return GH_E_INVALID_ARGUMENT;
+  if (r->wire) {
+    if ((r->ops & GH_OP_COMMIT) && r->t_cap < wire_t_bytes(*r)) return GH_E_INVALID_ARGUMENT;
+    if ((r->ops & GH_OP_GRAM) && r->g_cap < wire_g_bytes(*r)) return GH_E_INVALID_ARGUMENT;
+    return GH_OK;
+  }
   if (r->mem == GH_MEM_HOST)
     return check_canonical_host(r->ring, r->witness, (size_t)r->r * r->n * 64);
   return GH_OK;
@@ -229,7 +294,9 @@ int launch_wave(gh_server* s, Slot& sl, const std::vector<SrvReq*>& wave) {
     std::vector<GramDesc> gd;
     std::vector<GramWork> gw;
     std::vector<GramFin> gf;
-    size_t o_xd = 0, o_seg = 0, o_gd = 0, o_gw = 0, o_gf = 0;
+    std::vector<WireSeg> ws;  // wire witnesses: one run per part
+    std::vector<WireChunk> wc;
+    size_t o_xd = 0, o_seg = 0, o_gd = 0, o_gw = 0, o_gf = 0, o_ws = 0, o_wc = 0;
   };
   struct KeyPlan {
     gh_key* key;
@@ -241,7 +308,7 @@ int launch_wave(gh_server* s, Slot& sl, const std::vector<SrvReq*>& wave) {
   auto ring_plan = [&](gh_ring* g) -> RingPlan& {
     for (RingPlan& p : rp)
       if (p.ring == g) return p;
-    rp.push_back(RingPlan{g, {}, {}, {}, {}, {}});
+    rp.push_back(RingPlan{g, {}, {}, {}, {}, {}, {}, {}});
     return rp.back();
   };
   auto key_plan = [&](gh_key* k) -> KeyPlan& {
@@ -251,14 +318,23 @@ int launch_wave(gh_server* s, Slot& sl, const std::vector<SrvReq*>& wave) {
     return kp.back();
   };
   size_t stage = 0;  // host-output staging bytes
+  uint32_t nwire = 0;
   for (SrvReq* q : wave) {
     const gh_request& r = q->rq;
     const size_t wb = r.ring->word_bytes;
     char* blk = s->base + q->off;
-    const uint64_t src = r.mem == GH_MEM_HOST ? (uint64_t)(uintptr_t)(blk + q->o_in)
-                                              : (uint64_t)(uintptr_t)r.witness;
     char* sntt = blk + q->o_sntt;
+    uint64_t src = r.mem == GH_MEM_HOST ? (uint64_t)(uintptr_t)(blk + q->o_in)
+                                        : (uint64_t)(uintptr_t)r.witness;
     RingPlan& p = ring_plan(r.ring);
+    if (r.wire) {  // parts unpacked into the NTT buffer, then transformed in place
+      for (uint32_t i = 0; i < r.r; i++)
+        p.ws.push_back(WireSeg{(uint64_t)(uintptr_t)(blk + q->o_in + wire_part(r, i)),
+                               (uint64_t)(uintptr_t)(sntt + (size_t)i * r.n * 64 * wb),
+                               (uint64_t)r.n * 64, nwire, 0});
+      q->stage_f = nwire++;
+      src = (uint64_t)(uintptr_t)sntt;
+    }
     const uint32_t len = r.r * r.n;
     const uint32_t xi = (uint32_t)p.xd.size();
     p.xd.push_back(BatchDesc{(uint64_t)(uintptr_t)sntt, src, 0, 0, len, 0});
@@ -307,7 +383,15 @@ int launch_wave(gh_server* s, Slot& sl, const std::vector<SrvReq*>& wave) {
     p.o_gf = put(img, p.gf);
   }
   for (KeyPlan& k : kp) k.o_md = put(img, k.md);
+  for (RingPlan& p : rp) {
+    wire_chunks(p.ws, &p.wc);
+    p.o_ws = put(img, p.ws);
+    p.o_wc = put(img, p.wc);
+  }
+  const size_t o_flags = put(img, std::vector<uint32_t>(nwire, 0u));
   const size_t dbytes = img.size();
+  const size_t stage_flags = stage;  // wire flags come back after the outputs
+  stage += al((size_t)nwire * sizeof(uint32_t));
   int st = grow_slot(sl, dbytes, dbytes + stage);
   if (st) return st;
   char* hb = (char*)sl.hbuf;
@@ -316,17 +400,27 @@ int launch_wave(gh_server* s, Slot& sl, const std::vector<SrvReq*>& wave) {
   for (SrvReq* q : wave) {  // host-memory outputs land after the descriptor image
     q->stage_t += dbytes;
     q->stage_g += dbytes;
+    if (q->rq.wire) q->stage_f = dbytes + stage_flags + q->stage_f * sizeof(uint32_t);
   }
 
   hipStream_t stream = sl.stream;
   GH_HIP(hipEventRecord(sl.ev0, stream));
   for (SrvReq* q : wave) {
     if (q->rq.mem != GH_MEM_HOST) continue;
-    const size_t bytes = (size_t)q->rq.r * q->rq.n * 64 * q->rq.ring->word_bytes;
+    const size_t bytes = q->rq.wire ? q->rq.witness_len
+                                    : (size_t)q->rq.r * q->rq.n * 64 * q->rq.ring->word_bytes;
     GH_HIP(hipMemcpyAsync(s->base + q->off + q->o_in, q->rq.witness, bytes, hipMemcpyHostToDevice,
                           stream));
   }
   GH_HIP(hipMemcpyAsync(db, hb, dbytes, hipMemcpyHostToDevice, stream));
+  for (RingPlan& p : rp) {
+    if (p.wc.empty()) continue;
+    if (p.wc.size() > 0xffffffffull) return GH_E_INVALID_ARGUMENT;
+    st = enqueue_wire_unpack(p.ring, stream, (const WireSeg*)(db + p.o_ws),
+                             (const WireChunk*)(db + p.o_wc), (uint32_t)p.wc.size(),
+                             (uint32_t*)(db + o_flags));
+    if (st) return st;
+  }
   for (RingPlan& p : rp) {
     st = enqueue_batch_xform(p.ring, stream, (const BatchDesc*)(db + p.o_xd),
                              (const BatchSeg*)(db + p.o_seg), (uint32_t)p.seg.size());
@@ -357,8 +451,34 @@ int launch_wave(gh_server* s, Slot& sl, const std::vector<SrvReq*>& wave) {
       GH_HIP(hipMemcpyAsync(hb + q->stage_g, blk + q->o_gout, q->g_bytes, hipMemcpyDeviceToHost,
                             stream));
   }
+  if (nwire)
+    GH_HIP(hipMemcpyAsync(hb + dbytes + stage_flags, db + o_flags, nwire * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, stream));
   GH_HIP(hipEventRecord(sl.ev1, stream));
   return GH_OK;
+}
+
+// Writes a wire request's outputs: headers, and the staged words, which are
+// the encoded elements when the element width equals the word width.
+void emit_wire(SrvReq* q, const char* hb) {
+  const gh_request& r = q->rq;
+  if (r.ops & GH_OP_COMMIT) {
+    uint8_t* o = (uint8_t*)r.t_out;
+    const uint32_t rows = r.key->rows;
+    const size_t tb = (size_t)rows * 64 * r.ring->word_bytes;
+    put_le(o, r.r, 4);
+    size_t at = 4;
+    for (uint32_t i = 0; i < r.r; i++) {
+      put_le(o + at, rows, 8);
+      std::memcpy(o + at + 8, hb + q->stage_t + i * tb, tb);
+      at += 8 + tb;
+    }
+  }
+  if (r.ops & GH_OP_GRAM) {
+    uint8_t* o = (uint8_t*)r.g_out;
+    put_le(o, (uint64_t)r.r * (r.r + 1) / 2, 4);
+    std::memcpy(o + 4, hb + q->stage_g, q->g_bytes);
+  }
 }
 
 // Completes a wave's requests (lock held by the caller).
@@ -366,11 +486,11 @@ void finish(gh_server* s, std::vector<SrvReq*>& reqs, int status) {
   const Clock::time_point now = Clock::now();
   for (SrvReq* q : reqs) {
     s->arena.release(q->off, q->foot);
-    q->status = status;
+    q->status = (status == GH_OK && q->bad) ? GH_E_INVALID_DATA : status;
     q->done = true;
     q->t_done = now;
     s->admitted--;
-    if (status == GH_OK) {
+    if (q->status == GH_OK) {
       s->st.completed++;
       if (q->rq.ops & GH_OP_COMMIT) s->st.commits += q->rq.r;
       if (q->rq.ops & GH_OP_GRAM) s->st.grams++;
@@ -398,6 +518,13 @@ void run(gh_server* s) {
         if (hipEventElapsedTime(&ms, sl.ev0, sl.ev1) == hipSuccess) s->st.device_ms += ms;
         for (SrvReq* q : sl.reqs) {  // host-memory outputs leave the pinned staging
           if (q->rq.mem != GH_MEM_HOST) continue;
+          if (q->rq.wire) {
+            uint32_t flag;
+            std::memcpy(&flag, (char*)sl.hbuf + q->stage_f, sizeof flag);
+            q->bad = flag != 0;
+            if (!q->bad) emit_wire(q, (const char*)sl.hbuf);
+            continue;
+          }
           if (q->rq.ops & GH_OP_COMMIT)
             std::memcpy(q->rq.t_out, (char*)sl.hbuf + q->stage_t, q->t_bytes);
           if (q->rq.ops & GH_OP_GRAM)
@@ -534,12 +661,11 @@ GH_API int gh_server_destroy(gh_server* s) {
 
 GH_API int gh_server_submit(gh_server* s, const gh_request* req, uint64_t* ticket) {
   if (!s || !req || !ticket) return GH_E_INVALID_ARGUMENT;
-  int st = validate(s, req);
-  if (st) return st;
   SrvReq* q = new (std::nothrow) SrvReq();
   if (!q) return GH_E_OOM;
   q->rq = *req;
-  st = plan_request(s, q);
+  int st = validate(s, &q->rq);
+  if (st == GH_OK) st = plan_request(s, q);
   if (st) {
     delete q;
     return st;
