@@ -308,6 +308,33 @@ GH_API int gh_jl_project(gh_ring *ring, const uint8_t seed[32], uint32_t rows, c
                          uint32_t num_parts, uint64_t cols, void *p_modq, int64_t *p_exact,
                          int8_t *flat, int mem);
 
+/* ---- JL rows for aggregation --------------------------------------------------
+ * gh_jl_extract_flat: JLMatrix::extract_jl_rows_flat (labrador/src/jl.rs:311-383):
+ *   the entries alone, rows * num_parts * cols int8 in the layout
+ *   [m][part][coeff][poly] (what gh_jl_project's `flat` holds); rows <= 256.
+ * gh_jl_conjugate: jl_rows_flat_to_conjugated_polys
+ *   (labrador/src/main_protocol/mod.rs:885-923): rows * num_parts * n polynomials
+ *   (n = cols / d) laid out [m][part][poly]; coefficient k is e(0) for k = 0
+ *   and -e(d - k) otherwise, e(k) = Pi[m][part][poly * d + k] as a residue:
+ *   sigma_{-1} of the row's polynomial (labrador/src/relation.rs:636-655).
+ *   `flat` (layout above, in `mem`) is the input; flat NULL draws the
+ *   entries from `seed` instead, with the same result and no int8 array.
+ * gh_jl_evals: the prover's JL evaluations (labrador/src/main_protocol/step_prover.rs:196-205,
+ *   last_level/prover.rs:187-196): jl_eval[m] = sum_i sum_j
+ *   sigma_{-1}(pi_i^(m))_j * s_ij over R_q, for rows = 256, the count
+ *   JLProfile::validate requires (labrador/src/params/jl_profile.rs:47-62);
+ *   parts: num_parts x cols words (cols = n * d), coefficient domain; evals:
+ *   256 polynomials, coefficient domain. Their constant terms are the p of
+ *   gh_jl_project.
+ * All three need d = 64 and cols % d == 0. */
+GH_API int gh_jl_extract_flat(gh_ring *ring, const uint8_t seed[32], uint32_t rows,
+                              uint32_t num_parts, uint64_t cols, int8_t *flat, int mem);
+GH_API int gh_jl_conjugate(gh_ring *ring, const uint8_t seed[32], const int8_t *flat,
+                           uint32_t rows, uint32_t num_parts, uint64_t cols, void *polys,
+                           int mem);
+GH_API int gh_jl_evals(gh_ring *ring, const uint8_t seed[32], uint32_t rows, const void *parts,
+                       uint32_t num_parts, uint64_t cols, void *evals, int mem);
+
 /* ---- canonical wire formats ----------------------------------------------------
  * The reference's canonical encodings (serialize/src/lib.rs:1-7), byte for byte:
  *   element     PrimeField: gh_ring_info.wire_bytes = ceil(bits(q - 1) / 8) bytes,

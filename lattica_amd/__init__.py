@@ -56,7 +56,7 @@ EXPORTS = (
     "gh_server_poll", "gh_server_query", "gh_batch_plan_create", "gh_batch_plan_launch",
     "gh_batch_plan_destroy", "gh_bdlop_commit", "gh_decompose_balanced", "gh_garbage_h",
     "gh_amortize", "gh_wire_size", "gh_wire_encode", "gh_wire_peek", "gh_wire_decode",
-    "gh_key_save", "gh_key_load",
+    "gh_key_save", "gh_key_load", "gh_jl_extract_flat", "gh_jl_conjugate", "gh_jl_evals",
 )
 
 
@@ -181,6 +181,9 @@ def lib():
         "gh_wire_decode": (i32, [vp, i32, i32, vp, u64, u64, u64, vp, i32, ctypes.POINTER(u64)]),
         "gh_key_save": (i32, [vp, i32, vp, u64, ctypes.POINTER(u64)]),
         "gh_key_load": (i32, [vp, vp, u64, i32, ctypes.POINTER(u64), ctypes.POINTER(vp)]),
+        "gh_jl_extract_flat": (i32, [vp, vp, u32, u32, u64, vp, i32]),
+        "gh_jl_conjugate": (i32, [vp, vp, vp, u32, u32, u64, vp, i32]),
+        "gh_jl_evals": (i32, [vp, vp, u32, vp, u32, u64, vp, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -416,6 +419,40 @@ class Ring:
                                    _ptr(fl) if fl is not None else None, MEM_HOST),
                "gh_jl_project")
         return pm, pe, fl
+
+    # ---- JL rows for aggregation ----
+    def jl_extract_flat(self, seed: bytes, rows: int, num_parts: int, cols: int) -> np.ndarray:
+        """gh_jl_extract_flat: int8 entries, [m][part][coeff][poly]."""
+        sd = np.frombuffer(bytes(seed), dtype=np.uint8).copy()
+        out = np.zeros(rows * num_parts * cols, dtype=np.int8)
+        _check(lib().gh_jl_extract_flat(self.h, _ptr(sd), rows, num_parts, cols,
+                                        _ptr(out) if out.size else None, MEM_HOST),
+               "gh_jl_extract_flat")
+        return out
+
+    def jl_conjugate(self, rows: int, num_parts: int, cols: int, flat=None, seed: bytes = None):
+        """gh_jl_conjugate: (rows, num_parts, cols / d, d) conjugated JL polynomials,
+        from a flat int8 array or straight from the seed."""
+        n = cols // self.d
+        out = np.zeros((rows, num_parts, n, self.d), dtype=self.dtype)
+        sd = np.frombuffer(bytes(seed), dtype=np.uint8).copy() if seed is not None else None
+        fl = np.ascontiguousarray(flat, dtype=np.int8) if flat is not None else None
+        _check(lib().gh_jl_conjugate(self.h, _ptr(sd) if sd is not None else None,
+                                     _ptr(fl) if fl is not None else None, rows, num_parts, cols,
+                                     _ptr(out) if out.size else None, MEM_HOST),
+               "gh_jl_conjugate")
+        return out
+
+    def jl_evals(self, seed: bytes, parts, rows: int = 256) -> np.ndarray:
+        """gh_jl_evals: parts (num_parts, cols) or (num_parts, n, d) -> (rows, d)."""
+        p = self.words(parts)
+        num_parts = p.shape[0]
+        cols = p.size // max(1, num_parts)
+        sd = np.frombuffer(bytes(seed), dtype=np.uint8).copy()
+        out = np.zeros((rows, self.d), dtype=self.dtype)
+        _check(lib().gh_jl_evals(self.h, _ptr(sd), rows, _ptr(p) if p.size else None, num_parts,
+                                 cols, _ptr(out), MEM_HOST), "gh_jl_evals")
+        return out
 
     # ---- canonical wire formats (gh_wire_*) ----
     def wire_size(self, kind: int, rows: int, cols: int = 1) -> int:

@@ -1269,3 +1269,105 @@ GH_API int gh_key_load(gh_ring* ring, const void* bytes, uint64_t len, int domai
 }
 
 }  // extern "C"
+
+// ---- JL rows for aggregation --------------------------------------------------
+
+namespace {
+
+// shared argument checks of the JL entry points (gh_jl_project's limits)
+int jl_args(gh_ring* ring, const uint8_t* seed, uint32_t rows, uint64_t cols,
+            uint32_t num_parts) {
+  if (!ring || !seed) return GH_E_INVALID_ARGUMENT;
+  if (num_parts > 65535) return GH_E_INVALID_ARGUMENT;  // one grid row per part
+  if (rows == 0 || rows > 256) return GH_E_INVALID_ARGUMENT;
+  if (ring->d != 64 || cols % ring->d != 0) return GH_E_INVALID_ARGUMENT;
+  if (cols > 0xffffffffull / ((rows + 3) / 4)) return GH_E_INVALID_ARGUMENT;
+  return GH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+GH_API int gh_jl_extract_flat(gh_ring* ring, const uint8_t seed[32], uint32_t rows,
+                              uint32_t num_parts, uint64_t cols, int8_t* flat, int mem) {
+  TRY(jl_args(ring, seed, rows, cols, num_parts));
+  TRY(check_mem(mem));
+  const size_t bytes = (size_t)rows * num_parts * cols;
+  if (bytes == 0) return GH_OK;
+  if (!flat) return GH_E_INVALID_ARGUMENT;
+  gh_ctx* ctx = ring->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  if (mem == GH_MEM_DEVICE) return launch_jl_flat(ring, seed, rows, num_parts, cols, flat);
+  void* dev;
+  TRY(ctx_staging(ctx, bytes, &dev));
+  TRY(launch_jl_flat(ring, seed, rows, num_parts, cols, (int8_t*)dev));
+  TRY(d2h(ctx, flat, dev, bytes));
+  return sync(ctx);
+}
+
+GH_API int gh_jl_conjugate(gh_ring* ring, const uint8_t seed[32], const int8_t* flat,
+                           uint32_t rows, uint32_t num_parts, uint64_t cols, void* polys,
+                           int mem) {
+  if (!ring || (!flat && !seed)) return GH_E_INVALID_ARGUMENT;
+  static const uint8_t kNoSeed[32] = {0};
+  TRY(jl_args(ring, flat ? kNoSeed : seed, rows, cols, num_parts));
+  TRY(check_mem(mem));
+  const uint64_t n = cols / ring->d;
+  const size_t f_bytes = (size_t)rows * num_parts * cols;
+  const size_t o_bytes = (size_t)rows * num_parts * n * ring->d * ring->word_bytes;
+  if (o_bytes == 0) return GH_OK;
+  if (!polys) return GH_E_INVALID_ARGUMENT;
+  gh_ctx* ctx = ring->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  if (mem == GH_MEM_DEVICE)
+    return flat ? launch_jl_conj_flat(ring, flat, rows, num_parts, n, polys)
+                : launch_jl_conj_seed(ring, seed, rows, num_parts, n, polys);
+  void* dev;
+  TRY(ctx_staging(ctx, o_bytes + (flat ? f_bytes : 0), &dev));
+  char* out = (char*)dev;
+  if (flat) {
+    int8_t* fd = (int8_t*)(out + o_bytes);
+    TRY(h2d(ctx, fd, flat, f_bytes));
+    TRY(launch_jl_conj_flat(ring, fd, rows, num_parts, n, out));
+  } else {
+    TRY(launch_jl_conj_seed(ring, seed, rows, num_parts, n, out));
+  }
+  TRY(d2h(ctx, polys, out, o_bytes));
+  return sync(ctx);
+}
+
+GH_API int gh_jl_evals(gh_ring* ring, const uint8_t seed[32], uint32_t rows, const void* parts,
+                       uint32_t num_parts, uint64_t cols, void* evals, int mem) {
+  TRY(jl_args(ring, seed, rows, cols, num_parts));
+  TRY(check_mem(mem));
+  if (rows != 256 || !evals) return GH_E_INVALID_ARGUMENT;
+  if (num_parts && cols && !parts) return GH_E_INVALID_ARGUMENT;
+  gh_ctx* ctx = ring->ctx;
+  const size_t wb = ring->word_bytes;
+  const size_t e_bytes = (size_t)256 * ring->d * wb;
+  const uint64_t n = cols / ring->d;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  if (num_parts == 0 || n == 0) {
+    if (mem == GH_MEM_HOST) {
+      std::memset(evals, 0, e_bytes);
+      return GH_OK;
+    }
+    GH_HIP(hipMemsetAsync(evals, 0, e_bytes, ctx->stream));
+    return GH_OK;
+  }
+  if (mem == GH_MEM_DEVICE) return launch_jl_evals(ring, seed, parts, num_parts, n, evals);
+  const size_t in_bytes = (size_t)num_parts * cols * wb;
+  TRY(check_canonical_host(ring, parts, (size_t)num_parts * cols));
+  void* dev;
+  TRY(ctx_staging(ctx, in_bytes + e_bytes, &dev));
+  TRY(h2d(ctx, dev, parts, in_bytes));
+  TRY(launch_jl_evals(ring, seed, dev, num_parts, n, (char*)dev + in_bytes));
+  TRY(d2h(ctx, evals, (char*)dev + in_bytes, e_bytes));
+  return sync(ctx);
+}
+
+}  // extern "C"

@@ -184,6 +184,16 @@ int enqueue_wire_pack(gh_ring* ring, hipStream_t stream, const WireSeg* segs,
 // dst (row-major rows x cols polys) = key columns [col_offset, col_offset + cols)
 int launch_key_rowmajor(gh_ring* ring, const void* key_a, uint32_t rows, uint32_t cols,
                         uint32_t col_offset, void* dst);
+// JL rows for aggregation (gh_jl.hip); d = 64
+int launch_jl_flat(gh_ring* ring, const uint8_t seed[32], uint32_t rows, uint32_t num_parts,
+                   uint64_t cols, int8_t* flat);
+int launch_jl_conj_seed(gh_ring* ring, const uint8_t seed[32], uint32_t rows, uint32_t num_parts,
+                        uint64_t n, void* out);
+int launch_jl_conj_flat(gh_ring* ring, const int8_t* flat, uint32_t rows, uint32_t num_parts,
+                        uint64_t n, void* out);
+// rows = 256; parts: num_parts x n polys, coefficient domain; out: 256 polys
+int launch_jl_evals(gh_ring* ring, const uint8_t seed[32], const void* parts, uint32_t num_parts,
+                    uint64_t n, void* out);
 int launch_jl(gh_ring* ring, const uint8_t seed[32], uint32_t rows, const void* parts,
               uint32_t num_parts, uint64_t cols, void* p_modq, int64_t* p_exact, int8_t* flat);
 

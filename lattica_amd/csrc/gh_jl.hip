@@ -1,0 +1,443 @@
+// JL rows for aggregation: the entries of the JL matrix in the reference's
+// flat layout, its conjugated polynomials, and the polynomial JL evaluations
+// the prover folds into b'' (labrador/src/jl.rs:311-383,
+// labrador/src/main_protocol/mod.rs:835-923, step_prover.rs:196-205).
+//
+// Column col = poly * d + coeff of part p is keystream bytes
+// [col * pk, col * pk + pk) of ChaCha20(seed, nonce = p), pk = ceil(rows / 4),
+// two bits per row: 00 -> +1, 01 -> -1, else 0. With 256 rows a column is one
+// keystream block, block counter = col.
+//
+// Kernels:
+//   k_jl_flat        one thread per column, 256 consecutive polynomials of one
+//                    coefficient per block: each row's bytes are written as one
+//                    contiguous run of the flat layout [m][part][coeff][poly];
+//   k_jl_conj_seed   one wave per polynomial (lane = column within it): row m's
+//                    conjugated polynomial sigma_{-1}(pi) lands as 64 words, lane
+//                    k' writing coefficient (64 - k') mod 64;
+//   k_jl_conj_flat   the same from a flat int8 array, a 64 x 64 (coeff x poly)
+//                    tile transposed through LDS;
+//   k_jl_evals       jl_eval[m][t] = sum over columns c = (part, poly j, k) of
+//                    Pi[m][c] * W[c][t], W[(j, k)][t] = s_j[k + t] for k + t < 64
+//                    and -s_j[k + t - 64] otherwise (the negacyclic product
+//                    sigma_{-1}(pi) * s written out), an exact integer GEMM on
+//                    the int8 matrix cores: Pi is ternary, and each W entry is
+//                    split into signed base-256 digits (4 for the 32-bit prime,
+//                    8 for Goldilocks) whose products sum exactly in int32.
+//                    The accumulator tile of a block (256 rows x 256 digit
+//                    columns of int32) fills half a CU's register file, so
+//                    Goldilocks runs two passes of four digits each;
+//   k_jl_evals_fin   sums the per-block partials and recombines the digits mod q.
+#include <algorithm>
+
+#include "gh_internal.h"
+#include "gh_chacha.h"
+
+namespace gh {
+
+namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+// entry of a 2-bit code (labrador/src/jl.rs:356-362)
+__device__ __forceinline__ int jl_entry(uint32_t code) { return code == 0 ? 1 : (code == 1 ? -1 : 0); }
+
+// The keystream bytes of column `col` (pk of them) into 16 little-endian words
+// of `ks` (LDS, stride 17 words per thread): two blocks when the column
+// straddles a block boundary. Only used when pk < 64.
+__device__ void jl_column_lds(const ChaKey& key, uint64_t col, uint32_t pk, uint32_t* ks) {
+  const uint64_t off = col * pk;
+  const uint32_t b0 = (uint32_t)(off >> 6), o0 = (uint32_t)(off & 63);
+  uint32_t blk[16];
+  chacha_block(key, b0, blk);
+  uint8_t* out = reinterpret_cast<uint8_t*>(ks);
+  const uint8_t* bytes = reinterpret_cast<const uint8_t*>(blk);
+  const uint32_t first = 64 - o0 < pk ? 64 - o0 : pk;
+  for (uint32_t i = 0; i < first; i++) out[i] = bytes[o0 + i];
+  if (first < pk) {
+    chacha_block(key, b0 + 1, blk);
+    for (uint32_t i = first; i < pk; i++) out[i] = bytes[i - first];
+  }
+}
+
+// grid (ceil(n / 256), d, parts), 256 threads; rows <= 256
+__global__ __launch_bounds__(256) void k_jl_flat(ChaKey key0, uint32_t rows, uint32_t num_parts,
+                                                 uint64_t n, uint32_t d, int8_t* __restrict__ flat) {
+  __shared__ uint32_t ks[256 * 17];
+  const uint32_t t = threadIdx.x;
+  const uint64_t poly = (uint64_t)blockIdx.x * 256 + t;
+  const uint32_t coeff = blockIdx.y, part = blockIdx.z;
+  ChaKey key = key0;
+  key.n0 = part;
+  const uint64_t col = poly * d + coeff;
+  const uint32_t pk = (rows + 3) / 4;
+  const uint64_t row_stride = (uint64_t)num_parts * d * n;
+  int8_t* dst = flat + ((uint64_t)part * d + coeff) * n + poly;
+  if (pk == 64) {
+    if (poly >= n) return;
+    uint32_t blk[16];
+    chacha_block(key, (uint32_t)col, blk);
+#pragma unroll
+    for (int m = 0; m < 256; m++) {
+      if ((uint32_t)m < rows)
+        dst[(uint64_t)m * row_stride] = (int8_t)jl_entry((blk[m >> 4] >> (2 * (m & 15))) & 3);
+    }
+    return;
+  }
+  uint32_t* my = ks + t * 17;
+  if (poly < n) jl_column_lds(key, col, pk, my);
+  if (poly >= n) return;
+  const uint8_t* b = reinterpret_cast<const uint8_t*>(my);
+  for (uint32_t m = 0; m < rows; m++)
+    dst[(uint64_t)m * row_stride] = (int8_t)jl_entry((b[m >> 2] >> (2 * (m & 3))) & 3);
+}
+
+template <class W>
+__device__ __forceinline__ W conj_word(int e, bool k0, W q) {
+  // coefficient k of sigma_{-1}(pi): e at k = 0, -e elsewhere, as a residue
+  const int v = k0 ? e : -e;
+  return v == 0 ? W(0) : (v > 0 ? W(1) : q - W(1));
+}
+
+// grid (ceil(n / 4), parts), 256 threads = 4 polynomials x 64 columns; d = 64
+template <class W>
+__global__ __launch_bounds__(256) void k_jl_conj_seed(ChaKey key0, uint32_t rows,
+                                                      uint32_t num_parts, uint64_t n, W q,
+                                                      W* __restrict__ out) {
+  __shared__ uint32_t ks[256 * 17];
+  const uint32_t t = threadIdx.x;
+  const uint64_t j = (uint64_t)blockIdx.x * 4 + (t >> 6);
+  const uint32_t kc = t & 63, part = blockIdx.y;
+  ChaKey key = key0;
+  key.n0 = part;
+  const uint64_t col = j * 64 + kc;
+  const uint32_t pk = (rows + 3) / 4;
+  const uint32_t kout = (64 - kc) & 63;
+  // output polynomial (m, part, j) at ((m * parts + part) * n + j) * 64
+  W* dst = out + ((uint64_t)part * n + j) * 64 + kout;
+  const uint64_t row_stride = (uint64_t)num_parts * n * 64;
+  if (pk == 64) {
+    if (j >= n) return;
+    uint32_t blk[16];
+    chacha_block(key, (uint32_t)col, blk);
+#pragma unroll
+    for (int m = 0; m < 256; m++) {
+      if ((uint32_t)m < rows)
+        dst[(uint64_t)m * row_stride] =
+            conj_word<W>(jl_entry((blk[m >> 4] >> (2 * (m & 15))) & 3), kc == 0, q);
+    }
+    return;
+  }
+  uint32_t* my = ks + t * 17;
+  if (j < n) jl_column_lds(key, col, pk, my);
+  if (j >= n) return;
+  const uint8_t* b = reinterpret_cast<const uint8_t*>(my);
+  for (uint32_t m = 0; m < rows; m++)
+    dst[(uint64_t)m * row_stride] = conj_word<W>(jl_entry((b[m >> 2] >> (2 * (m & 3))) & 3), kc == 0, q);
+}
+
+// grid (ceil(n / 64), parts, rows), 256 threads; d = 64
+template <class W>
+__global__ __launch_bounds__(256) void k_jl_conj_flat(const int8_t* __restrict__ flat,
+                                                      uint32_t num_parts, uint64_t n, W q,
+                                                      W* __restrict__ out) {
+  constexpr int kStride = 68;  // bytes per coefficient row of the tile (17 dwords)
+  __shared__ int8_t tile[64 * kStride];
+  const uint32_t t = threadIdx.x;
+  const uint64_t j0 = (uint64_t)blockIdx.x * 64;
+  const uint32_t part = blockIdx.y, m = blockIdx.z;
+  const int8_t* src = flat + ((uint64_t)m * num_parts + part) * 64 * n;
+  for (int e = 0; e < 16; e++) {
+    const uint32_t idx = e * 256 + t, c = idx >> 6, jj = idx & 63;
+    tile[c * kStride + jj] = j0 + jj < n ? src[(uint64_t)c * n + j0 + jj] : (int8_t)0;
+  }
+  __syncthreads();
+  W* dst = out + (((uint64_t)m * num_parts + part) * n + j0) * 64;
+  for (int e = 0; e < 16; e++) {
+    const uint32_t idx = e * 256 + t, jj = idx >> 6, k = idx & 63;
+    if (j0 + jj >= n) continue;
+    const uint32_t srck = (64 - k) & 63;
+    dst[(uint64_t)jj * 64 + k] = conj_word<W>(tile[srck * kStride + jj], k == 0, q);
+  }
+}
+
+// ---- JL evaluations on the int8 matrix cores ---------------------------------
+
+constexpr int kEvPolys = 8;       // witness polynomials per LDS round
+constexpr int kEvRowBytes = 64;   // A tile: [poly][m][k] int8, k contiguous
+
+// Centred lift of a residue into the window that D signed base-256 digits
+// cover exactly ([-128 (256^D - 1) / 255, 127 (256^D - 1) / 255], 2^(8D)
+// integers), then its digits, least significant first.
+template <int D>
+__device__ __forceinline__ void lift_digits(uint64_t x, uint64_t q, int8_t (&dig)[D]) {
+  int64_t v;
+  if (D == 4) {
+    v = x <= 2139062143ull ? (int64_t)x : (int64_t)x - (int64_t)q;
+  } else {
+    v = x <= 0x7F7F7F7F7F7F7F7Full ? (int64_t)x : (int64_t)(x - q);
+  }
+#pragma unroll
+  for (int b = 0; b < D; b++) {
+    const int64_t lo = (int64_t)(int8_t)(uint8_t)(v & 0xff);  // in [-128, 127]
+    dig[b] = (int8_t)lo;
+    v = (v - lo) >> 8;
+  }
+}
+
+// Block: 8 waves for digits dbase .. dbase + 3 of DT; wave w owns digit
+// dbase + w / 2 (both 32-wide t halves) and m-tiles 4 (w % 2) .. + 3 (32 rows
+// each): 8 accumulator tiles of 32 x 32. Each round: every thread generates
+// one of the 512 keystream columns of 8 polynomials into the A tile and one
+// coefficient's digit windows E ([poly][digit][128] bytes: digits of s_u, then
+// of -s_u); then every wave runs its MFMAs over the round's 8 x 64 columns.
+// Partial sums leave per block.
+template <class W, int DT>
+__global__ __launch_bounds__(512) void k_jl_evals(ChaKey key0, const W* __restrict__ parts,
+                                                  uint64_t n, uint64_t total_polys,
+                                                  uint64_t polys_per_block, uint64_t q,
+                                                  uint32_t dbase, int32_t* __restrict__ partial) {
+  constexpr int D = 4;  // digits per pass
+  constexpr int kN = 64 * DT;
+  __shared__ __attribute__((aligned(16))) int8_t a_tile[kEvPolys * 256 * kEvRowBytes];
+  __shared__ __attribute__((aligned(16))) int8_t e_tile[kEvPolys * D * 128];
+  const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const uint32_t digit = wave >> 1, mgrp = wave & 1;
+  const uint64_t g_begin = (uint64_t)blockIdx.x * polys_per_block;
+  const uint64_t g_end = std::min<uint64_t>(g_begin + polys_per_block, total_polys);
+  v16i acc[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int h = 0; h < 2; h++) acc[i][h] = v16i{};
+
+  for (uint64_t g0 = g_begin; g0 < g_end; g0 += kEvPolys) {
+    // -- generation: A tile and digit windows, one column / coefficient per thread
+    {
+      const uint32_t ps = t >> 6, k = t & 63;
+      const uint64_t g = g0 + ps;
+      int8_t* arow = a_tile + ps * 256 * kEvRowBytes + k;
+      if (g < g_end) {
+        ChaKey key = key0;
+        key.n0 = (uint32_t)(g / n);
+        const uint64_t col = (g % n) * 64 + k;
+        uint32_t blk[16];
+        chacha_block(key, (uint32_t)col, blk);
+#pragma unroll
+        for (int m = 0; m < 256; m++)
+          arow[m * kEvRowBytes] = (int8_t)jl_entry((blk[m >> 4] >> (2 * (m & 15))) & 3);
+      } else {
+#pragma unroll 8
+        for (int m = 0; m < 256; m++) arow[m * kEvRowBytes] = 0;
+      }
+    }
+    {
+      const uint32_t ps = t >> 6, u = t & 63;
+      const uint64_t g = g0 + ps;
+      int8_t dp[DT], dn[DT];
+      if (g < g_end) {
+        const uint64_t x = (uint64_t)parts[g * 64 + u];
+        lift_digits<DT>(x, q, dp);
+        lift_digits<DT>(x == 0 ? 0 : q - x, q, dn);
+      } else {
+#pragma unroll
+        for (int b = 0; b < DT; b++) dp[b] = dn[b] = 0;
+      }
+      int8_t* e = e_tile + ps * D * 128;
+#pragma unroll
+      for (int b = 0; b < D; b++) {
+        e[b * 128 + u] = dp[dbase + b];
+        e[b * 128 + 64 + u] = dn[dbase + b];
+      }
+    }
+    __syncthreads();
+    // -- matrix-core phase
+    const uint32_t row = lane & 31, kg = lane >> 5;
+    for (int ps = 0; ps < kEvPolys; ps++) {
+      if (g0 + ps >= g_end) break;
+      const int8_t* at = a_tile + ps * 256 * kEvRowBytes;
+      const uint32_t* ew = reinterpret_cast<const uint32_t*>(e_tile + (ps * D + digit) * 128);
+#pragma unroll
+      for (int kstep = 0; kstep < 2; kstep++) {
+        const uint32_t k0 = kstep * 32 + kg * 16;
+        v4i bfrag[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          // bytes E[t + k0 .. t + k0 + 15], t = 32 h + row: five aligned words
+          const uint32_t o = 32 * h + row + k0, wi = o >> 2, sh = o & 3;
+          uint32_t w5[5];
+#pragma unroll
+          for (int x = 0; x < 5; x++) w5[x] = ew[wi + x];
+#pragma unroll
+          for (int x = 0; x < 4; x++)
+            bfrag[h][x] = (int)__builtin_amdgcn_alignbyte(w5[x + 1], w5[x], sh);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const uint32_t m = (mgrp * 4 + i) * 32 + row;
+          const v4i afrag = *reinterpret_cast<const v4i*>(at + m * kEvRowBytes + k0);
+#pragma unroll
+          for (int h = 0; h < 2; h++)
+            acc[i][h] = __builtin_amdgcn_mfma_i32_32x32x32_i8(afrag, bfrag[h], acc[i][h], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // -- partial sums: partial[block][m][n], n = digit * 64 + t
+  int32_t* pb = partial + (uint64_t)blockIdx.x * 256 * kN;
+  const uint32_t col_in = lane & 31, rgrp = lane >> 5;
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const uint32_t m = (mgrp * 4 + i) * 32 + 8 * (r >> 2) + 4 * rgrp + (r & 3);
+        const uint32_t nn = (dbase + digit) * 64 + h * 32 + col_in;
+        pb[(uint64_t)m * kN + nn] = acc[i][h][r];
+      }
+}
+
+// Goldilocks: 256 r mod p for r < p (2^64 = 2^32 - 1 mod p)
+__device__ __forceinline__ uint64_t gold_shift8(uint64_t r) {
+  const uint64_t p = 0xFFFFFFFF00000001ull;
+  const uint64_t lo = r << 8, t = (r >> 56) * 0xFFFFFFFFull;
+  uint64_t s = lo + t;
+  if (s < lo) s += 0xFFFFFFFFull;
+  return s >= p ? s - p : s;
+}
+
+// evals[m][t] = sum_b 256^b sum_blocks partial[block][m][b * 64 + t] mod q.
+// Block m, one thread per digit column n = b * 64 + t summing its column over
+// the blocks (eight loads in flight), then 64 threads recombine the digits
+// from the top one (Horner). Digit sums are below 2^40 in magnitude (at most
+// 2^31 per block partial, 256 blocks).
+template <class W, int D>
+__global__ __launch_bounds__(64 * D) void k_jl_evals_fin(const int32_t* __restrict__ partial,
+                                                         uint32_t nblk, uint64_t q,
+                                                         W* __restrict__ out) {
+  constexpr int kN = 64 * D;
+  __shared__ int64_t sums[kN];
+  const uint32_t m = blockIdx.x, nn = threadIdx.x;
+  const int32_t* p = partial + (uint64_t)m * kN + nn;
+  const uint64_t stride = (uint64_t)256 * kN;
+  int64_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint32_t blk = 0;
+  for (; blk + 8 <= nblk; blk += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; u++) acc[u] += p[(uint64_t)(blk + u) * stride];
+  }
+  for (; blk < nblk; blk++) acc[0] += p[(uint64_t)blk * stride];
+  int64_t s = 0;
+#pragma unroll
+  for (int u = 0; u < 8; u++) s += acc[u];
+  sums[nn] = s;
+  __syncthreads();
+  if (nn >= 64) return;
+  uint64_t r = 0;
+#pragma unroll
+  for (int b = D - 1; b >= 0; b--) {
+    const int64_t sb = sums[b * 64 + nn];
+    if (D == 4) {
+      const int64_t v = ((int64_t)r * 256 + sb) % (int64_t)q;
+      r = (uint64_t)(v < 0 ? v + (int64_t)q : v);
+    } else {
+      r = gold_shift8(r);
+      const uint64_t a = sb < 0 ? (uint64_t)(-sb) : (uint64_t)sb;  // < 2^40 < q
+      if (sb >= 0) {
+        const uint64_t x = r + a;
+        r = (x < r || x >= q) ? x - q : x;
+      } else {
+        r = r >= a ? r - a : r + (q - a);
+      }
+    }
+  }
+  out[(uint64_t)m * 64 + nn] = (W)r;
+}
+
+}  // namespace
+
+int launch_jl_flat(gh_ring* ring, const uint8_t seed[32], uint32_t rows, uint32_t num_parts,
+                   uint64_t cols, int8_t* flat) {
+  gh_ctx* ctx = ring->ctx;
+  const uint64_t n = cols / ring->d;
+  if (n == 0 || num_parts == 0) return GH_OK;
+  dim3 grid((unsigned)((n + 255) / 256), ring->d, num_parts);
+  hipLaunchKernelGGL(k_jl_flat, grid, dim3(256), 0, ctx->stream, chacha_key(seed), rows,
+                     num_parts, n, ring->d, flat);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+int launch_jl_conj_seed(gh_ring* ring, const uint8_t seed[32], uint32_t rows, uint32_t num_parts,
+                        uint64_t n, void* out) {
+  gh_ctx* ctx = ring->ctx;
+  if (n == 0 || num_parts == 0) return GH_OK;
+  dim3 grid((unsigned)((n + 3) / 4), num_parts);
+  if (ring->word_bytes == 4)
+    hipLaunchKernelGGL(k_jl_conj_seed<uint32_t>, grid, dim3(256), 0, ctx->stream,
+                       chacha_key(seed), rows, num_parts, n, (uint32_t)ring->q, (uint32_t*)out);
+  else
+    hipLaunchKernelGGL(k_jl_conj_seed<uint64_t>, grid, dim3(256), 0, ctx->stream,
+                       chacha_key(seed), rows, num_parts, n, (uint64_t)ring->q, (uint64_t*)out);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+int launch_jl_conj_flat(gh_ring* ring, const int8_t* flat, uint32_t rows, uint32_t num_parts,
+                        uint64_t n, void* out) {
+  gh_ctx* ctx = ring->ctx;
+  if (n == 0 || num_parts == 0 || rows == 0) return GH_OK;
+  dim3 grid((unsigned)((n + 63) / 64), num_parts, rows);
+  if (ring->word_bytes == 4)
+    hipLaunchKernelGGL(k_jl_conj_flat<uint32_t>, grid, dim3(256), 0, ctx->stream, flat,
+                       num_parts, n, (uint32_t)ring->q, (uint32_t*)out);
+  else
+    hipLaunchKernelGGL(k_jl_conj_flat<uint64_t>, grid, dim3(256), 0, ctx->stream, flat,
+                       num_parts, n, (uint64_t)ring->q, (uint64_t*)out);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+int launch_jl_evals(gh_ring* ring, const uint8_t seed[32], const void* parts, uint32_t num_parts,
+                    uint64_t n, void* out) {
+  gh_ctx* ctx = ring->ctx;
+  const bool gold = ring->kind == GH_RING_GOLDILOCKS;
+  const int D = gold ? 8 : 4;  // digits of a lifted witness word
+  const uint64_t total = (uint64_t)num_parts * n;
+  // one block per CU (the accumulators fill half the register file), polys
+  // split evenly in rounds of 8
+  uint64_t nblk = std::min<uint64_t>((uint64_t)ctx->num_cus, (total + kEvPolys - 1) / kEvPolys);
+  if (nblk == 0) nblk = 1;
+  uint64_t ppb = (total + nblk - 1) / nblk;
+  ppb = (ppb + kEvPolys - 1) / kEvPolys * kEvPolys;
+  nblk = (total + ppb - 1) / ppb;
+  if (nblk == 0) nblk = 1;
+  const size_t p_bytes = (size_t)nblk * 256 * 64 * D * sizeof(int32_t);
+  void* ws;
+  int st = ctx_workspace(ctx, p_bytes, &ws);
+  if (st) return st;
+  const ChaKey key = chacha_key(seed);
+  if (!gold) {
+    hipLaunchKernelGGL((k_jl_evals<uint32_t, 4>), dim3((unsigned)nblk), dim3(512), 0, ctx->stream,
+                       key, (const uint32_t*)parts, n, total, ppb, ring->q, 0u, (int32_t*)ws);
+    GH_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_jl_evals_fin<uint32_t, 4>), dim3(256), dim3(256), 0, ctx->stream,
+                       (const int32_t*)ws, (uint32_t)nblk, ring->q, (uint32_t*)out);
+  } else {
+    for (uint32_t dbase = 0; dbase < 8; dbase += 4) {
+      hipLaunchKernelGGL((k_jl_evals<uint64_t, 8>), dim3((unsigned)nblk), dim3(512), 0,
+                         ctx->stream, key, (const uint64_t*)parts, n, total, ppb, ring->q, dbase,
+                         (int32_t*)ws);
+      GH_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL((k_jl_evals_fin<uint64_t, 8>), dim3(256), dim3(512), 0, ctx->stream,
+                       (const int32_t*)ws, (uint32_t)nblk, ring->q, (uint64_t*)out);
+  }
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+}  // namespace gh

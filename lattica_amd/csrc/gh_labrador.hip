@@ -19,6 +19,7 @@
 #include "gh_internal.h"
 #include "gh_ntt.h"
 #include "gh_common_kernels.h"
+#include "gh_chacha.h"
 
 namespace gh {
 
@@ -477,37 +478,8 @@ int launch_amortize(gh_ring* ring, const void* c, const void* s, uint32_t r, uin
 }
 
 // ---------------------------------------------------------------------------
-// ChaCha20 (RFC 8439 section 2.3) and JL
+// JL projection
 // ---------------------------------------------------------------------------
-
-__device__ __forceinline__ uint32_t rotl(uint32_t x, int n) { return __builtin_rotateleft32(x, n); }
-
-#define GH_QR(a, b, c, d)                 \
-  a += b; d ^= a; d = rotl(d, 16);        \
-  c += d; b ^= c; b = rotl(b, 12);        \
-  a += b; d ^= a; d = rotl(d, 8);         \
-  c += d; b ^= c; b = rotl(b, 7);
-
-struct ChaKey {
-  uint32_t k[8];
-  uint32_t n0;  // nonce word 0 = part index; words 1, 2 are zero
-};
-
-__device__ __forceinline__ void chacha_block(const ChaKey& key, uint32_t counter, uint32_t (&out)[16]) {
-  uint32_t x0 = 0x61707865, x1 = 0x3320646e, x2 = 0x79622d32, x3 = 0x6b206574;
-  uint32_t x4 = key.k[0], x5 = key.k[1], x6 = key.k[2], x7 = key.k[3];
-  uint32_t x8 = key.k[4], x9 = key.k[5], x10 = key.k[6], x11 = key.k[7];
-  uint32_t x12 = counter, x13 = key.n0, x14 = 0, x15 = 0;
-#pragma unroll
-  for (int i = 0; i < 10; i++) {
-    GH_QR(x0, x4, x8, x12) GH_QR(x1, x5, x9, x13) GH_QR(x2, x6, x10, x14) GH_QR(x3, x7, x11, x15)
-    GH_QR(x0, x5, x10, x15) GH_QR(x1, x6, x11, x12) GH_QR(x2, x7, x8, x13) GH_QR(x3, x4, x9, x14)
-  }
-  out[0] = x0 + 0x61707865; out[1] = x1 + 0x3320646e; out[2] = x2 + 0x79622d32; out[3] = x3 + 0x6b206574;
-  out[4] = x4 + key.k[0]; out[5] = x5 + key.k[1]; out[6] = x6 + key.k[2]; out[7] = x7 + key.k[3];
-  out[8] = x8 + key.k[4]; out[9] = x9 + key.k[5]; out[10] = x10 + key.k[6]; out[11] = x11 + key.k[7];
-  out[12] = x12 + counter; out[13] = x13 + key.n0; out[14] = x14; out[15] = x15;
-}
 
 constexpr int kJlCols = 256;  // columns per LDS sweep (one per thread)
 
@@ -516,9 +488,8 @@ constexpr int kJlCols = 256;  // columns per LDS sweep (one per thread)
 template <class W, bool GOLD>
 __global__ __launch_bounds__(256) void k_jl(ChaKey key0, uint32_t rows, uint64_t q,
                                             const W* __restrict__ parts, uint64_t cols,
-                                            uint64_t chunk_cols, uint32_t d,
-                                            int64_t* __restrict__ part_acc,
-                                            int8_t* __restrict__ flat, uint32_t num_parts) {
+                                            uint64_t chunk_cols,
+                                            int64_t* __restrict__ part_acc) {
   __shared__ uint8_t ks[kJlCols][64];
   __shared__ int64_t sval[kJlCols];
   __shared__ uint64_t sres[kJlCols];
@@ -533,7 +504,6 @@ __global__ __launch_bounds__(256) void k_jl(ChaKey key0, uint32_t rows, uint64_t
   if (c_end > cols) c_end = cols;
   int64_t acc = 0;
   uint64_t accm = 0;  // Goldilocks: modular
-  const uint64_t n = cols / d;
   for (uint64_t c0 = c_begin; c0 < c_end; c0 += kJlCols) {
     const uint64_t col = c0 + t;
     if (col < c_end) {
@@ -571,12 +541,6 @@ __global__ __launch_bounds__(256) void k_jl(ChaKey key0, uint32_t rows, uint64_t
         } else {
           const int64_t v = sval[c];
           acc += code == 0 ? v : (code == 1 ? -v : 0);
-        }
-        if (flat) {
-          const uint64_t cc = c0 + c;
-          const uint64_t poly = cc / d, coeff = cc % d;
-          flat[(uint64_t)t * num_parts * cols + ((uint64_t)part * d + coeff) * n + poly] =
-              (int8_t)(code == 0 ? 1 : (code == 1 ? -1 : 0));
         }
       }
     }
@@ -633,11 +597,7 @@ __global__ __launch_bounds__(256) void k_jl_reduce(const int64_t* __restrict__ p
 int launch_jl(gh_ring* ring, const uint8_t seed[32], uint32_t rows, const void* parts,
               uint32_t num_parts, uint64_t cols, void* p_modq, int64_t* p_exact, int8_t* flat) {
   gh_ctx* ctx = ring->ctx;
-  ChaKey key;
-  for (int i = 0; i < 8; i++)
-    key.k[i] = (uint32_t)seed[4 * i] | ((uint32_t)seed[4 * i + 1] << 8) |
-               ((uint32_t)seed[4 * i + 2] << 16) | ((uint32_t)seed[4 * i + 3] << 24);
-  key.n0 = 0;
+  const ChaKey key = chacha_key(seed);
   // chunks per part so that the device holds a few blocks per CU
   uint64_t want = (uint64_t)ctx->num_cus * 4 / (num_parts ? num_parts : 1);
   if (want < 1) want = 1;
@@ -654,12 +614,10 @@ int launch_jl(gh_ring* ring, const uint8_t seed[32], uint32_t rows, const void* 
   const bool gold = ring->kind == GH_RING_GOLDILOCKS;
   if (!gold)
     hipLaunchKernelGGL((k_jl<uint32_t, false>), grid, dim3(256), 0, ctx->stream, key, rows,
-                       ring->q, (const uint32_t*)parts, cols, chunk_cols, ring->d, (int64_t*)ws,
-                       flat, num_parts);
+                       ring->q, (const uint32_t*)parts, cols, chunk_cols, (int64_t*)ws);
   else
     hipLaunchKernelGGL((k_jl<uint64_t, true>), grid, dim3(256), 0, ctx->stream, key, rows,
-                       ring->q, (const uint64_t*)parts, cols, chunk_cols, ring->d, (int64_t*)ws,
-                       flat, num_parts);
+                       ring->q, (const uint64_t*)parts, cols, chunk_cols, (int64_t*)ws);
   GH_HIP(hipGetLastError());
   if (!gold)
     hipLaunchKernelGGL((k_jl_reduce<false>), dim3(rows), dim3(256), 0, ctx->stream,
@@ -668,6 +626,8 @@ int launch_jl(gh_ring* ring, const uint8_t seed[32], uint32_t rows, const void* 
     hipLaunchKernelGGL((k_jl_reduce<true>), dim3(rows), dim3(256), 0, ctx->stream,
                        (const int64_t*)ws, nblk, rows, ring->q, p_exact, p_modq);
   GH_HIP(hipGetLastError());
+  // the entries in the flat layout, written row-contiguous by their own kernel
+  if (flat) return launch_jl_flat(ring, seed, rows, num_parts, cols, flat);
   return GH_OK;
 }
 

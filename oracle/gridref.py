@@ -261,6 +261,54 @@ def jl_project(q, seed: bytes, rows, parts, d, want_flat=False):
     return pm, pe, flat
 
 
+def jl_extract_flat(seed: bytes, rows, num_parts, cols, d):
+    """JLMatrix::extract_jl_rows_flat (labrador/src/jl.rs:311-383): the entries
+    alone, [m][part][coeff][poly] int8. The entries do not depend on the
+    witness, so this is jl_project's flat output over a zero witness."""
+    _, _, flat = jl_project(4294966657, seed, rows, np.zeros((num_parts, cols), np.uint64), d,
+                            want_flat=True)
+    return flat
+
+
+def _addmod(a, b, q):
+    """(a + b) mod q elementwise for uint64 arrays of residues, q up to 2^64."""
+    s = a + b  # wraps mod 2^64
+    wrap = (s < a) | (s >= np.uint64(q))
+    return np.where(wrap, s - np.uint64(q), s)
+
+
+def jl_conjugate_flat(q, flat, rows, num_parts, n, d):
+    """jl_rows_flat_to_conjugated_polys (labrador/src/main_protocol/mod.rs:885-923):
+    flat [m][part][coeff][poly] -> (rows, num_parts, n, d) polynomials with
+    coefficient k = entry(src) for k = 0 and q - entry(src) otherwise (0 stays
+    0), src = 0 for k = 0 else d - k, entry lifted to [0, q)."""
+    f = np.asarray(flat, dtype=np.int64).reshape(rows, num_parts, d, n)
+    src = np.array([0] + [d - k for k in range(1, d)])
+    e = f[:, :, src, :]                       # (rows, parts, d(k), n)
+    u = np.where(e < 0, np.uint64(q) - np.uint64(1), e.astype(np.uint64))  # v in {-1, 0, 1}
+    neg = np.where(u == 0, np.uint64(0), np.uint64(q) - u)
+    out = np.where((np.arange(d) == 0)[None, None, :, None], u, neg)
+    return np.ascontiguousarray(out.transpose(0, 1, 3, 2)).astype(np.uint64)
+
+
+def jl_evals(q, d, seed: bytes, rows, parts_polys, naive=False):
+    """jl_eval[m] = sum_i sum_j conj(pi_i^(m))_j * s_ij over Z_q[X]/(X^d + 1)
+    (labrador/src/main_protocol/step_prover.rs:196-205): the conjugated JL
+    polynomials times the witness polynomials through the oracle's negacyclic
+    product, summed in the reference's order. parts_polys: (num_parts, n, d)."""
+    s = _u64(parts_polys)
+    num_parts, n = s.shape[0], s.shape[1]
+    flat = jl_extract_flat(seed, rows, num_parts, n * d, d)
+    conj = jl_conjugate_flat(q, flat, rows, num_parts, n, d)
+    b = np.broadcast_to(s[None], conj.shape).reshape(-1, d)
+    prod = negacyclic_mul(q, d, conj.reshape(-1, d), np.ascontiguousarray(b), naive=naive)
+    prod = prod.reshape(rows, num_parts * n, d)
+    acc = np.zeros((rows, d), dtype=np.uint64)
+    for c in range(num_parts * n):
+        acc = _addmod(acc, prod[:, c, :], q)
+    return acc
+
+
 def chacha20_block(key: bytes, counter: int, nonce: bytes) -> bytes:
     k = np.frombuffer(bytes(key), dtype=np.uint8).copy()
     n = np.frombuffer(bytes(nonce), dtype=np.uint8).copy()
