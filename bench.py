@@ -212,9 +212,13 @@ def labrador_calls(ctx, la, ring, log, reps=10):
     """LaBRADOR scale: r = 32 witness vectors x 1024 polynomials (n = 2^15 in all),
     coefficients in [-2^7, 2^7), kappa = 16. Times the 32 inner commitments
     t_i = A*s_i (gh_inner_commit), the full Gram matrix (528 polynomials,
-    gh_gram) and the 256 x 2^21 ternary JL projection (gh_jl_project), each on
-    resident device buffers. The oracle checks one Gram entry and, for the JL,
-    p mod q against the exact sum."""
+    gh_gram), the 256 x 2^21 ternary JL projection (gh_jl_project), the step
+    after it (the 256 JL evaluations gh_jl_evals, the 2^23 conjugated JL
+    polynomials gh_jl_conjugate, the flat int8 entries gh_jl_extract_flat) and
+    the step around the commitment (h garbage, amortization z = sum c_i s_i,
+    balanced decomposition of the 512 t polynomials), each on resident device
+    buffers. The oracle checks one Gram entry; the JL p mod q must equal the
+    exact sum mod q and the constant terms of the JL evaluations."""
     rng = np.random.default_rng(33)
     r_, n_, rows = 32, 1024, 256
     a = rng.integers(0, Q, size=(KAPPA, n_, D), dtype=np.uint32)
@@ -225,6 +229,14 @@ def labrador_calls(ctx, la, ring, log, reps=10):
     npairs = r_ * (r_ + 1) // 2
     g_dev = ctx.alloc(npairs * D * 4)
     pm_dev, pe_dev = ctx.alloc(rows * 4), ctx.alloc(rows * 8)
+    ev_dev = ctx.alloc(rows * D * 4)
+    conj_dev = ctx.alloc(rows * r_ * n_ * D * 4)          # 2 GiB of conjugated polynomials
+    flat_dev = ctx.alloc(rows * r_ * n_ * D)              # 512 MiB of int8 entries
+    h_dev = ctx.alloc(npairs * D * 4)
+    c = witness(rng, r_)
+    c_dev = ctx.to_device(c)
+    z_dev = ctx.alloc(n_ * D * 4)
+    dec_dev = ctx.alloc(r_ * KAPPA * 4 * D * 4)
     seed = np.frombuffer(bytes(range(32)), dtype=np.uint8).copy()
     L = la.lib()
     import ctypes
@@ -237,6 +249,21 @@ def labrador_calls(ctx, la, ring, log, reps=10):
                                                s_dev.ptr, r_, n_ * D, pm_dev.ptr,
                                                ctypes.c_void_p(pe_dev.ptr.value), None,
                                                la.MEM_DEVICE),
+        "jl_evals_256x32x1024": lambda: L.gh_jl_evals(ring.h, ctypes.c_void_p(seed.ctypes.data),
+                                                      rows, s_dev.ptr, r_, n_ * D, ev_dev.ptr,
+                                                      la.MEM_DEVICE),
+        "jl_conjugate_256x32x1024": lambda: L.gh_jl_conjugate(
+            ring.h, ctypes.c_void_p(seed.ctypes.data), None, rows, r_, n_ * D, conj_dev.ptr,
+            la.MEM_DEVICE),
+        "jl_extract_flat_256x2^21": lambda: L.gh_jl_extract_flat(
+            ring.h, ctypes.c_void_p(seed.ctypes.data), rows, r_, n_ * D, flat_dev.ptr,
+            la.MEM_DEVICE),
+        "garbage_h_32x1024": lambda: L.gh_garbage_h(ring.h, s_dev.ptr, s_dev.ptr, r_, n_,
+                                                    la.DOMAIN_COEFF, h_dev.ptr, la.MEM_DEVICE),
+        "amortize_32x1024": lambda: L.gh_amortize(ring.h, c_dev.ptr, s_dev.ptr, r_, n_,
+                                                  la.DOMAIN_COEFF, z_dev.ptr, la.MEM_DEVICE),
+        "decompose_t_512_base2^8_4limbs": lambda: L.gh_decompose_balanced(
+            ring.h, t_dev.ptr, r_ * KAPPA, 256, 4, dec_dev.ptr, la.MEM_DEVICE),
     }
     res = {}
     for name, fn in calls.items():
@@ -260,13 +287,23 @@ def labrador_calls(ctx, la, ring, log, reps=10):
     pe = pe_dev.download(np.int64, (rows,))
     res["oracle_gram_entry"] = bool(np.array_equal(g[idx].astype(np.uint64), exp[1]))
     res["jl_modq_matches_exact"] = bool(np.array_equal(pm, pe % Q))
+    ev = ev_dev.download(np.uint32, (rows, D)).astype(np.int64)
+    res["jl_evals_constant_terms_equal_p"] = bool(np.array_equal(ev[:, 0], pm))
     log(f"LaBRADOR r 32 x 1024: inner commits {res['inner_commit_32x1024']['us_per_call']:,.0f} us "
         f"({res['inner_commit_32x1024']['commits_per_s']:,.0f} commits/s), Gram "
         f"{res['gram_32x1024']['us_per_call']:,.0f} us, JL {res['jl_256x2^21']['us_per_call']:,.0f} us; "
         f"oracle Gram entry {res['oracle_gram_entry']}, JL mod q == exact mod q "
         f"{res['jl_modq_matches_exact']}")
+    log(f"  JL evals {res['jl_evals_256x32x1024']['us_per_call']:,.0f} us (ct == p "
+        f"{res['jl_evals_constant_terms_equal_p']}), conjugated polys "
+        f"{res['jl_conjugate_256x32x1024']['us_per_call']:,.0f} us, flat "
+        f"{res['jl_extract_flat_256x2^21']['us_per_call']:,.0f} us, h garbage "
+        f"{res['garbage_h_32x1024']['us_per_call']:,.0f} us, amortize "
+        f"{res['amortize_32x1024']['us_per_call']:,.0f} us, decompose "
+        f"{res['decompose_t_512_base2^8_4limbs']['us_per_call']:,.0f} us")
     key.close()
-    for b in (s_dev, t_dev, g_dev, pm_dev, pe_dev):
+    for b in (s_dev, t_dev, g_dev, pm_dev, pe_dev, ev_dev, conj_dev, flat_dev, h_dev, c_dev,
+              z_dev, dec_dev):
         b.free()
     return res
 

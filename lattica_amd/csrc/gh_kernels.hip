@@ -356,18 +356,40 @@ __global__ __launch_bounds__(64 * WAVES) void k_mac4_q32(
 
 // Fused witness transform + MAC (q32). A block owns a contiguous range of at
 // most kFuseChunk key columns (the launcher sizes the grid so). In issue order:
-// the lane twiddles and the witness of the block's columns (which the transform
-// waits for), then the key loads of each wave's first PF columns, then the
-// transform of the witness polynomials into LDS (four lanes per polynomial)
-// while those key loads are in flight, then the stream over the remaining
-// columns against the LDS copy. PF = DEPTH * 16 / WAVES columns per wave: at
-// depth 2 with two 4-wave blocks per CU that is 256 KB of key in flight per CU
-// during the transform, enough to keep HBM busy through it. Results equal
-// k_mac4_q32 on a separately transformed witness.
+// the lane twiddles and the first witness polynomial of each quad (which the
+// transform waits for), then the key loads of each wave's first PF columns,
+// then the transform of the witness polynomials into LDS (four lanes per
+// polynomial) while those key loads are in flight, then the MAC over the
+// wave's columns against the LDS copy with PF columns kept in flight: a
+// column's registers are reloaded with the column PF further on as soon as
+// they are used. PF = DEPTH * 16 / WAVES columns per wave: at depth 2 with two
+// 4-wave blocks per CU that is 256 KB of key in flight per CU for the whole
+// sweep.
+//
+// Vector loads complete in issue order and the wait counter is one number, so
+// the code between the prefetch and the MAC is kept free of branches that
+// load: the first polynomial's transform is straight-line, loads past the
+// block's end are clamped to its last column (and multiplied by a zero
+// witness), and with NW = columns per wave known at compile time the MAC is
+// fully unrolled, letting each wait name exactly the column it needs. NW = 0
+// is the general loop (blocks of any width, conservative waits). Results
+// equal k_mac4_q32 on a separately transformed witness.
 constexpr int kFuseChunk = 128;
 constexpr int kFuseStride = 68;  // LDS row stride of s_ntt (16-byte aligned rows)
 
-template <int RB, int WAVES, int DEPTH>
+template <int NI>
+__device__ __forceinline__ void mac_column(uint64_t (&lo)[NI][4], uint32_t (&hi)[NI][4],
+                                           const u32x4 (&a)[NI], const u32x4 s0) {
+#pragma unroll
+  for (int i = 0; i < NI; i++) {
+    mac96(lo[i][0], hi[i][0], a[i].x, s0.x);
+    mac96(lo[i][1], hi[i][1], a[i].y, s0.y);
+    mac96(lo[i][2], hi[i][2], a[i].z, s0.z);
+    mac96(lo[i][3], hi[i][3], a[i].w, s0.w);
+  }
+}
+
+template <int RB, int WAVES, int DEPTH, int NW>
 __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_q32(
     FQ32 f, const uint32_t* __restrict__ tab, const uint32_t* __restrict__ A,
     const uint32_t* __restrict__ s_coeff, uint32_t* __restrict__ partial, uint32_t rows,
@@ -385,6 +407,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_q32(
   const size_t col_stride = (size_t)rows * 64;
   const u32x4* akey = reinterpret_cast<const u32x4*>(A + (size_t)row0 * 64) + lane;
   const size_t col4 = col_stride / 4;  // one key column, in u32x4
+  // column j of this wave is wave + j * WAVES; loads clamp to the block's last
+  const uint32_t nwm = NW > 0 ? (uint32_t)NW : (ncol + WAVES - 1) / WAVES;
+  auto col_of = [&](uint32_t j) -> uint32_t {
+    const uint32_t c = wave + j * WAVES;
+    return c < ncol ? c : ncol - 1;
+  };
 
   uint64_t lo[NI][4];
   uint32_t hi[NI][4];
@@ -394,7 +422,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_q32(
     for (int j = 0; j < 4; j++) { lo[i][j] = 0; hi[i][j] = 0; }
 
   // (a) loads the transform waits for: lane twiddles, this thread's first
-  // witness polynomial (vector loads complete in issue order)
+  // witness polynomial (clamped into the block)
   uint32_t ltw[16][2];
 #pragma unroll
   for (int k = 0; k < 16; k++) {
@@ -402,82 +430,77 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_q32(
     ltw[k][1] = tab[(kTabQuad + 16 * t + k) * 2 + 1];
   }
   const uint32_t p0 = threadIdx.x >> 2;
-  uint32_t x0[16];
-  if (p0 < ncol) {
-    const uint32_t* sp = s_coeff + (size_t)(ch + p0) * 64 + t;
+  uint32_t x[16];
+  {
+    const uint32_t pc = p0 < ncol ? p0 : ncol - 1;
+    const uint32_t* sp = s_coeff + (size_t)(ch + pc) * 64 + t;
 #pragma unroll
-    for (int i = 0; i < 16; i++) x0[i] = sp[4 * i];
+    for (int i = 0; i < 16; i++) x[i] = sp[4 * i];
   }
   // (b) key prefetch: this wave's first PF columns
   u32x4 pa[PF][NI];
 #pragma unroll
   for (int q = 0; q < PF; q++) {
-    const uint32_t cq = wave + q * WAVES;
-    if (cq < ncol) {
+    if (NW > 0 ? q < NW : (uint32_t)q < nwm) {
+      const uint32_t cq = col_of(q);
 #pragma unroll
       for (int i = 0; i < NI; i++)
         pa[q][i] = __builtin_nontemporal_load(akey + (size_t)(ch + cq) * col4 + i * 64);
     }
   }
-  // (c) witness transform into LDS while (b) is in flight
-  for (uint32_t p = p0; p < ncol; p += 16 * WAVES) {
-    uint32_t x[16];
-    if (p == p0) {
+  // (c) witness transform into LDS while (b) is in flight: the first
+  // polynomial straight-line, further ones (blocks wider than 16 * WAVES) after
+  quad_fwd64(f, x, tab, ltw, t);
+  if (p0 < ncol) {
+    u32x4* dp = reinterpret_cast<u32x4*>(lds + p0 * kFuseStride + 16 * bitrev2(t));
 #pragma unroll
-      for (int i = 0; i < 16; i++) x[i] = x0[i];
-    } else {
-      const uint32_t* sp = s_coeff + (size_t)(ch + p) * 64 + t;
+    for (int v = 0; v < 4; v++) dp[v] = u32x4{x[4 * v], x[4 * v + 1], x[4 * v + 2], x[4 * v + 3]};
+  }
+  for (uint32_t p = p0 + 16 * WAVES; p < ncol; p += 16 * WAVES) {
+    const uint32_t* sp = s_coeff + (size_t)(ch + p) * 64 + t;
 #pragma unroll
-      for (int i = 0; i < 16; i++) x[i] = sp[4 * i];
-    }
+    for (int i = 0; i < 16; i++) x[i] = sp[4 * i];
     quad_fwd64(f, x, tab, ltw, t);
     u32x4* dp = reinterpret_cast<u32x4*>(lds + p * kFuseStride + 16 * bitrev2(t));
 #pragma unroll
     for (int v = 0; v < 4; v++) dp[v] = u32x4{x[4 * v], x[4 * v + 1], x[4 * v + 2], x[4 * v + 3]};
   }
   __syncthreads();
-  // (d) MAC: the prefetched columns, then the rest two per step
+  // (d) MAC over the wave's columns, PF in flight; a clamped column meets a
+  // zero witness
+  auto witness = [&](uint32_t j) -> u32x4 {
+    const uint32_t c = wave + j * WAVES;
+    const u32x4 s0 = *reinterpret_cast<const u32x4*>(lds + col_of(j) * kFuseStride + k0);
+    const uint32_t keep = c < ncol ? 0xffffffffu : 0u;
+    return u32x4{s0.x & keep, s0.y & keep, s0.z & keep, s0.w & keep};
+  };
+  if constexpr (NW > 0) {
 #pragma unroll
-  for (int q = 0; q < PF; q++) {
-    const uint32_t cq = wave + q * WAVES;
-    if (cq < ncol) {
-      const u32x4 s0 = *reinterpret_cast<const u32x4*>(lds + cq * kFuseStride + k0);
+    for (int j = 0; j < NW; j++) {
+      constexpr int kPF = PF;
+      const int q = j % kPF;
+      mac_column<NI>(lo, hi, pa[q], witness(j));
+      if (j + kPF < NW) {
+        const uint32_t cn = col_of(j + kPF);
 #pragma unroll
-      for (int i = 0; i < NI; i++) {
-        mac96(lo[i][0], hi[i][0], pa[q][i].x, s0.x);
-        mac96(lo[i][1], hi[i][1], pa[q][i].y, s0.y);
-        mac96(lo[i][2], hi[i][2], pa[q][i].z, s0.z);
-        mac96(lo[i][3], hi[i][3], pa[q][i].w, s0.w);
+        for (int i = 0; i < NI; i++)
+          pa[q][i] = __builtin_nontemporal_load(akey + (size_t)(ch + cn) * col4 + i * 64);
       }
     }
-  }
-  for (uint32_t c = wave + PF * WAVES; c < ncol; c += 2 * WAVES) {
-    const bool two = c + WAVES < ncol;
-    u32x4 va[NI], vb[NI];
+  } else {
+    for (uint32_t j0 = 0; j0 < nwm; j0 += PF) {
 #pragma unroll
-    for (int i = 0; i < NI; i++)
-      va[i] = __builtin_nontemporal_load(akey + (size_t)(ch + c) * col4 + i * 64);
-    if (two) {
+      for (int q = 0; q < PF; q++) {
+        const uint32_t j = j0 + q;
+        if (j < nwm) {
+          mac_column<NI>(lo, hi, pa[q], witness(j));
+          if (j + PF < nwm) {
+            const uint32_t cn = col_of(j + PF);
 #pragma unroll
-      for (int i = 0; i < NI; i++)
-        vb[i] = __builtin_nontemporal_load(akey + (size_t)(ch + c + WAVES) * col4 + i * 64);
-    }
-    const u32x4 s0 = *reinterpret_cast<const u32x4*>(lds + c * kFuseStride + k0);
-#pragma unroll
-    for (int i = 0; i < NI; i++) {
-      mac96(lo[i][0], hi[i][0], va[i].x, s0.x);
-      mac96(lo[i][1], hi[i][1], va[i].y, s0.y);
-      mac96(lo[i][2], hi[i][2], va[i].z, s0.z);
-      mac96(lo[i][3], hi[i][3], va[i].w, s0.w);
-    }
-    if (two) {
-      const u32x4 s1 = *reinterpret_cast<const u32x4*>(lds + (c + WAVES) * kFuseStride + k0);
-#pragma unroll
-      for (int i = 0; i < NI; i++) {
-        mac96(lo[i][0], hi[i][0], vb[i].x, s1.x);
-        mac96(lo[i][1], hi[i][1], vb[i].y, s1.y);
-        mac96(lo[i][2], hi[i][2], vb[i].z, s1.z);
-        mac96(lo[i][3], hi[i][3], vb[i].w, s1.w);
+            for (int i = 0; i < NI; i++)
+              pa[q][i] = __builtin_nontemporal_load(akey + (size_t)(ch + cn) * col4 + i * 64);
+          }
+        }
       }
     }
   }
@@ -759,16 +782,31 @@ static int mac_launch(gh_key* key, const void* s_ntt, uint32_t cols, void* parti
   return GH_OK;
 }
 
+template <int RB, int WAVES, int DEPTH>
+static void fused_launch_nw(gh_ctx* ctx, dim3 grid, size_t lds_bytes, hipEvent_t e0,
+                            hipEvent_t e1, FQ32 f, const uint32_t* tab, const uint32_t* A,
+                            const uint32_t* S, uint32_t* P, uint32_t rows, uint32_t cols,
+                            uint32_t cpb) {
+  // the unrolled sweep when every full block has the same columns per wave
+  const uint32_t nw = cpb % WAVES == 0 ? cpb / WAVES : 0;
+#define GH_FUSED_NW(N)                                                                          \
+  hipExtLaunchKernelGGL((k_commit_fused_q32<RB, WAVES, DEPTH, N>), grid, dim3(64 * WAVES),      \
+                        lds_bytes, ctx->stream, e0, e1, 0, f, tab, A, S, P, rows, cols, cpb)
+  if (nw == 8) GH_FUSED_NW(8);
+  else if (nw == 16) GH_FUSED_NW(16);
+  else if (nw == 32) GH_FUSED_NW(32);
+  else GH_FUSED_NW(0);
+#undef GH_FUSED_NW
+}
+
 template <int RB, int WAVES>
 static void fused_launch(gh_ctx* ctx, dim3 grid, size_t lds_bytes, hipEvent_t e0, hipEvent_t e1,
                          FQ32 f, const uint32_t* tab, const uint32_t* A, const uint32_t* S,
                          uint32_t* P, uint32_t rows, uint32_t cols, uint32_t cpb) {
   if (ctx->tune.fused_depth == 2)
-    hipExtLaunchKernelGGL((k_commit_fused_q32<RB, WAVES, 2>), grid, dim3(64 * WAVES), lds_bytes,
-                          ctx->stream, e0, e1, 0, f, tab, A, S, P, rows, cols, cpb);
+    fused_launch_nw<RB, WAVES, 2>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb);
   else
-    hipExtLaunchKernelGGL((k_commit_fused_q32<RB, WAVES, 1>), grid, dim3(64 * WAVES), lds_bytes,
-                          ctx->stream, e0, e1, 0, f, tab, A, S, P, rows, cols, cpb);
+    fused_launch_nw<RB, WAVES, 1>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb);
 }
 
 // q32 commitment in two launches on the context stream: the fused
