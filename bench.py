@@ -15,11 +15,12 @@ baseline (the reference's AVX2 prepared-commit path restated in C, run on this
 host), and the side workloads, each checked against the oracle or the plain
 form: the batched form at the same shape, kappa 8 x n 256 commitments
 (`small`), 2^18 negacyclic products (`negacyclic`), LaBRADOR-scale inner
-commitments, Gram matrix and JL projection (`labrador`), 2000 Goldilocks
-commitments batched, graph-replayed and per call (`goldilocks`), and the serving
-loop against the plain per-call form on a Poisson stream of mixed requests
-(`serving`). `--only NAME` runs one side workload alone and prints its JSON
-(for profiling).
+commitments, Gram matrix, JL projection and the steps around them
+(`labrador`), 2000 Goldilocks commitments batched, graph-replayed and per call
+(`goldilocks`), the serving loop against the plain per-call form on a Poisson
+stream of mixed requests (`serving`), and key persistence and RingVec wire
+encodings at the headline shape (`wire`). `--only NAME` runs one side workload
+alone and prints its JSON (for profiling).
 """
 import argparse
 import json
@@ -308,6 +309,71 @@ def labrador_calls(ctx, la, ring, log, reps=10):
     return res
 
 
+def wire_formats(ctx, la, ring, log, reps=5):
+    """Canonical wire formats and key persistence at the headline shape: the
+    kappa 16 x 2^15 q32 key saved as LinearCommitmentKey<TwistedNttPoly> bytes
+    (134 MB) and loaded back into a resident key, and a 2^15-polynomial witness
+    RingVec (8 MB) encoded from and decoded into device memory. Host bytes are
+    pageable numpy buffers, so the rates include the PCIe copies; the loaded
+    key's commitment must equal the original's."""
+    import ctypes
+    rng = np.random.default_rng(91)
+    a = rng.integers(0, Q, size=(KAPPA, N, D), dtype=np.uint32)
+    key = la.Key(ring, a, domain=la.DOMAIN_NTT)
+    res = {}
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        blob = key.save(la.DOMAIN_NTT)
+    dt = (time.perf_counter() - t0) / reps
+    res["key_save_ntt"] = {"ms": round(dt * 1e3, 2), "bytes": len(blob),
+                           "GBps": round(len(blob) / dt / 1e9, 2)}
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        k2 = ring.load_key(blob, la.DOMAIN_NTT)
+        k2.close()
+    dt = (time.perf_counter() - t0) / reps
+    res["key_load_ntt"] = {"ms": round(dt * 1e3, 2), "GBps": round(len(blob) / dt / 1e9, 2)}
+    k2 = ring.load_key(blob, la.DOMAIN_NTT)
+    s = witness(rng, N)
+    res["loaded_key_commits_equal"] = bool(np.array_equal(k2.commit(s), key.commit(s)))
+    k2.close()
+    # RingVec of the witness, device words <-> host bytes
+    s_dev = ctx.to_device(s)
+    out_dev = ctx.alloc(s.nbytes)
+    L = la.lib()
+    size = ring.wire_size(la.WIRE_RINGVEC, N)
+    enc = np.zeros(size, dtype=np.uint8)
+    wr = ctypes.c_uint64()
+    t0 = time.perf_counter()
+    for _ in range(reps * 4):
+        st = L.gh_wire_encode(ring.h, la.WIRE_RINGVEC, N, 1, s_dev.ptr, la.MEM_DEVICE,
+                              enc.ctypes.data, size, ctypes.byref(wr))
+        if st:
+            raise la.GridHipError(st, "gh_wire_encode")
+    dt = (time.perf_counter() - t0) / (reps * 4)
+    res["ringvec_encode_2^15_polys"] = {"us": round(dt * 1e6, 1), "GBps": round(size / dt / 1e9, 2)}
+    t0 = time.perf_counter()
+    for _ in range(reps * 4):
+        st = L.gh_wire_decode(ring.h, la.WIRE_RINGVEC, la.DOMAIN_COEFF, enc.ctypes.data, size, N,
+                              1, out_dev.ptr, la.MEM_DEVICE, None)
+        if st:
+            raise la.GridHipError(st, "gh_wire_decode")
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / (reps * 4)
+    res["ringvec_decode_2^15_polys"] = {"us": round(dt * 1e6, 1), "GBps": round(size / dt / 1e9, 2)}
+    res["ringvec_round_trip_equal"] = bool(np.array_equal(out_dev.download(np.uint32, s.shape), s))
+    log(f"wire: key save {res['key_save_ntt']['ms']:.1f} ms ({res['key_save_ntt']['GBps']:.1f} GB/s), "
+        f"load {res['key_load_ntt']['ms']:.1f} ms ({res['key_load_ntt']['GBps']:.1f} GB/s), loaded key "
+        f"commits equal {res['loaded_key_commits_equal']}; RingVec 2^15 polys encode "
+        f"{res['ringvec_encode_2^15_polys']['us']:,.0f} us, decode "
+        f"{res['ringvec_decode_2^15_polys']['us']:,.0f} us, round trip "
+        f"{res['ringvec_round_trip_equal']}")
+    key.close()
+    s_dev.free()
+    out_dev.free()
+    return res
+
+
 def serving(ctx, la, log, count=1000, seed=55):
     """Serving: a Poisson arrival stream of mixed commit and Gram requests, half
     q32 and half Goldilocks (d = 64, kappa = 16 keys of 2^15 columns, both
@@ -489,7 +555,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-check", action="store_true")
-    ap.add_argument("--only", choices=["small", "negacyclic", "labrador", "goldilocks", "serving"],
+    ap.add_argument("--only", choices=["small", "negacyclic", "labrador", "goldilocks", "serving",
+                                       "wire"],
                     help="run one side workload alone (for profiling) and print its JSON")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -510,7 +577,8 @@ def main():
                 "negacyclic": lambda: negacyclic_batch(ctx, la, ring, log),
                 "labrador": lambda: labrador_calls(ctx, la, ring, log),
                 "goldilocks": lambda: goldilocks_batch(ctx, la, log),
-                "serving": lambda: serving(ctx, la, log)}
+                "serving": lambda: serving(ctx, la, log),
+                "wire": lambda: wire_formats(ctx, la, ring, log)}
         print(json.dumps({"workload": args.only, "result": legs[args.only]()}), flush=True)
         ring.close()
         ctx.close()
@@ -621,6 +689,7 @@ def main():
         extra["labrador_q32_r32_n1024"] = labrador_calls(ctx, la, ring, log)
         extra["goldilocks_batch_2000"] = goldilocks_batch(ctx, la, log)
         extra["serving_mixed_1000"] = serving(ctx, la, log)
+        extra["wire_formats"] = wire_formats(ctx, la, ring, log)
 
     cpu = None
     if not args.no_cpu:

@@ -893,6 +893,7 @@ GH_API int gh_jl_project(gh_ring* ring, const uint8_t seed[32], uint32_t rows, c
   if (cols % ring->d != 0) return GH_E_INVALID_ARGUMENT;
   if (p_exact && ring->kind != GH_RING_Q32) return GH_E_INVALID_ARGUMENT;
   if (cols > 0xffffffffull / ((rows + 3) / 4)) return GH_E_INVALID_ARGUMENT;  // 32-bit block counter
+  if (num_parts > 65535) return GH_E_INVALID_ARGUMENT;  // one grid row per part
   gh_ctx* ctx = ring->ctx;
   const size_t wb = ring->word_bytes;
   const size_t in_bytes = (size_t)num_parts * cols * wb;

@@ -191,6 +191,10 @@ int launch_jl_conj_seed(gh_ring* ring, const uint8_t seed[32], uint32_t rows, ui
                         uint64_t n, void* out);
 int launch_jl_conj_flat(gh_ring* ring, const int8_t* flat, uint32_t rows, uint32_t num_parts,
                         uint64_t n, void* out);
+// p = Pi * s on the matrix cores; rows in [253, 256] (one keystream block per
+// column); p_exact (q32 only, nullable): the exact centred sum
+int launch_jl_proj(gh_ring* ring, const uint8_t seed[32], uint32_t rows, const void* parts,
+                   uint32_t num_parts, uint64_t n, void* p_modq, int64_t* p_exact);
 // rows = 256; parts: num_parts x n polys, coefficient domain; out: 256 polys
 int launch_jl_evals(gh_ring* ring, const uint8_t seed[32], const void* parts, uint32_t num_parts,
                     uint64_t n, void* out);

@@ -61,6 +61,43 @@ __device__ void jl_column_lds(const ChaKey& key, uint64_t col, uint32_t pk, uint
   }
 }
 
+constexpr int kEvPolys = 8;       // witness polynomials per LDS round
+constexpr int kEvRowBytes = 64;   // A tile: [poly][m][k] int8, k contiguous
+
+// Decodes this lane's keystream column (16 words, 2 bits per row, 256 rows)
+// into int8 entries stored as rows of columns: the dword holding row m of the
+// four columns of this lane's quad goes to base + m * row_stride + 4 * (k / 4),
+// k = the lane's column. Four rows at a time: their four codes are spread one
+// per byte and mapped through a byte table (00 -> +1, 01 -> -1, else 0) by
+// v_perm; then a 4 x 4 byte transpose across the lane quad (two DPP
+// exchanges) leaves each lane one dword holding four columns of one row,
+// stored whole: lane r of the quad stores rows 4 g + r. Rows >= `rows` are not
+// stored. All 64 lanes of the wave must be active (the exchanges read
+// neighbours); `base` + offsets must be 4-byte aligned.
+__device__ __forceinline__ void jl_quad_store(const uint32_t (&blk)[16], int8_t* base,
+                                              uint64_t row_stride, uint32_t k, uint32_t rows) {
+  const uint32_t r = k & 3, a4 = k & ~3u;
+  const uint32_t sel1 = (r & 1) ? 0x03070105u : 0x06020400u;  // pair columns (r, r ^ 1)
+  const uint32_t sel2 = (r & 2) ? 0x03020706u : 0x05040100u;  // then pairs (r, r ^ 2)
+#pragma unroll
+  for (int g = 0; g < 64; g++) {
+    const uint32_t x = (blk[g >> 2] >> (8 * (g & 3))) & 0xffu;
+    const uint32_t y = (x | (x << 6) | (x << 12) | (x << 18)) & 0x03030303u;
+    const uint32_t e = __builtin_amdgcn_perm(0u, 0x0000FF01u, y);
+    const uint32_t pn = (uint32_t)__builtin_amdgcn_mov_dpp((int)e, 0xB1, 0xF, 0xF, false);
+    const uint32_t z = __builtin_amdgcn_perm(pn, e, sel1);
+    const uint32_t qn = (uint32_t)__builtin_amdgcn_mov_dpp((int)z, 0x4E, 0xF, 0xF, false);
+    const uint32_t w = __builtin_amdgcn_perm(qn, z, sel2);
+    const uint32_t m = 4 * g + r;
+    if (m < rows) *reinterpret_cast<uint32_t*>(base + (uint64_t)m * row_stride + a4) = w;
+  }
+}
+
+__device__ __forceinline__ void jl_tile_store(const uint32_t (&blk)[16], int8_t* tile_poly,
+                                              uint32_t k) {
+  jl_quad_store(blk, tile_poly, kEvRowBytes, k, 256);
+}
+
 // grid (ceil(n / 256), d, parts), 256 threads; rows <= 256
 __global__ __launch_bounds__(256) void k_jl_flat(ChaKey key0, uint32_t rows, uint32_t num_parts,
                                                  uint64_t n, uint32_t d, int8_t* __restrict__ flat) {
@@ -74,6 +111,16 @@ __global__ __launch_bounds__(256) void k_jl_flat(ChaKey key0, uint32_t rows, uin
   const uint32_t pk = (rows + 3) / 4;
   const uint64_t row_stride = (uint64_t)num_parts * d * n;
   int8_t* dst = flat + ((uint64_t)part * d + coeff) * n + poly;
+  if (pk == 64 && n % 4 == 0 && ((uintptr_t)flat & 3) == 0 &&
+      (uint64_t)blockIdx.x * 256 + 256 <= n) {
+    // every polynomial of the block exists and rows start 4-byte aligned:
+    // dword stores of four polynomials per row through the quad transpose
+    uint32_t blk[16];
+    chacha_block(key, (uint32_t)col, blk);
+    jl_quad_store(blk, flat + ((uint64_t)part * d + coeff) * n + (uint64_t)blockIdx.x * 256,
+                  row_stride, t, rows);
+    return;
+  }
   if (pk == 64) {
     if (poly >= n) return;
     uint32_t blk[16];
@@ -164,12 +211,20 @@ __global__ __launch_bounds__(256) void k_jl_conj_flat(const int8_t* __restrict__
 
 // ---- JL evaluations on the int8 matrix cores ---------------------------------
 
-constexpr int kEvPolys = 8;       // witness polynomials per LDS round
-constexpr int kEvRowBytes = 64;   // A tile: [poly][m][k] int8, k contiguous
 
 // Centred lift of a residue into the window that D signed base-256 digits
 // cover exactly ([-128 (256^D - 1) / 255, 127 (256^D - 1) / 255], 2^(8D)
 // integers), then its digits, least significant first.
+template <int D>
+__device__ __forceinline__ void signed_digits(int64_t v, int8_t (&dig)[D]) {
+#pragma unroll
+  for (int b = 0; b < D; b++) {
+    const int64_t lo = (int64_t)(int8_t)(uint8_t)(v & 0xff);  // in [-128, 127]
+    dig[b] = (int8_t)lo;
+    v = (v - lo) >> 8;
+  }
+}
+
 template <int D>
 __device__ __forceinline__ void lift_digits(uint64_t x, uint64_t q, int8_t (&dig)[D]) {
   int64_t v;
@@ -178,12 +233,7 @@ __device__ __forceinline__ void lift_digits(uint64_t x, uint64_t q, int8_t (&dig
   } else {
     v = x <= 0x7F7F7F7F7F7F7F7Full ? (int64_t)x : (int64_t)(x - q);
   }
-#pragma unroll
-  for (int b = 0; b < D; b++) {
-    const int64_t lo = (int64_t)(int8_t)(uint8_t)(v & 0xff);  // in [-128, 127]
-    dig[b] = (int8_t)lo;
-    v = (v - lo) >> 8;
-  }
+  signed_digits<D>(v, dig);
 }
 
 // Block: 8 waves for digits dbase .. dbase + 3 of DT; wave w owns digit
@@ -216,21 +266,17 @@ __global__ __launch_bounds__(512) void k_jl_evals(ChaKey key0, const W* __restri
     // -- generation: A tile and digit windows, one column / coefficient per thread
     {
       const uint32_t ps = t >> 6, k = t & 63;
-      const uint64_t g = g0 + ps;
-      int8_t* arow = a_tile + ps * 256 * kEvRowBytes + k;
+      const uint64_t g = g0 + ps;  // wave-uniform
+      uint32_t blk[16];
       if (g < g_end) {
         ChaKey key = key0;
         key.n0 = (uint32_t)(g / n);
-        const uint64_t col = (g % n) * 64 + k;
-        uint32_t blk[16];
-        chacha_block(key, (uint32_t)col, blk);
-#pragma unroll
-        for (int m = 0; m < 256; m++)
-          arow[m * kEvRowBytes] = (int8_t)jl_entry((blk[m >> 4] >> (2 * (m & 15))) & 3);
+        chacha_block(key, (uint32_t)((g % n) * 64 + k), blk);
       } else {
-#pragma unroll 8
-        for (int m = 0; m < 256; m++) arow[m * kEvRowBytes] = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) blk[i] = 0xAAAAAAAAu;  // code 10 everywhere: entry 0
       }
+      jl_tile_store(blk, a_tile + ps * 256 * kEvRowBytes, k);
     }
     {
       const uint32_t ps = t >> 6, u = t & 63;
@@ -357,7 +403,159 @@ __global__ __launch_bounds__(64 * D) void k_jl_evals_fin(const int32_t* __restri
   out[(uint64_t)m * 64 + nn] = (W)r;
 }
 
+// ---- JL projection on the int8 matrix cores -----------------------------------
+//
+// p[m] = sum_c Pi[m][c] * s_c: the same A tile as the evaluations against one
+// B column per digit of s_c (a 32-wide tile, 8 columns used): the centred
+// lift in 5 signed digits for the 32-bit prime, so the int64 sum is the
+// reference's exact p (jl.rs:388-508), or the Goldilocks window lift in 8
+// digits, reduced mod p. Wave w owns rows 32 w .. 32 w + 31.
+constexpr int kProjDigits = 8;  // digit slots per witness word (B columns used)
+
+template <class W, bool GOLD>
+__global__ __launch_bounds__(512) void k_jl_proj(ChaKey key0, const W* __restrict__ parts,
+                                                 uint64_t n, uint64_t total_polys,
+                                                 uint64_t polys_per_block, uint64_t q,
+                                                 int32_t* __restrict__ partial) {
+  __shared__ __attribute__((aligned(16))) int8_t a_tile[kEvPolys * 256 * kEvRowBytes];
+  __shared__ __attribute__((aligned(16))) int8_t d_tile[kEvPolys * kProjDigits * 64];
+  const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const uint64_t g_begin = (uint64_t)blockIdx.x * polys_per_block;
+  const uint64_t g_end = std::min<uint64_t>(g_begin + polys_per_block, total_polys);
+  v16i acc = v16i{};
+  for (uint64_t g0 = g_begin; g0 < g_end; g0 += kEvPolys) {
+    {
+      const uint32_t ps = t >> 6, k = t & 63;
+      const uint64_t g = g0 + ps;  // wave-uniform
+      uint32_t blk[16];
+      int64_t v = 0;
+      if (g < g_end) {
+        ChaKey key = key0;
+        key.n0 = (uint32_t)(g / n);
+        chacha_block(key, (uint32_t)((g % n) * 64 + k), blk);
+        const uint64_t x = (uint64_t)parts[g * 64 + k];
+        if (GOLD)
+          v = x <= 0x7F7F7F7F7F7F7F7Full ? (int64_t)x : (int64_t)(x - q);
+        else
+          v = x > q / 2 ? (int64_t)x - (int64_t)q : (int64_t)x;  // centred, as p_exact
+      } else {
+#pragma unroll
+        for (int i = 0; i < 16; i++) blk[i] = 0xAAAAAAAAu;
+      }
+      jl_tile_store(blk, a_tile + ps * 256 * kEvRowBytes, k);
+      int8_t dig[kProjDigits];
+      signed_digits<kProjDigits>(v, dig);
+#pragma unroll
+      for (int b = 0; b < kProjDigits; b++) d_tile[(ps * kProjDigits + b) * 64 + k] = dig[b];
+    }
+    __syncthreads();
+    const uint32_t row = lane & 31, kg = lane >> 5;
+    for (int ps = 0; ps < kEvPolys; ps++) {
+      if (g0 + ps >= g_end) break;
+#pragma unroll
+      for (int kstep = 0; kstep < 2; kstep++) {
+        const uint32_t k0 = kstep * 32 + kg * 16;
+        const v4i afrag = *reinterpret_cast<const v4i*>(
+            a_tile + (ps * 256 + wave * 32 + row) * kEvRowBytes + k0);
+        const v4i bfrag = row < kProjDigits
+                              ? *reinterpret_cast<const v4i*>(
+                                    d_tile + (ps * kProjDigits + row) * 64 + k0)
+                              : v4i{0, 0, 0, 0};
+        acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(afrag, bfrag, acc, 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  const uint32_t col_in = lane & 31, rgrp = lane >> 5;
+  if (col_in < kProjDigits) {
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const uint32_t m = wave * 32 + 8 * (r >> 2) + 4 * rgrp + (r & 3);
+      partial[((uint64_t)blockIdx.x * 256 + m) * kProjDigits + col_in] = acc[r];
+    }
+  }
+}
+
+// p[m] from the block partials: 32 lanes per digit sum the blocks, a tree
+// in LDS, then the digits recombine (exactly for the 32-bit prime, mod p for
+// Goldilocks). Block m, 256 threads.
+template <bool GOLD>
+__global__ __launch_bounds__(256) void k_jl_proj_fin(const int32_t* __restrict__ partial,
+                                                     uint32_t nblk, uint64_t q, void* p_modq,
+                                                     int64_t* __restrict__ p_exact) {
+  __shared__ int64_t red[kProjDigits][32];
+  const uint32_t m = blockIdx.x, b = threadIdx.x >> 5, l = threadIdx.x & 31;
+  int64_t s = 0;
+  for (uint32_t blk = l; blk < nblk; blk += 32)
+    s += partial[((uint64_t)blk * 256 + m) * kProjDigits + b];
+  red[b][l] = s;
+  __syncthreads();
+  for (uint32_t h = 16; h > 0; h >>= 1) {
+    if (l < h) red[b][l] += red[b][l + h];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  if (!GOLD) {
+    int64_t e = 0;
+#pragma unroll
+    for (int d = kProjDigits - 1; d >= 0; d--) e = e * 256 + red[d][0];
+    if (p_exact) p_exact[m] = e;
+    int64_t r = e % (int64_t)q;
+    if (r < 0) r += (int64_t)q;
+    ((uint32_t*)p_modq)[m] = (uint32_t)r;
+  } else {
+    uint64_t r = 0;
+#pragma unroll
+    for (int d = kProjDigits - 1; d >= 0; d--) {
+      const int64_t sb = red[d][0];
+      r = gold_shift8(r);
+      const uint64_t a = sb < 0 ? (uint64_t)(-sb) : (uint64_t)sb;
+      if (sb >= 0) {
+        const uint64_t x = r + a;
+        r = (x < r || x >= q) ? x - q : x;
+      } else {
+        r = r >= a ? r - a : r + (q - a);
+      }
+    }
+    ((uint64_t*)p_modq)[m] = r;
+  }
+}
+
 }  // namespace
+
+int launch_jl_proj(gh_ring* ring, const uint8_t seed[32], uint32_t rows, const void* parts,
+                   uint32_t num_parts, uint64_t n, void* p_modq, int64_t* p_exact) {
+  gh_ctx* ctx = ring->ctx;
+  const bool gold = ring->kind == GH_RING_GOLDILOCKS;
+  const uint64_t total = (uint64_t)num_parts * n;
+  uint64_t nblk = std::min<uint64_t>((uint64_t)ctx->num_cus * 2, (total + kEvPolys - 1) / kEvPolys);
+  if (nblk == 0) nblk = 1;
+  uint64_t ppb = (total + nblk - 1) / nblk;
+  ppb = (ppb + kEvPolys - 1) / kEvPolys * kEvPolys;
+  nblk = (total + ppb - 1) / ppb;
+  if (nblk == 0) nblk = 1;
+  void* ws;
+  int st = ctx_workspace(ctx, (size_t)nblk * 256 * kProjDigits * sizeof(int32_t), &ws);
+  if (st) return st;
+  const ChaKey key = chacha_key(seed);
+  if (!gold)
+    hipLaunchKernelGGL((k_jl_proj<uint32_t, false>), dim3((unsigned)nblk), dim3(512), 0,
+                       ctx->stream, key, (const uint32_t*)parts, n, total, ppb, ring->q,
+                       (int32_t*)ws);
+  else
+    hipLaunchKernelGGL((k_jl_proj<uint64_t, true>), dim3((unsigned)nblk), dim3(512), 0,
+                       ctx->stream, key, (const uint64_t*)parts, n, total, ppb, ring->q,
+                       (int32_t*)ws);
+  GH_HIP(hipGetLastError());
+  if (!gold)
+    hipLaunchKernelGGL((k_jl_proj_fin<false>), dim3(rows), dim3(256), 0, ctx->stream,
+                       (const int32_t*)ws, (uint32_t)nblk, ring->q, p_modq, p_exact);
+  else
+    hipLaunchKernelGGL((k_jl_proj_fin<true>), dim3(rows), dim3(256), 0, ctx->stream,
+                       (const int32_t*)ws, (uint32_t)nblk, ring->q, p_modq, nullptr);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
 
 int launch_jl_flat(gh_ring* ring, const uint8_t seed[32], uint32_t rows, uint32_t num_parts,
                    uint64_t cols, int8_t* flat) {

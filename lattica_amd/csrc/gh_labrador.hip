@@ -597,6 +597,14 @@ __global__ __launch_bounds__(256) void k_jl_reduce(const int64_t* __restrict__ p
 int launch_jl(gh_ring* ring, const uint8_t seed[32], uint32_t rows, const void* parts,
               uint32_t num_parts, uint64_t cols, void* p_modq, int64_t* p_exact, int8_t* flat) {
   gh_ctx* ctx = ring->ctx;
+  // one keystream block per column (rows 253..256, the reference profile's 256
+  // among them): the matrix-core projection; otherwise the row-per-thread kernel
+  if ((rows + 3) / 4 == 64) {
+    int st = launch_jl_proj(ring, seed, rows, parts, num_parts, cols / ring->d, p_modq, p_exact);
+    if (st) return st;
+    if (flat) return launch_jl_flat(ring, seed, rows, num_parts, cols, flat);
+    return GH_OK;
+  }
   const ChaKey key = chacha_key(seed);
   // chunks per part so that the device holds a few blocks per CU
   uint64_t want = (uint64_t)ctx->num_cus * 4 / (num_parts ? num_parts : 1);

@@ -17,21 +17,45 @@ pytestmark = pytest.mark.gpu
 SEED = bytes(range(7, 39))
 
 
-@pytest.mark.parametrize("rows,num_parts,n", [(256, 2, 5), (256, 3, 300), (253, 1, 7),
-                                              (100, 2, 9), (7, 3, 257), (1, 1, 1)])
+@pytest.mark.parametrize("rows,num_parts,n", [(256, 2, 5), (256, 3, 300), (256, 2, 512),
+                                              (254, 1, 256), (253, 1, 7), (100, 2, 9),
+                                              (7, 3, 257), (1, 1, 1)])
 def test_extract_flat_matches_oracle(oracle, ring_q32, rows, num_parts, n):
     got = ring_q32.jl_extract_flat(SEED, rows, num_parts, n * 64)
     want = oracle.jl_extract_flat(SEED, rows, num_parts, n * 64, 64)
     assert np.array_equal(got, want)
 
 
-def test_project_flat_output_matches_oracle(oracle, ring_q32):
-    rng = np.random.default_rng(4)
-    parts = rand_residues(rng, Q32, (3, 70 * 64))
-    pm, pe, fl = ring_q32.jl_project(SEED, 256, parts, want_flat=True)
-    opm, ope, ofl = oracle.jl_project(Q32, SEED, 256, parts, 64, want_flat=True)
+@pytest.mark.parametrize("rows,n", [(256, 70), (256, 256), (255, 9), (200, 33)])
+def test_project_flat_output_matches_oracle(oracle, ring_q32, rows, n):
+    """256-row projections run on the matrix cores, others on the row-per-thread kernel."""
+    rng = np.random.default_rng(4 + n)
+    parts = rand_residues(rng, Q32, (3, n * 64))
+    pm, pe, fl = ring_q32.jl_project(SEED, rows, parts, want_flat=True)
+    opm, ope, ofl = oracle.jl_project(Q32, SEED, rows, parts, 64, want_flat=True)
     assert np.array_equal(pm.astype(np.uint64), opm) and np.array_equal(pe, ope)
     assert np.array_equal(fl, ofl)
+
+
+@pytest.mark.parametrize("n", [1, 9, 300])
+def test_project_goldilocks_matches_oracle(oracle, ring_gold, n):
+    rng = np.random.default_rng(40 + n)
+    parts = rand_residues(rng, GOLDILOCKS, (2, n * 64))
+    pm, _, _ = ring_gold.jl_project(SEED, 256, parts, want_exact=False)
+    opm, _, _ = oracle.jl_project(GOLDILOCKS, SEED, 256, parts, 64)
+    assert np.array_equal(pm.astype(np.uint64), opm)
+
+
+def test_project_extreme_words(oracle, ring_q32):
+    """Centred lifts at the edges: 0, 1, q - 1, (q - 1) / 2, (q + 1) / 2 in 5 digits."""
+    q = Q32
+    edges = np.array([0, 1, q - 1, (q - 1) // 2, (q + 1) // 2, 2139062143, 2139062144],
+                     dtype=np.uint64)
+    rng = np.random.default_rng(5)
+    parts = edges[rng.integers(0, len(edges), size=(2, 50 * 64))]
+    pm, pe, _ = ring_q32.jl_project(SEED, 256, parts)
+    opm, ope, _ = oracle.jl_project(q, SEED, 256, parts, 64)
+    assert np.array_equal(pm.astype(np.uint64), opm) and np.array_equal(pe, ope)
 
 
 @pytest.mark.parametrize("name", ["q32", "gold"])
