@@ -262,44 +262,21 @@ __global__ __launch_bounds__(512) void k_jl_evals(ChaKey key0, const W* __restri
 #pragma unroll
     for (int h = 0; h < 2; h++) acc[i][h] = v16i{};
 
-  for (uint64_t g0 = g_begin; g0 < g_end; g0 += kEvPolys) {
-    // -- generation: A tile and digit windows, one column / coefficient per thread
-    {
-      const uint32_t ps = t >> 6, k = t & 63;
-      const uint64_t g = g0 + ps;  // wave-uniform
-      uint32_t blk[16];
-      if (g < g_end) {
-        ChaKey key = key0;
-        key.n0 = (uint32_t)(g / n);
-        chacha_block(key, (uint32_t)((g % n) * 64 + k), blk);
-      } else {
+  // keystream of this thread's column of round g0 (polynomial g0 + t / 64),
+  // code 10 (entry 0) past the block's end
+  auto keystream = [&](uint64_t g0, uint32_t (&blk)[16]) {
+    const uint64_t g = g0 + (t >> 6);  // wave-uniform
+    if (g < g_end) {
+      ChaKey key = key0;
+      key.n0 = (uint32_t)(g / n);
+      chacha_block(key, (uint32_t)((g % n) * 64 + (t & 63)), blk);
+    } else {
 #pragma unroll
-        for (int i = 0; i < 16; i++) blk[i] = 0xAAAAAAAAu;  // code 10 everywhere: entry 0
-      }
-      jl_tile_store(blk, a_tile + ps * 256 * kEvRowBytes, k);
+      for (int i = 0; i < 16; i++) blk[i] = 0xAAAAAAAAu;
     }
-    {
-      const uint32_t ps = t >> 6, u = t & 63;
-      const uint64_t g = g0 + ps;
-      int8_t dp[DT], dn[DT];
-      if (g < g_end) {
-        const uint64_t x = (uint64_t)parts[g * 64 + u];
-        lift_digits<DT>(x, q, dp);
-        lift_digits<DT>(x == 0 ? 0 : q - x, q, dn);
-      } else {
-#pragma unroll
-        for (int b = 0; b < DT; b++) dp[b] = dn[b] = 0;
-      }
-      int8_t* e = e_tile + ps * D * 128;
-#pragma unroll
-      for (int b = 0; b < D; b++) {
-        e[b * 128 + u] = dp[dbase + b];
-        e[b * 128 + 64 + u] = dn[dbase + b];
-      }
-    }
-    __syncthreads();
-    // -- matrix-core phase
-    const uint32_t row = lane & 31, kg = lane >> 5;
+  };
+  const uint32_t row = lane & 31, kg = lane >> 5;
+  auto mfma_round = [&](uint64_t g0) {
     for (int ps = 0; ps < kEvPolys; ps++) {
       if (g0 + ps >= g_end) break;
       const int8_t* at = a_tile + ps * 256 * kEvRowBytes;
@@ -328,6 +305,46 @@ __global__ __launch_bounds__(512) void k_jl_evals(ChaKey key0, const W* __restri
             acc[i][h] = __builtin_amdgcn_mfma_i32_32x32x32_i8(afrag, bfrag[h], acc[i][h], 0, 0, 0);
         }
       }
+    }
+  };
+
+  // Each round: decode the keystream held in registers into the A tile and
+  // the round's digit windows, then run the matrix cores on it while drawing
+  // the next round's keystream (waves 0-3 draw first, waves 4-7 multiply
+  // first, so the two kinds of work can share a SIMD). The multiply phase
+  // reads about 0.8 KB of operands from LDS per MFMA, close to the LDS
+  // bandwidth, which is what bounds it.
+  uint32_t blk[16];
+  keystream(g_begin, blk);
+  for (uint64_t g0 = g_begin; g0 < g_end; g0 += kEvPolys) {
+    jl_tile_store(blk, a_tile + (t >> 6) * 256 * kEvRowBytes, t & 63);
+    {
+      const uint32_t ps = t >> 6, u = t & 63;
+      const uint64_t g = g0 + ps;
+      int8_t dp[DT], dn[DT];
+      if (g < g_end) {
+        const uint64_t x = (uint64_t)parts[g * 64 + u];
+        lift_digits<DT>(x, q, dp);
+        lift_digits<DT>(x == 0 ? 0 : q - x, q, dn);
+      } else {
+#pragma unroll
+        for (int b = 0; b < DT; b++) dp[b] = dn[b] = 0;
+      }
+      int8_t* e = e_tile + ps * D * 128;
+#pragma unroll
+      for (int b = 0; b < D; b++) {
+        e[b * 128 + u] = dp[dbase + b];
+        e[b * 128 + 64 + u] = dn[dbase + b];
+      }
+    }
+    __syncthreads();
+    const uint64_t gn = g0 + kEvPolys;  // block-uniform
+    if (wave < 4) {
+      if (gn < g_end) keystream(gn, blk);
+      mfma_round(g0);
+    } else {
+      mfma_round(g0);
+      if (gn < g_end) keystream(gn, blk);
     }
     __syncthreads();
   }
