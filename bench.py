@@ -141,6 +141,17 @@ def goldilocks_batch(ctx, la, log, count=2000, kappa=16, max_cols=1 << 14):
             "mean_len": float(lens.mean()), "oracle_spot_check": ok}
 
 
+def cpu_seconds(fn, min_s=0.5):
+    """Seconds per call of fn on this host (the oracle, one core), at least min_s of runs."""
+    fn()
+    n, a = 0, time.perf_counter()
+    while True:
+        fn()
+        n += 1
+        if time.perf_counter() - a >= min_s:
+            return (time.perf_counter() - a) / n
+
+
 def negacyclic_batch(ctx, la, ring, log, pairs=1 << 18, reps=20):
     """Batched negacyclic products a*b mod (X^64 + 1), 2^18 pairs of uniform
     residues on resident device buffers (one gh_negacyclic_mul call); a sample
@@ -166,10 +177,16 @@ def negacyclic_batch(ctx, la, ring, log, pairs=1 << 18, reps=20):
                                                          hb[idx].astype(np.uint64))))
     for b in (pa, pb, po):
         b.free()
+    # the reference's CPU path (poly_mul_ntt) as restated by the oracle, one core
+    sa, sb = ha[:4096].astype(np.uint64), hb[:4096].astype(np.uint64)
+    cpu_pairs_s = 4096 / cpu_seconds(lambda: gridref.negacyclic_mul(Q, D, sa, sb))
     log(f"negacyclic x{pairs}: {dt * 1e6:,.1f} us ({pairs / dt / 1e9:.2f} G pairs/s, "
-        f"{3 * pairs * D * 4 / dt / 1e9:,.0f} GB/s algorithmic); oracle sample {ok}")
+        f"{3 * pairs * D * 4 / dt / 1e9:,.0f} GB/s algorithmic); oracle sample {ok}; "
+        f"CPU 1 core {cpu_pairs_s / 1e6:.2f} M pairs/s")
     return {"pairs_per_s": round(pairs / dt, 1), "us_per_batch": round(dt * 1e6, 1),
-            "algorithmic_GBps": round(3 * pairs * D * 4 / dt / 1e9, 1), "oracle_sample": ok}
+            "algorithmic_GBps": round(3 * pairs * D * 4 / dt / 1e9, 1), "oracle_sample": ok,
+            "cpu_1core": {"pairs_per_s": round(cpu_pairs_s, 1), "kind": "port",
+                          "sample": "4096 pairs through the oracle's NTT product"}}
 
 
 def commit_small(ctx, la, ring, log, batch=1024, reps=200):
@@ -290,11 +307,34 @@ def labrador_calls(ctx, la, ring, log, reps=10):
     res["jl_modq_matches_exact"] = bool(np.array_equal(pm, pe % Q))
     ev = ev_dev.download(np.uint32, (rows, D)).astype(np.int64)
     res["jl_evals_constant_terms_equal_p"] = bool(np.array_equal(ev[:, 0], pm))
+    # the reference's CPU path as restated by the oracle, one core, on bounded
+    # samples scaled to the full call (Gram per pair, JL per part, evaluations
+    # per witness polynomial, inner commitments on the AVX2 restatement)
+    s64 = s.astype(np.uint64)
+    seed_b = bytes(range(32))
+    g_s = cpu_seconds(lambda: gridref.gram(Q, D, s64[:4])) / 10 * npairs
+    jl_s = cpu_seconds(lambda: gridref.jl_project(Q, seed_b, rows, s64[:1].reshape(1, -1), D)) * r_
+    ev_s = cpu_seconds(lambda: gridref.jl_evals(Q, D, seed_b, rows, s64[:1, :4])) / 4 * r_ * n_
+    c = gridref.Avx2Commit(Q, 64, a.astype(np.uint64), D)
+    c.load(s64[0])
+    ic_s = cpu_seconds(lambda: c.run(1)) * r_
+    res["cpu_1core_us"] = {"inner_commit_32x1024": round(ic_s * 1e6, 1),
+                           "gram_32x1024": round(g_s * 1e6, 1),
+                           "jl_256x2^21": round(jl_s * 1e6, 1),
+                           "jl_evals_256x32x1024": round(ev_s * 1e6, 1),
+                           "kind": "port",
+                           "sample": "Gram over 4 of the 32 parts (10 pairs), JL over 1 part, "
+                                     "evaluations over 4 polynomials of 1 part, 1 inner "
+                                     "commitment on the AVX2 restatement; scaled to the call"}
     log(f"LaBRADOR r 32 x 1024: inner commits {res['inner_commit_32x1024']['us_per_call']:,.0f} us "
         f"({res['inner_commit_32x1024']['commits_per_s']:,.0f} commits/s), Gram "
         f"{res['gram_32x1024']['us_per_call']:,.0f} us, JL {res['jl_256x2^21']['us_per_call']:,.0f} us; "
         f"oracle Gram entry {res['oracle_gram_entry']}, JL mod q == exact mod q "
         f"{res['jl_modq_matches_exact']}")
+    cpu1 = res["cpu_1core_us"]
+    log(f"  CPU 1 core (oracle, scaled samples): inner commits {cpu1['inner_commit_32x1024'] / 1e3:,.1f} "
+        f"ms, Gram {cpu1['gram_32x1024'] / 1e3:,.1f} ms, JL {cpu1['jl_256x2^21'] / 1e3:,.1f} ms, "
+        f"JL evals {cpu1['jl_evals_256x32x1024'] / 1e6:,.1f} s")
     log(f"  JL evals {res['jl_evals_256x32x1024']['us_per_call']:,.0f} us (ct == p "
         f"{res['jl_evals_constant_terms_equal_p']}), conjugated polys "
         f"{res['jl_conjugate_256x32x1024']['us_per_call']:,.0f} us, flat "

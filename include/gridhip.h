@@ -108,9 +108,9 @@ GH_API int gh_ctx_profile_read(gh_ctx *ctx, int phase, double *total_ms, uint64_
  * "mac_waves" (4, 8, 16), "mac_blocks_per_cu" (1..8), "quad_max" (largest
  * witness batch transformed four lanes per polynomial), "fused" (1: q32
  * commitments transform the witness inside the MAC kernel; 0: separate
- * transform and MAC kernels), "fused_depth" (1, 2: key columns the fused
- * kernel prefetches per wave before its witness transform, in units of
- * 16 / mac_waves), "batch_chunk_cols"
+ * transform and MAC kernels), "fused_depth" (1..4: the fused
+ * kernel keeps max(1, depth * 8 / mac_waves) key columns in flight per wave),
+ * "batch_chunk_cols"
  * (16..4096, key columns per block of gh_commit_batch), "batch_group" (4, 8,
  * 16: witnesses that share each loaded key column). Values are checked;
  * an unknown key or value gives GH_E_INVALID_ARGUMENT. Every setting yields

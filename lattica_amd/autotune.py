@@ -45,7 +45,7 @@ def tune_commit(ctx, kappa=16, n=1 << 15, reps=60, quick=False, log=print):
     bpcs = [1, 2] if quick else [1, 2, 3, 4]
     configs = []
     # fused transform+MAC (16-byte loads): block shape and key prefetch depth
-    for waves, bpc, depth in itertools.product([16, 8, 4], bpcs, [2, 1]):
+    for waves, bpc, depth in itertools.product([16, 8, 4], bpcs, [1, 2, 3, 4]):
         configs.append({"fused": 1, "fused_depth": depth, "mac_variant": 1, "mac_waves": waves,
                         "mac_blocks_per_cu": bpc, "quad_max": 1 << 17})
     # separate kernels: MAC load width, block shape, witness transform kernel

@@ -73,7 +73,7 @@ struct gh_tuning {
   int mac_blocks_per_cu = 1;  // MAC blocks (= partial sums) per compute unit
   int64_t quad_max = 1 << 17; // largest batch transformed four lanes per polynomial
   int fused = 1;              // q32 commit: witness transform fused into the MAC kernel
-  int fused_depth = 2;        // fused kernel: key columns prefetched per wave, x 16 / mac_waves
+  int fused_depth = 2;        // fused kernel: key columns in flight per wave, x 8 / mac_waves
   int batch_chunk_cols = 256; // batched MAC: key columns per block
   int batch_group = 8;        // batched MAC: witnesses sharing each loaded key column (4, 8, 16)
 };

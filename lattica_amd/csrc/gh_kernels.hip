@@ -362,9 +362,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_mac4_q32(
 // polynomial) while those key loads are in flight, then the MAC over the
 // wave's columns against the LDS copy with PF columns kept in flight: a
 // column's registers are reloaded with the column PF further on as soon as
-// they are used. PF = DEPTH * 16 / WAVES columns per wave: at depth 2 with two
-// 4-wave blocks per CU that is 256 KB of key in flight per CU for the whole
-// sweep.
+// they are used. PF = max(1, DEPTH * 8 / WAVES) columns per wave: with two
+// 4-wave blocks per CU, depth d keeps d * 64 KB of key in flight per CU for
+// the whole sweep (128 KB, depth 2, measures fastest at kappa 16).
 //
 // Vector loads complete in issue order and the wait counter is one number, so
 // the code between the prefetch and the MAC is kept free of branches that
@@ -395,7 +395,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_q32(
     const uint32_t* __restrict__ s_coeff, uint32_t* __restrict__ partial, uint32_t rows,
     uint32_t cols, uint32_t cols_per_block) {
   constexpr int NI = RB / 4;
-  constexpr int PF = DEPTH * 16 / WAVES;
+  constexpr int PF = DEPTH * 8 / WAVES > 0 ? DEPTH * 8 / WAVES : 1;
   extern __shared__ uint32_t lds[];  // max(cols_per_block * kFuseStride, WAVES * RB * 64) words
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -788,14 +788,19 @@ static void fused_launch_nw(gh_ctx* ctx, dim3 grid, size_t lds_bytes, hipEvent_t
                             const uint32_t* S, uint32_t* P, uint32_t rows, uint32_t cols,
                             uint32_t cpb) {
   // the unrolled sweep when every full block has the same columns per wave
-  const uint32_t nw = cpb % WAVES == 0 ? cpb / WAVES : 0;
+  // (instantiated for 16-row blocks, kappa a multiple of 16)
+  const uint32_t nw = RB == 16 && cpb % WAVES == 0 ? cpb / WAVES : 0;
 #define GH_FUSED_NW(N)                                                                          \
   hipExtLaunchKernelGGL((k_commit_fused_q32<RB, WAVES, DEPTH, N>), grid, dim3(64 * WAVES),      \
                         lds_bytes, ctx->stream, e0, e1, 0, f, tab, A, S, P, rows, cols, cpb)
-  if (nw == 8) GH_FUSED_NW(8);
-  else if (nw == 16) GH_FUSED_NW(16);
-  else if (nw == 32) GH_FUSED_NW(32);
-  else GH_FUSED_NW(0);
+  if constexpr (RB == 16) {
+    if (nw == 8) GH_FUSED_NW(8);
+    else if (nw == 16) GH_FUSED_NW(16);
+    else if (nw == 32) GH_FUSED_NW(32);
+    else GH_FUSED_NW(0);
+  } else {
+    GH_FUSED_NW(0);
+  }
 #undef GH_FUSED_NW
 }
 
@@ -803,10 +808,12 @@ template <int RB, int WAVES>
 static void fused_launch(gh_ctx* ctx, dim3 grid, size_t lds_bytes, hipEvent_t e0, hipEvent_t e1,
                          FQ32 f, const uint32_t* tab, const uint32_t* A, const uint32_t* S,
                          uint32_t* P, uint32_t rows, uint32_t cols, uint32_t cpb) {
-  if (ctx->tune.fused_depth == 2)
-    fused_launch_nw<RB, WAVES, 2>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb);
-  else
-    fused_launch_nw<RB, WAVES, 1>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb);
+  switch (ctx->tune.fused_depth) {
+    case 1: fused_launch_nw<RB, WAVES, 1>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb); break;
+    case 3: fused_launch_nw<RB, WAVES, 3>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb); break;
+    case 4: fused_launch_nw<RB, WAVES, 4>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb); break;
+    default: fused_launch_nw<RB, WAVES, 2>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb); break;
+  }
 }
 
 // q32 commitment in two launches on the context stream: the fused

@@ -237,6 +237,39 @@ def test_north_star_commit_full_size(oracle, ctx, ring_q32):
     s_dev.free(); t_dev.free(); key.close()
 
 
+@pytest.fixture(scope="module")
+def full_key(oracle, ring_q32):
+    """The kappa 16 x 2^15 key of the headline shape and one commitment of it."""
+    import lattica_amd as la
+    rng = np.random.default_rng(62)
+    a = rand_residues(rng, Q32, (16, 1 << 15, 64))
+    s = small_signed(rng, Q32, (1 << 15, 64), -128, 128)
+    z = np.zeros((16, 1, 64), dtype=np.uint64)
+    zr = np.zeros((1, 64), dtype=np.uint64)
+    exp = oracle.commit(Q32, 64, a, z, s, zr, key_domain=1)
+    exp_tail = oracle.commit(Q32, 64, a[:, :32763], z, s[:32763], zr, key_domain=1)
+    key = la.Key(ring_q32, a, domain=la.DOMAIN_NTT)
+    yield key, s, exp, exp_tail
+    key.close()
+
+
+@pytest.mark.parametrize("waves,bpc", [(4, 1), (4, 2), (4, 4), (8, 1), (8, 2)])
+def test_unrolled_sweeps_at_the_headline_shape(ctx, full_key, waves, bpc):
+    """The fused kernel's unrolled sweeps (8, 16 and 32 columns per wave) at every
+    depth, on the full 2^15 columns and on 32763 (a last block 5 columns short)."""
+    key, s, exp, exp_tail = full_key
+    saved = ctx.get_tuning()
+    try:
+        for depth in (1, 2, 3, 4):
+            ctx.set_tuning(fused=1, mac_waves=waves, mac_blocks_per_cu=bpc, fused_depth=depth)
+            assert np.array_equal(key.commit(s).astype(np.uint64), exp), depth
+            # a one-item batch takes the plain path over the key's first 32763 columns
+            got = key.commit_batch([s[:32763]])[0]
+            assert np.array_equal(got.astype(np.uint64), exp_tail), depth
+    finally:
+        ctx.set_tuning(**saved)
+
+
 def test_every_tuning_setting_gives_the_same_commitment(oracle, ctx, ring_q32):
     import lattica_amd as la
     r = ring_q32
@@ -254,7 +287,8 @@ def test_every_tuning_setting_gives_the_same_commitment(oracle, ctx, ring_q32):
                 (0, 1, 16, 4, 1 << 17, 1), (0, 1, 16, 8, 0, 1), (1, 1, 4, 1, 0, 1),
                 (1, 1, 8, 2, 0, 1), (1, 1, 16, 1, 0, 1), (1, 1, 16, 8, 0, 1), (1, 1, 4, 3, 0, 1),
                 (1, 1, 8, 3, 0, 1), (1, 1, 4, 1, 0, 2), (1, 1, 4, 2, 0, 2), (1, 1, 4, 3, 0, 2),
-                (1, 1, 8, 1, 0, 2), (1, 1, 16, 2, 0, 2)]:
+                (1, 1, 8, 1, 0, 2), (1, 1, 16, 2, 0, 2), (1, 1, 4, 2, 0, 3), (1, 1, 4, 2, 0, 4),
+                (1, 1, 16, 1, 0, 4)]:
             ctx.set_tuning(fused=fused, mac_variant=variant, mac_waves=waves,
                            mac_blocks_per_cu=bpc, quad_max=quad, fused_depth=depth)
             got = key.commit(s).astype(np.uint64)
