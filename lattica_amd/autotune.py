@@ -96,8 +96,8 @@ def tune_commit(ctx, kappa=16, n=1 << 15, reps=60, quick=False, log=print):
 def tune_batch(ctx, quick=False, log=print):
     """Sweep batch_chunk_cols x batch_group on two batched workloads and rank by
     the summed time: (a) 16 q32 commitments at kappa = 16, n = 2^15 per call;
-    (b) 400 Goldilocks commitments, kappa = 16, log-uniform lengths in
-    [2^8, 2^14] (the C4 distribution). Results must not change with the tuning."""
+    (b) 2000 Goldilocks commitments, kappa = 16, log-uniform lengths in
+    [2^8, 2^14] (the C4 workload). Results must not change with the tuning."""
     import lattica_amd as la
     rng = np.random.default_rng(9)
     work = []
@@ -117,11 +117,11 @@ def tune_batch(ctx, quick=False, log=print):
     del hi, lo
     t = rng.integers(-1, 2, size=(1 << 15, 64), dtype=np.int64)
     pool = ctx.to_device(np.where(t < 0, np.uint64(p - 1), t.astype(np.uint64)))
-    n_items = 400
+    n_items = 2000
     lens = np.floor(2.0 ** rng.uniform(8, 14, size=n_items)).astype(np.int64)
     offs = rng.integers(0, (1 << 15) - lens + 1)
     og = ctx.alloc(n_items * 16 * 64 * 8)
-    work.append(("gold_c4_400", kg, [pool.ptr.value + int(o) * 512 for o in offs], list(lens),
+    work.append(("gold_c4_2000", kg, [pool.ptr.value + int(o) * 512 for o in offs], list(lens),
                  [og.ptr.value + i * 16 * 64 * 8 for i in range(n_items)], None))
 
     chunks = [64, 128, 256] if quick else [32, 64, 128, 256, 512]
