@@ -110,8 +110,9 @@ GH_API int gh_ctx_profile_read(gh_ctx *ctx, int phase, double *total_ms, uint64_
  * commitments transform the witness inside the MAC kernel; 0: separate
  * transform and MAC kernels), "fused_depth" (1..4: the fused
  * kernel keeps max(1, depth * 8 / mac_waves) key columns in flight per wave),
- * "batch_chunk_cols"
- * (16..4096, key columns per block of gh_commit_batch), "batch_group" (4, 8,
+ * "batch_chunk_cols" and "batch_chunk_cols_gold" (16..4096, key columns per
+ * block of gh_commit_batch and of the server's waves, for q32 and Goldilocks
+ * keys; halved for batches too small to fill the device), "batch_group" (4, 8,
  * 16: witnesses that share each loaded key column). Values are checked;
  * an unknown key or value gives GH_E_INVALID_ARGUMENT. Every setting yields
  * identical results; only the speed changes. */

@@ -434,16 +434,16 @@ static int batch_enqueue(gh_key* key, hipStream_t stream, const BatchLayout& L, 
   return enqueue_batch_finalize(key->ring, stream, ddesc, n, key->rows, cc);
 }
 
-// Key columns per block of the grouped MAC: the tuned width, halved (not below
+// Key columns per block of the grouped MAC: the field's tuned width, halved (not below
 // 32) while the grid would hold fewer blocks than half the device's CUs, so a
 // batch of short witnesses still spreads over the device.
 static uint32_t batch_chunk(gh_ctx* ctx, const gh_commit_item* items, size_t count,
-                            uint32_t rows) {
+                            uint32_t rows, int ring_kind) {
   uint32_t maxlen = 0;
   for (size_t i = 0; i < count; i++) maxlen = std::max(maxlen, items[i].len);
   const uint64_t groups = (count + ctx->tune.batch_group - 1) / ctx->tune.batch_group;
   const uint64_t rblocks = rows >= 16 ? rows / 16 : 1;
-  uint32_t cc = (uint32_t)ctx->tune.batch_chunk_cols;
+  uint32_t cc = tuned_batch_chunk(ctx->tune, ring_kind);
   while (cc > 32 &&
          (uint64_t)((maxlen + cc - 1) / cc) * groups * rblocks < (uint64_t)ctx->num_cus / 2)
     cc /= 2;
@@ -463,7 +463,7 @@ int launch_commit_batch(gh_key* key, const gh_commit_item* items, size_t count) 
     }
     return GH_OK;
   }
-  const uint32_t cc = batch_chunk(ctx, items, count, key->rows);
+  const uint32_t cc = batch_chunk(ctx, items, count, key->rows, key->ring->kind);
   BatchLayout sizes;
   batch_layout(key, items, (uint32_t)count, cc, nullptr, sizes);
   void* ws;
@@ -524,7 +524,7 @@ GH_API int gh_batch_plan_create(gh_key* key, const gh_commit_item* items, size_t
   if (!p) return GH_E_OOM;
   p->key = key;
   p->count = count;
-  const uint32_t cc = batch_chunk(ctx, items, count, key->rows);
+  const uint32_t cc = batch_chunk(ctx, items, count, key->rows, key->ring->kind);
   BatchLayout L;
   batch_layout(key, items, (uint32_t)count, cc, nullptr, L);
   if (hipMalloc(&p->mem, L.total()) != hipSuccess) {

@@ -273,6 +273,9 @@ GH_API int gh_ctx_set_tuning(gh_ctx* ctx, const char* key, int64_t value) {
   } else if (!std::strcmp(key, "batch_chunk_cols")) {
     if (value < 16 || value > 4096) return GH_E_INVALID_ARGUMENT;
     t.batch_chunk_cols = (int)value;
+  } else if (!std::strcmp(key, "batch_chunk_cols_gold")) {
+    if (value < 16 || value > 4096) return GH_E_INVALID_ARGUMENT;
+    t.batch_chunk_cols_gold = (int)value;
   } else if (!std::strcmp(key, "batch_group")) {
     if (value != 4 && value != 8 && value != 16) return GH_E_INVALID_ARGUMENT;
     t.batch_group = (int)value;
@@ -293,6 +296,7 @@ GH_API int gh_ctx_get_tuning(gh_ctx* ctx, const char* key, int64_t* value) {
   else if (!std::strcmp(key, "fused")) *value = t.fused;
   else if (!std::strcmp(key, "fused_depth")) *value = t.fused_depth;
   else if (!std::strcmp(key, "batch_chunk_cols")) *value = t.batch_chunk_cols;
+  else if (!std::strcmp(key, "batch_chunk_cols_gold")) *value = t.batch_chunk_cols_gold;
   else if (!std::strcmp(key, "batch_group")) *value = t.batch_group;
   else return GH_E_INVALID_ARGUMENT;
   return GH_OK;

@@ -74,9 +74,15 @@ struct gh_tuning {
   int64_t quad_max = 1 << 17; // largest batch transformed four lanes per polynomial
   int fused = 1;              // q32 commit: witness transform fused into the MAC kernel
   int fused_depth = 2;        // fused kernel: key columns in flight per wave, x 8 / mac_waves
-  int batch_chunk_cols = 256; // batched MAC: key columns per block
+  int batch_chunk_cols = 256; // batched MAC: key columns per block (q32 keys)
+  int batch_chunk_cols_gold = 256; // batched MAC: key columns per block (Goldilocks keys)
   int batch_group = 8;        // batched MAC: witnesses sharing each loaded key column (4, 8, 16)
 };
+
+// Tuned key columns per block of the batched MAC for keys over `ring`.
+inline uint32_t tuned_batch_chunk(const gh_tuning& t, int ring_kind) {
+  return (uint32_t)(ring_kind == GH_RING_Q32 ? t.batch_chunk_cols : t.batch_chunk_cols_gold);
+}
 
 struct gh_ctx {
   int device = 0;

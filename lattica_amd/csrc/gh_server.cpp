@@ -120,7 +120,8 @@ struct Slot {
 struct gh_server {
   gh_ctx* ctx = nullptr;
   gh_server_config cfg{};
-  uint32_t cc = 256;  // batched MAC chunking, fixed for the server's life
+  uint32_t cc[2] = {256, 256};  // batched MAC chunking for q32 and Goldilocks keys, fixed
+  uint32_t chunk(const gh_ring* ring) const { return cc[ring->kind == GH_RING_Q32 ? 0 : 1]; }
   int group = 8, variant = 1;
   char* base = nullptr;  // arena
   Arena arena;
@@ -152,7 +153,8 @@ int plan_request(gh_server* s, SrvReq* q) {
   at += al(m * wb);
   if (r.ops & GH_OP_COMMIT) {
     const uint32_t rows = r.key->rows;
-    q->nch = (r.n + s->cc - 1) / s->cc;
+    const uint32_t cc = s->chunk(r.ring);
+    q->nch = (r.n + cc - 1) / cc;
     q->o_cpart = at;
     at += al((size_t)r.r * q->nch * rows * 64 * wb);
     q->t_bytes = (size_t)r.r * rows * 64 * wb;
@@ -429,9 +431,10 @@ int launch_wave(gh_server* s, Slot& sl, const std::vector<SrvReq*>& wave) {
   for (KeyPlan& k : kp) {
     const BatchDesc* md = (const BatchDesc*)(db + k.o_md);
     const uint32_t count = (uint32_t)k.md.size();
-    st = enqueue_batch_mac(k.key, stream, md, count, k.md[0].len, s->cc, s->group, s->variant);
+    const uint32_t cc = s->chunk(k.key->ring);
+    st = enqueue_batch_mac(k.key, stream, md, count, k.md[0].len, cc, s->group, s->variant);
     if (st) return st;
-    st = enqueue_batch_finalize(k.key->ring, stream, md, count, k.key->rows, s->cc);
+    st = enqueue_batch_finalize(k.key->ring, stream, md, count, k.key->rows, cc);
     if (st) return st;
   }
   for (RingPlan& p : rp) {
@@ -620,7 +623,8 @@ GH_API int gh_server_create(gh_ctx* ctx, const gh_server_config* cfg, gh_server*
   if (!s) return GH_E_OOM;
   s->ctx = ctx;
   s->cfg = c;
-  s->cc = (uint32_t)ctx->tune.batch_chunk_cols;
+  s->cc[0] = tuned_batch_chunk(ctx->tune, GH_RING_Q32);
+  s->cc[1] = tuned_batch_chunk(ctx->tune, GH_RING_GOLDILOCKS);
   s->group = ctx->tune.batch_group;
   s->variant = ctx->tune.mac_variant;
   if (hipMalloc((void**)&s->base, c.mem_bytes) != hipSuccess) {

@@ -375,7 +375,9 @@ def test_commit_batch_device_many_items_all_tunings(oracle, ctx, ring_q32, ring_
     saved = ctx.get_tuning()
     try:
         for group, chunk in [(8, 256), (4, 16), (16, 100), (8, 4096)]:
-            ctx.set_tuning(batch_group=group, batch_chunk_cols=chunk)
+            # each field reads its own width; the other field's is set elsewhere
+            ctx.set_tuning(batch_group=group, batch_chunk_cols=chunk if name == "q32" else 64,
+                           batch_chunk_cols_gold=chunk if name != "q32" else 64)
             key.commit_batch_device(wd, lens, outs)
             ctx.synchronize()
             for i in range(count):
