@@ -141,6 +141,33 @@ def goldilocks_batch(ctx, la, log, count=2000, kappa=16, max_cols=1 << 14):
             "mean_len": float(lens.mean()), "oracle_spot_check": ok}
 
 
+def ct_pair_sum(a, b, q):
+    """sum over rows g of the constant term of a_g * b_g in Z_q[X]/(X^64 + 1), q < 2^32."""
+    a = np.asarray(a).reshape(-1, 64).astype(np.uint64)
+    b = np.asarray(b).reshape(-1, 64).astype(np.uint64)
+    qq = np.uint64(q)
+    first = (a[:, 0] * b[:, 0]) % qq
+    rest = np.zeros(a.shape[0], dtype=np.uint64)
+    for k in range(1, 64):
+        rest += (a[:, k] * b[:, 64 - k]) % qq
+    return (int(first.sum()) - int(rest.sum())) % q
+
+
+def host_cpu():
+    """The host's CPU model, logical CPUs and the CPUs this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(),
+            "usable_cpus": len(os.sched_getaffinity(0))}
+
+
 def cpu_seconds(fn, min_s=0.5):
     """Seconds per call of fn on this host (the oracle, one core), at least min_s of runs."""
     fn()
@@ -255,6 +282,9 @@ def labrador_calls(ctx, la, ring, log, reps=10):
     c_dev = ctx.to_device(c)
     z_dev = ctx.alloc(n_ * D * 4)
     dec_dev = ctx.alloc(r_ * KAPPA * 4 * D * 4)
+    scale = rng.integers(0, Q, size=(rows, D), dtype=np.uint32)  # aggregation scale polys
+    sc_dev = ctx.to_device(scale)
+    phi_dev = ctx.to_device(np.zeros((r_, n_, D), dtype=np.uint32))
     seed = np.frombuffer(bytes(range(32)), dtype=np.uint8).copy()
     L = la.lib()
     import ctypes
@@ -270,6 +300,9 @@ def labrador_calls(ctx, la, ring, log, reps=10):
         "jl_evals_256x32x1024": lambda: L.gh_jl_evals(ring.h, ctypes.c_void_p(seed.ctypes.data),
                                                       rows, s_dev.ptr, r_, n_ * D, ev_dev.ptr,
                                                       la.MEM_DEVICE),
+        "jl_aggregate_256x32x1024": lambda: L.gh_jl_aggregate(
+            ring.h, ctypes.c_void_p(seed.ctypes.data), rows, sc_dev.ptr, r_, n_ * D, phi_dev.ptr,
+            la.MEM_DEVICE),
         "jl_conjugate_256x32x1024": lambda: L.gh_jl_conjugate(
             ring.h, ctypes.c_void_p(seed.ctypes.data), None, rows, r_, n_ * D, conj_dev.ptr,
             la.MEM_DEVICE),
@@ -307,6 +340,14 @@ def labrador_calls(ctx, la, ring, log, reps=10):
     res["jl_modq_matches_exact"] = bool(np.array_equal(pm, pe % Q))
     ev = ev_dev.download(np.uint32, (rows, D)).astype(np.int64)
     res["jl_evals_constant_terms_equal_p"] = bool(np.array_equal(ev[:, 0], pm))
+    # aggregation from phi = 0: sum_ij ct(phi_ij s_ij) = sum_m ct(scale_m jl_eval[m])
+    phi_dev.upload(np.zeros((r_, n_, D), dtype=np.uint32))
+    st = calls["jl_aggregate_256x32x1024"]()
+    if st:
+        raise la.GridHipError(st, "gh_jl_aggregate")
+    phi = phi_dev.download(np.uint32, (r_ * n_, D))
+    res["jl_aggregate_pairs_with_evals"] = bool(
+        ct_pair_sum(phi, s.reshape(-1, D), Q) == ct_pair_sum(scale, ev, Q))
     # the reference's CPU path as restated by the oracle, one core, on bounded
     # samples scaled to the full call (Gram per pair, JL per part, evaluations
     # per witness polynomial, inner commitments on the AVX2 restatement)
@@ -315,17 +356,42 @@ def labrador_calls(ctx, la, ring, log, reps=10):
     g_s = cpu_seconds(lambda: gridref.gram(Q, D, s64[:4])) / 10 * npairs
     jl_s = cpu_seconds(lambda: gridref.jl_project(Q, seed_b, rows, s64[:1].reshape(1, -1), D)) * r_
     ev_s = cpu_seconds(lambda: gridref.jl_evals(Q, D, seed_b, rows, s64[:1, :4])) / 4 * r_ * n_
+    # the aggregation's JL term is rows x r x n ring products (aggregation.rs:237-245);
+    # timed as 16384 of the oracle's NTT products, conjugation and sums not counted
+    pa = rng.integers(0, Q, size=(16384, D), dtype=np.uint64)
+    pb = rng.integers(0, Q, size=(16384, D), dtype=np.uint64)
+    ag_s = cpu_seconds(lambda: gridref.negacyclic_mul(Q, D, pa, pb)) / 16384 * rows * r_ * n_
     c = gridref.Avx2Commit(Q, 64, a.astype(np.uint64), D)
     c.load(s64[0])
     ic_s = cpu_seconds(lambda: c.run(1)) * r_
+    c.close()
+    # the reference's own algorithm for these two calls multiplies in the
+    # coefficient domain, a full NTT product per ring product (crs.rs:211-256,
+    # garbage.rs:76-93); timed on 1 commitment over 128 columns and on 2 parts
+    # of 128 polynomials (3 pairs), scaled to the call
+    zk = np.zeros((KAPPA, 1, D), dtype=np.uint64)
+    zr = np.zeros((1, D), dtype=np.uint64)
+    a128 = a[:, :128].astype(np.uint64)   # the key's words taken as coefficients
+    icc_s = cpu_seconds(lambda: gridref.commit(Q, D, a128, zk, s64[0, :128], zr,
+                                               path="coeff")) * r_ * (n_ / 128)
+    gc_s = cpu_seconds(lambda: gridref.gram(Q, D, s64[:2, :128], ntt_once=False)) / 3 * npairs * (n_ / 128)
     res["cpu_1core_us"] = {"inner_commit_32x1024": round(ic_s * 1e6, 1),
                            "gram_32x1024": round(g_s * 1e6, 1),
                            "jl_256x2^21": round(jl_s * 1e6, 1),
                            "jl_evals_256x32x1024": round(ev_s * 1e6, 1),
+                           "jl_aggregate_256x32x1024": round(ag_s * 1e6, 1),
+                           "coefficient_domain": {
+                               "inner_commit_32x1024": round(icc_s * 1e6, 1),
+                               "gram_32x1024": round(gc_s * 1e6, 1),
+                               "note": "the reference's algorithm: every ring product a full "
+                                       "NTT multiply; 1 commitment over 128 columns and 3 pairs "
+                                       "of 128 polynomials, scaled"},
                            "kind": "port",
-                           "sample": "Gram over 4 of the 32 parts (10 pairs), JL over 1 part, "
-                                     "evaluations over 4 polynomials of 1 part, 1 inner "
-                                     "commitment on the AVX2 restatement; scaled to the call"}
+                           "sample": "NTT once: Gram over 4 of the 32 parts (10 pairs), JL over "
+                                     "1 part, evaluations over 4 polynomials of 1 part, "
+                                     "aggregation as 16384 NTT products scaled to its "
+                                     "256 x 32768 products, 1 inner commitment on "
+                                     "the AVX2 restatement; scaled to the call"}
     log(f"LaBRADOR r 32 x 1024: inner commits {res['inner_commit_32x1024']['us_per_call']:,.0f} us "
         f"({res['inner_commit_32x1024']['commits_per_s']:,.0f} commits/s), Gram "
         f"{res['gram_32x1024']['us_per_call']:,.0f} us, JL {res['jl_256x2^21']['us_per_call']:,.0f} us; "
@@ -334,9 +400,14 @@ def labrador_calls(ctx, la, ring, log, reps=10):
     cpu1 = res["cpu_1core_us"]
     log(f"  CPU 1 core (oracle, scaled samples): inner commits {cpu1['inner_commit_32x1024'] / 1e3:,.1f} "
         f"ms, Gram {cpu1['gram_32x1024'] / 1e3:,.1f} ms, JL {cpu1['jl_256x2^21'] / 1e3:,.1f} ms, "
-        f"JL evals {cpu1['jl_evals_256x32x1024'] / 1e6:,.1f} s")
+        f"JL evals {cpu1['jl_evals_256x32x1024'] / 1e6:,.1f} s, JL aggregation "
+        f"{cpu1['jl_aggregate_256x32x1024'] / 1e6:,.1f} s; coefficient-domain products: "
+        f"inner commits {cpu1['coefficient_domain']['inner_commit_32x1024'] / 1e3:,.1f} ms, "
+        f"Gram {cpu1['coefficient_domain']['gram_32x1024'] / 1e3:,.1f} ms")
     log(f"  JL evals {res['jl_evals_256x32x1024']['us_per_call']:,.0f} us (ct == p "
-        f"{res['jl_evals_constant_terms_equal_p']}), conjugated polys "
+        f"{res['jl_evals_constant_terms_equal_p']}), JL aggregation "
+        f"{res['jl_aggregate_256x32x1024']['us_per_call']:,.0f} us (pairs with evals "
+        f"{res['jl_aggregate_pairs_with_evals']}), conjugated polys "
         f"{res['jl_conjugate_256x32x1024']['us_per_call']:,.0f} us, flat "
         f"{res['jl_extract_flat_256x2^21']['us_per_call']:,.0f} us, h garbage "
         f"{res['garbage_h_32x1024']['us_per_call']:,.0f} us, amortize "
@@ -344,7 +415,7 @@ def labrador_calls(ctx, la, ring, log, reps=10):
         f"{res['decompose_t_512_base2^8_4limbs']['us_per_call']:,.0f} us")
     key.close()
     for b in (s_dev, t_dev, g_dev, pm_dev, pe_dev, ev_dev, conj_dev, flat_dev, h_dev, c_dev,
-              z_dev, dec_dev):
+              z_dev, dec_dev, sc_dev, phi_dev):
         b.free()
     return res
 
@@ -752,12 +823,28 @@ def main():
             n_all += 1
         cpu_all = n_all / (time.perf_counter() - a)
         c.close()
+        # the reference's L = u32 limb (8 lanes, the wide Montgomery path for q >= 2^31)
+        c32 = gridref.Avx2Commit(Q, 32, a_host0.astype(np.uint64), D)
+        c32.load(s_host[0].astype(np.uint64))
+        c32.run(1)
+        n32, a = 0, time.perf_counter()
+        while time.perf_counter() - a < args.cpu_seconds / 2:
+            c32.run(1)
+            n32 += 1
+        cpu_l32 = n32 / (time.perf_counter() - a)
+        c32.close()
         cpu = {"value": round(cpu_rate, 3), "unit": "commits/s", "cores": 1, "kind": "port",
                "sample": f"{n_done} prepared commits (kappa=16, n=2^15, d=64, q32) of the reference's "
                          f"AVX2 L=u64 path restated in C (oracle/gridref_avx2.c), 1 thread",
                "all_cores": {"value": round(cpu_all, 3), "threads": threads_all,
-                             "note": "this build's row/column split over threads"}}
-        log(f"cpu baseline: {cpu_rate:.2f} commits/s on 1 core, {cpu_all:.2f} on {threads_all}")
+                             "note": "this build's row/column split over threads"},
+               "limb_u32_1core": {"value": round(cpu_l32, 3), "commits": n32,
+                                  "note": "the reference's L = u32 path, 1 thread: 8-lane "
+                                          "transforms, scalar wide Montgomery products in the "
+                                          "dot product for q >= 2^31"},
+               "host": host_cpu()}
+        log(f"cpu baseline: {cpu_rate:.2f} commits/s on 1 core (L = u64), {cpu_l32:.2f} "
+            f"(L = u32), {cpu_all:.2f} on {threads_all} threads; {cpu['host']['model']}")
 
     out = {
         "metric": "Ajtai commits/sec (d=64)",

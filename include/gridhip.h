@@ -327,7 +327,15 @@ GH_API int gh_jl_project(gh_ring *ring, const uint8_t seed[32], uint32_t rows, c
  *   parts: num_parts x cols words (cols = n * d), coefficient domain; evals:
  *   256 polynomials, coefficient domain. Their constant terms are the p of
  *   gh_jl_project.
- * All three need d = 64 and cols % d == 0. */
+ * gh_jl_aggregate: the JL term of the aggregated function, prover and verifier
+ *   (labrador/src/main_protocol/aggregation.rs:237-245, called from
+ *   step_prover.rs:187 and step_verifier.rs:359 through aggregate_step2_flat):
+ *   phi[i][j] += sum_m scale_m * sigma_{-1}(pi_i^(m))_j over R_q, for rows =
+ *   256. scale: 256 polynomials, the loop's jl_scale_m = sum_k beta_k *
+ *   omega_k[m] formed by the caller; phi: num_parts x n polynomials (n = cols
+ *   / d), read and written in place (the caller's agg_phi). Both coefficient
+ *   domain. The conjugated polynomials are never materialised.
+ * All four need d = 64 and cols % d == 0. */
 GH_API int gh_jl_extract_flat(gh_ring *ring, const uint8_t seed[32], uint32_t rows,
                               uint32_t num_parts, uint64_t cols, int8_t *flat, int mem);
 GH_API int gh_jl_conjugate(gh_ring *ring, const uint8_t seed[32], const int8_t *flat,
@@ -335,6 +343,9 @@ GH_API int gh_jl_conjugate(gh_ring *ring, const uint8_t seed[32], const int8_t *
                            int mem);
 GH_API int gh_jl_evals(gh_ring *ring, const uint8_t seed[32], uint32_t rows, const void *parts,
                        uint32_t num_parts, uint64_t cols, void *evals, int mem);
+GH_API int gh_jl_aggregate(gh_ring *ring, const uint8_t seed[32], uint32_t rows,
+                           const void *scale, uint32_t num_parts, uint64_t cols, void *phi,
+                           int mem);
 
 /* ---- canonical wire formats ----------------------------------------------------
  * The reference's canonical encodings (serialize/src/lib.rs:1-7), byte for byte:

@@ -57,6 +57,7 @@ EXPORTS = (
     "gh_batch_plan_destroy", "gh_bdlop_commit", "gh_decompose_balanced", "gh_garbage_h",
     "gh_amortize", "gh_wire_size", "gh_wire_encode", "gh_wire_peek", "gh_wire_decode",
     "gh_key_save", "gh_key_load", "gh_jl_extract_flat", "gh_jl_conjugate", "gh_jl_evals",
+    "gh_jl_aggregate",
 )
 
 
@@ -184,6 +185,7 @@ def lib():
         "gh_jl_extract_flat": (i32, [vp, vp, u32, u32, u64, vp, i32]),
         "gh_jl_conjugate": (i32, [vp, vp, vp, u32, u32, u64, vp, i32]),
         "gh_jl_evals": (i32, [vp, vp, u32, vp, u32, u64, vp, i32]),
+        "gh_jl_aggregate": (i32, [vp, vp, u32, vp, u32, u64, vp, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -453,6 +455,18 @@ class Ring:
         _check(lib().gh_jl_evals(self.h, _ptr(sd), rows, _ptr(p) if p.size else None, num_parts,
                                  cols, _ptr(out), MEM_HOST), "gh_jl_evals")
         return out
+
+    def jl_aggregate(self, seed: bytes, scale, phi, rows: int = 256) -> np.ndarray:
+        """gh_jl_aggregate: phi (num_parts, n, d) + sum_m scale_m * sigma(pi^(m));
+        scale (rows, d). Returns the new phi; the argument is not modified."""
+        sc = self.words(scale)
+        ph = self.words(phi).copy()
+        num_parts = ph.shape[0] if ph.ndim == 3 else 1
+        cols = ph.size // max(1, num_parts)
+        sd = np.frombuffer(bytes(seed), dtype=np.uint8).copy()
+        _check(lib().gh_jl_aggregate(self.h, _ptr(sd), rows, _ptr(sc), num_parts, cols,
+                                     _ptr(ph) if ph.size else None, MEM_HOST), "gh_jl_aggregate")
+        return ph
 
     # ---- canonical wire formats (gh_wire_*) ----
     def wire_size(self, kind: int, rows: int, cols: int = 1) -> int:

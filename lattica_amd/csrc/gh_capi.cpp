@@ -1375,4 +1375,33 @@ GH_API int gh_jl_evals(gh_ring* ring, const uint8_t seed[32], uint32_t rows, con
   return sync(ctx);
 }
 
+GH_API int gh_jl_aggregate(gh_ring* ring, const uint8_t seed[32], uint32_t rows,
+                           const void* scale, uint32_t num_parts, uint64_t cols, void* phi,
+                           int mem) {
+  TRY(jl_args(ring, seed, rows, cols, num_parts));
+  TRY(check_mem(mem));
+  if (rows != 256 || !scale) return GH_E_INVALID_ARGUMENT;
+  const uint64_t n = cols / ring->d;
+  const size_t words = (size_t)num_parts * n * ring->d;
+  if (words == 0) return GH_OK;
+  if (!phi) return GH_E_INVALID_ARGUMENT;
+  gh_ctx* ctx = ring->ctx;
+  const size_t wb = ring->word_bytes;
+  const size_t s_bytes = (size_t)256 * ring->d * wb, p_bytes = words * wb;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  if (mem == GH_MEM_DEVICE) return launch_jl_aggregate(ring, seed, scale, num_parts, n, phi);
+  TRY(check_canonical_host(ring, scale, (size_t)256 * ring->d));
+  TRY(check_canonical_host(ring, phi, words));
+  void* dev;
+  TRY(ctx_staging(ctx, s_bytes + p_bytes, &dev));
+  char* ds = (char*)dev;
+  char* dp = ds + s_bytes;
+  TRY(h2d(ctx, ds, scale, s_bytes));
+  TRY(h2d(ctx, dp, phi, p_bytes));
+  TRY(launch_jl_aggregate(ring, seed, ds, num_parts, n, dp));
+  TRY(d2h(ctx, phi, dp, p_bytes));
+  return sync(ctx);
+}
+
 }  // extern "C"

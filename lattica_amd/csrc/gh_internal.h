@@ -204,6 +204,9 @@ int launch_jl_proj(gh_ring* ring, const uint8_t seed[32], uint32_t rows, const v
 // rows = 256; parts: num_parts x n polys, coefficient domain; out: 256 polys
 int launch_jl_evals(gh_ring* ring, const uint8_t seed[32], const void* parts, uint32_t num_parts,
                     uint64_t n, void* out);
+// phi[g] += sum_m scale_m * sigma_{-1}(pi_g^(m)) over the 256 JL rows (device pointers)
+int launch_jl_aggregate(gh_ring* ring, const uint8_t seed[32], const void* scale,
+                        uint32_t num_parts, uint64_t n, void* phi);
 int launch_jl(gh_ring* ring, const uint8_t seed[32], uint32_t rows, const void* parts,
               uint32_t num_parts, uint64_t cols, void* p_modq, int64_t* p_exact, int8_t* flat);
 

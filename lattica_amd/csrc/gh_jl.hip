@@ -1,7 +1,8 @@
 // JL rows for aggregation: the entries of the JL matrix in the reference's
-// flat layout, its conjugated polynomials, and the polynomial JL evaluations
-// the prover folds into b'' (labrador/src/jl.rs:311-383,
-// labrador/src/main_protocol/mod.rs:835-923, step_prover.rs:196-205).
+// flat layout, its conjugated polynomials, the polynomial JL evaluations the
+// prover folds into b'' and the JL term of the aggregated phi
+// (labrador/src/jl.rs:311-383, labrador/src/main_protocol/mod.rs:835-923,
+// step_prover.rs:196-205, main_protocol/aggregation.rs:237-245).
 //
 // Column col = poly * d + coeff of part p is keystream bytes
 // [col * pk, col * pk + pk) of ChaCha20(seed, nonce = p), pk = ceil(rows / 4),
@@ -27,7 +28,13 @@
 //                    The accumulator tile of a block (256 rows x 256 digit
 //                    columns of int32) fills half a CU's register file, so
 //                    Goldilocks runs two passes of four digits each;
-//   k_jl_evals_fin   sums the per-block partials and recombines the digits mod q.
+//   k_jl_evals_fin   sums the per-block partials and recombines the digits mod q;
+//   k_jl_agg         the aggregation's JL term phi_ij += sum_m c_m sigma_{-1}(pi_ij^(m))
+//                    (main_protocol/aggregation.rs:237-245) as the same kind of
+//                    GEMM with the polynomials as rows: the entries go from the
+//                    keystream into the matrix-core A operand in registers, the
+//                    digits of the rotated c_m sit in LDS, and the 2 GiB of
+//                    conjugated polynomials of r 32 x 1024 are never written.
 #include <algorithm>
 
 #include "gh_internal.h"
@@ -420,6 +427,166 @@ __global__ __launch_bounds__(64 * D) void k_jl_evals_fin(const int32_t* __restri
   out[(uint64_t)m * 64 + nn] = (W)r;
 }
 
+// ---- JL aggregation on the int8 matrix cores ----------------------------------
+//
+// phi_ij += sum_m c_m * sigma_{-1}(pi_ij^(m)). With sigma(pi) = sum_k e(k) X^-k
+// and (c X^-k)[t] = c[t + k] for t + k < 64, -c[t + k - 64] otherwise, this is
+// phi[g][t] += sum_(k, m) Pi[m][g, k] * R[w = t + k][m], R[w][m] the digits of
+// c_m[w] (w < 64) or of -c_m[w - 64]: a GEMM with the polynomials g as M,
+// (k, m) as K (m fastest, 32 per step) and t as N. The A operand comes straight
+// from the keystream in registers: lane l draws column (g, k) of its
+// polynomial, l < 32 for even k and l >= 32 for odd k, so the two halves of a
+// wave swap one word per step and every block is drawn once. R sits in LDS.
+
+constexpr int kAggPolys = 128;   // polynomials per block: 4 tiles of 32
+constexpr int kAggRow = 272;     // bytes per R row: 256 m + 16 pad (bank spread)
+
+// 16 ternary codes (2 bits each, m ascending) -> 16 int8 entries
+__device__ __forceinline__ v4i jl_decode16(uint32_t x) {
+  const uint32_t e0 = __builtin_amdgcn_perm(0u, 0x0000FF01u, x & 0x03030303u);          // codes 0, 4, 8, 12
+  const uint32_t e1 = __builtin_amdgcn_perm(0u, 0x0000FF01u, (x >> 2) & 0x03030303u);   // 1, 5, 9, 13
+  const uint32_t e2 = __builtin_amdgcn_perm(0u, 0x0000FF01u, (x >> 4) & 0x03030303u);
+  const uint32_t e3 = __builtin_amdgcn_perm(0u, 0x0000FF01u, (x >> 6) & 0x03030303u);
+  const uint32_t lo01 = __builtin_amdgcn_perm(e1, e0, 0x05010400u);
+  const uint32_t hi01 = __builtin_amdgcn_perm(e1, e0, 0x07030602u);
+  const uint32_t lo23 = __builtin_amdgcn_perm(e3, e2, 0x05010400u);
+  const uint32_t hi23 = __builtin_amdgcn_perm(e3, e2, 0x07030602u);
+  v4i r;
+  r[0] = (int)__builtin_amdgcn_perm(lo23, lo01, 0x05040100u);
+  r[1] = (int)__builtin_amdgcn_perm(lo23, lo01, 0x07060302u);
+  r[2] = (int)__builtin_amdgcn_perm(hi23, hi01, 0x05040100u);
+  r[3] = (int)__builtin_amdgcn_perm(hi23, hi01, 0x07060302u);
+  return r;
+}
+
+// sum_b 256^(dbase + b) acc[b] mod q for four digit sums of magnitude < 2^22
+template <bool GOLD>
+__device__ __forceinline__ uint64_t agg_combine(const int32_t (&a)[4], uint32_t dbase, uint64_t q) {
+  const int64_t v = (int64_t)a[0] + ((int64_t)a[1] << 8) + ((int64_t)a[2] << 16) +
+                    ((int64_t)a[3] << 24);  // below 2^47 in magnitude
+  if (!GOLD) {
+    const int64_t r = v % (int64_t)q;
+    return (uint64_t)(r < 0 ? r + (int64_t)q : r);
+  }
+  uint64_t r = v < 0 ? q - (uint64_t)(-v) : (uint64_t)v;
+  for (uint32_t b = 0; b < dbase; b++) r = gold_shift8(r);
+  return r;
+}
+
+// Grid: one block per 128 polynomials, 8 waves; wave w multiplies tile w & 3
+// over half w >> 2 of the coefficients (k-pairs 16 h .. 16 h + 15) for digits
+// dbase .. dbase + 3; the halves meet in LDS and the first adds its sums to phi.
+template <class W, int DT>
+__global__ __launch_bounds__(512) void k_jl_agg(ChaKey key0, const W* __restrict__ scale,
+                                                uint64_t n, uint64_t total_polys, uint64_t q,
+                                                uint32_t dbase, W* __restrict__ phi) {
+  constexpr int D = 4;  // digits per pass
+  constexpr bool GOLD = DT == 8;
+  __shared__ __attribute__((aligned(16))) int8_t tab[D * 128 * kAggRow];  // 139,264 B
+  const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+
+  // R: tab[(b * 128 + w) * kAggRow + m] = digit dbase + b of c_m[w] or -c_m[w - 64];
+  // a task covers four consecutive m of one coefficient
+  for (uint32_t task = t; task < 64 * 64; task += 512) {
+    const uint32_t u = task & 63, m4 = task >> 6;
+    uint32_t pw[D] = {0, 0, 0, 0}, nw[D] = {0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const uint64_t x = (uint64_t)scale[(uint64_t)(4 * m4 + i) * 64 + u];
+      int8_t dp[DT], dn[DT];
+      lift_digits<DT>(x, q, dp);
+      lift_digits<DT>(x == 0 ? 0 : q - x, q, dn);
+#pragma unroll
+      for (int b = 0; b < D; b++) {
+        pw[b] |= (uint32_t)(uint8_t)dp[dbase + b] << (8 * i);
+        nw[b] |= (uint32_t)(uint8_t)dn[dbase + b] << (8 * i);
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < D; b++) {
+      *reinterpret_cast<uint32_t*>(tab + (b * 128 + u) * kAggRow + 4 * m4) = pw[b];
+      *reinterpret_cast<uint32_t*>(tab + (b * 128 + 64 + u) * kAggRow + 4 * m4) = nw[b];
+    }
+  }
+  __syncthreads();
+
+  const uint32_t tile = wave & 3, half = wave >> 2, hi = lane >> 5, col = lane & 31;
+  const uint64_t g = (uint64_t)blockIdx.x * kAggPolys + tile * 32 + col;  // this lane's A row
+  const bool live = g < total_polys;
+  ChaKey key = key0;
+  key.n0 = live ? (uint32_t)(g / n) : 0u;
+  const uint32_t jcol = live ? (uint32_t)((g % n) * 64) : 0u;
+  v16i acc[2][D];
+#pragma unroll
+  for (int h = 0; h < 2; h++)
+#pragma unroll
+    for (int b = 0; b < D; b++) acc[h][b] = v16i{};
+
+  for (uint32_t kp = 16 * half; kp < 16 * half + 16; kp++) {
+    const uint32_t k0 = 2 * kp;
+    uint32_t blk[16];
+    if (live) {
+      chacha_block(key, jcol + k0 + hi, blk);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 16; i++) blk[i] = 0xAAAAAAAAu;  // code 10: entry 0
+    }
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+      // step (k, m = 32 s .. 32 s + 31): half hi of the wave holds m 32 s + 16 hi + 0..15,
+      // word 2 s + hi of column k's block; the low half drew k0, the high half k0 + 1
+      const uint32_t give = hi ? blk[2 * s] : blk[2 * s + 1];
+      const uint32_t got = (uint32_t)__shfl_xor((int)give, 32);
+      const v4i a0 = jl_decode16(hi ? got : blk[2 * s]);       // coefficient k0
+      const v4i a1 = jl_decode16(hi ? blk[2 * s + 1] : got);   // coefficient k0 + 1
+      const int8_t* rb = tab + 32 * s + 16 * hi;
+#pragma unroll
+      for (int kk = 0; kk < 2; kk++) {
+        const v4i& a = kk ? a1 : a0;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          const uint32_t w = 32 * h + col + k0 + kk;  // t + k, below 128
+#pragma unroll
+          for (int b = 0; b < D; b++) {
+            const v4i bf = *reinterpret_cast<const v4i*>(rb + (b * 128 + w) * kAggRow);
+            acc[h][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bf, acc[h][b], 0, 0, 0);
+          }
+        }
+      }
+    }
+  }
+
+  // the second half's sums through LDS (R is no longer read), lane-major
+  __syncthreads();
+  int32_t* xs = reinterpret_cast<int32_t*>(tab) + (uint64_t)tile * (2 * D * 16 * 64);
+  if (half == 1) {
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int b = 0; b < D; b++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) xs[((h * D + b) * 16 + r) * 64 + lane] = acc[h][b][r];
+  }
+  __syncthreads();
+  if (half == 1) return;
+  const uint64_t g0 = (uint64_t)blockIdx.x * kAggPolys + tile * 32;
+#pragma unroll
+  for (int h = 0; h < 2; h++)
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const uint64_t gg = g0 + 8 * (r >> 2) + 4 * hi + (r & 3);  // D-tile row
+      int32_t sums[4];
+#pragma unroll
+      for (int b = 0; b < D; b++) sums[b] = acc[h][b][r] + xs[((h * D + b) * 16 + r) * 64 + lane];
+      if (gg >= total_polys) continue;
+      W* p = phi + gg * 64 + 32 * h + col;
+      const uint64_t add = agg_combine<GOLD>(sums, dbase, q);
+      const uint64_t cur = (uint64_t)*p;
+      const uint64_t s2 = cur + add;  // both below q; wraps 2^64 only for Goldilocks
+      *p = (W)((s2 < cur || s2 >= q) ? s2 - q : s2);
+    }
+}
+
 // ---- JL projection on the int8 matrix cores -----------------------------------
 //
 // p[m] = sum_c Pi[m][c] * s_c: the same A tile as the evaluations against one
@@ -613,6 +780,29 @@ int launch_jl_conj_flat(gh_ring* ring, const int8_t* flat, uint32_t rows, uint32
     hipLaunchKernelGGL(k_jl_conj_flat<uint64_t>, grid, dim3(256), 0, ctx->stream, flat,
                        num_parts, n, (uint64_t)ring->q, (uint64_t*)out);
   GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+int launch_jl_aggregate(gh_ring* ring, const uint8_t seed[32], const void* scale,
+                        uint32_t num_parts, uint64_t n, void* phi) {
+  gh_ctx* ctx = ring->ctx;
+  const uint64_t total = (uint64_t)num_parts * n;
+  if (total == 0) return GH_OK;
+  const uint64_t nblk = (total + kAggPolys - 1) / kAggPolys;
+  if (nblk > 0x7fffffffull) return GH_E_INVALID_ARGUMENT;
+  const ChaKey key = chacha_key(seed);
+  if (ring->kind == GH_RING_Q32) {
+    hipLaunchKernelGGL((k_jl_agg<uint32_t, 4>), dim3((unsigned)nblk), dim3(512), 0, ctx->stream,
+                       key, (const uint32_t*)scale, n, total, ring->q, 0u, (uint32_t*)phi);
+    GH_HIP(hipGetLastError());
+  } else {
+    for (uint32_t dbase = 0; dbase < 8; dbase += 4) {
+      hipLaunchKernelGGL((k_jl_agg<uint64_t, 8>), dim3((unsigned)nblk), dim3(512), 0,
+                         ctx->stream, key, (const uint64_t*)scale, n, total, ring->q, dbase,
+                         (uint64_t*)phi);
+      GH_HIP(hipGetLastError());
+    }
+  }
   return GH_OK;
 }
 

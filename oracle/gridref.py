@@ -309,6 +309,26 @@ def jl_evals(q, d, seed: bytes, rows, parts_polys, naive=False):
     return acc
 
 
+def jl_aggregate(q, d, seed: bytes, rows, scale, num_parts, n, phi=None, naive=False):
+    """The JL term of aggregate_step2_flat (labrador/src/main_protocol/aggregation.rs:237-245):
+    phi[i][j] += scale_m * conj(pi_i^(m))_j summed over the rows m, with the
+    conjugated JL polynomials of jl_rows_flat_to_conjugated_polys
+    (main_protocol/mod.rs:885-923) and the oracle's negacyclic product, in the
+    reference's order (m outer, then i, j). scale: (rows, d) polynomials, the
+    caller's jl_scale_m = sum_k beta_k * omega_k[m]; phi: (num_parts, n, d) or
+    None for zeros. Returns the new phi."""
+    sc = _u64(scale).reshape(rows, d)
+    acc = (np.zeros((num_parts * n, d), dtype=np.uint64) if phi is None
+           else _u64(phi).reshape(num_parts * n, d).copy())
+    flat = jl_extract_flat(seed, rows, num_parts, n * d, d)
+    conj = jl_conjugate_flat(q, flat, rows, num_parts, n, d).reshape(rows * num_parts * n, d)
+    a = np.repeat(sc, num_parts * n, axis=0)     # scale_m beside each (m, i, j)
+    prod = negacyclic_mul(q, d, a, conj, naive=naive).reshape(rows, num_parts * n, d)
+    for m in range(rows):
+        acc = _addmod(acc, prod[m], q)
+    return acc.reshape(num_parts, n, d)
+
+
 def chacha20_block(key: bytes, counter: int, nonce: bytes) -> bytes:
     k = np.frombuffer(bytes(key), dtype=np.uint8).copy()
     n = np.frombuffer(bytes(nonce), dtype=np.uint8).copy()

@@ -1,11 +1,14 @@
 """JL rows for aggregation through the C ABI: flat extraction, conjugated
-polynomials and the prover's JL evaluations against the oracle, bit for bit.
+polynomials, the prover's JL evaluations and the aggregation's JL term against
+the oracle, bit for bit.
 
 Sizes cross the kernels' tiles: 256-polynomial extraction blocks, 4- and
 64-polynomial conjugation tiles, 8-polynomial matrix-core rounds and the
-per-block split of the evaluation GEMM. At the LaBRADOR scale (32 parts of
-1024 polynomials) the evaluations are checked through their constant terms,
-which must equal the projection p.
+per-block split of the evaluation GEMM, and the aggregation's 32-polynomial
+tiles and 128-polynomial blocks. At the LaBRADOR scale (32 parts of 1024
+polynomials) the evaluations are checked through their constant terms, which
+must equal the projection p, and the aggregation through its pairing with the
+witness, which must equal the evaluations' pairing with the scale.
 """
 import numpy as np
 import pytest
@@ -137,3 +140,112 @@ def test_jl_evals_device_memory_and_errors(ctx, oracle, ring_q32):
     finally:
         inb.free()
         outb.free()
+
+
+def _ct_sum(a, b, q):
+    """sum over polynomials of the constant term of a_g * b_g in Z_q[X]/(X^64 + 1), q < 2^32."""
+    a = a.reshape(-1, 64).astype(np.uint64)
+    b = b.reshape(-1, 64).astype(np.uint64)
+    qq = np.uint64(q)
+    first = (a[:, 0] * b[:, 0]) % qq
+    rest = np.zeros(a.shape[0], dtype=np.uint64)
+    for k in range(1, 64):
+        rest += (a[:, k] * b[:, 64 - k]) % qq
+    return (int(first.sum()) - int(rest.sum())) % q
+
+
+@pytest.mark.parametrize("name", ["q32", "gold"])
+@pytest.mark.parametrize("num_parts,n", [(1, 1), (2, 3), (1, 40), (3, 50), (2, 129)])
+def test_jl_aggregate_matches_oracle(oracle, ring_q32, ring_gold, name, num_parts, n):
+    """Sizes cross the 32-polynomial tiles and 128-polynomial blocks, with part
+    boundaries inside a tile; phi starts nonzero (the caller's agg_phi)."""
+    ring = ring_q32 if name == "q32" else ring_gold
+    rng = np.random.default_rng(num_parts * 7 + n)
+    scale = rand_residues(rng, ring.q, (256, 64))
+    phi0 = rand_residues(rng, ring.q, (num_parts, n, 64))
+    got = ring.jl_aggregate(SEED, scale, phi0)
+    want = oracle.jl_aggregate(ring.q, 64, SEED, 256, scale, num_parts, n, phi=phi0)
+    assert np.array_equal(got.astype(np.uint64), want)
+
+
+@pytest.mark.parametrize("name", ["q32", "gold"])
+def test_jl_aggregate_extreme_scale_words(oracle, ring_q32, ring_gold, name):
+    ring = ring_q32 if name == "q32" else ring_gold
+    q = ring.q
+    edges = [0, 1, 2, q - 1, q - 2, q // 2, q // 2 + 1]
+    edges += ([2139062143, 2139062144] if name == "q32"
+              else [0x7F7F7F7F7F7F7F7F, 0x7F7F7F7F7F7F7F80])
+    rng = np.random.default_rng(78)
+    scale = np.array(edges, dtype=np.uint64)[rng.integers(0, len(edges), size=(256, 64))]
+    phi0 = np.array(edges, dtype=np.uint64)[rng.integers(0, len(edges), size=(2, 20, 64))]
+    got = ring.jl_aggregate(SEED, scale, phi0)
+    want = oracle.jl_aggregate(q, 64, SEED, 256, scale, 2, 20, phi=phi0)
+    assert np.array_equal(got.astype(np.uint64), want)
+
+
+@pytest.mark.parametrize("name", ["q32", "gold"])
+def test_jl_aggregate_is_linear_in_the_scale(ring_q32, ring_gold, name):
+    """Two calls with scales c and c' on the same phi equal one call with c + c'."""
+    ring = ring_q32 if name == "q32" else ring_gold
+    q = ring.q
+    rng = np.random.default_rng(79)
+    c1 = rand_residues(rng, q, (256, 64))
+    c2 = rand_residues(rng, q, (256, 64))
+    csum = np.array([(int(a) + int(b)) % q for a, b in zip(c1.ravel(), c2.ravel())],
+                    dtype=np.uint64).reshape(256, 64)
+    phi0 = rand_residues(rng, q, (5, 77, 64))
+    two = ring.jl_aggregate(SEED, c2, ring.jl_aggregate(SEED, c1, phi0))
+    one = ring.jl_aggregate(SEED, csum, phi0)
+    assert np.array_equal(two, one)
+
+
+def test_jl_aggregate_pairs_with_the_evaluations_at_labrador_scale(ring_q32):
+    """r 32 x 1024: sum_ij ct(phi_ij s_ij) = sum_m ct(scale_m jl_eval[m]), with
+    phi from zero, both sides on the device."""
+    ring = ring_q32
+    q = ring.q
+    rng = np.random.default_rng(124)
+    num_parts, n = 32, 1024
+    s = rand_residues(rng, q, (num_parts, n, 64)).astype(np.uint32)
+    scale = rand_residues(rng, q, (256, 64)).astype(np.uint32)
+    phi = ring.jl_aggregate(SEED, scale, np.zeros((num_parts, n, 64), dtype=np.uint32))
+    ev = ring.jl_evals(SEED, s)
+    assert _ct_sum(phi, s, q) == _ct_sum(scale, ev, q)
+
+
+def test_jl_aggregate_device_memory_and_errors(ctx, oracle, ring_q32):
+    import lattica_amd as la
+    rng = np.random.default_rng(80)
+    scale = rand_residues(rng, Q32, (256, 64)).astype(np.uint32)
+    phi0 = rand_residues(rng, Q32, (2, 9, 64)).astype(np.uint32)
+    sb = la.DeviceBuffer(ctx, scale.nbytes)
+    pb = la.DeviceBuffer(ctx, phi0.nbytes)
+    try:
+        sb.upload(scale)
+        pb.upload(phi0)
+        sd = np.frombuffer(SEED, dtype=np.uint8).copy()
+        L = la.lib()
+        assert L.gh_jl_aggregate(ring_q32.h, sd.ctypes.data, 256, sb.ptr, 2, 9 * 64, pb.ptr,
+                                 la.MEM_DEVICE) == 0
+        got = pb.download(np.uint32, (2, 9, 64))
+        want = oracle.jl_aggregate(Q32, 64, SEED, 256, scale, 2, 9, phi=phi0)
+        assert np.array_equal(got.astype(np.uint64), want)
+        ph = phi0.copy()
+        assert L.gh_jl_aggregate(ring_q32.h, sd.ctypes.data, 128, scale.ctypes.data, 2, 9 * 64,
+                                 ph.ctypes.data, la.MEM_HOST) == -20   # JLProfile: 256 rows
+        assert L.gh_jl_aggregate(ring_q32.h, sd.ctypes.data, 256, scale.ctypes.data, 2, 9 * 64 - 1,
+                                 ph.ctypes.data, la.MEM_HOST) == -20   # cols % d
+        bad = scale.copy()
+        bad[3, 5] = Q32
+        assert L.gh_jl_aggregate(ring_q32.h, sd.ctypes.data, 256, bad.ctypes.data, 2, 9 * 64,
+                                 ph.ctypes.data, la.MEM_HOST) == -21   # not canonical
+        bad = phi0.copy()
+        bad[1, 8, 63] = 0xFFFFFFFF
+        assert L.gh_jl_aggregate(ring_q32.h, sd.ctypes.data, 256, scale.ctypes.data, 2, 9 * 64,
+                                 bad.ctypes.data, la.MEM_HOST) == -21
+        assert np.array_equal(ph, phi0)   # rejected calls leave phi alone
+        assert L.gh_jl_aggregate(ring_q32.h, sd.ctypes.data, 256, scale.ctypes.data, 0, 9 * 64,
+                                 None, la.MEM_HOST) == 0          # nothing to aggregate
+    finally:
+        sb.free()
+        pb.free()

@@ -93,3 +93,75 @@ def test_jl_evals_evaluation_homomorphism(oracle, q):
                 pi = [int(v) % q for v in e[m, i, :, j]]
                 want = (want + poly_eval(pi, psi_inv, q) * sv[i][j]) % q
         assert poly_eval(ev[m], psi, q) == want, m
+
+
+def ct_product(a, b, q):
+    """Constant term of a * b in Z_q[X]/(X^d + 1)."""
+    a = [int(v) for v in a]
+    b = [int(v) for v in b]
+    d = len(a)
+    return (a[0] * b[0] - sum(a[k] * b[d - k] for k in range(1, d))) % q
+
+
+@pytest.mark.parametrize("q", [Q32, GOLDILOCKS])
+def test_jl_aggregate_evaluation_homomorphism(oracle, q):
+    """aggregation.rs:237-245 at a root w of X^d + 1: phi_ij(w) = phi0_ij(w) +
+    sum_m scale_m(w) * pi_ij^(m)(w^-1), with pi read from the flat entries."""
+    rng = np.random.default_rng(21)
+    rows, num_parts, n, d = 256, 2, 3, 64
+    scale = rand_residues(rng, q, (rows, d))
+    phi0 = rand_residues(rng, q, (num_parts, n, d))
+    phi = oracle.jl_aggregate(q, d, SEED, rows, scale, num_parts, n, phi=phi0)
+    e = entries(oracle.jl_extract_flat(SEED, rows, num_parts, n * d, d), rows, num_parts, d, n)
+    psi = oracle.root_of_unity(q, 2 * d)
+    psi_inv = pow(psi, 2 * d - 1, q)
+    sc = [poly_eval(scale[m], psi, q) for m in range(rows)]
+    for i in range(num_parts):
+        for j in range(n):
+            want = poly_eval(phi0[i, j], psi, q)
+            for m in range(rows):
+                pi = [int(v) % q for v in e[m, i, :, j]]
+                want = (want + sc[m] * poly_eval(pi, psi_inv, q)) % q
+            assert poly_eval(phi[i, j], psi, q) == want, (i, j)
+
+
+def test_jl_aggregate_constant_scale_gives_the_projection(oracle):
+    """With scale_m = omega_m (constants) the aggregated phi pairs with the
+    witness to sum_m omega_m p[m], the check the verifier makes on b''
+    (aggregation.rs:158-178)."""
+    q = Q32
+    rng = np.random.default_rng(22)
+    rows, num_parts, n, d = 256, 3, 2, 64
+    omega = rand_residues(rng, q, (rows,))
+    scale = np.zeros((rows, d), dtype=np.uint64)
+    scale[:, 0] = omega
+    s = rand_residues(rng, q, (num_parts, n, d))
+    phi = oracle.jl_aggregate(q, d, SEED, rows, scale, num_parts, n)
+    pm, _, _ = oracle.jl_project(q, SEED, rows, s.reshape(num_parts, n * d), d)
+    lhs = sum(ct_product(phi[i, j], s[i, j], q) for i in range(num_parts) for j in range(n)) % q
+    rhs = sum(int(omega[m]) * int(pm[m]) for m in range(rows)) % q
+    assert lhs == rhs
+
+
+@pytest.mark.parametrize("q", [Q32, GOLDILOCKS])
+def test_jl_aggregate_pairs_with_the_jl_evaluations(oracle, q):
+    """sum_ij ct(phi_ij * s_ij) = sum_m ct(scale_m * jl_eval[m]): the two sums
+    regroup the same products sigma(pi_ij^(m)) * scale_m * s_ij."""
+    rng = np.random.default_rng(23)
+    rows, num_parts, n, d = 256, 2, 3, 64
+    scale = rand_residues(rng, q, (rows, d))
+    s = rand_residues(rng, q, (num_parts, n, d))
+    phi = oracle.jl_aggregate(q, d, SEED, rows, scale, num_parts, n)
+    ev = oracle.jl_evals(q, d, SEED, rows, s)
+    lhs = sum(ct_product(phi[i, j], s[i, j], q) for i in range(num_parts) for j in range(n)) % q
+    rhs = sum(ct_product(scale[m], ev[m], q) for m in range(rows)) % q
+    assert lhs == rhs
+
+
+def test_jl_aggregate_naive_products_agree(oracle):
+    q = Q32
+    rng = np.random.default_rng(24)
+    scale = rand_residues(rng, q, (256, 64))
+    a = oracle.jl_aggregate(q, 64, SEED, 256, scale, 1, 2)
+    b = oracle.jl_aggregate(q, 64, SEED, 256, scale, 1, 2, naive=True)
+    assert np.array_equal(a, b)
