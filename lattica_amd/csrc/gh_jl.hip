@@ -433,29 +433,26 @@ __global__ __launch_bounds__(64 * D) void k_jl_evals_fin(const int32_t* __restri
 // and (c X^-k)[t] = c[t + k] for t + k < 64, -c[t + k - 64] otherwise, this is
 // phi[g][t] += sum_(k, m) Pi[m][g, k] * R[w = t + k][m], R[w][m] the digits of
 // c_m[w] (w < 64) or of -c_m[w - 64]: a GEMM with the polynomials g as M,
-// (k, m) as K (m fastest, 32 per step) and t as N. The A operand comes straight
-// from the keystream in registers: lane l draws column (g, k) of its
-// polynomial, l < 32 for even k and l >= 32 for odd k, so the two halves of a
-// wave swap one word per step and every block is drawn once. R sits in LDS.
+// (k, m) as K (32 m per step) and t as N. The A operand comes straight from
+// the keystream in registers: lane l draws column (g, k) of its polynomial,
+// l < 32 for even k and l >= 32 for odd k; one v_permlane32_swap per step
+// hands each half the word it needs, so every block is drawn once. Within a
+// 16-row group the K order is m = 4 o + i at byte 4 i + o, which is what four
+// masked shifts of a keystream word give, and R is stored in the same order.
+// R sits in LDS. The next block's rounds are spread between the current
+// block's matrix-core steps.
 
 constexpr int kAggPolys = 128;   // polynomials per block: 4 tiles of 32
 constexpr int kAggRow = 272;     // bytes per R row: 256 m + 16 pad (bank spread)
 
-// 16 ternary codes (2 bits each, m ascending) -> 16 int8 entries
+// 16 ternary codes of a keystream word -> 16 int8 entries, byte 4 i + o
+// holding code 4 o + i (00 -> +1, 01 -> -1, else 0)
 __device__ __forceinline__ v4i jl_decode16(uint32_t x) {
-  const uint32_t e0 = __builtin_amdgcn_perm(0u, 0x0000FF01u, x & 0x03030303u);          // codes 0, 4, 8, 12
-  const uint32_t e1 = __builtin_amdgcn_perm(0u, 0x0000FF01u, (x >> 2) & 0x03030303u);   // 1, 5, 9, 13
-  const uint32_t e2 = __builtin_amdgcn_perm(0u, 0x0000FF01u, (x >> 4) & 0x03030303u);
-  const uint32_t e3 = __builtin_amdgcn_perm(0u, 0x0000FF01u, (x >> 6) & 0x03030303u);
-  const uint32_t lo01 = __builtin_amdgcn_perm(e1, e0, 0x05010400u);
-  const uint32_t hi01 = __builtin_amdgcn_perm(e1, e0, 0x07030602u);
-  const uint32_t lo23 = __builtin_amdgcn_perm(e3, e2, 0x05010400u);
-  const uint32_t hi23 = __builtin_amdgcn_perm(e3, e2, 0x07030602u);
   v4i r;
-  r[0] = (int)__builtin_amdgcn_perm(lo23, lo01, 0x05040100u);
-  r[1] = (int)__builtin_amdgcn_perm(lo23, lo01, 0x07060302u);
-  r[2] = (int)__builtin_amdgcn_perm(hi23, hi01, 0x05040100u);
-  r[3] = (int)__builtin_amdgcn_perm(hi23, hi01, 0x07060302u);
+  r[0] = (int)__builtin_amdgcn_perm(0u, 0x0000FF01u, x & 0x03030303u);
+  r[1] = (int)__builtin_amdgcn_perm(0u, 0x0000FF01u, (x >> 2) & 0x03030303u);
+  r[2] = (int)__builtin_amdgcn_perm(0u, 0x0000FF01u, (x >> 4) & 0x03030303u);
+  r[3] = (int)__builtin_amdgcn_perm(0u, 0x0000FF01u, (x >> 6) & 0x03030303u);
   return r;
 }
 
@@ -473,9 +470,13 @@ __device__ __forceinline__ uint64_t agg_combine(const int32_t (&a)[4], uint32_t 
   return r;
 }
 
-// Grid: one block per 128 polynomials, 8 waves; wave w multiplies tile w & 3
-// over half w >> 2 of the coefficients (k-pairs 16 h .. 16 h + 15) for digits
-// dbase .. dbase + 3; the halves meet in LDS and the first adds its sums to phi.
+// Grid: one block per 128 polynomials, 8 waves (two per SIMD, so one wave's
+// keystream rounds can run beside the other's matrix-core ops); wave w
+// multiplies tile w & 3 over half w >> 2 of the coefficients (k-pairs
+// 16 h .. 16 h + 15) for digits dbase .. dbase + 3; the halves meet in LDS and
+// the first adds its sums to phi. Lanes past the last polynomial draw
+// keystream like the others: an A row only reaches the D row of the same
+// polynomial, which is not stored.
 template <class W, int DT>
 __global__ __launch_bounds__(512) void k_jl_agg(ChaKey key0, const W* __restrict__ scale,
                                                 uint64_t n, uint64_t total_polys, uint64_t q,
@@ -485,27 +486,28 @@ __global__ __launch_bounds__(512) void k_jl_agg(ChaKey key0, const W* __restrict
   __shared__ __attribute__((aligned(16))) int8_t tab[D * 128 * kAggRow];  // 139,264 B
   const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
 
-  // R: tab[(b * 128 + w) * kAggRow + m] = digit dbase + b of c_m[w] or -c_m[w - 64];
-  // a task covers four consecutive m of one coefficient
+  // R: row (b, w) = digit dbase + b of c_m[w] or -c_m[w - 64]; within each
+  // 16-m group byte 4 i + o holds m = 4 o + i. A task writes one word: i of
+  // group c, m = 16 c + i + 4 o for o = 0..3.
   for (uint32_t task = t; task < 64 * 64; task += 512) {
-    const uint32_t u = task & 63, m4 = task >> 6;
+    const uint32_t u = task & 63, c = task >> 8, i = (task >> 6) & 3;
     uint32_t pw[D] = {0, 0, 0, 0}, nw[D] = {0, 0, 0, 0};
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const uint64_t x = (uint64_t)scale[(uint64_t)(4 * m4 + i) * 64 + u];
+    for (int o = 0; o < 4; o++) {
+      const uint64_t x = (uint64_t)scale[(uint64_t)(16 * c + i + 4 * o) * 64 + u];
       int8_t dp[DT], dn[DT];
       lift_digits<DT>(x, q, dp);
       lift_digits<DT>(x == 0 ? 0 : q - x, q, dn);
 #pragma unroll
       for (int b = 0; b < D; b++) {
-        pw[b] |= (uint32_t)(uint8_t)dp[dbase + b] << (8 * i);
-        nw[b] |= (uint32_t)(uint8_t)dn[dbase + b] << (8 * i);
+        pw[b] |= (uint32_t)(uint8_t)dp[dbase + b] << (8 * o);
+        nw[b] |= (uint32_t)(uint8_t)dn[dbase + b] << (8 * o);
       }
     }
 #pragma unroll
     for (int b = 0; b < D; b++) {
-      *reinterpret_cast<uint32_t*>(tab + (b * 128 + u) * kAggRow + 4 * m4) = pw[b];
-      *reinterpret_cast<uint32_t*>(tab + (b * 128 + 64 + u) * kAggRow + 4 * m4) = nw[b];
+      *reinterpret_cast<uint32_t*>(tab + (b * 128 + u) * kAggRow + 16 * c + 4 * i) = pw[b];
+      *reinterpret_cast<uint32_t*>(tab + (b * 128 + 64 + u) * kAggRow + 16 * c + 4 * i) = nw[b];
     }
   }
   __syncthreads();
@@ -522,39 +524,48 @@ __global__ __launch_bounds__(512) void k_jl_agg(ChaKey key0, const W* __restrict
 #pragma unroll
     for (int b = 0; b < D; b++) acc[h][b] = v16i{};
 
-  for (uint32_t kp = 16 * half; kp < 16 * half + 16; kp++) {
-    const uint32_t k0 = 2 * kp;
-    uint32_t blk[16];
-    if (live) {
-      chacha_block(key, jcol + k0 + hi, blk);
-    } else {
+  // one K step: coefficient pair (k0, k0 + 1), rows m = 32 s .. 32 s + 31, the
+  // half hi of the wave on m = 32 s + 16 hi .. + 15 (word 2 s + hi of a block)
+  auto step = [&](const uint32_t (&blk)[16], uint32_t k0, int s) {
+    const auto sw = __builtin_amdgcn_permlane32_swap(blk[2 * s], blk[2 * s + 1], false, false);
+    const v4i a0 = jl_decode16(sw[0]);  // coefficient k0
+    const v4i a1 = jl_decode16(sw[1]);  // coefficient k0 + 1
+    const int8_t* rb = tab + 32 * s + 16 * hi;
 #pragma unroll
-      for (int i = 0; i < 16; i++) blk[i] = 0xAAAAAAAAu;  // code 10: entry 0
-    }
+    for (int kk = 0; kk < 2; kk++) {
+      const v4i& a = kk ? a1 : a0;
 #pragma unroll
-    for (int s = 0; s < 8; s++) {
-      // step (k, m = 32 s .. 32 s + 31): half hi of the wave holds m 32 s + 16 hi + 0..15,
-      // word 2 s + hi of column k's block; the low half drew k0, the high half k0 + 1
-      const uint32_t give = hi ? blk[2 * s] : blk[2 * s + 1];
-      const uint32_t got = (uint32_t)__shfl_xor((int)give, 32);
-      const v4i a0 = jl_decode16(hi ? got : blk[2 * s]);       // coefficient k0
-      const v4i a1 = jl_decode16(hi ? blk[2 * s + 1] : got);   // coefficient k0 + 1
-      const int8_t* rb = tab + 32 * s + 16 * hi;
+      for (int h = 0; h < 2; h++) {
+        const uint32_t w = 32 * h + col + k0 + kk;  // t + k, below 128
 #pragma unroll
-      for (int kk = 0; kk < 2; kk++) {
-        const v4i& a = kk ? a1 : a0;
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-          const uint32_t w = 32 * h + col + k0 + kk;  // t + k, below 128
-#pragma unroll
-          for (int b = 0; b < D; b++) {
-            const v4i bf = *reinterpret_cast<const v4i*>(rb + (b * 128 + w) * kAggRow);
-            acc[h][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bf, acc[h][b], 0, 0, 0);
-          }
+        for (int b = 0; b < D; b++) {
+          const v4i bf = *reinterpret_cast<const v4i*>(rb + (b * 128 + w) * kAggRow);
+          acc[h][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bf, acc[h][b], 0, 0, 0);
         }
       }
     }
+  };
+
+  const uint32_t kp_begin = 16 * half, kp_end = kp_begin + 16;
+  uint32_t cur[16];
+  chacha_block(key, jcol + 2 * kp_begin + hi, cur);
+  for (uint32_t kp = kp_begin; kp + 1 < kp_end; kp++) {
+    // steps of block kp with the ten double rounds of block kp + 1 between them
+    const uint32_t ctr = jcol + 2 * (kp + 1) + hi;
+    uint32_t nx[16];
+    chacha_init(key, ctr, nx);
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+      step(cur, 2 * kp, s);
+      chacha_double_round(nx);
+      if (s < 2) chacha_double_round(nx);
+    }
+    chacha_finish(key, ctr, nx);
+#pragma unroll
+    for (int i = 0; i < 16; i++) cur[i] = nx[i];
   }
+#pragma unroll
+  for (int s = 0; s < 8; s++) step(cur, 2 * (kp_end - 1), s);
 
   // the second half's sums through LDS (R is no longer read), lane-major
   __syncthreads();
@@ -581,9 +592,9 @@ __global__ __launch_bounds__(512) void k_jl_agg(ChaKey key0, const W* __restrict
       if (gg >= total_polys) continue;
       W* p = phi + gg * 64 + 32 * h + col;
       const uint64_t add = agg_combine<GOLD>(sums, dbase, q);
-      const uint64_t cur = (uint64_t)*p;
-      const uint64_t s2 = cur + add;  // both below q; wraps 2^64 only for Goldilocks
-      *p = (W)((s2 < cur || s2 >= q) ? s2 - q : s2);
+      const uint64_t cur_w = (uint64_t)*p;
+      const uint64_t s2 = cur_w + add;  // both below q; wraps 2^64 only for Goldilocks
+      *p = (W)((s2 < cur_w || s2 >= q) ? s2 - q : s2);
     }
 }
 
