@@ -37,29 +37,6 @@ __device__ __forceinline__ void chacha_block(const ChaKey& key, uint32_t counter
   out[12] = x12 + counter; out[13] = x13 + key.n0; out[14] = x14; out[15] = x15;
 }
 
-// The block function in pieces, so a kernel can spread the rounds of its next
-// block between other work: init, ten double rounds, finish.
-__device__ __forceinline__ void chacha_init(const ChaKey& key, uint32_t counter, uint32_t (&x)[16]) {
-  x[0] = 0x61707865; x[1] = 0x3320646e; x[2] = 0x79622d32; x[3] = 0x6b206574;
-#pragma unroll
-  for (int i = 0; i < 8; i++) x[4 + i] = key.k[i];
-  x[12] = counter; x[13] = key.n0; x[14] = 0; x[15] = 0;
-}
-
-__device__ __forceinline__ void chacha_double_round(uint32_t (&x)[16]) {
-  GH_QR(x[0], x[4], x[8], x[12]) GH_QR(x[1], x[5], x[9], x[13])
-  GH_QR(x[2], x[6], x[10], x[14]) GH_QR(x[3], x[7], x[11], x[15])
-  GH_QR(x[0], x[5], x[10], x[15]) GH_QR(x[1], x[6], x[11], x[12])
-  GH_QR(x[2], x[7], x[8], x[13]) GH_QR(x[3], x[4], x[9], x[14])
-}
-
-__device__ __forceinline__ void chacha_finish(const ChaKey& key, uint32_t counter, uint32_t (&x)[16]) {
-  x[0] += 0x61707865; x[1] += 0x3320646e; x[2] += 0x79622d32; x[3] += 0x6b206574;
-#pragma unroll
-  for (int i = 0; i < 8; i++) x[4 + i] += key.k[i];
-  x[12] += counter; x[13] += key.n0;
-}
-
 // key words from the 32-byte seed (little-endian)
 inline ChaKey chacha_key(const uint8_t seed[32]) {
   ChaKey key;

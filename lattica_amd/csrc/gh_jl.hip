@@ -439,8 +439,7 @@ __global__ __launch_bounds__(64 * D) void k_jl_evals_fin(const int32_t* __restri
 // hands each half the word it needs, so every block is drawn once. Within a
 // 16-row group the K order is m = 4 o + i at byte 4 i + o, which is what four
 // masked shifts of a keystream word give, and R is stored in the same order.
-// R sits in LDS. The next block's rounds are spread between the current
-// block's matrix-core steps.
+// R sits in LDS.
 
 constexpr int kAggPolys = 128;   // polynomials per block: 4 tiles of 32
 constexpr int kAggRow = 272;     // bytes per R row: 256 m + 16 pad (bank spread)
@@ -471,7 +470,7 @@ __device__ __forceinline__ uint64_t agg_combine(const int32_t (&a)[4], uint32_t 
 }
 
 // Grid: one block per 128 polynomials, 8 waves (two per SIMD, so one wave's
-// keystream rounds can run beside the other's matrix-core ops); wave w
+// keystream can run beside the other's matrix-core ops); wave w
 // multiplies tile w & 3 over half w >> 2 of the coefficients (k-pairs
 // 16 h .. 16 h + 15) for digits dbase .. dbase + 3; the halves meet in LDS and
 // the first adds its sums to phi. Lanes past the last polynomial draw
@@ -546,26 +545,24 @@ __global__ __launch_bounds__(512) void k_jl_agg(ChaKey key0, const W* __restrict
     }
   };
 
-  const uint32_t kp_begin = 16 * half, kp_end = kp_begin + 16;
-  uint32_t cur[16];
-  chacha_block(key, jcol + 2 * kp_begin + hi, cur);
-  for (uint32_t kp = kp_begin; kp + 1 < kp_end; kp++) {
-    // steps of block kp with the ten double rounds of block kp + 1 between them
-    const uint32_t ctr = jcol + 2 * (kp + 1) + hi;
-    uint32_t nx[16];
-    chacha_init(key, ctr, nx);
+  const uint32_t kp_begin = 16 * half;
+  for (uint32_t kp = kp_begin; kp < kp_begin + 16; kp++) {
+    uint32_t blk[16];
+    chacha_block(key, jcol + 2 * kp + hi, blk);
 #pragma unroll
-    for (int s = 0; s < 8; s++) {
-      step(cur, 2 * kp, s);
-      chacha_double_round(nx);
-      if (s < 2) chacha_double_round(nx);
+    for (int s = 0; s < 8; s++) step(blk, 2 * kp, s);
+    // matrix-core order for the 128 MFMAs of the pair: each followed by the R
+    // read four MFMAs ahead, so four reads are in flight instead of one
+#pragma unroll
+    for (int i = 0; i < 4; i++) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
+#pragma unroll
+    for (int i = 0; i < 124; i++) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
     }
-    chacha_finish(key, ctr, nx);
 #pragma unroll
-    for (int i = 0; i < 16; i++) cur[i] = nx[i];
+    for (int i = 0; i < 4; i++) __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
   }
-#pragma unroll
-  for (int s = 0; s < 8; s++) step(cur, 2 * (kp_end - 1), s);
 
   // the second half's sums through LDS (R is no longer read), lane-major
   __syncthreads();
