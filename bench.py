@@ -659,8 +659,9 @@ def serving(ctx, la, log, count=1000, seed=55):
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400)
-    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=1000,
+                    help="untimed steps first; about 30 ms, so the clocks have ramped")
     ap.add_argument("--keys", type=int, default=3, help="resident keys used in rotation")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
