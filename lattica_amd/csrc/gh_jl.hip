@@ -597,74 +597,138 @@ __global__ __launch_bounds__(512) void k_jl_agg(ChaKey key0, const W* __restrict
 
 // ---- JL projection on the int8 matrix cores -----------------------------------
 //
-// p[m] = sum_c Pi[m][c] * s_c: the same A tile as the evaluations against one
-// B column per digit of s_c (a 32-wide tile, 8 columns used): the centred
-// lift in 5 signed digits for the 32-bit prime, so the int64 sum is the
-// reference's exact p (jl.rs:388-508), or the Goldilocks window lift in 8
-// digits, reduced mod p. Wave w owns rows 32 w .. 32 w + 31.
+// p[m] = sum_c Pi[m][c] * s_c as an int8 GEMM: Pi (256 rows x the columns) as
+// A, one B column per digit of s_c: the centred lift in 5 signed digits for
+// the 32-bit prime, so the int64 sum is the reference's exact p
+// (jl.rs:388-508), or the Goldilocks window lift in 8 digits, reduced mod p.
+// The output is small (256 rows x 8 digit columns), so every wave holds all of
+// it and works through whole polynomials of its own, with no LDS tile: lane l
+// draws the keystream of one coefficient, and the keystream words, which hold
+// one column and many rows, are turned into A fragments, which hold one row
+// and many columns, by a matrix-core transpose (an MFMA against a constant 0/1
+// matrix; the lanes draw their coefficients in the order that makes the
+// transposed columns come out natural). The witness digits are transposed
+// the same way into the B fragments.
 constexpr int kProjDigits = 8;  // digit slots per witness word (B columns used)
+constexpr int kProjWaves = 4;   // waves per block
 
-template <class W, bool GOLD>
-__global__ __launch_bounds__(512) void k_jl_proj(ChaKey key0, const W* __restrict__ parts,
-                                                 uint64_t n, uint64_t total_polys,
-                                                 uint64_t polys_per_block, uint64_t q,
-                                                 int32_t* __restrict__ partial) {
-  __shared__ __attribute__((aligned(16))) int8_t a_tile[kEvPolys * 256 * kEvRowBytes];
-  __shared__ __attribute__((aligned(16))) int8_t d_tile[kEvPolys * kProjDigits * 64];
-  const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const uint64_t g_begin = (uint64_t)blockIdx.x * polys_per_block;
-  const uint64_t g_end = std::min<uint64_t>(g_begin + polys_per_block, total_polys);
-  v16i acc = v16i{};
-  for (uint64_t g0 = g_begin; g0 < g_end; g0 += kEvPolys) {
-    {
-      const uint32_t ps = t >> 6, k = t & 63;
-      const uint64_t g = g0 + ps;  // wave-uniform
-      uint32_t blk[16];
-      int64_t v = 0;
-      if (g < g_end) {
-        ChaKey key = key0;
-        key.n0 = (uint32_t)(g / n);
-        chacha_block(key, (uint32_t)((g % n) * 64 + k), blk);
-        const uint64_t x = (uint64_t)parts[g * 64 + k];
-        if (GOLD)
-          v = x <= 0x7F7F7F7F7F7F7F7Full ? (int64_t)x : (int64_t)(x - q);
-        else
-          v = x > q / 2 ? (int64_t)x - (int64_t)q : (int64_t)x;  // centred, as p_exact
-      } else {
-#pragma unroll
-        for (int i = 0; i < 16; i++) blk[i] = 0xAAAAAAAAu;
-      }
-      jl_tile_store(blk, a_tile + ps * 256 * kEvRowBytes, k);
-      int8_t dig[kProjDigits];
-      signed_digits<kProjDigits>(v, dig);
-#pragma unroll
-      for (int b = 0; b < kProjDigits; b++) d_tile[(ps * kProjDigits + b) * 64 + k] = dig[b];
-    }
-    __syncthreads();
-    const uint32_t row = lane & 31, kg = lane >> 5;
-    for (int ps = 0; ps < kEvPolys; ps++) {
-      if (g0 + ps >= g_end) break;
-#pragma unroll
-      for (int kstep = 0; kstep < 2; kstep++) {
-        const uint32_t k0 = kstep * 32 + kg * 16;
-        const v4i afrag = *reinterpret_cast<const v4i*>(
-            a_tile + (ps * 256 + wave * 32 + row) * kEvRowBytes + k0);
-        const v4i bfrag = row < kProjDigits
-                              ? *reinterpret_cast<const v4i*>(
-                                    d_tile + (ps * kProjDigits + row) * 64 + k0)
-                              : v4i{0, 0, 0, 0};
-        acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(afrag, bfrag, acc, 0, 0, 0);
-      }
-    }
-    __syncthreads();
+// Coefficient drawn by `lane` within a polynomial: lanes 0-31 the first 32,
+// 32-63 the second, lane i of a half on 16 ((i >> 2) & 1) + 4 (i >> 3) + (i & 3):
+// a transpose then leaves register r of lane half h on coefficient 16 h + r.
+__device__ __forceinline__ uint32_t jl_tcoeff(uint32_t lane) {
+  const uint32_t i = lane & 31;
+  return 32 * (lane >> 5) + 16 * ((i >> 2) & 1) + 4 * (i >> 3) + (i & 3);
+}
+
+// B operand of the keystream transpose: a 1 where K position (jl_decode16's
+// order, m = 4 o + i at byte 4 i + o of each 16-group) holds row j = lane & 31
+__device__ __forceinline__ v4i jl_tperm(uint32_t lane) {
+  const uint32_t j = lane & 31, h = lane >> 5;
+  v4i r = {0, 0, 0, 0};
+  if ((j >> 4) == h) {
+    const uint32_t jj = j & 15, b = 4 * (jj & 3) + (jj >> 2);
+    r[b >> 2] = (int)(1u << (8 * (b & 3)));
   }
-  const uint32_t col_in = lane & 31, rgrp = lane >> 5;
-  if (col_in < kProjDigits) {
+  return r;
+}
+
+// B operand of the digit transpose: the 32 x 32 identity
+__device__ __forceinline__ v4i jl_tident(uint32_t lane) {
+  const uint32_t j = lane & 31, h = lane >> 5;
+  v4i r = {0, 0, 0, 0};
+  if ((j >> 4) == h) r[(j & 15) >> 2] = (int)(1u << (8 * (j & 3)));
+  return r;
+}
+
+// the low bytes of 16 small int32 sums as an int8 operand, register r -> byte r
+__device__ __forceinline__ v4i jl_pack(const v16i& d) {
+  v4i r;
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-      const uint32_t m = wave * 32 + 8 * (r >> 2) + 4 * rgrp + (r & 3);
-      partial[((uint64_t)blockIdx.x * 256 + m) * kProjDigits + col_in] = acc[r];
+  for (int k = 0; k < 4; k++)
+    r[k] = (int)(__builtin_amdgcn_perm((uint32_t)d[4 * k + 1], (uint32_t)d[4 * k], 0x0C0C0400u) |
+                 __builtin_amdgcn_perm((uint32_t)d[4 * k + 3], (uint32_t)d[4 * k + 2], 0x04000C0Cu));
+  return r;
+}
+
+// Grid: blocks of 4 waves, wave w of block b on polynomials
+// [(4 b + w) ppw, + ppw); per polynomial: 16 keystream transposes, 2 digit
+// transposes and 16 accumulating MFMAs (8 row tiles x 2 coefficient halves).
+// The waves' sums meet in LDS; one partial [256][8] per block.
+template <class W, bool GOLD>
+__global__ __launch_bounds__(256, 2) void k_jl_proj(ChaKey key0, const W* __restrict__ parts,
+                                                 uint64_t n, uint64_t total_polys,
+                                                 uint64_t polys_per_wave, uint64_t q,
+                                                 int32_t* __restrict__ partial) {
+  __shared__ int32_t red[kProjWaves - 1][256 * kProjDigits];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t g_begin = ((uint64_t)blockIdx.x * kProjWaves + wave) * polys_per_wave;
+  const uint64_t g_end = std::min<uint64_t>(g_begin + polys_per_wave, total_polys);
+  const uint32_t coeff = jl_tcoeff(lane);
+  const v4i tperm = jl_tperm(lane), tident = jl_tident(lane);
+  v16i acc[8];
+#pragma unroll
+  for (int mc = 0; mc < 8; mc++) acc[mc] = v16i{};
+
+  for (uint64_t g = g_begin; g < g_end; g++) {  // wave-uniform
+    ChaKey key = key0;
+    key.n0 = (uint32_t)(g / n);
+    uint32_t blk[16];
+    chacha_block(key, (uint32_t)((g % n) * 64 + coeff), blk);
+    const uint64_t x = (uint64_t)parts[g * 64 + coeff];
+    int64_t v;
+    if (GOLD)
+      v = x <= 0x7F7F7F7F7F7F7F7Full ? (int64_t)x : (int64_t)(x - q);
+    else
+      v = x > q / 2 ? (int64_t)x - (int64_t)q : (int64_t)x;  // centred, as p_exact
+    int8_t dig[kProjDigits];
+    signed_digits<kProjDigits>(v, dig);
+    uint32_t dw[2] = {0, 0};
+#pragma unroll
+    for (int d = 0; d < kProjDigits; d++) dw[d >> 2] |= (uint32_t)(uint8_t)dig[d] << (8 * (d & 3));
+    // digit rows: the low half of the wave for the first 32 coefficients, the
+    // high half's digits brought down for the second
+    const auto s0 = __builtin_amdgcn_permlane32_swap(dw[0], 0u, false, false);
+    const auto s1 = __builtin_amdgcn_permlane32_swap(dw[1], 0u, false, false);
+    const v4i bd1 = jl_pack(__builtin_amdgcn_mfma_i32_32x32x32_i8(
+        v4i{(int)s0[0], (int)s1[0], 0, 0}, tident, v16i{}, 0, 0, 0));
+    const v4i bd2 = jl_pack(__builtin_amdgcn_mfma_i32_32x32x32_i8(
+        v4i{(int)s0[1], (int)s1[1], 0, 0}, tident, v16i{}, 0, 0, 0));
+#pragma unroll
+    for (int mc = 0; mc < 8; mc++) {
+      // rows 32 mc .. 32 mc + 31: word 2 mc + hi of each block, one swap per pair
+      const auto sw = __builtin_amdgcn_permlane32_swap(blk[2 * mc], blk[2 * mc + 1], false, false);
+      const v4i a1 = jl_pack(__builtin_amdgcn_mfma_i32_32x32x32_i8(jl_decode16(sw[0]), tperm,
+                                                                   v16i{}, 0, 0, 0));
+      const v4i a2 = jl_pack(__builtin_amdgcn_mfma_i32_32x32x32_i8(jl_decode16(sw[1]), tperm,
+                                                                   v16i{}, 0, 0, 0));
+      acc[mc] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, bd1, acc[mc], 0, 0, 0);
+      acc[mc] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a2, bd2, acc[mc], 0, 0, 0);
     }
+  }
+
+  // digit column d = lane & 31 < 8; row m = 32 mc + 8 (r / 4) + 4 (lane / 32) + r % 4
+  const uint32_t d = lane & 31, hi = lane >> 5;
+  if (wave > 0 && d < kProjDigits) {
+#pragma unroll
+    for (int mc = 0; mc < 8; mc++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const uint32_t m = 32 * mc + 8 * (r >> 2) + 4 * hi + (r & 3);
+        red[wave - 1][m * kProjDigits + d] = acc[mc][r];
+      }
+  }
+  __syncthreads();
+  if (wave == 0 && d < kProjDigits) {
+#pragma unroll
+    for (int mc = 0; mc < 8; mc++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const uint32_t m = 32 * mc + 8 * (r >> 2) + 4 * hi + (r & 3);
+        int32_t sum = acc[mc][r];
+#pragma unroll
+        for (int w = 0; w < kProjWaves - 1; w++) sum += red[w][m * kProjDigits + d];
+        partial[((uint64_t)blockIdx.x * 256 + m) * kProjDigits + d] = sum;
+      }
   }
 }
 
@@ -720,23 +784,23 @@ int launch_jl_proj(gh_ring* ring, const uint8_t seed[32], uint32_t rows, const v
   gh_ctx* ctx = ring->ctx;
   const bool gold = ring->kind == GH_RING_GOLDILOCKS;
   const uint64_t total = (uint64_t)num_parts * n;
-  uint64_t nblk = std::min<uint64_t>((uint64_t)ctx->num_cus * 2, (total + kEvPolys - 1) / kEvPolys);
+  // two waves per SIMD, whole polynomials per wave
+  const uint64_t waves = (uint64_t)ctx->num_cus * 2 * kProjWaves;
+  const uint64_t ppw = std::max<uint64_t>(1, (total + waves - 1) / waves);
+  uint64_t nblk = (total + ppw * kProjWaves - 1) / (ppw * kProjWaves);
   if (nblk == 0) nblk = 1;
-  uint64_t ppb = (total + nblk - 1) / nblk;
-  ppb = (ppb + kEvPolys - 1) / kEvPolys * kEvPolys;
-  nblk = (total + ppb - 1) / ppb;
-  if (nblk == 0) nblk = 1;
+  if (nblk > 0x7fffffffull) return GH_E_INVALID_ARGUMENT;
   void* ws;
   int st = ctx_workspace(ctx, (size_t)nblk * 256 * kProjDigits * sizeof(int32_t), &ws);
   if (st) return st;
   const ChaKey key = chacha_key(seed);
   if (!gold)
-    hipLaunchKernelGGL((k_jl_proj<uint32_t, false>), dim3((unsigned)nblk), dim3(512), 0,
-                       ctx->stream, key, (const uint32_t*)parts, n, total, ppb, ring->q,
+    hipLaunchKernelGGL((k_jl_proj<uint32_t, false>), dim3((unsigned)nblk), dim3(64 * kProjWaves),
+                       0, ctx->stream, key, (const uint32_t*)parts, n, total, ppw, ring->q,
                        (int32_t*)ws);
   else
-    hipLaunchKernelGGL((k_jl_proj<uint64_t, true>), dim3((unsigned)nblk), dim3(512), 0,
-                       ctx->stream, key, (const uint64_t*)parts, n, total, ppb, ring->q,
+    hipLaunchKernelGGL((k_jl_proj<uint64_t, true>), dim3((unsigned)nblk), dim3(64 * kProjWaves),
+                       0, ctx->stream, key, (const uint64_t*)parts, n, total, ppw, ring->q,
                        (int32_t*)ws);
   GH_HIP(hipGetLastError());
   if (!gold)
