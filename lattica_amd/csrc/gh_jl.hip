@@ -105,6 +105,64 @@ __device__ __forceinline__ void jl_tile_store(const uint32_t (&blk)[16], int8_t*
   jl_quad_store(blk, tile_poly, kEvRowBytes, k, 256);
 }
 
+// ---- matrix-core transposes of keystream columns ------------------------------
+//
+// A keystream block holds one column for all 256 rows; an MFMA A fragment
+// holds one row across 16 columns. An MFMA of the decoded keystream against a
+// constant 0/1 matrix turns one into the other in registers: its D layout puts
+// register r of lane half h on A row 8 (r / 4) + 4 h + r % 4, so the lanes draw
+// their columns in the order that makes those rows consecutive, and the int32
+// results are packed back to bytes.
+
+// 16 ternary codes of a keystream word -> 16 int8 entries, byte 4 i + o
+// holding code 4 o + i (00 -> +1, 01 -> -1, else 0)
+__device__ __forceinline__ v4i jl_decode16(uint32_t x) {
+  v4i r;
+  r[0] = (int)__builtin_amdgcn_perm(0u, 0x0000FF01u, x & 0x03030303u);
+  r[1] = (int)__builtin_amdgcn_perm(0u, 0x0000FF01u, (x >> 2) & 0x03030303u);
+  r[2] = (int)__builtin_amdgcn_perm(0u, 0x0000FF01u, (x >> 4) & 0x03030303u);
+  r[3] = (int)__builtin_amdgcn_perm(0u, 0x0000FF01u, (x >> 6) & 0x03030303u);
+  return r;
+}
+
+// Column (of 64) drawn by `lane`: lanes 0-31 the first 32, 32-63 the second,
+// lane i of a half on 16 ((i >> 2) & 1) + 4 (i >> 3) + (i & 3): a transpose then
+// leaves register r of lane half h on column 16 h + r.
+__device__ __forceinline__ uint32_t jl_tcoeff(uint32_t lane) {
+  const uint32_t i = lane & 31;
+  return 32 * (lane >> 5) + 16 * ((i >> 2) & 1) + 4 * (i >> 3) + (i & 3);
+}
+
+// B operand of the keystream transpose: a 1 where K position (jl_decode16's
+// order, m = 4 o + i at byte 4 i + o of each 16-group) holds row j = lane & 31
+__device__ __forceinline__ v4i jl_tperm(uint32_t lane) {
+  const uint32_t j = lane & 31, h = lane >> 5;
+  v4i r = {0, 0, 0, 0};
+  if ((j >> 4) == h) {
+    const uint32_t jj = j & 15, b = 4 * (jj & 3) + (jj >> 2);
+    r[b >> 2] = (int)(1u << (8 * (b & 3)));
+  }
+  return r;
+}
+
+// B operand of the digit transpose: the 32 x 32 identity
+__device__ __forceinline__ v4i jl_tident(uint32_t lane) {
+  const uint32_t j = lane & 31, h = lane >> 5;
+  v4i r = {0, 0, 0, 0};
+  if ((j >> 4) == h) r[(j & 15) >> 2] = (int)(1u << (8 * (j & 3)));
+  return r;
+}
+
+// the low bytes of 16 small int32 sums as an int8 operand, register r -> byte r
+__device__ __forceinline__ v4i jl_pack(const v16i& d) {
+  v4i r;
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    r[k] = (int)(__builtin_amdgcn_perm((uint32_t)d[4 * k + 1], (uint32_t)d[4 * k], 0x0C0C0400u) |
+                 __builtin_amdgcn_perm((uint32_t)d[4 * k + 3], (uint32_t)d[4 * k + 2], 0x04000C0Cu));
+  return r;
+}
+
 // grid (ceil(n / 256), d, parts), 256 threads; rows <= 256
 __global__ __launch_bounds__(256) void k_jl_flat(ChaKey key0, uint32_t rows, uint32_t num_parts,
                                                  uint64_t n, uint32_t d, int8_t* __restrict__ flat) {
@@ -444,17 +502,6 @@ __global__ __launch_bounds__(64 * D) void k_jl_evals_fin(const int32_t* __restri
 constexpr int kAggPolys = 128;   // polynomials per block: 4 tiles of 32
 constexpr int kAggRow = 272;     // bytes per R row: 256 m + 16 pad (bank spread)
 
-// 16 ternary codes of a keystream word -> 16 int8 entries, byte 4 i + o
-// holding code 4 o + i (00 -> +1, 01 -> -1, else 0)
-__device__ __forceinline__ v4i jl_decode16(uint32_t x) {
-  v4i r;
-  r[0] = (int)__builtin_amdgcn_perm(0u, 0x0000FF01u, x & 0x03030303u);
-  r[1] = (int)__builtin_amdgcn_perm(0u, 0x0000FF01u, (x >> 2) & 0x03030303u);
-  r[2] = (int)__builtin_amdgcn_perm(0u, 0x0000FF01u, (x >> 4) & 0x03030303u);
-  r[3] = (int)__builtin_amdgcn_perm(0u, 0x0000FF01u, (x >> 6) & 0x03030303u);
-  return r;
-}
-
 // sum_b 256^(dbase + b) acc[b] mod q for four digit sums of magnitude < 2^22
 template <bool GOLD>
 __device__ __forceinline__ uint64_t agg_combine(const int32_t (&a)[4], uint32_t dbase, uint64_t q) {
@@ -611,44 +658,6 @@ __global__ __launch_bounds__(512) void k_jl_agg(ChaKey key0, const W* __restrict
 // the same way into the B fragments.
 constexpr int kProjDigits = 8;  // digit slots per witness word (B columns used)
 constexpr int kProjWaves = 4;   // waves per block
-
-// Coefficient drawn by `lane` within a polynomial: lanes 0-31 the first 32,
-// 32-63 the second, lane i of a half on 16 ((i >> 2) & 1) + 4 (i >> 3) + (i & 3):
-// a transpose then leaves register r of lane half h on coefficient 16 h + r.
-__device__ __forceinline__ uint32_t jl_tcoeff(uint32_t lane) {
-  const uint32_t i = lane & 31;
-  return 32 * (lane >> 5) + 16 * ((i >> 2) & 1) + 4 * (i >> 3) + (i & 3);
-}
-
-// B operand of the keystream transpose: a 1 where K position (jl_decode16's
-// order, m = 4 o + i at byte 4 i + o of each 16-group) holds row j = lane & 31
-__device__ __forceinline__ v4i jl_tperm(uint32_t lane) {
-  const uint32_t j = lane & 31, h = lane >> 5;
-  v4i r = {0, 0, 0, 0};
-  if ((j >> 4) == h) {
-    const uint32_t jj = j & 15, b = 4 * (jj & 3) + (jj >> 2);
-    r[b >> 2] = (int)(1u << (8 * (b & 3)));
-  }
-  return r;
-}
-
-// B operand of the digit transpose: the 32 x 32 identity
-__device__ __forceinline__ v4i jl_tident(uint32_t lane) {
-  const uint32_t j = lane & 31, h = lane >> 5;
-  v4i r = {0, 0, 0, 0};
-  if ((j >> 4) == h) r[(j & 15) >> 2] = (int)(1u << (8 * (j & 3)));
-  return r;
-}
-
-// the low bytes of 16 small int32 sums as an int8 operand, register r -> byte r
-__device__ __forceinline__ v4i jl_pack(const v16i& d) {
-  v4i r;
-#pragma unroll
-  for (int k = 0; k < 4; k++)
-    r[k] = (int)(__builtin_amdgcn_perm((uint32_t)d[4 * k + 1], (uint32_t)d[4 * k], 0x0C0C0400u) |
-                 __builtin_amdgcn_perm((uint32_t)d[4 * k + 3], (uint32_t)d[4 * k + 2], 0x04000C0Cu));
-  return r;
-}
 
 // Grid: blocks of 4 waves, wave w of block b on polynomials
 // [(4 b + w) ppw, + ppw); per polynomial: 16 keystream transposes, 2 digit

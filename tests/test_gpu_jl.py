@@ -21,8 +21,8 @@ SEED = bytes(range(7, 39))
 
 
 @pytest.mark.parametrize("rows,num_parts,n", [(256, 2, 5), (256, 3, 300), (256, 2, 512),
-                                              (254, 1, 256), (253, 1, 7), (100, 2, 9),
-                                              (7, 3, 257), (1, 1, 1)])
+                                              (254, 1, 256), (253, 3, 1024), (256, 2, 272),
+                                              (253, 1, 7), (100, 2, 9), (7, 3, 257), (1, 1, 1)])
 def test_extract_flat_matches_oracle(oracle, ring_q32, rows, num_parts, n):
     got = ring_q32.jl_extract_flat(SEED, rows, num_parts, n * 64)
     want = oracle.jl_extract_flat(SEED, rows, num_parts, n * 64, 64)
