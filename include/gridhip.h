@@ -26,7 +26,10 @@
  *    call returns when results are in host memory) or GH_MEM_DEVICE (device
  *    pointers from gh_alloc or any HIP allocation on the context's device;
  *    work is enqueued on the context stream and the call returns without
- *    waiting — gh_ctx_synchronize() waits).
+ *    waiting — gh_ctx_synchronize() waits). Device results are complete once
+ *    gh_ctx_synchronize() returns, and visible to every later call on the
+ *    context; a device-memory gh_commit may hold its last step back until
+ *    that point, so hipDeviceSynchronize() alone does not finish it.
  *  - Return value: GH_OK (0) or a negative gh_status. Errors that the
  *    reference reports map 1:1 onto NttError (arith/ntt.rs:40-49) and
  *    CommitmentError (commit/src/error.rs:7-18).
@@ -207,7 +210,11 @@ GH_API int gh_key_dims(const gh_key *key, uint32_t *rows, uint32_t *msg_len, uin
  * message: msg_len polys, randomness: open_len polys (may be NULL when the
  * key has open_len 0), both in `domain` (GH_DOMAIN_NTT is the
  * commit_with_opening_ntt path, commit/src/ajtai/mod.rs:256-269).
- * DIMENSION_MISMATCH when lengths differ from the key (linear.rs:270-327). */
+ * DIMENSION_MISMATCH when lengths differ from the key (linear.rs:270-327).
+ * With GH_MEM_DEVICE and open_len 0 the finalize (sum of the MAC partials and
+ * the inverse NTT) is held back and runs inside the next such gh_commit's
+ * launch, or before any other call on the context: back-to-back commitments
+ * cost one launch each. */
 GH_API int gh_commit(gh_key *key, const void *message, uint32_t msg_len, const void *randomness,
                      uint32_t open_len, int domain, void *t_out, int mem);
 

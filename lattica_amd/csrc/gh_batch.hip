@@ -520,6 +520,7 @@ GH_API int gh_batch_plan_create(gh_key* key, const gh_commit_item* items, size_t
   gh_ctx* ctx = key->ring->ctx;
   std::lock_guard<std::mutex> lk(ctx->mu);
   GH_HIP(hipSetDevice(ctx->device));
+  if (int fst = ctx_flush(ctx)) return fst;
   gh_batch_plan* p = new (std::nothrow) gh_batch_plan();
   if (!p) return GH_E_OOM;
   p->key = key;
@@ -563,6 +564,7 @@ GH_API int gh_batch_plan_launch(gh_batch_plan* p) {
   gh_ctx* ctx = p->key->ring->ctx;
   std::lock_guard<std::mutex> lk(ctx->mu);
   GH_HIP(hipSetDevice(ctx->device));
+  if (int fst = ctx_flush(ctx)) return fst;
   GH_HIP(hipGraphLaunch(p->exec, ctx->stream));
   return GH_OK;
 }
@@ -573,6 +575,7 @@ GH_API int gh_batch_plan_destroy(gh_batch_plan* p) {
   {
     std::lock_guard<std::mutex> lk(ctx->mu);
     hipSetDevice(ctx->device);
+    ctx_flush(ctx);
     hipStreamSynchronize(ctx->stream);
   }
   plan_free(p);

@@ -32,7 +32,11 @@ static int grow(gh_ctx* ctx, void** buf, size_t* have, size_t need) {
 }
 
 int ctx_workspace(gh_ctx* ctx, size_t bytes, void** out) {
-  int st = grow(ctx, &ctx->work, &ctx->work_bytes, bytes);
+  // a pending finalize reads its partial sums from the workspace: issue it
+  // before the caller reuses or regrows the buffer
+  int st = ctx_flush(ctx);
+  if (st) return st;
+  st = grow(ctx, &ctx->work, &ctx->work_bytes, bytes);
   if (st) return st;
   *out = ctx->work;
   return GH_OK;
@@ -95,9 +99,12 @@ namespace {
 
 constexpr int kVersion = 1;
 
+// Every call but a device-memory gh_commit starts here: a finalize held back
+// by the previous commitment is issued first, so whatever the call reads or
+// frees sees the finished commitment.
 int set_device(gh_ctx* ctx) {
   GH_HIP(hipSetDevice(ctx->device));
-  return GH_OK;
+  return ctx_flush(ctx);
 }
 
 int check_mem(int mem) { return (mem == GH_MEM_HOST || mem == GH_MEM_DEVICE) ? GH_OK : GH_E_INVALID_ARGUMENT; }
@@ -201,6 +208,10 @@ GH_API int gh_ctx_create(int device, gh_ctx** out) {
 GH_API int gh_ctx_destroy(gh_ctx* ctx) {
   if (!ctx) return GH_E_INVALID_ARGUMENT;
   hipSetDevice(ctx->device);
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ctx->stream) ctx_flush(ctx);
+  }
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   if (ctx->work) hipFree(ctx->work);
   if (ctx->stage) hipFree(ctx->stage);
@@ -213,6 +224,7 @@ GH_API int gh_ctx_destroy(gh_ctx* ctx) {
 
 GH_API int gh_ctx_synchronize(gh_ctx* ctx) {
   if (!ctx) return GH_E_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(ctx->mu);
   TRY(set_device(ctx));
   return sync(ctx);
 }
@@ -411,7 +423,10 @@ GH_API int gh_ring_create(gh_ctx* ctx, uint64_t q, uint32_t d, gh_ring** out) {
 
 GH_API int gh_ring_destroy(gh_ring* ring) {
   if (!ring) return GH_E_INVALID_ARGUMENT;
-  hipSetDevice(ring->ctx->device);
+  {
+    std::lock_guard<std::mutex> lk(ring->ctx->mu);
+    TRY(set_device(ring->ctx));  // a pending finalize may read this ring's table
+  }
   hipStreamSynchronize(ring->ctx->stream);
   if (ring->dev_tab) hipFree(ring->dev_tab);
   delete ring;
@@ -575,7 +590,10 @@ GH_API int gh_key_create(gh_ring* ring, uint32_t rows, const void* a_msg, uint32
 
 GH_API int gh_key_destroy(gh_key* key) {
   if (!key) return GH_E_INVALID_ARGUMENT;
-  hipSetDevice(key->ring->ctx->device);
+  {
+    std::lock_guard<std::mutex> lk(key->ring->ctx->mu);
+    TRY(set_device(key->ring->ctx));
+  }
   hipStreamSynchronize(key->ring->ctx->stream);
   if (key->dev_a) hipFree(key->dev_a);
   delete key;
@@ -600,12 +618,17 @@ GH_API int gh_commit(gh_key* key, const void* message, uint32_t msg_len, const v
   gh_ring* ring = key->ring;
   gh_ctx* ctx = ring->ctx;
   std::lock_guard<std::mutex> lk(ctx->mu);
-  TRY(set_device(ctx));
+  GH_HIP(hipSetDevice(ctx->device));
   const size_t wb = ring->word_bytes, d = ring->d;
   const uint32_t cols = msg_len + open_len;
   const size_t msg_bytes = (size_t)msg_len * d * wb, open_bytes = (size_t)open_len * d * wb;
   const size_t t_bytes = (size_t)key->rows * d * wb;
-  if (mem == GH_MEM_DEVICE && open_len == 0) return launch_commit(key, message, cols, domain, t_out);
+  // device memory, no opening: asynchronous, and the finalize may be held
+  // back until the next call on the context (the next such commitment runs it
+  // inside its own launch)
+  if (mem == GH_MEM_DEVICE && open_len == 0)
+    return launch_commit(key, message, cols, domain, t_out, true);
+  TRY(ctx_flush(ctx));
   void* dev;
   TRY(ctx_staging(ctx, msg_bytes + open_bytes + t_bytes, &dev));
   void* ds = dev;

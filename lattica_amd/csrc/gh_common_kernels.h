@@ -40,36 +40,46 @@ __device__ __forceinline__ uint64_t gold_reduce(const FGold& f, const GoldAcc& s
 }
 
 // t[row] = INTT(sum_p partial[p][row]) with the untwist table `post`, by one
-// block of 1024 threads.
-template <class F>
-__device__ __forceinline__ void finalize_row(F f, const typename F::W* __restrict__ tab,
-                                             const typename F::W* __restrict__ partial,
-                                             uint32_t nparts, uint32_t rows, uint32_t row, int post,
-                                             typename F::W* __restrict__ t_out) {
+// block of G waves; `red` holds G * 64 words of LDS.
+template <class F, int G>
+__device__ __forceinline__ void finalize_row_g(F f, const typename F::W* __restrict__ tab,
+                                               const typename F::W* __restrict__ partial,
+                                               uint32_t nparts, uint32_t rows, uint32_t row,
+                                               int post, typename F::W* __restrict__ t_out,
+                                               typename F::W* red) {
   using W = typename F::W;
   const int k = threadIdx.x & 63, g = threadIdx.x >> 6;
-  // up to 32 independent loads in flight per thread: one round trip for the
-  // partial counts a device-wide MAC produces (<= 512)
+  // up to 32 independent loads in flight per thread: one round trip (16
+  // waves) or a few (4 waves) for the partial counts a device-wide MAC
+  // produces (<= 512)
   W acc = 0;
-  for (uint32_t p0 = g; p0 < nparts; p0 += 16 * 32) {
+  for (uint32_t p0 = g; p0 < nparts; p0 += G * 32) {
     W v[32];
 #pragma unroll
     for (int u = 0; u < 32; u++) {
-      const uint32_t p = p0 + 16 * u;
+      const uint32_t p = p0 + G * u;
       v[u] = p < nparts ? partial[((size_t)p * rows + row) * 64 + k] : W(0);
     }
 #pragma unroll
     for (int u = 0; u < 32; u++) acc = f.add(acc, v[u]);
   }
-  __shared__ W red[16][64];
-  red[g][k] = acc;
+  red[g * 64 + k] = acc;
   __syncthreads();
   if (g == 0) {
-    W s = red[0][k];
+    W s = red[k];
 #pragma unroll
-    for (int i = 1; i < 16; i++) s = f.add(s, red[i][k]);
+    for (int i = 1; i < G; i++) s = f.add(s, red[i * 64 + k]);
     t_out[(size_t)row * 64 + k] = wave_inv64(f, s, tab, post, k);
   }
+}
+
+template <class F>
+__device__ __forceinline__ void finalize_row(F f, const typename F::W* __restrict__ tab,
+                                             const typename F::W* __restrict__ partial,
+                                             uint32_t nparts, uint32_t rows, uint32_t row, int post,
+                                             typename F::W* __restrict__ t_out) {
+  __shared__ typename F::W red[16 * 64];
+  finalize_row_g<F, 16>(f, tab, partial, nparts, rows, row, post, t_out, red);
 }
 
 template <class F>

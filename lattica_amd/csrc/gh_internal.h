@@ -84,6 +84,19 @@ inline uint32_t tuned_batch_chunk(const gh_tuning& t, int ring_kind) {
   return (uint32_t)(ring_kind == GH_RING_Q32 ? t.batch_chunk_cols : t.batch_chunk_cols_gold);
 }
 
+// The finalize of a device-memory q32 commitment, held back so that it runs
+// as extra blocks of the next such commitment's MAC launch instead of as a
+// launch of its own. Its partial sums stay at work + off (bytes long) until it
+// has run; every other call on the context issues it first (ctx_flush).
+struct PendingFin {
+  bool active = false;
+  uint32_t q = 0, qinv = 0;
+  const void* tab = nullptr;
+  size_t off = 0, bytes = 0;
+  uint32_t nparts = 0, rows = 0;
+  void* t_out = nullptr;
+};
+
 struct gh_ctx {
   int device = 0;
   gh_tuning tune;
@@ -105,6 +118,7 @@ struct gh_ctx {
   size_t prof_used = 0;
   struct ProfRec { int phase; hipEvent_t a, b; };
   std::vector<ProfRec> prof_recs;
+  PendingFin fin;
   std::mutex mu;
 };
 
@@ -152,7 +166,12 @@ int launch_negacyclic_mul(gh_ring* ring, const void* a, const void* b, void* out
 int launch_pointwise_mul(gh_ring* ring, const void* a, const void* b, void* out, size_t count);
 int launch_key_layout(gh_ring* ring, const void* src_rowmajor, uint32_t rows, uint32_t cols,
                       uint32_t col_offset, uint32_t total_cols, void* dst, int domain);
-int launch_commit(gh_key* key, const void* s_dev, uint32_t cols, int domain, void* t_dev);
+// defer: a q32 commitment on the fused path leaves its finalize pending (see
+// PendingFin); other paths flush a pending one and finalize in place
+int launch_commit(gh_key* key, const void* s_dev, uint32_t cols, int domain, void* t_dev,
+                  bool defer = false);
+// issues a pending finalize on the context stream (context lock held)
+int ctx_flush(gh_ctx* ctx);
 // items hold device pointers; every len in [1, key columns]
 int launch_commit_batch(gh_key* key, const gh_commit_item* items, size_t count);
 

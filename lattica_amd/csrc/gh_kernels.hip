@@ -389,20 +389,39 @@ __device__ __forceinline__ void mac_column(uint64_t (&lo)[NI][4], uint32_t (&hi)
   }
 }
 
+// The previous commitment's finalize, run by the first `nblk` blocks (one
+// per row) of a fused launch; nblk = 0: none.
+struct FinArgs {
+  FQ32 f;
+  const uint32_t* tab;
+  const uint32_t* partial;
+  uint32_t* out;
+  uint32_t nparts, rows, nblk;
+};
+
 template <int RB, int WAVES, int DEPTH, int NW>
 __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_q32(
     FQ32 f, const uint32_t* __restrict__ tab, const uint32_t* __restrict__ A,
     const uint32_t* __restrict__ s_coeff, uint32_t* __restrict__ partial, uint32_t rows,
-    uint32_t cols, uint32_t cols_per_block) {
+    uint32_t cols, uint32_t cols_per_block, FinArgs fin) {
   constexpr int NI = RB / 4;
   constexpr int PF = DEPTH * 8 / WAVES > 0 ? DEPTH * 8 / WAVES : 1;
   extern __shared__ uint32_t lds[];  // max(cols_per_block * kFuseStride, WAVES * RB * 64) words
+  if (blockIdx.x < fin.nblk) {
+    // row blockIdx.x of the previous commitment: its partial sums were
+    // complete when this launch began
+    if (blockIdx.y == 0)
+      finalize_row_g<FQ32, WAVES>(fin.f, fin.tab, fin.partial, fin.nparts, fin.rows, blockIdx.x,
+                                  (int)kTabUntwistMac, fin.out, lds);
+    return;
+  }
+  const uint32_t bx = blockIdx.x - fin.nblk;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int rsub = lane >> 4, k0 = 4 * (lane & 15);
   const int t = threadIdx.x & 3;
   const uint32_t row0 = blockIdx.y * RB;
-  const uint32_t ch = blockIdx.x * cols_per_block;  // < cols: every block owns a column
+  const uint32_t ch = bx * cols_per_block;  // < cols: every block owns a column
   const uint32_t ncol = ch + cols_per_block < cols ? cols_per_block : cols - ch;
   const size_t col_stride = (size_t)rows * 64;
   const u32x4* akey = reinterpret_cast<const u32x4*>(A + (size_t)row0 * 64) + lane;
@@ -518,7 +537,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_q32(
     uint32_t acc = red[o];
 #pragma unroll
     for (int w = 1; w < WAVES; w++) acc = f.add(acc, red[w * RB * 64 + o]);
-    partial[((size_t)blockIdx.x * rows + row0) * 64 + o] = acc;
+    partial[((size_t)bx * rows + row0) * 64 + o] = acc;
   }
 }
 
@@ -786,13 +805,14 @@ template <int RB, int WAVES, int DEPTH>
 static void fused_launch_nw(gh_ctx* ctx, dim3 grid, size_t lds_bytes, hipEvent_t e0,
                             hipEvent_t e1, FQ32 f, const uint32_t* tab, const uint32_t* A,
                             const uint32_t* S, uint32_t* P, uint32_t rows, uint32_t cols,
-                            uint32_t cpb) {
+                            uint32_t cpb, const FinArgs& fin) {
   // the unrolled sweep when every full block has the same columns per wave
   // (instantiated for 16-row blocks, kappa a multiple of 16)
   const uint32_t nw = RB == 16 && cpb % WAVES == 0 ? cpb / WAVES : 0;
 #define GH_FUSED_NW(N)                                                                          \
   hipExtLaunchKernelGGL((k_commit_fused_q32<RB, WAVES, DEPTH, N>), grid, dim3(64 * WAVES),      \
-                        lds_bytes, ctx->stream, e0, e1, 0, f, tab, A, S, P, rows, cols, cpb)
+                        lds_bytes, ctx->stream, e0, e1, 0, f, tab, A, S, P, rows, cols, cpb,    \
+                        fin)
   if constexpr (RB == 16) {
     if (nw == 8) GH_FUSED_NW(8);
     else if (nw == 16) GH_FUSED_NW(16);
@@ -807,19 +827,36 @@ static void fused_launch_nw(gh_ctx* ctx, dim3 grid, size_t lds_bytes, hipEvent_t
 template <int RB, int WAVES>
 static void fused_launch(gh_ctx* ctx, dim3 grid, size_t lds_bytes, hipEvent_t e0, hipEvent_t e1,
                          FQ32 f, const uint32_t* tab, const uint32_t* A, const uint32_t* S,
-                         uint32_t* P, uint32_t rows, uint32_t cols, uint32_t cpb) {
+                         uint32_t* P, uint32_t rows, uint32_t cols, uint32_t cpb,
+                         const FinArgs& fin) {
   switch (ctx->tune.fused_depth) {
-    case 1: fused_launch_nw<RB, WAVES, 1>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb); break;
-    case 3: fused_launch_nw<RB, WAVES, 3>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb); break;
-    case 4: fused_launch_nw<RB, WAVES, 4>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb); break;
-    default: fused_launch_nw<RB, WAVES, 2>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb); break;
+    case 1: fused_launch_nw<RB, WAVES, 1>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, fin); break;
+    case 3: fused_launch_nw<RB, WAVES, 3>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, fin); break;
+    case 4: fused_launch_nw<RB, WAVES, 4>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, fin); break;
+    default: fused_launch_nw<RB, WAVES, 2>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, fin); break;
   }
 }
 
-// q32 commitment in two launches on the context stream: the fused
-// transform+MAC kernel, then the finalize. (A second stream for the finalize
-// was measured: the cross-queue event hand-off costs more than it overlaps.)
-static int launch_commit_fused(gh_key* key, const void* s_dev, uint32_t cols, void* t_dev) {
+// The standalone finalize of a q32 commitment: 16 waves per row.
+static int launch_finalize_q32(gh_ctx* ctx, FQ32 f, const uint32_t* tab, const uint32_t* P,
+                               uint32_t nparts, uint32_t rows, uint32_t* t_dev) {
+  hipEvent_t f0, f1;
+  prof_pair(ctx, 2, &f0, &f1);
+  hipExtLaunchKernelGGL((k_finalize<FQ32>), dim3(rows), dim3(1024), 0, ctx->stream, f0, f1, 0, f,
+                        tab, P, nparts, rows, (int)kTabUntwistMac, t_dev);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+// q32 commitment: the fused transform+MAC kernel, then the finalize. With
+// `defer` the finalize is left pending and runs as the first blocks of the
+// next deferred commitment's MAC launch, so back-to-back commitments cost one
+// launch each: the next MAC's partial sums go beside the pending ones, and the
+// finalize blocks, dispatched first, read them while the MAC blocks stream
+// the key. (A second stream for the finalize was measured: the cross-queue
+// event hand-off costs more than it overlaps.)
+static int launch_commit_fused(gh_key* key, const void* s_dev, uint32_t cols, void* t_dev,
+                               bool defer) {
   gh_ring* ring = key->ring;
   gh_ctx* ctx = ring->ctx;
   const uint32_t rows = key->rows;
@@ -836,43 +873,71 @@ static int launch_commit_fused(gh_key* key, const void* s_dev, uint32_t cols, vo
   if (nparts == 0) nparts = 1;
   const uint32_t cpb = (cols + nparts - 1) / nparts;
   nparts = (cols + cpb - 1) / cpb;
-  void* ws;
-  int st = ctx_workspace(ctx, (size_t)nparts * rows * 64 * 4, &ws);
-  if (st) return st;
+  const size_t pbytes = ((size_t)nparts * rows * 64 * 4 + 255) & ~(size_t)255;
+
+  // this commitment's partial sums: beside a pending finalize's, in the
+  // workspace as it is (growing it would free them), else from offset 0
+  PendingFin& pf = ctx->fin;
+  size_t off = 0;
+  if (pf.active) {
+    if (defer && pbytes <= pf.off) off = 0;
+    else if (defer && pf.off + pf.bytes + pbytes <= ctx->work_bytes) off = pf.off + pf.bytes;
+    else if (int st = ctx_flush(ctx)) return st;
+  }
+  FinArgs fin{};
+  if (pf.active) {
+    fin = FinArgs{FQ32{pf.q, pf.qinv}, (const uint32_t*)pf.tab,
+                  (const uint32_t*)((const char*)ctx->work + pf.off), (uint32_t*)pf.t_out,
+                  pf.nparts, pf.rows, pf.rows};
+  } else {
+    void* ws;
+    int st = ctx_workspace(ctx, (defer ? 2 : 1) * pbytes, &ws);  // room for the next one too
+    if (st) return st;
+  }
 
   FQ32 f{(uint32_t)ring->q, ring->qinv32};
   const uint32_t* tab = (const uint32_t*)ring->dev_tab;
   const uint32_t* A = (const uint32_t*)key->dev_a;
   const uint32_t* S = (const uint32_t*)s_dev;
-  uint32_t* P = (uint32_t*)ws;
-  dim3 grid(nparts, rows / rb);
+  uint32_t* P = (uint32_t*)((char*)ctx->work + off);
+  dim3 grid(fin.nblk + nparts, rows / rb);
   size_t lds_words = (size_t)cpb * kFuseStride;
   if ((size_t)waves * rb * 64 > lds_words) lds_words = (size_t)waves * rb * 64;
   const size_t lds_bytes = lds_words * 4;
   hipEvent_t e0, e1;
   prof_pair(ctx, 1, &e0, &e1);
-#define GH_FUSED(R, W) fused_launch<R, W>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb)
+#define GH_FUSED(R, W) fused_launch<R, W>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, fin)
   if (rb == 16) { if (waves == 4) GH_FUSED(16, 4); else if (waves == 8) GH_FUSED(16, 8); else GH_FUSED(16, 16); }
   else if (rb == 8) { if (waves == 4) GH_FUSED(8, 4); else if (waves == 8) GH_FUSED(8, 8); else GH_FUSED(8, 16); }
   else { if (waves == 4) GH_FUSED(4, 4); else if (waves == 8) GH_FUSED(4, 8); else GH_FUSED(4, 16); }
 #undef GH_FUSED
   GH_HIP(hipGetLastError());
-  hipEvent_t f0, f1;
-  prof_pair(ctx, 2, &f0, &f1);
-  hipExtLaunchKernelGGL((k_finalize<FQ32>), dim3(rows), dim3(1024), 0, ctx->stream, f0, f1, 0, f,
-                        tab, (const uint32_t*)P, nparts, rows, (int)kTabUntwistMac,
-                        (uint32_t*)t_dev);
-  GH_HIP(hipGetLastError());
+  pf.active = false;  // a pending finalize ran inside this launch
+  if (!defer) return launch_finalize_q32(ctx, f, tab, P, nparts, rows, (uint32_t*)t_dev);
+  pf = PendingFin{true, (uint32_t)ring->q, ring->qinv32, ring->dev_tab, off, pbytes, nparts, rows,
+                  t_dev};
   return GH_OK;
 }
 
+int ctx_flush(gh_ctx* ctx) {
+  PendingFin& pf = ctx->fin;
+  if (!pf.active) return GH_OK;
+  pf.active = false;
+  return launch_finalize_q32(ctx, FQ32{pf.q, pf.qinv}, (const uint32_t*)pf.tab,
+                             (const uint32_t*)((const char*)ctx->work + pf.off), pf.nparts,
+                             pf.rows, (uint32_t*)pf.t_out);
+}
+
 // t = A[:, :cols] * s; s (cols polys) and t (rows polys) are device pointers.
-int launch_commit(gh_key* key, const void* s_dev, uint32_t cols, int domain, void* t_dev) {
+int launch_commit(gh_key* key, const void* s_dev, uint32_t cols, int domain, void* t_dev,
+                  bool defer) {
   gh_ring* ring = key->ring;
   gh_ctx* ctx = ring->ctx;
   if (ring->kind == GH_RING_Q32 && ctx->tune.fused && key->rows % 4 == 0 &&
       domain == GH_DOMAIN_COEFF)
-    return launch_commit_fused(key, s_dev, cols, t_dev);
+    return launch_commit_fused(key, s_dev, cols, t_dev, defer);
+  int fst = ctx_flush(ctx);  // this path finalizes in its own launch
+  if (fst) return fst;
   const size_t wb = ring->word_bytes;
   const uint32_t rows = key->rows;
   // block shape: the tuned wave count for the 16-byte q32 kernel, else 16
