@@ -262,7 +262,8 @@ def labrador_calls(ctx, la, ring, log, reps=10):
     polynomials gh_jl_conjugate, the flat int8 entries gh_jl_extract_flat) and
     the step around the commitment (h garbage, amortization z = sum c_i s_i,
     balanced decomposition of the 512 t polynomials), each on resident device
-    buffers. The oracle checks one Gram entry; the JL p mod q must equal the
+    buffers (the JL evaluations also on uniform residues, which need every
+    digit plane where the witness needs two). The oracle checks one Gram entry; the JL p mod q must equal the
     exact sum mod q and the constant terms of the JL evaluations."""
     rng = np.random.default_rng(33)
     r_, n_, rows = 32, 1024, 256
@@ -285,6 +286,8 @@ def labrador_calls(ctx, la, ring, log, reps=10):
     scale = rng.integers(0, Q, size=(rows, D), dtype=np.uint32)  # aggregation scale polys
     sc_dev = ctx.to_device(scale)
     phi_dev = ctx.to_device(np.zeros((r_, n_, D), dtype=np.uint32))
+    su_dev = ctx.to_device(rng.integers(0, Q, size=(r_, n_, D), dtype=np.uint32))
+    evu_dev = ctx.alloc(rows * D * 4)
     seed = np.frombuffer(bytes(range(32)), dtype=np.uint8).copy()
     L = la.lib()
     import ctypes
@@ -300,6 +303,10 @@ def labrador_calls(ctx, la, ring, log, reps=10):
         "jl_evals_256x32x1024": lambda: L.gh_jl_evals(ring.h, ctypes.c_void_p(seed.ctypes.data),
                                                       rows, s_dev.ptr, r_, n_ * D, ev_dev.ptr,
                                                       la.MEM_DEVICE),
+        # the same on residues drawn uniformly mod q: every digit plane is live
+        "jl_evals_256x32x1024_uniform": lambda: L.gh_jl_evals(
+            ring.h, ctypes.c_void_p(seed.ctypes.data), rows, su_dev.ptr, r_, n_ * D, evu_dev.ptr,
+            la.MEM_DEVICE),
         "jl_aggregate_256x32x1024": lambda: L.gh_jl_aggregate(
             ring.h, ctypes.c_void_p(seed.ctypes.data), rows, sc_dev.ptr, r_, n_ * D, phi_dev.ptr,
             la.MEM_DEVICE),
@@ -405,7 +412,8 @@ def labrador_calls(ctx, la, ring, log, reps=10):
         f"inner commits {cpu1['coefficient_domain']['inner_commit_32x1024'] / 1e3:,.1f} ms, "
         f"Gram {cpu1['coefficient_domain']['gram_32x1024'] / 1e3:,.1f} ms")
     log(f"  JL evals {res['jl_evals_256x32x1024']['us_per_call']:,.0f} us (ct == p "
-        f"{res['jl_evals_constant_terms_equal_p']}), JL aggregation "
+        f"{res['jl_evals_constant_terms_equal_p']}; uniform residues "
+        f"{res['jl_evals_256x32x1024_uniform']['us_per_call']:,.0f} us), JL aggregation "
         f"{res['jl_aggregate_256x32x1024']['us_per_call']:,.0f} us (pairs with evals "
         f"{res['jl_aggregate_pairs_with_evals']}), conjugated polys "
         f"{res['jl_conjugate_256x32x1024']['us_per_call']:,.0f} us, flat "
@@ -415,7 +423,7 @@ def labrador_calls(ctx, la, ring, log, reps=10):
         f"{res['decompose_t_512_base2^8_4limbs']['us_per_call']:,.0f} us")
     key.close()
     for b in (s_dev, t_dev, g_dev, pm_dev, pe_dev, ev_dev, conj_dev, flat_dev, h_dev, c_dev,
-              z_dev, dec_dev, sc_dev, phi_dev):
+              z_dev, dec_dev, sc_dev, phi_dev, su_dev, evu_dev):
         b.free()
     return res
 

@@ -100,9 +100,30 @@ __device__ __forceinline__ void jl_quad_store(const uint32_t (&blk)[16], int8_t*
   }
 }
 
+// The evaluation A tile ([m][64] int8 per polynomial) keeps 16-byte chunk c of
+// row m at chunk c ^ jl_tile_swz(m): the matrix-core fragment reads (rows
+// m .. m + 31 of one chunk) and these stores then hit distinct LDS banks.
+__device__ __forceinline__ uint32_t jl_tile_swz(uint32_t m) { return (m ^ (m >> 2)) & 3; }
+
+// jl_quad_store's column transpose into the swizzled evaluation tile.
 __device__ __forceinline__ void jl_tile_store(const uint32_t (&blk)[16], int8_t* tile_poly,
                                               uint32_t k) {
-  jl_quad_store(blk, tile_poly, kEvRowBytes, k, 256);
+  const uint32_t r = k & 3, a4 = k & ~3u;
+  const uint32_t sel1 = (r & 1) ? 0x03070105u : 0x06020400u;
+  const uint32_t sel2 = (r & 2) ? 0x03020706u : 0x05040100u;
+#pragma unroll
+  for (int g = 0; g < 64; g++) {
+    const uint32_t x = (blk[g >> 2] >> (8 * (g & 3))) & 0xffu;
+    const uint32_t y = (x | (x << 6) | (x << 12) | (x << 18)) & 0x03030303u;
+    const uint32_t e = __builtin_amdgcn_perm(0u, 0x0000FF01u, y);
+    const uint32_t pn = (uint32_t)__builtin_amdgcn_mov_dpp((int)e, 0xB1, 0xF, 0xF, false);
+    const uint32_t z = __builtin_amdgcn_perm(pn, e, sel1);
+    const uint32_t qn = (uint32_t)__builtin_amdgcn_mov_dpp((int)z, 0x4E, 0xF, 0xF, false);
+    const uint32_t w = __builtin_amdgcn_perm(qn, z, sel2);
+    const uint32_t m = 4 * g + r;  // jl_tile_swz(m) = (r ^ g) & 3
+    const uint32_t c = (a4 >> 4) ^ ((r ^ (uint32_t)g) & 3);
+    *reinterpret_cast<uint32_t*>(tile_poly + m * kEvRowBytes + c * 16 + (a4 & 15)) = w;
+  }
 }
 
 // ---- matrix-core transposes of keystream columns ------------------------------
@@ -301,29 +322,64 @@ __device__ __forceinline__ void lift_digits(uint64_t x, uint64_t q, int8_t (&dig
   signed_digits<D>(v, dig);
 }
 
-// Block: 8 waves for digits dbase .. dbase + 3 of DT; wave w owns digit
-// dbase + w / 2 (both 32-wide t halves) and m-tiles 4 (w % 2) .. + 3 (32 rows
-// each): 8 accumulator tiles of 32 x 32. Each round: every thread generates
-// one of the 512 keystream columns of 8 polynomials into the A tile and one
-// coefficient's digit windows E ([poly][digit][128] bytes: digits of s_u, then
-// of -s_u); then every wave runs its MFMAs over the round's 8 x 64 columns.
-// Partial sums leave per block.
+// Signed base-256 digits the lifted word v needs: D digits cover
+// [-128 (256^D - 1) / 255, 127 (256^D - 1) / 255], nested ranges, so the count
+// is one plus the number of ranges below DT digits that v falls outside.
+template <int DT>
+__device__ __forceinline__ uint32_t digits_needed(int64_t v) {
+  uint32_t d = 1;
+  int64_t lo = -128, hi = 127;
+#pragma unroll
+  for (int i = 1; i < DT; i++) {
+    d += (v < lo || v > hi) ? 1u : 0u;
+    if (i + 1 < DT) {
+      lo = lo * 256 - 128;
+      hi = hi * 256 + 127;
+    }
+  }
+  return d;
+}
+
+// Digits the lifted word x (or its negation: the evaluations multiply by the
+// digits of s and of -s) needs.
 template <class W, int DT>
-__global__ __launch_bounds__(512) void k_jl_evals(ChaKey key0, const W* __restrict__ parts,
-                                                  uint64_t n, uint64_t total_polys,
-                                                  uint64_t polys_per_block, uint64_t q,
-                                                  uint32_t dbase, int32_t* __restrict__ partial) {
-  constexpr int D = 4;  // digits per pass
+__device__ __forceinline__ uint32_t word_span(uint64_t x, uint64_t q) {
+  const uint64_t xn = x == 0 ? 0 : q - x;
+  int64_t v, vn;
+  if (DT == 4) {
+    v = x <= 2139062143ull ? (int64_t)x : (int64_t)x - (int64_t)q;
+    vn = xn <= 2139062143ull ? (int64_t)xn : (int64_t)xn - (int64_t)q;
+  } else {
+    v = x <= 0x7F7F7F7F7F7F7F7Full ? (int64_t)x : (int64_t)(x - q);
+    vn = xn <= 0x7F7F7F7F7F7F7F7Full ? (int64_t)xn : (int64_t)(xn - q);
+  }
+  return max(digits_needed<DT>(v), digits_needed<DT>(vn));
+}
+
+// One pass over ND digits dbase .. dbase + ND - 1 of DT (ND = 1, 2 or 4):
+// block of 8 waves, wave w on digit dbase + w / (8 / ND) (both 32-wide t
+// halves) and m-tiles ND (w % (8 / ND)) .. + ND - 1 (32 rows each): 2 ND
+// accumulator tiles of 32 x 32, every wave busy whatever ND is. Each round:
+// every thread generates one of the 512 keystream columns of 8 polynomials
+// into the A tile and one coefficient's digit windows E ([poly][digit][128]
+// bytes: digits of s_u, then of -s_u); then every wave runs its MFMAs over the
+// round's 8 x 64 columns. Partial sums leave per block.
+constexpr int kEvDigits = 4;  // digit rows of E per polynomial (one pass)
+
+template <class W, int DT, int ND>
+__device__ __forceinline__ void evals_pass(ChaKey key0, const W* __restrict__ parts, uint64_t n,
+                                           uint64_t total_polys, uint64_t polys_per_block,
+                                           uint64_t q, uint32_t dbase, int32_t* __restrict__ partial,
+                                           int8_t* a_tile, int8_t* e_tile, uint32_t (&blk)[16]) {
   constexpr int kN = 64 * DT;
-  __shared__ __attribute__((aligned(16))) int8_t a_tile[kEvPolys * 256 * kEvRowBytes];
-  __shared__ __attribute__((aligned(16))) int8_t e_tile[kEvPolys * D * 128];
+  constexpr int kWavesPerDigit = 8 / ND;
   const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const uint32_t digit = wave >> 1, mgrp = wave & 1;
+  const uint32_t digit = wave / kWavesPerDigit, mgrp = wave % kWavesPerDigit;
   const uint64_t g_begin = (uint64_t)blockIdx.x * polys_per_block;
   const uint64_t g_end = std::min<uint64_t>(g_begin + polys_per_block, total_polys);
-  v16i acc[4][2];
+  v16i acc[ND][2];
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+  for (int i = 0; i < ND; i++)
 #pragma unroll
     for (int h = 0; h < 2; h++) acc[i][h] = v16i{};
 
@@ -345,7 +401,8 @@ __global__ __launch_bounds__(512) void k_jl_evals(ChaKey key0, const W* __restri
     for (int ps = 0; ps < kEvPolys; ps++) {
       if (g0 + ps >= g_end) break;
       const int8_t* at = a_tile + ps * 256 * kEvRowBytes;
-      const uint32_t* ew = reinterpret_cast<const uint32_t*>(e_tile + (ps * D + digit) * 128);
+      const uint32_t* ew =
+          reinterpret_cast<const uint32_t*>(e_tile + (ps * kEvDigits + digit) * 128);
 #pragma unroll
       for (int kstep = 0; kstep < 2; kstep++) {
         const uint32_t k0 = kstep * 32 + kg * 16;
@@ -362,9 +419,10 @@ __global__ __launch_bounds__(512) void k_jl_evals(ChaKey key0, const W* __restri
             bfrag[h][x] = (int)__builtin_amdgcn_alignbyte(w5[x + 1], w5[x], sh);
         }
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-          const uint32_t m = (mgrp * 4 + i) * 32 + row;
-          const v4i afrag = *reinterpret_cast<const v4i*>(at + m * kEvRowBytes + k0);
+        for (int i = 0; i < ND; i++) {
+          const uint32_t m = (mgrp * ND + i) * 32 + row;
+          const uint32_t c = (k0 >> 4) ^ jl_tile_swz(row);  // row = m mod 32
+          const v4i afrag = *reinterpret_cast<const v4i*>(at + m * kEvRowBytes + c * 16);
 #pragma unroll
           for (int h = 0; h < 2; h++)
             acc[i][h] = __builtin_amdgcn_mfma_i32_32x32x32_i8(afrag, bfrag[h], acc[i][h], 0, 0, 0);
@@ -376,11 +434,8 @@ __global__ __launch_bounds__(512) void k_jl_evals(ChaKey key0, const W* __restri
   // Each round: decode the keystream held in registers into the A tile and
   // the round's digit windows, then run the matrix cores on it while drawing
   // the next round's keystream (waves 0-3 draw first, waves 4-7 multiply
-  // first, so the two kinds of work can share a SIMD). The multiply phase
-  // reads about 0.8 KB of operands from LDS per MFMA, close to the LDS
-  // bandwidth, which is what bounds it.
-  uint32_t blk[16];
-  keystream(g_begin, blk);
+  // first, so the two kinds of work can share a SIMD). The tile transpose and
+  // the digit windows run before the barrier, on their own.
   for (uint64_t g0 = g_begin; g0 < g_end; g0 += kEvPolys) {
     jl_tile_store(blk, a_tile + (t >> 6) * 256 * kEvRowBytes, t & 63);
     {
@@ -395,9 +450,9 @@ __global__ __launch_bounds__(512) void k_jl_evals(ChaKey key0, const W* __restri
 #pragma unroll
         for (int b = 0; b < DT; b++) dp[b] = dn[b] = 0;
       }
-      int8_t* e = e_tile + ps * D * 128;
+      int8_t* e = e_tile + ps * kEvDigits * 128;
 #pragma unroll
-      for (int b = 0; b < D; b++) {
+      for (int b = 0; b < ND; b++) {
         e[b * 128 + u] = dp[dbase + b];
         e[b * 128 + 64 + u] = dn[dbase + b];
       }
@@ -417,15 +472,84 @@ __global__ __launch_bounds__(512) void k_jl_evals(ChaKey key0, const W* __restri
   int32_t* pb = partial + (uint64_t)blockIdx.x * 256 * kN;
   const uint32_t col_in = lane & 31, rgrp = lane >> 5;
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+  for (int i = 0; i < ND; i++)
 #pragma unroll
     for (int h = 0; h < 2; h++)
 #pragma unroll
       for (int r = 0; r < 16; r++) {
-        const uint32_t m = (mgrp * 4 + i) * 32 + 8 * (r >> 2) + 4 * rgrp + (r & 3);
+        const uint32_t m = (mgrp * ND + i) * 32 + 8 * (r >> 2) + 4 * rgrp + (r & 3);
         const uint32_t nn = (dbase + digit) * 64 + h * 32 + col_in;
         pb[(uint64_t)m * kN + nn] = acc[i][h][r];
       }
+}
+
+// Digits dbase .. dbase + 3 of DT, of which only those below the block's
+// digit span (the most any of its witness words needs) can be nonzero: the
+// pass runs over 1, 2 or 4 of them (3 as 4), or not at all. The dbase = 0 pass
+// finds the span (its loads in flight while the first keystream is drawn) and
+// records it in span[block] for the later passes and the reduction.
+template <class W, int DT>
+__global__ __launch_bounds__(512) void k_jl_evals(ChaKey key0, const W* __restrict__ parts,
+                                                  uint64_t n, uint64_t total_polys,
+                                                  uint64_t polys_per_block, uint64_t q,
+                                                  uint32_t dbase, uint32_t* __restrict__ span,
+                                                  int32_t* __restrict__ partial) {
+  __shared__ __attribute__((aligned(16))) int8_t a_tile[kEvPolys * 256 * kEvRowBytes];
+  __shared__ __attribute__((aligned(16))) int8_t e_tile[kEvPolys * kEvDigits * 128];
+  __shared__ uint32_t wmax[8];
+  const uint32_t t = threadIdx.x;
+  const uint64_t g_begin = (uint64_t)blockIdx.x * polys_per_block;
+  const uint64_t g_end = std::min<uint64_t>(g_begin + polys_per_block, total_polys);
+  uint32_t deff;
+  uint32_t blk[16];
+  auto first_keystream = [&]() {
+    const uint64_t g = g_begin + (t >> 6);  // wave-uniform
+    if (g < g_end) {
+      ChaKey key = key0;
+      key.n0 = (uint32_t)(g / n);
+      chacha_block(key, (uint32_t)((g % n) * 64 + (t & 63)), blk);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 16; i++) blk[i] = 0xAAAAAAAAu;
+    }
+  };
+  if (dbase == 0) {
+    const uint64_t w_end = g_end * 64;
+    uint32_t d = 1;
+    for (uint64_t i0 = g_begin * 64 + t; i0 < w_end; i0 += 16 * 512) {
+      uint64_t x[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const uint64_t i = i0 + u * 512;
+        x[u] = i < w_end ? (uint64_t)parts[i] : 0;
+      }
+      if (i0 == g_begin * 64 + t) first_keystream();
+#pragma unroll
+      for (int u = 0; u < 16; u++) d = max(d, word_span<W, DT>(x[u], q));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) d = max(d, (uint32_t)__shfl_xor((int)d, o));
+    if ((t & 63) == 0) wmax[t >> 6] = d;
+    __syncthreads();
+    deff = wmax[0];
+#pragma unroll
+    for (int w = 1; w < 8; w++) deff = max(deff, wmax[w]);
+    if (t == 0) span[blockIdx.x] = deff;
+  } else {
+    deff = span[blockIdx.x];
+    if (dbase >= deff) return;  // block-uniform
+    first_keystream();
+  }
+  const uint32_t nd = deff - dbase;
+  if (nd == 1)
+    evals_pass<W, DT, 1>(key0, parts, n, total_polys, polys_per_block, q, dbase, partial, a_tile,
+                         e_tile, blk);
+  else if (nd == 2)
+    evals_pass<W, DT, 2>(key0, parts, n, total_polys, polys_per_block, q, dbase, partial, a_tile,
+                         e_tile, blk);
+  else
+    evals_pass<W, DT, 4>(key0, parts, n, total_polys, polys_per_block, q, dbase, partial, a_tile,
+                         e_tile, blk);
 }
 
 // Goldilocks: 256 r mod p for r < p (2^64 = 2^32 - 1 mod p)
@@ -442,32 +566,63 @@ __device__ __forceinline__ uint64_t gold_shift8(uint64_t r) {
 // the blocks (eight loads in flight), then 64 threads recombine the digits
 // from the top one (Horner). Digit sums are below 2^40 in magnitude (at most
 // 2^31 per block partial, 256 blocks).
+constexpr uint32_t kEvMaxBlocks = 1024;  // evaluation blocks (the reduction lists them)
+
 template <class W, int D>
-__global__ __launch_bounds__(64 * D) void k_jl_evals_fin(const int32_t* __restrict__ partial,
-                                                         uint32_t nblk, uint64_t q,
-                                                         W* __restrict__ out) {
-  constexpr int kN = 64 * D;
-  __shared__ int64_t sums[kN];
-  const uint32_t m = blockIdx.x, nn = threadIdx.x;
-  const int32_t* p = partial + (uint64_t)m * kN + nn;
-  const uint64_t stride = (uint64_t)256 * kN;
-  int64_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  uint32_t blk = 0;
-  for (; blk + 8 <= nblk; blk += 8) {
-#pragma unroll
-    for (int u = 0; u < 8; u++) acc[u] += p[(uint64_t)(blk + u) * stride];
+__global__ __launch_bounds__(512) void k_jl_evals_fin(const int32_t* __restrict__ partial,
+                                                      uint32_t nblk, uint64_t q,
+                                                      const uint32_t* __restrict__ span,
+                                                      W* __restrict__ out) {
+  constexpr int kN = 64 * D, kWaves = 8, kU = 16;
+  __shared__ uint16_t blocks[D][kEvMaxBlocks];
+  __shared__ uint32_t counts[D];
+  __shared__ int64_t sums[kWaves][kN];
+  const uint32_t m = blockIdx.x, t = threadIdx.x, lane = t & 63;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  // plane pl lists the blocks whose span covers it (the others did not
+  // compute it); wave pl builds the list
+  if (wave < (uint32_t)D) {
+    const uint32_t pl = wave;
+    uint32_t cnt = 0;
+    for (uint32_t b0 = 0; b0 < nblk; b0 += 64) {
+      const uint32_t b = b0 + lane;
+      const bool has = b < nblk && pl < span[b];
+      const uint64_t mask = __ballot(has);
+      if (has) blocks[pl][cnt + __popcll(mask & ((1ull << lane) - 1))] = (uint16_t)b;
+      cnt += (uint32_t)__popcll(mask);
+    }
+    if (lane == 0) counts[pl] = cnt;
   }
-  for (; blk < nblk; blk++) acc[0] += p[(uint64_t)blk * stride];
-  int64_t s = 0;
-#pragma unroll
-  for (int u = 0; u < 8; u++) s += acc[u];
-  sums[nn] = s;
   __syncthreads();
-  if (nn >= 64) return;
+  // every wave takes every 8th listed block of each plane, kU loads in flight
+  const uint64_t stride = (uint64_t)256 * kN;
+  for (int pl = 0; pl < D; pl++) {
+    const int32_t* p = partial + (uint64_t)m * kN + pl * 64 + lane;
+    const uint16_t* list = blocks[pl];
+    const uint32_t cnt = counts[pl];
+    int64_t acc[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) acc[u] = 0;
+    uint32_t i = wave;
+    for (; i + (kU - 1) * kWaves < cnt; i += kU * kWaves) {
+#pragma unroll
+      for (int u = 0; u < kU; u++) acc[u] += p[(uint64_t)list[i + u * kWaves] * stride];
+    }
+    for (; i < cnt; i += kWaves) acc[0] += p[(uint64_t)list[i] * stride];
+    int64_t s = 0;
+#pragma unroll
+    for (int u = 0; u < kU; u++) s += acc[u];
+    sums[wave][pl * 64 + lane] = s;
+  }
+  __syncthreads();
+  if (t >= 64) return;
+  const uint32_t nn = t;
   uint64_t r = 0;
 #pragma unroll
   for (int b = D - 1; b >= 0; b--) {
-    const int64_t sb = sums[b * 64 + nn];
+    int64_t sb = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) sb += sums[w][b * 64 + nn];
     if (D == 4) {
       const int64_t v = ((int64_t)r * 256 + sb) % (int64_t)q;
       r = (uint64_t)(v < 0 ? v + (int64_t)q : v);
@@ -895,7 +1050,8 @@ int launch_jl_evals(gh_ring* ring, const uint8_t seed[32], const void* parts, ui
   const uint64_t total = (uint64_t)num_parts * n;
   // one block per CU (the accumulators fill half the register file), polys
   // split evenly in rounds of 8
-  uint64_t nblk = std::min<uint64_t>((uint64_t)ctx->num_cus, (total + kEvPolys - 1) / kEvPolys);
+  uint64_t nblk = std::min<uint64_t>(std::min<uint64_t>(ctx->num_cus, kEvMaxBlocks),
+                                     (total + kEvPolys - 1) / kEvPolys);
   if (nblk == 0) nblk = 1;
   uint64_t ppb = (total + nblk - 1) / nblk;
   ppb = (ppb + kEvPolys - 1) / kEvPolys * kEvPolys;
@@ -903,24 +1059,28 @@ int launch_jl_evals(gh_ring* ring, const uint8_t seed[32], const void* parts, ui
   if (nblk == 0) nblk = 1;
   const size_t p_bytes = (size_t)nblk * 256 * 64 * D * sizeof(int32_t);
   void* ws;
-  int st = ctx_workspace(ctx, p_bytes, &ws);
+  int st = ctx_workspace(ctx, p_bytes + nblk * sizeof(uint32_t), &ws);
   if (st) return st;
+  // per-block digit spans after the partial sums: small-norm witnesses (the
+  // LaBRADOR case) need one digit of four, and the passes skip the zero planes
+  uint32_t* span = (uint32_t*)((char*)ws + p_bytes);
   const ChaKey key = chacha_key(seed);
   if (!gold) {
     hipLaunchKernelGGL((k_jl_evals<uint32_t, 4>), dim3((unsigned)nblk), dim3(512), 0, ctx->stream,
-                       key, (const uint32_t*)parts, n, total, ppb, ring->q, 0u, (int32_t*)ws);
+                       key, (const uint32_t*)parts, n, total, ppb, ring->q, 0u, span,
+                       (int32_t*)ws);
     GH_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_jl_evals_fin<uint32_t, 4>), dim3(256), dim3(256), 0, ctx->stream,
-                       (const int32_t*)ws, (uint32_t)nblk, ring->q, (uint32_t*)out);
+    hipLaunchKernelGGL((k_jl_evals_fin<uint32_t, 4>), dim3(256), dim3(512), 0, ctx->stream,
+                       (const int32_t*)ws, (uint32_t)nblk, ring->q, span, (uint32_t*)out);
   } else {
     for (uint32_t dbase = 0; dbase < 8; dbase += 4) {
       hipLaunchKernelGGL((k_jl_evals<uint64_t, 8>), dim3((unsigned)nblk), dim3(512), 0,
                          ctx->stream, key, (const uint64_t*)parts, n, total, ppb, ring->q, dbase,
-                         (int32_t*)ws);
+                         span, (int32_t*)ws);
       GH_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL((k_jl_evals_fin<uint64_t, 8>), dim3(256), dim3(512), 0, ctx->stream,
-                       (const int32_t*)ws, (uint32_t)nblk, ring->q, (uint64_t*)out);
+                       (const int32_t*)ws, (uint32_t)nblk, ring->q, span, (uint64_t*)out);
   }
   GH_HIP(hipGetLastError());
   return GH_OK;

@@ -249,3 +249,57 @@ def test_jl_aggregate_device_memory_and_errors(ctx, oracle, ring_q32):
     finally:
         sb.free()
         pb.free()
+
+
+def _lifted(q, vals):
+    """Signed integers as residues mod q."""
+    v = np.asarray(vals, dtype=object)
+    return np.array([int(x) % q for x in v.ravel()], dtype=np.uint64).reshape(v.shape)
+
+
+@pytest.mark.parametrize("name", ["q32", "gold"])
+@pytest.mark.parametrize("case", ["byte", "minus128", "short", "one_large", "wide", "mixed"])
+def test_jl_evals_by_witness_digit_span(oracle, ring_q32, ring_gold, name, case):
+    """Each evaluation block skips the digit planes past its witness words'
+    span: small words (one digit), -128 (its negation needs two), 16-bit words
+    (two), one full-width word among small ones (all, in one block only),
+    40-bit words (Goldilocks: six of eight, both passes), and polynomials of
+    every width at random over 32 blocks (the reduction then sums a different
+    set of blocks for each digit plane)."""
+    ring = ring_q32 if name == "q32" else ring_gold
+    q = ring.q
+    rng = np.random.default_rng({"byte": 1, "minus128": 2, "short": 3, "one_large": 4,
+                                 "wide": 5, "mixed": 6}[case])
+    shape = (4, 64, 64) if case == "mixed" else (3, 37, 64)
+    if case == "mixed":
+        bits = rng.choice([7, 15, 23, 31, 40, 62], size=shape[:2])
+        if name == "q32":
+            bits = np.minimum(bits, 31)
+        lim = (1 << bits.astype(object))[..., None]
+        s = _lifted(q, np.vectorize(lambda b: int(rng.integers(-b, b)), otypes=[object])(
+            np.broadcast_to(lim, shape)))
+    elif case == "byte":
+        s = _lifted(q, rng.integers(-128, 128, size=shape))
+    elif case == "minus128":
+        s = _lifted(q, np.full(shape, -128))
+    elif case == "short":
+        s = _lifted(q, rng.integers(-(1 << 15), 1 << 15, size=shape))
+    elif case == "one_large":
+        s = _lifted(q, rng.integers(-3, 4, size=shape))
+        s[2, 36, 63] = q // 2
+    else:
+        s = _lifted(q, rng.integers(-(1 << 40), 1 << 40, size=shape)) if name == "gold" else \
+            rand_residues(rng, q, shape)
+    got = ring.jl_evals(SEED, s)
+    want = oracle.jl_evals(q, 64, SEED, 256, s)
+    assert np.array_equal(got.astype(np.uint64), want)
+
+
+def test_jl_evals_small_witness_at_labrador_scale(ring_q32):
+    """r 32 x 1024 with coefficients in [-2^7, 2^7) (one digit): the constant
+    terms are still the projection."""
+    rng = np.random.default_rng(125)
+    s = _lifted(Q32, rng.integers(-128, 128, size=(32, 1024 * 64))).astype(np.uint32)
+    ev = ring_q32.jl_evals(SEED, s)
+    pm, _, _ = ring_q32.jl_project(SEED, 256, s)
+    assert np.array_equal(ev[:, 0], pm)
