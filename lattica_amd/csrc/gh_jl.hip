@@ -359,50 +359,82 @@ __device__ __forceinline__ uint32_t word_span(uint64_t x, uint64_t q) {
 // One pass over ND digits dbase .. dbase + ND - 1 of DT (ND = 1, 2 or 4):
 // block of 8 waves, wave w on digit dbase + w / (8 / ND) (both 32-wide t
 // halves) and m-tiles ND (w % (8 / ND)) .. + ND - 1 (32 rows each): 2 ND
-// accumulator tiles of 32 x 32, every wave busy whatever ND is. Each round:
-// every thread generates one of the 512 keystream columns of 8 polynomials
-// into the A tile and one coefficient's digit windows E ([poly][digit][128]
-// bytes: digits of s_u, then of -s_u); then every wave runs its MFMAs over the
-// round's 8 x 64 columns. Partial sums leave per block.
+// accumulator tiles of 32 x 32, every wave busy whatever ND is.
+//
+// The polynomials go through in steps of kEvStep = 4, double-buffered in LDS:
+// A tiles (the decoded keystream, 256 x 64 int8 per polynomial) and digit
+// windows E ([poly][digit][128] bytes: digits of s_u, then of -s_u). Threads
+// t / 256 = g draw the keystream columns (one each: polynomial (t / 64) % 4,
+// column t % 64) of every other step. During step s, with every wave running
+// the MFMAs of step s from buffer s % 2:
+//   the half that drew step s + 1 stores its A tile and E into buffer
+//   (s + 1) % 2 first, then multiplies;
+//   the other half multiplies first, then draws step s + 2 (keystream and
+//   witness word held in registers until it stores them in step s + 1).
+// Waves w and w + 4 share a SIMD, so on each SIMD one wave's vector work runs
+// beside the other's matrix-core work. Partial sums leave per block.
 constexpr int kEvDigits = 4;  // digit rows of E per polynomial (one pass)
+constexpr int kEvStep = 4;    // polynomials per pipeline step (two LDS buffers)
+static_assert(2 * kEvStep == kEvPolys, "the two step buffers fill the LDS tiles");
 
 template <class W, int DT, int ND>
 __device__ __forceinline__ void evals_pass(ChaKey key0, const W* __restrict__ parts, uint64_t n,
                                            uint64_t total_polys, uint64_t polys_per_block,
                                            uint64_t q, uint32_t dbase, int32_t* __restrict__ partial,
-                                           int8_t* a_tile, int8_t* e_tile, uint32_t (&blk)[16]) {
+                                           int8_t* a_tile, int8_t* e_tile, uint32_t (&blk)[16],
+                                           uint64_t& xw) {
   constexpr int kN = 64 * DT;
   constexpr int kWavesPerDigit = 8 / ND;
   const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const uint32_t digit = wave / kWavesPerDigit, mgrp = wave % kWavesPerDigit;
+  const uint32_t half = t >> 8, ps_own = (t >> 6) & 3, u = t & 63;
   const uint64_t g_begin = (uint64_t)blockIdx.x * polys_per_block;
   const uint64_t g_end = std::min<uint64_t>(g_begin + polys_per_block, total_polys);
+  const uint32_t nsteps = (uint32_t)((g_end - g_begin + kEvStep - 1) / kEvStep);
   v16i acc[ND][2];
 #pragma unroll
   for (int i = 0; i < ND; i++)
 #pragma unroll
     for (int h = 0; h < 2; h++) acc[i][h] = v16i{};
 
-  // keystream of this thread's column of round g0 (polynomial g0 + t / 64),
-  // code 10 (entry 0) past the block's end
-  auto keystream = [&](uint64_t g0, uint32_t (&blk)[16]) {
-    const uint64_t g = g0 + (t >> 6);  // wave-uniform
+  // this thread's keystream column and witness word of step s (code 10, entry
+  // 0, and a zero word past the block's end)
+  auto draw = [&](uint32_t step) {
+    const uint64_t g = g_begin + (uint64_t)step * kEvStep + ps_own;  // wave-uniform
     if (g < g_end) {
+      xw = (uint64_t)parts[g * 64 + u];
       ChaKey key = key0;
       key.n0 = (uint32_t)(g / n);
-      chacha_block(key, (uint32_t)((g % n) * 64 + (t & 63)), blk);
+      chacha_block(key, (uint32_t)((g % n) * 64 + u), blk);
     } else {
+      xw = 0;
 #pragma unroll
       for (int i = 0; i < 16; i++) blk[i] = 0xAAAAAAAAu;
     }
   };
+  // store what draw() left into buffer step % 2
+  auto store = [&](uint32_t step) {
+    const uint32_t b = step & 1;
+    jl_tile_store(blk, a_tile + (b * kEvStep + ps_own) * 256 * kEvRowBytes, u);
+    int8_t dp[DT], dn[DT];
+    lift_digits<DT>(xw, q, dp);
+    lift_digits<DT>(xw == 0 ? 0 : q - xw, q, dn);
+    int8_t* e = e_tile + (b * kEvStep + ps_own) * kEvDigits * 128;
+#pragma unroll
+    for (int d = 0; d < ND; d++) {
+      e[d * 128 + u] = dp[dbase + d];
+      e[d * 128 + 64 + u] = dn[dbase + d];
+    }
+  };
   const uint32_t row = lane & 31, kg = lane >> 5;
-  auto mfma_round = [&](uint64_t g0) {
-    for (int ps = 0; ps < kEvPolys; ps++) {
+  auto mfma_step = [&](uint32_t step) {
+    const uint32_t b = step & 1;
+    const uint64_t g0 = g_begin + (uint64_t)step * kEvStep;
+    for (int ps = 0; ps < kEvStep; ps++) {
       if (g0 + ps >= g_end) break;
-      const int8_t* at = a_tile + ps * 256 * kEvRowBytes;
+      const int8_t* at = a_tile + (b * kEvStep + ps) * 256 * kEvRowBytes;
       const uint32_t* ew =
-          reinterpret_cast<const uint32_t*>(e_tile + (ps * kEvDigits + digit) * 128);
+          reinterpret_cast<const uint32_t*>(e_tile + ((b * kEvStep + ps) * kEvDigits + digit) * 128);
 #pragma unroll
       for (int kstep = 0; kstep < 2; kstep++) {
         const uint32_t k0 = kstep * 32 + kg * 16;
@@ -431,40 +463,16 @@ __device__ __forceinline__ void evals_pass(ChaKey key0, const W* __restrict__ pa
     }
   };
 
-  // Each round: decode the keystream held in registers into the A tile and
-  // the round's digit windows, then run the matrix cores on it while drawing
-  // the next round's keystream (waves 0-3 draw first, waves 4-7 multiply
-  // first, so the two kinds of work can share a SIMD). The tile transpose and
-  // the digit windows run before the barrier, on their own.
-  for (uint64_t g0 = g_begin; g0 < g_end; g0 += kEvPolys) {
-    jl_tile_store(blk, a_tile + (t >> 6) * 256 * kEvRowBytes, t & 63);
-    {
-      const uint32_t ps = t >> 6, u = t & 63;
-      const uint64_t g = g0 + ps;
-      int8_t dp[DT], dn[DT];
-      if (g < g_end) {
-        const uint64_t x = (uint64_t)parts[g * 64 + u];
-        lift_digits<DT>(x, q, dp);
-        lift_digits<DT>(x == 0 ? 0 : q - x, q, dn);
-      } else {
-#pragma unroll
-        for (int b = 0; b < DT; b++) dp[b] = dn[b] = 0;
-      }
-      int8_t* e = e_tile + ps * kEvDigits * 128;
-#pragma unroll
-      for (int b = 0; b < ND; b++) {
-        e[b * 128 + u] = dp[dbase + b];
-        e[b * 128 + 64 + u] = dn[dbase + b];
-      }
-    }
-    __syncthreads();
-    const uint64_t gn = g0 + kEvPolys;  // block-uniform
-    if (wave < 4) {
-      if (gn < g_end) keystream(gn, blk);
-      mfma_round(g0);
+  // the caller drew steps 0 (half 0) and 1 (half 1)
+  if (half == 0) store(0);
+  __syncthreads();
+  for (uint32_t s = 0; s < nsteps; s++) {  // block-uniform
+    if (half != (s & 1)) {
+      if (s + 1 < nsteps) store(s + 1);
+      mfma_step(s);
     } else {
-      mfma_round(g0);
-      if (gn < g_end) keystream(gn, blk);
+      mfma_step(s);
+      if (s + 2 < nsteps) draw(s + 2);
     }
     __syncthreads();
   }
@@ -502,13 +510,18 @@ __global__ __launch_bounds__(512) void k_jl_evals(ChaKey key0, const W* __restri
   const uint64_t g_end = std::min<uint64_t>(g_begin + polys_per_block, total_polys);
   uint32_t deff;
   uint32_t blk[16];
+  uint64_t xw;
+  // steps 0 and 1 of the pass (polynomials g_begin + t / 64): keystream
+  // column and witness word
   auto first_keystream = [&]() {
     const uint64_t g = g_begin + (t >> 6);  // wave-uniform
     if (g < g_end) {
+      xw = (uint64_t)parts[g * 64 + (t & 63)];
       ChaKey key = key0;
       key.n0 = (uint32_t)(g / n);
       chacha_block(key, (uint32_t)((g % n) * 64 + (t & 63)), blk);
     } else {
+      xw = 0;
 #pragma unroll
       for (int i = 0; i < 16; i++) blk[i] = 0xAAAAAAAAu;
     }
@@ -543,13 +556,13 @@ __global__ __launch_bounds__(512) void k_jl_evals(ChaKey key0, const W* __restri
   const uint32_t nd = deff - dbase;
   if (nd == 1)
     evals_pass<W, DT, 1>(key0, parts, n, total_polys, polys_per_block, q, dbase, partial, a_tile,
-                         e_tile, blk);
+                         e_tile, blk, xw);
   else if (nd == 2)
     evals_pass<W, DT, 2>(key0, parts, n, total_polys, polys_per_block, q, dbase, partial, a_tile,
-                         e_tile, blk);
+                         e_tile, blk, xw);
   else
     evals_pass<W, DT, 4>(key0, parts, n, total_polys, polys_per_block, q, dbase, partial, a_tile,
-                         e_tile, blk);
+                         e_tile, blk, xw);
 }
 
 // Goldilocks: 256 r mod p for r < p (2^64 = 2^32 - 1 mod p)
