@@ -29,7 +29,7 @@
 namespace gh {
 
 template <class F>
-__global__ __launch_bounds__(256) void k_batch_fwd_quad(F f, const typename F::W* __restrict__ tab,
+__global__ __launch_bounds__(256, 4) void k_batch_fwd_quad(F f, const typename F::W* __restrict__ tab,
                                                         const BatchDesc* __restrict__ desc,
                                                         const BatchSeg* __restrict__ seg,
                                                         uint32_t nseg) {

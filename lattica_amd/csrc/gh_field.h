@@ -30,6 +30,7 @@ constexpr uint64_t kGoldEps = 0xffffffffull;  // 2^64 mod p
 struct FQ32 {
   using W = uint32_t;
   static constexpr int kTwWords = 2;  // twiddle entry: (w*2^32 mod q, that * q^-1 mod 2^32)
+  static constexpr bool kPow2Roots = false;
   uint32_t q;
   uint32_t qinv;  // q^-1 mod 2^32
 
@@ -78,6 +79,11 @@ struct FQ32 {
 struct FGold {
   using W = uint64_t;
   static constexpr int kTwWords = 1;  // twiddles are canonical values
+  // The forward transform's roots are powers of two: omega = psi^2 = 2^39 for
+  // the plan's psi = 7^((p-1)/128) = 2^43 - 2^91 (the plan checks it), and
+  // 2^96 = -1, 2^192 = 1. Multiplying by them is shifting (mul_pow2).
+  static constexpr bool kPow2Roots = true;
+  static constexpr int kOmegaLog2 = 39;
 
   __device__ __forceinline__ W add(W a, W b) const {
     W s = a + b;
@@ -99,6 +105,18 @@ struct FGold {
     return t2 >= kGoldilocks ? t2 - kGoldilocks : t2;
   }
   __device__ __forceinline__ W mul(W a, W b) const { return reduce128(a * b, __umul64hi(a, b)); }
+  // x * 2^e mod p for 0 <= e < 192 (e a compile-time constant after
+  // unrolling): 2^96 = -1 folds the upper half into a negation, and for
+  // 64 <= e < 96, 2^64 = 2^32 - 1 gives x 2^(e-32) - x 2^(e-64).
+  __device__ __forceinline__ W shl_reduce(W x, int s) const {  // 0 <= s < 64
+    return s == 0 ? x : reduce128(x << s, x >> (64 - s));
+  }
+  __device__ __forceinline__ W mul_pow2(W x, int e) const {
+    const bool neg = e >= 96;
+    if (neg) e -= 96;
+    const W y = e < 64 ? shl_reduce(x, e) : sub(shl_reduce(x, e - 32), shl_reduce(x, e - 64));
+    return neg ? (y == 0 ? 0 : kGoldilocks - y) : y;
+  }
   __device__ __forceinline__ W mont(W a, W b) const { return mul(a, b); }
   __device__ __forceinline__ W mul_tw(W a, const W* tw) const { return mul(a, tw[0]); }
 };

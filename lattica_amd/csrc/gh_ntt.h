@@ -170,7 +170,12 @@ __device__ __forceinline__ void quad_fwd64(const F& f, typename F::W (&x)[16],
 #pragma unroll
   for (int i = 0; i < 16; i++) {
     W v = x[i];
-    if (i != 0) v = f.mul_tw(v, tab + (kTabTwist + 4 * i) * TW);
+    if (i != 0) {
+      if constexpr (F::kPow2Roots)  // psi^(4i) = omega^(2i)
+        v = f.mul_pow2(v, (2 * F::kOmegaLog2 * i) % 192);
+      else
+        v = f.mul_tw(v, tab + (kTabTwist + 4 * i) * TW);
+    }
     y[bitrev4(i)] = v;
   }
 #pragma unroll
@@ -182,7 +187,12 @@ __device__ __forceinline__ void quad_fwd64(const F& f, typename F::W (&x)[16],
       for (int j = 0; j < half; j++) {
         W u = y[st + j];
         W v = y[st + half + j];
-        if (j != 0) v = f.mul_tw(v, tab + (kTabFwd + half - 1 + j) * TW);
+        if (j != 0) {
+          if constexpr (F::kPow2Roots)  // (omega^(64 / len))^j
+            v = f.mul_pow2(v, (F::kOmegaLog2 * (64 / len) * j) % 192);
+          else
+            v = f.mul_tw(v, tab + (kTabFwd + half - 1 + j) * TW);
+        }
         y[st + j] = f.add(u, v);
         y[st + half + j] = f.sub(u, v);
       }
@@ -199,7 +209,12 @@ __device__ __forceinline__ void quad_fwd64(const F& f, typename F::W (&x)[16],
   // ... lane 3 holds C1 - C3 and takes the factor iota ...
   if (t == 3) {
 #pragma unroll
-    for (int k = 0; k < 16; k++) y[k] = f.mul_tw(y[k], tab + (kTabTwist + 32) * TW);
+    for (int k = 0; k < 16; k++) {
+      if constexpr (F::kPow2Roots)  // psi^32 = omega^16
+        y[k] = f.mul_pow2(y[k], (16 * F::kOmegaLog2) % 192);
+      else
+        y[k] = f.mul_tw(y[k], tab + (kTabTwist + 32) * TW);
+    }
   }
   // ... and t, t^1 combine: lanes 0..3 end with Y_0, Y_2, Y_1, Y_3.
 #pragma unroll

@@ -98,6 +98,10 @@ int plan_ring(uint64_t q, uint32_t d, gh_ring* ring) {
   if (q < (1ull << 32)) kind = GH_RING_Q32;
   else if (q == kGoldilocks) kind = GH_RING_GOLDILOCKS;
   else return GH_E_UNSUPPORTED_MODULUS;
+  // the Goldilocks forward transform multiplies by its roots as shifts
+  // (FGold::kOmegaLog2): they must be this psi's
+  if (kind == GH_RING_GOLDILOCKS && omega != powmod(2, FGold::kOmegaLog2, q))
+    return GH_E_UNSUPPORTED_TWIST;
 
   ring->q = q;
   ring->d = d;

@@ -80,6 +80,25 @@ def test_ntt_batches_match_oracle(oracle, ring_q32, ring_gold, name, count):
 
 
 @pytest.mark.parametrize("name", ["q32", "goldilocks"])
+def test_twisted_forward_extreme_words(oracle, ring_q32, ring_gold, name):
+    """Words at the edges of the reductions (0, 1, q - 1, q - 2, 2^31, 2^32 - 1,
+    2^32, 2^63 where below q) in every slot pattern: the Goldilocks transform
+    multiplies by its roots as shifts, whose carries these words exercise."""
+    r = ring_for(name, ring_q32, ring_gold)
+    q = r.q
+    edges = [v for v in (0, 1, q - 1, q - 2, 1 << 31, (1 << 32) - 1, 1 << 32, 1 << 63,
+                         q - (1 << 32), (q + 1) // 2) if 0 <= v < q]
+    rng = np.random.default_rng(5)
+    x = np.array(edges, dtype=np.uint64)[rng.integers(0, len(edges), size=(97, 64))]
+    x[0] = q - 1
+    x[1] = 0
+    x[1, 7] = q - 1
+    got = r.twisted_forward(r.words(x)).astype(np.uint64)
+    assert np.array_equal(got, oracle.twisted_forward(q, 64, x))
+    assert np.array_equal(r.twisted_inverse(r.words(got)).astype(np.uint64), x)
+
+
+@pytest.mark.parametrize("name", ["q32", "goldilocks"])
 def test_large_batch_round_trip_and_spot_check(oracle, ctx, ring_q32, ring_gold, name):
     # enough polynomials for the wide-tile kernels; round trip over all of them,
     # exact comparison on a sample
