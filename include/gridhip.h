@@ -28,8 +28,9 @@
  *    work is enqueued on the context stream and the call returns without
  *    waiting — gh_ctx_synchronize() waits). Device results are complete once
  *    gh_ctx_synchronize() returns, and visible to every later call on the
- *    context; a device-memory gh_commit may hold its last step back until
- *    that point, so hipDeviceSynchronize() alone does not finish it.
+ *    context; a device-memory gh_commit (or single-item gh_commit_batch) may
+ *    hold its last step back until that point, so hipDeviceSynchronize()
+ *    alone does not finish it.
  *  - Return value: GH_OK (0) or a negative gh_status. Errors that the
  *    reference reports map 1:1 onto NttError (arith/ntt.rs:40-49) and
  *    CommitmentError (commit/src/error.rs:7-18).
@@ -212,9 +213,9 @@ GH_API int gh_key_dims(const gh_key *key, uint32_t *rows, uint32_t *msg_len, uin
  * commit_with_opening_ntt path, commit/src/ajtai/mod.rs:256-269).
  * DIMENSION_MISMATCH when lengths differ from the key (linear.rs:270-327).
  * With GH_MEM_DEVICE and open_len 0 the finalize (sum of the MAC partials and
- * the inverse NTT) is held back and runs inside the next such gh_commit's
- * launch, or before any other call on the context: back-to-back commitments
- * cost one launch each. */
+ * the inverse NTT) is held back and runs inside the next such commitment's
+ * launch over the same field (q32 or Goldilocks), or before any other call on
+ * the context: back-to-back commitments cost one launch each. */
 GH_API int gh_commit(gh_key *key, const void *message, uint32_t msg_len, const void *randomness,
                      uint32_t open_len, int domain, void *t_out, int mem);
 
@@ -237,8 +238,9 @@ GH_API int gh_bdlop_commit(gh_key *mask, gh_key *key, const void *message, uint3
  * Item i commits witness_i (len_i polys, coefficient domain) against the first
  * len_i columns of the key: t_i = A[:, :len_i] * w_i. All pointers in one call
  * share `mem`. The per-item result equals gh_commit on the same inputs with a
- * key of len_i columns. No reference counterpart: the reference commits one
- * message per call (commit/src/ajtai/mod.rs:443-452). */
+ * key of len_i columns. A single device-memory item takes gh_commit's path
+ * (its finalize may be held back the same way). No reference counterpart: the
+ * reference commits one message per call (commit/src/ajtai/mod.rs:443-452). */
 typedef struct gh_commit_item {
   const void *witness;
   uint32_t len;

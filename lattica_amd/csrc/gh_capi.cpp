@@ -854,6 +854,13 @@ GH_API int gh_commit_batch(gh_key* key, const gh_commit_item* items, size_t coun
   gh_ring* ring = key->ring;
   gh_ctx* ctx = ring->ctx;
   std::lock_guard<std::mutex> lk(ctx->mu);
+  if (mem == GH_MEM_DEVICE && count == 1) {
+    // one item: gh_commit's path, its finalize held back the same way (so no
+    // flush here: the launch issues a pending finalize itself)
+    GH_HIP(hipSetDevice(ctx->device));
+    return launch_commit(key, items[0].witness, items[0].len, GH_DOMAIN_COEFF, items[0].t_out,
+                         true);
+  }
   TRY(set_device(ctx));
   const size_t wb = ring->word_bytes, d = ring->d;
   const size_t t_bytes = (size_t)key->rows * d * wb;

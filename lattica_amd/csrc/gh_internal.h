@@ -84,12 +84,14 @@ inline uint32_t tuned_batch_chunk(const gh_tuning& t, int ring_kind) {
   return (uint32_t)(ring_kind == GH_RING_Q32 ? t.batch_chunk_cols : t.batch_chunk_cols_gold);
 }
 
-// The finalize of a device-memory q32 commitment, held back so that it runs
-// as extra blocks of the next such commitment's MAC launch instead of as a
-// launch of its own. Its partial sums stay at work + off (bytes long) until it
-// has run; every other call on the context issues it first (ctx_flush).
+// The finalize of a device-memory commitment (fused path, q32 or Goldilocks),
+// held back so that it runs as extra blocks of the next such commitment's
+// launch over the same field instead of as a launch of its own. Its partial
+// sums stay at work + off (bytes long) until it has run; every other call on
+// the context issues it first (ctx_flush).
 struct PendingFin {
   bool active = false;
+  int kind = GH_RING_Q32;
   uint32_t q = 0, qinv = 0;
   const void* tab = nullptr;
   size_t off = 0, bytes = 0;

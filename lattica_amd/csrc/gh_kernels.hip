@@ -541,6 +541,99 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_q32(
   }
 }
 
+// Fused witness transform + MAC (Goldilocks), the per-call form: a block
+// transforms its contiguous range of at most kFuseChunkGold witness
+// polynomials into LDS (four lanes per polynomial), then each wave runs the
+// modular multiply-add over its columns (wave, wave + WAVES, ...) against
+// that copy, two columns of key loads in flight, and the waves' sums meet in
+// the same LDS. The first fin.nblk blocks run the previous commitment's
+// finalize instead. Results equal k_mac_gold on a separately transformed
+// witness.
+constexpr int kFuseChunkGold = 64;
+constexpr int kFuseStrideGold = 66;  // LDS row stride of the transformed witness, words
+
+struct FinArgsGold {
+  const uint64_t* tab;
+  const uint64_t* partial;
+  uint64_t* out;
+  uint32_t nparts, rows, nblk;
+};
+
+template <int RB, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_commit_fused_gold(
+    FGold f, const uint64_t* __restrict__ tab, const uint64_t* __restrict__ A,
+    const uint64_t* __restrict__ s_coeff, uint64_t* __restrict__ partial, uint32_t rows,
+    uint32_t cols, uint32_t cols_per_block, FinArgsGold fin) {
+  extern __shared__ uint64_t lds64[];  // max(cols_per_block * kFuseStrideGold, WAVES * RB * 64)
+  if (blockIdx.x < fin.nblk) {
+    if (blockIdx.y == 0)
+      finalize_row_g<FGold, WAVES>(f, fin.tab, fin.partial, fin.nparts, fin.rows, blockIdx.x,
+                                   (int)kTabUntwist, fin.out, lds64);
+    return;
+  }
+  const uint32_t bx = blockIdx.x - fin.nblk;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, t = threadIdx.x & 3;
+  const uint32_t row0 = blockIdx.y * RB;
+  const uint32_t ch = bx * cols_per_block;  // < cols: every block owns a column
+  const uint32_t ncol = ch + cols_per_block < cols ? cols_per_block : cols - ch;
+  uint64_t ltw[16][1];
+#pragma unroll
+  for (int k = 0; k < 16; k++) ltw[k][0] = tab[kTabQuad + 16 * t + k];
+  // whole quads share p, so they drop out together (the transform exchanges
+  // values within a quad)
+  for (uint32_t p = threadIdx.x >> 2; p < ncol; p += 16 * WAVES) {
+    uint64_t x[16];
+    const uint64_t* sp = s_coeff + (size_t)(ch + p) * 64 + t;
+#pragma unroll
+    for (int i = 0; i < 16; i++) x[i] = sp[4 * i];
+    quad_fwd64(f, x, tab, ltw, t);
+    uint64_t* dp = lds64 + p * kFuseStrideGold + 16 * bitrev2(t);
+#pragma unroll
+    for (int v = 0; v < 16; v++) dp[v] = x[v];
+  }
+  __syncthreads();
+  uint64_t acc[RB];
+#pragma unroll
+  for (int r = 0; r < RB; r++) acc[r] = 0;
+  const size_t col_stride = (size_t)rows * 64;
+  const uint64_t* akey = A + (size_t)row0 * 64 + lane;
+  // this wave's columns wave + j * WAVES, j < nw; the next column's key loads
+  // are issued before the current column's products (clamped to the block's
+  // last column, so the loads never branch)
+  const uint32_t nw = (uint32_t)wave < ncol ? (ncol - wave + WAVES - 1) / WAVES : 0;
+  auto load_col = [&](uint64_t (&v)[RB], uint32_t j) {
+    const uint32_t c = wave + j * WAVES;
+    const uint64_t* a0 = akey + (size_t)(ch + (c < ncol ? c : ncol - 1)) * col_stride;
+#pragma unroll
+    for (int r = 0; r < RB; r++) v[r] = __builtin_nontemporal_load(a0 + r * 64);
+  };
+  auto mac_col = [&](const uint64_t (&v)[RB], uint32_t j) {
+    const uint64_t s0 = lds64[(wave + j * WAVES) * kFuseStrideGold + lane];
+#pragma unroll
+    for (int r = 0; r < RB; r++) acc[r] = f.add(acc[r], f.mul(v[r], s0));
+  };
+  uint64_t va[RB], vb[RB];
+  if (nw > 0) load_col(va, 0);
+  for (uint32_t j = 0; j < nw; j += 2) {
+    load_col(vb, j + 1);
+    mac_col(va, j);
+    if (j + 1 >= nw) break;
+    load_col(va, j + 2);
+    mac_col(vb, j + 1);
+  }
+  __syncthreads();  // the witness copy is no longer read
+  uint64_t* red = lds64;  // [WAVES][RB][64]
+#pragma unroll
+  for (int r = 0; r < RB; r++) red[(wave * RB + r) * 64 + lane] = acc[r];
+  __syncthreads();
+  for (int o = threadIdx.x; o < RB * 64; o += 64 * WAVES) {
+    uint64_t a = red[o];
+#pragma unroll
+    for (int w = 1; w < WAVES; w++) a = f.add(a, red[w * RB * 64 + o]);
+    partial[((size_t)bx * rows + row0) * 64 + o] = a;
+  }
+}
+
 // Goldilocks: modular multiply-add per term, canonical partials.
 template <int RB>
 __global__ __launch_bounds__(64 * kMacWaves) void k_mac_gold(
@@ -879,6 +972,8 @@ static int launch_commit_fused(gh_key* key, const void* s_dev, uint32_t cols, vo
   // workspace as it is (growing it would free them), else from offset 0
   PendingFin& pf = ctx->fin;
   size_t off = 0;
+  if (pf.active && pf.kind != GH_RING_Q32)
+    if (int st = ctx_flush(ctx)) return st;
   if (pf.active) {
     if (defer && pbytes <= pf.off) off = 0;
     else if (defer && pf.off + pf.bytes + pbytes <= ctx->work_bytes) off = pf.off + pf.bytes;
@@ -914,8 +1009,84 @@ static int launch_commit_fused(gh_key* key, const void* s_dev, uint32_t cols, vo
   GH_HIP(hipGetLastError());
   pf.active = false;  // a pending finalize ran inside this launch
   if (!defer) return launch_finalize_q32(ctx, f, tab, P, nparts, rows, (uint32_t*)t_dev);
-  pf = PendingFin{true, (uint32_t)ring->q, ring->qinv32, ring->dev_tab, off, pbytes, nparts, rows,
-                  t_dev};
+  pf = PendingFin{true, GH_RING_Q32, (uint32_t)ring->q, ring->qinv32, ring->dev_tab, off, pbytes,
+                  nparts, rows, t_dev};
+  return GH_OK;
+}
+
+// The standalone finalize of a Goldilocks commitment: 16 waves per row.
+static int launch_finalize_gold(gh_ctx* ctx, const uint64_t* tab, const uint64_t* P,
+                                uint32_t nparts, uint32_t rows, uint64_t* t_dev) {
+  hipEvent_t f0, f1;
+  prof_pair(ctx, 2, &f0, &f1);
+  hipExtLaunchKernelGGL((k_finalize<FGold>), dim3(rows), dim3(1024), 0, ctx->stream, f0, f1, 0,
+                        FGold{}, tab, P, nparts, rows, (int)kTabUntwist, t_dev);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+// Goldilocks commitment, per-call form: k_commit_fused_gold, then the
+// finalize, held back with `defer` exactly as for q32 (the partial sums of
+// consecutive commitments alternate between two areas of the workspace).
+static int launch_commit_fused_gold(gh_key* key, const void* s_dev, uint32_t cols, void* t_dev,
+                                    bool defer) {
+  gh_ring* ring = key->ring;
+  gh_ctx* ctx = ring->ctx;
+  const uint32_t rows = key->rows;
+  constexpr int kWaves = 4;
+  const int rb = rows % 16 == 0 ? 16 : rows % 8 == 0 ? 8 : 4;
+  // blocks: the tuned count per CU, at least one per kFuseChunkGold columns,
+  // at most one per wave's column; every block owns at least one column
+  uint32_t nparts = (uint32_t)(ctx->num_cus * ctx->tune.mac_blocks_per_cu);
+  const uint32_t min_parts = (cols + kFuseChunkGold - 1) / kFuseChunkGold;
+  if (nparts < min_parts) nparts = min_parts;
+  const uint32_t max_parts = (cols + kWaves - 1) / kWaves;
+  if (nparts > max_parts) nparts = max_parts;
+  if (nparts == 0) nparts = 1;
+  const uint32_t cpb = (cols + nparts - 1) / nparts;
+  nparts = (cols + cpb - 1) / cpb;
+  const size_t pbytes = ((size_t)nparts * rows * 64 * 8 + 255) & ~(size_t)255;
+
+  PendingFin& pf = ctx->fin;
+  size_t off = 0;
+  if (pf.active && pf.kind != GH_RING_GOLDILOCKS)
+    if (int st = ctx_flush(ctx)) return st;
+  if (pf.active) {
+    if (defer && pbytes <= pf.off) off = 0;
+    else if (defer && pf.off + pf.bytes + pbytes <= ctx->work_bytes) off = pf.off + pf.bytes;
+    else if (int st = ctx_flush(ctx)) return st;
+  }
+  FinArgsGold fin{};
+  if (pf.active) {
+    fin = FinArgsGold{(const uint64_t*)pf.tab,
+                      (const uint64_t*)((const char*)ctx->work + pf.off), (uint64_t*)pf.t_out,
+                      pf.nparts, pf.rows, pf.rows};
+  } else {
+    void* ws;
+    int st = ctx_workspace(ctx, (defer ? 2 : 1) * pbytes, &ws);  // room for the next one too
+    if (st) return st;
+  }
+  const uint64_t* tab = (const uint64_t*)ring->dev_tab;
+  const uint64_t* A = (const uint64_t*)key->dev_a;
+  const uint64_t* S = (const uint64_t*)s_dev;
+  uint64_t* P = (uint64_t*)((char*)ctx->work + off);
+  dim3 grid(fin.nblk + nparts, rows / rb);
+  size_t lds_words = (size_t)cpb * kFuseStrideGold;
+  if ((size_t)kWaves * rb * 64 > lds_words) lds_words = (size_t)kWaves * rb * 64;
+  const size_t lds_bytes = lds_words * 8;
+  hipEvent_t e0, e1;
+  prof_pair(ctx, 1, &e0, &e1);
+#define GH_FUSED_GOLD(R)                                                                        \
+  hipExtLaunchKernelGGL((k_commit_fused_gold<R, kWaves>), grid, dim3(64 * kWaves), lds_bytes,   \
+                        ctx->stream, e0, e1, 0, FGold{}, tab, A, S, P, rows, cols, cpb, fin)
+  if (rb == 16) GH_FUSED_GOLD(16);
+  else if (rb == 8) GH_FUSED_GOLD(8);
+  else GH_FUSED_GOLD(4);
+#undef GH_FUSED_GOLD
+  GH_HIP(hipGetLastError());
+  pf.active = false;  // a pending finalize ran inside this launch
+  if (!defer) return launch_finalize_gold(ctx, tab, P, nparts, rows, (uint64_t*)t_dev);
+  pf = PendingFin{true, GH_RING_GOLDILOCKS, 0, 0, ring->dev_tab, off, pbytes, nparts, rows, t_dev};
   return GH_OK;
 }
 
@@ -923,6 +1094,10 @@ int ctx_flush(gh_ctx* ctx) {
   PendingFin& pf = ctx->fin;
   if (!pf.active) return GH_OK;
   pf.active = false;
+  if (pf.kind == GH_RING_GOLDILOCKS)
+    return launch_finalize_gold(ctx, (const uint64_t*)pf.tab,
+                                (const uint64_t*)((const char*)ctx->work + pf.off), pf.nparts,
+                                pf.rows, (uint64_t*)pf.t_out);
   return launch_finalize_q32(ctx, FQ32{pf.q, pf.qinv}, (const uint32_t*)pf.tab,
                              (const uint32_t*)((const char*)ctx->work + pf.off), pf.nparts,
                              pf.rows, (uint32_t*)pf.t_out);
@@ -936,6 +1111,9 @@ int launch_commit(gh_key* key, const void* s_dev, uint32_t cols, int domain, voi
   if (ring->kind == GH_RING_Q32 && ctx->tune.fused && key->rows % 4 == 0 &&
       domain == GH_DOMAIN_COEFF)
     return launch_commit_fused(key, s_dev, cols, t_dev, defer);
+  if (ring->kind == GH_RING_GOLDILOCKS && ctx->tune.fused && key->rows % 4 == 0 &&
+      domain == GH_DOMAIN_COEFF)
+    return launch_commit_fused_gold(key, s_dev, cols, t_dev, defer);
   int fst = ctx_flush(ctx);  // this path finalizes in its own launch
   if (fst) return fst;
   const size_t wb = ring->word_bytes;

@@ -22,26 +22,28 @@ def _case(ctx, ring, key, rng):
     return ctx.to_device(w), ctx.alloc(key.rows * 64 * ring.info.word_bytes), want
 
 
-def test_back_to_back_commitments_of_mixed_shapes(ctx, ring_q32):
+@pytest.mark.parametrize("name", ["q32", "goldilocks"])
+def test_back_to_back_commitments_of_mixed_shapes(ctx, ring_q32, ring_gold, name):
     """Keys of different row counts and lengths, so consecutive commitments
     need partial-sum areas of different sizes (the largest does not fit beside
     a pending one and takes the flush path); one synchronize at the end."""
     import lattica_amd as la
+    ring = ring_q32 if name == "q32" else ring_gold
     rng = np.random.default_rng(61)
-    keys = [_key(la, ring_q32, rng, 16, 300), _key(la, ring_q32, rng, 8, 300),
-            _key(la, ring_q32, rng, 16, 4096), _key(la, ring_q32, rng, 4, 40),
-            _key(la, ring_q32, rng, 32, 1000)]
+    keys = [_key(la, ring, rng, 16, 300), _key(la, ring, rng, 8, 300),
+            _key(la, ring, rng, 16, 4096), _key(la, ring, rng, 4, 40),
+            _key(la, ring, rng, 32, 1000), _key(la, ring, rng, 12, 1)]
     cases = []
     try:
-        for i in range(15):
+        for i in range(17):
             key = keys[i % len(keys)]
-            s, t, want = _case(ctx, ring_q32, key, rng)
+            s, t, want = _case(ctx, ring, key, rng)
             cases.append((key, s, t, want))
         for key, s, t, _ in cases:
             key.commit_device(s, key.msg_len, t)
         ctx.synchronize()
         for i, (key, s, t, want) in enumerate(cases):
-            got = t.download(ring_q32.dtype, (key.rows, 64))
+            got = t.download(ring.dtype, (key.rows, 64))
             assert np.array_equal(got, want), i
     finally:
         for _, s, t, _ in cases:
@@ -52,15 +54,17 @@ def test_back_to_back_commitments_of_mixed_shapes(ctx, ring_q32):
 
 
 def test_rings_of_different_moduli_interleaved(ctx):
-    """The held-back finalize carries its own ring's modulus and tables."""
+    """The held-back finalize carries its own ring's modulus and tables, and a
+    Goldilocks commitment after a q32 one (or before) issues the other field's
+    pending finalize first."""
     import lattica_amd as la
     rng = np.random.default_rng(62)
-    rings = [la.Ring(ctx, q, 64) for q in (Q32, 12289, 8380417)]
+    rings = [la.Ring(ctx, q, 64) for q in (Q32, 12289, la.GOLDILOCKS, 8380417)]
     keys = [_key(la, r, rng, 16, 257) for r in rings]
     cases = []
     try:
-        for i in range(9):
-            key = keys[i % 3]
+        for i in range(12):
+            key = keys[i % 4]
             cases.append((key,) + _case(ctx, key.ring, key, rng))
         for key, s, t, _ in cases:
             key.commit_device(s, key.msg_len, t)
@@ -156,4 +160,34 @@ def test_profiled_commitments(ctx, ring_q32):
     finally:
         s.free()
         t.free()
+        key.close()
+
+
+@pytest.mark.parametrize("name", ["q32", "goldilocks"])
+def test_single_item_device_batches_back_to_back(ctx, ring_q32, ring_gold, name):
+    """A one-item device-memory gh_commit_batch takes the per-call path with its
+    finalize held back: back-to-back single items of different lengths, then a
+    grouped batch (which issues the pending finalize first)."""
+    import lattica_amd as la
+    ring = ring_q32 if name == "q32" else ring_gold
+    rng = np.random.default_rng(65)
+    key = _key(la, ring, rng, 16, 700)
+    lens = [700, 1, 333, 64, 699]
+    ws = [small_signed(rng, ring.q, (n, 64), -128, 128).astype(ring.dtype) for n in lens]
+    bufs = []
+    try:
+        outs = []
+        for w in ws:
+            s = ctx.to_device(w)
+            t = ctx.alloc(16 * 64 * ring.info.word_bytes)
+            bufs += [s, t]
+            outs.append(t)
+            key.commit_batch_device([s], [w.shape[0]], [t])
+        grouped = key.commit_batch(ws)  # the grouped kernels: an independent path
+        ctx.synchronize()
+        for i in range(len(ws)):
+            assert np.array_equal(outs[i].download(ring.dtype, (16, 64)), grouped[i]), i
+    finally:
+        for b in bufs:
+            b.free()
         key.close()
