@@ -27,8 +27,11 @@
 //                    8 for Goldilocks) whose products sum exactly in int32.
 //                    The accumulator tile of a block (256 rows x 256 digit
 //                    columns of int32) fills half a CU's register file, so
-//                    Goldilocks runs two passes of four digits each;
-//   k_jl_evals_fin   sums the per-block partials and recombines the digits mod q;
+//                    Goldilocks runs two passes of four digits each; a block
+//                    runs only the digits its witness words need (one for
+//                    words in [-127, 127]);
+//   k_jl_evals_fin   sums each digit plane over the blocks that computed it and
+//                    recombines the digits mod q;
 //   k_jl_agg         the aggregation's JL term phi_ij += sum_m c_m sigma_{-1}(pi_ij^(m))
 //                    (main_protocol/aggregation.rs:237-245) as the same kind of
 //                    GEMM with the polynomials as rows: the entries go from the
