@@ -72,7 +72,7 @@ __device__ void jl_column_lds(const ChaKey& key, uint64_t col, uint32_t pk, uint
 }
 
 constexpr int kEvPolys = 8;       // witness polynomials per LDS round
-constexpr int kEvRowBytes = 64;   // A tile: [poly][m][k] int8, k contiguous
+constexpr int kEvRowBytes = 64;   // A tile bytes per row of a polynomial (256 rows x 64 coefficients)
 
 // Decodes this lane's keystream column (16 words, 2 bits per row, 256 rows)
 // into int8 entries stored as rows of columns: the dword holding row m of the
@@ -100,32 +100,6 @@ __device__ __forceinline__ void jl_quad_store(const uint32_t (&blk)[16], int8_t*
     const uint32_t w = __builtin_amdgcn_perm(qn, z, sel2);
     const uint32_t m = 4 * g + r;
     if (m < rows) *reinterpret_cast<uint32_t*>(base + (uint64_t)m * row_stride + a4) = w;
-  }
-}
-
-// The evaluation A tile ([m][64] int8 per polynomial) keeps 16-byte chunk c of
-// row m at chunk c ^ jl_tile_swz(m): the matrix-core fragment reads (rows
-// m .. m + 31 of one chunk) and these stores then hit distinct LDS banks.
-__device__ __forceinline__ uint32_t jl_tile_swz(uint32_t m) { return (m ^ (m >> 2)) & 3; }
-
-// jl_quad_store's column transpose into the swizzled evaluation tile.
-__device__ __forceinline__ void jl_tile_store(const uint32_t (&blk)[16], int8_t* tile_poly,
-                                              uint32_t k) {
-  const uint32_t r = k & 3, a4 = k & ~3u;
-  const uint32_t sel1 = (r & 1) ? 0x03070105u : 0x06020400u;
-  const uint32_t sel2 = (r & 2) ? 0x03020706u : 0x05040100u;
-#pragma unroll
-  for (int g = 0; g < 64; g++) {
-    const uint32_t x = (blk[g >> 2] >> (8 * (g & 3))) & 0xffu;
-    const uint32_t y = (x | (x << 6) | (x << 12) | (x << 18)) & 0x03030303u;
-    const uint32_t e = __builtin_amdgcn_perm(0u, 0x0000FF01u, y);
-    const uint32_t pn = (uint32_t)__builtin_amdgcn_mov_dpp((int)e, 0xB1, 0xF, 0xF, false);
-    const uint32_t z = __builtin_amdgcn_perm(pn, e, sel1);
-    const uint32_t qn = (uint32_t)__builtin_amdgcn_mov_dpp((int)z, 0x4E, 0xF, 0xF, false);
-    const uint32_t w = __builtin_amdgcn_perm(qn, z, sel2);
-    const uint32_t m = 4 * g + r;  // jl_tile_swz(m) = (r ^ g) & 3
-    const uint32_t c = (a4 >> 4) ^ ((r ^ (uint32_t)g) & 3);
-    *reinterpret_cast<uint32_t*>(tile_poly + m * kEvRowBytes + c * 16 + (a4 & 15)) = w;
   }
 }
 
@@ -388,9 +362,15 @@ __device__ __forceinline__ void evals_pass(ChaKey key0, const W* __restrict__ pa
                                            uint64_t& xw) {
   constexpr int kN = 64 * DT;
   constexpr int kWavesPerDigit = 8 / ND;
+  // The keystream columns become A fragments on the matrix cores (as in
+  // k_jl_proj: the lanes draw their coefficients in jl_tcoeff order, and an
+  // MFMA against a 0/1 matrix transposes them), stored fragment-major:
+  // [m-tile][k-step][lane] x 16 bytes per polynomial.
   const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const uint32_t digit = wave / kWavesPerDigit, mgrp = wave % kWavesPerDigit;
-  const uint32_t half = t >> 8, ps_own = (t >> 6) & 3, u = t & 63;
+  const uint32_t half = t >> 8, ps_own = (t >> 6) & 3;
+  const uint32_t u = jl_tcoeff(lane);  // this lane's coefficient
+  const v4i tperm = jl_tperm(lane);
   const uint64_t g_begin = (uint64_t)blockIdx.x * polys_per_block;
   const uint64_t g_end = std::min<uint64_t>(g_begin + polys_per_block, total_polys);
   const uint32_t nsteps = (uint32_t)((g_end - g_begin + kEvStep - 1) / kEvStep);
@@ -418,7 +398,17 @@ __device__ __forceinline__ void evals_pass(ChaKey key0, const W* __restrict__ pa
   // store what draw() left into buffer step % 2
   auto store = [&](uint32_t step) {
     const uint32_t b = step & 1;
-    jl_tile_store(blk, a_tile + (b * kEvStep + ps_own) * 256 * kEvRowBytes, u);
+    int8_t* tp = a_tile + (b * kEvStep + ps_own) * 256 * kEvRowBytes;
+#pragma unroll
+    for (int mc = 0; mc < 8; mc++) {  // rows 32 mc .. 32 mc + 31, both k-steps
+      const auto sw = __builtin_amdgcn_permlane32_swap(blk[2 * mc], blk[2 * mc + 1], false, false);
+      const v4i a1 = jl_pack(
+          __builtin_amdgcn_mfma_i32_32x32x32_i8(jl_decode16(sw[0]), tperm, v16i{}, 0, 0, 0));
+      const v4i a2 = jl_pack(
+          __builtin_amdgcn_mfma_i32_32x32x32_i8(jl_decode16(sw[1]), tperm, v16i{}, 0, 0, 0));
+      *reinterpret_cast<v4i*>(tp + ((2 * mc) * 64 + lane) * 16) = a1;
+      *reinterpret_cast<v4i*>(tp + ((2 * mc + 1) * 64 + lane) * 16) = a2;
+    }
     int8_t dp[DT], dn[DT];
     lift_digits<DT>(xw, q, dp);
     lift_digits<DT>(xw == 0 ? 0 : q - xw, q, dn);
@@ -455,9 +445,8 @@ __device__ __forceinline__ void evals_pass(ChaKey key0, const W* __restrict__ pa
         }
 #pragma unroll
         for (int i = 0; i < ND; i++) {
-          const uint32_t m = (mgrp * ND + i) * 32 + row;
-          const uint32_t c = (k0 >> 4) ^ jl_tile_swz(row);  // row = m mod 32
-          const v4i afrag = *reinterpret_cast<const v4i*>(at + m * kEvRowBytes + c * 16);
+          const v4i afrag = *reinterpret_cast<const v4i*>(
+              at + ((2 * (mgrp * ND + i) + kstep) * 64 + lane) * 16);
 #pragma unroll
           for (int h = 0; h < 2; h++)
             acc[i][h] = __builtin_amdgcn_mfma_i32_32x32x32_i8(afrag, bfrag[h], acc[i][h], 0, 0, 0);
@@ -515,14 +504,15 @@ __global__ __launch_bounds__(512) void k_jl_evals(ChaKey key0, const W* __restri
   uint32_t blk[16];
   uint64_t xw;
   // steps 0 and 1 of the pass (polynomials g_begin + t / 64): keystream
-  // column and witness word
+  // column and witness word of coefficient jl_tcoeff(lane)
   auto first_keystream = [&]() {
     const uint64_t g = g_begin + (t >> 6);  // wave-uniform
+    const uint32_t u = jl_tcoeff(t & 63);
     if (g < g_end) {
-      xw = (uint64_t)parts[g * 64 + (t & 63)];
+      xw = (uint64_t)parts[g * 64 + u];
       ChaKey key = key0;
       key.n0 = (uint32_t)(g / n);
-      chacha_block(key, (uint32_t)((g % n) * 64 + (t & 63)), blk);
+      chacha_block(key, (uint32_t)((g % n) * 64 + u), blk);
     } else {
       xw = 0;
 #pragma unroll
