@@ -90,4 +90,16 @@ __global__ __launch_bounds__(1024) void k_finalize(F f, const typename F::W* __r
   finalize_row(f, tab, partial, nparts, rows, blockIdx.x, post, t_out);
 }
 
+// The same with 4 waves per row, for many rows and few partials (the Gram
+// matrix's 528 polynomials): at 104-128 VGPRs a 16-wave block fills a CU, so
+// 528 rows would take three rounds of 256 blocks; 4-wave blocks take one.
+template <class F>
+__global__ __launch_bounds__(256) void k_finalize4(F f, const typename F::W* __restrict__ tab,
+                                                   const typename F::W* __restrict__ partial,
+                                                   uint32_t nparts, uint32_t rows, int post,
+                                                   typename F::W* __restrict__ t_out) {
+  __shared__ typename F::W red[4 * 64];
+  finalize_row_g<F, 4>(f, tab, partial, nparts, rows, blockIdx.x, post, t_out, red);
+}
+
 }  // namespace gh

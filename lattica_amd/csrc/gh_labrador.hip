@@ -232,14 +232,14 @@ int launch_gram(gh_ring* ring, const void* s_dev, uint32_t r, uint32_t n, int do
     hipLaunchKernelGGL(k_gram_q32, grid, dim3(64), 0, ctx->stream, f, (const uint32_t*)s_ntt, r,
                        n, chunk, (uint32_t*)partial);
     GH_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_finalize<FQ32>), dim3(npairs), dim3(1024), 0, ctx->stream, f,
+    hipLaunchKernelGGL((k_finalize4<FQ32>), dim3(npairs), dim3(256), 0, ctx->stream, f,
                        (const uint32_t*)ring->dev_tab, (const uint32_t*)partial, nchunks, npairs,
                        (int)kTabUntwistMac, (uint32_t*)g_dev);
   } else {
     hipLaunchKernelGGL(k_gram_gold, grid, dim3(64), 0, ctx->stream, FGold{},
                        (const uint64_t*)s_ntt, r, n, chunk, (uint64_t*)partial);
     GH_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_finalize<FGold>), dim3(npairs), dim3(1024), 0, ctx->stream, FGold{},
+    hipLaunchKernelGGL((k_finalize4<FGold>), dim3(npairs), dim3(256), 0, ctx->stream, FGold{},
                        (const uint64_t*)ring->dev_tab, (const uint64_t*)partial, nchunks, npairs,
                        (int)kTabUntwist, (uint64_t*)g_dev);
   }
@@ -442,14 +442,14 @@ int launch_garbage_h(gh_ring* ring, const void* phi, const void* s, uint32_t r, 
     hipLaunchKernelGGL(k_garbage_q32, grid, dim3(64), 0, ctx->stream, f, (const uint32_t*)pn,
                        (const uint32_t*)sn, r, n, chunk, (uint32_t*)partial);
     GH_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_finalize<FQ32>), dim3(npairs), dim3(1024), 0, ctx->stream, f,
+    hipLaunchKernelGGL((k_finalize4<FQ32>), dim3(npairs), dim3(256), 0, ctx->stream, f,
                        (const uint32_t*)ring->dev_tab, (const uint32_t*)partial, nchunks, npairs,
                        (int)kTabUntwistMacHalf, (uint32_t*)h_dev);
   } else {
     hipLaunchKernelGGL(k_garbage_gold, grid, dim3(64), 0, ctx->stream, FGold{},
                        (const uint64_t*)pn, (const uint64_t*)sn, r, n, chunk, (uint64_t*)partial);
     GH_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_finalize<FGold>), dim3(npairs), dim3(1024), 0, ctx->stream, FGold{},
+    hipLaunchKernelGGL((k_finalize4<FGold>), dim3(npairs), dim3(256), 0, ctx->stream, FGold{},
                        (const uint64_t*)ring->dev_tab, (const uint64_t*)partial, nchunks, npairs,
                        (int)kTabUntwistMacHalf, (uint64_t*)h_dev);
   }
