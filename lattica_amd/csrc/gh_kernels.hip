@@ -935,8 +935,12 @@ static int launch_finalize_q32(gh_ctx* ctx, FQ32 f, const uint32_t* tab, const u
                                uint32_t nparts, uint32_t rows, uint32_t* t_dev) {
   hipEvent_t f0, f1;
   prof_pair(ctx, 2, &f0, &f1);
-  hipExtLaunchKernelGGL((k_finalize<FQ32>), dim3(rows), dim3(1024), 0, ctx->stream, f0, f1, 0, f,
-                        tab, P, nparts, rows, (int)kTabUntwistMac, t_dev);
+  if (rows > (uint32_t)ctx->num_cus)  // 4-wave blocks: all rows in one round
+    hipExtLaunchKernelGGL((k_finalize4<FQ32>), dim3(rows), dim3(256), 0, ctx->stream, f0, f1, 0,
+                          f, tab, P, nparts, rows, (int)kTabUntwistMac, t_dev);
+  else
+    hipExtLaunchKernelGGL((k_finalize<FQ32>), dim3(rows), dim3(1024), 0, ctx->stream, f0, f1, 0,
+                          f, tab, P, nparts, rows, (int)kTabUntwistMac, t_dev);
   GH_HIP(hipGetLastError());
   return GH_OK;
 }
@@ -1019,8 +1023,12 @@ static int launch_finalize_gold(gh_ctx* ctx, const uint64_t* tab, const uint64_t
                                 uint32_t nparts, uint32_t rows, uint64_t* t_dev) {
   hipEvent_t f0, f1;
   prof_pair(ctx, 2, &f0, &f1);
-  hipExtLaunchKernelGGL((k_finalize<FGold>), dim3(rows), dim3(1024), 0, ctx->stream, f0, f1, 0,
-                        FGold{}, tab, P, nparts, rows, (int)kTabUntwist, t_dev);
+  if (rows > (uint32_t)ctx->num_cus)  // 4-wave blocks: all rows in one round
+    hipExtLaunchKernelGGL((k_finalize4<FGold>), dim3(rows), dim3(256), 0, ctx->stream, f0, f1, 0,
+                          FGold{}, tab, P, nparts, rows, (int)kTabUntwist, t_dev);
+  else
+    hipExtLaunchKernelGGL((k_finalize<FGold>), dim3(rows), dim3(1024), 0, ctx->stream, f0, f1, 0,
+                          FGold{}, tab, P, nparts, rows, (int)kTabUntwist, t_dev);
   GH_HIP(hipGetLastError());
   return GH_OK;
 }
@@ -1154,15 +1162,29 @@ int launch_commit(gh_key* key, const void* s_dev, uint32_t cols, int domain, voi
   }
   if (st) return st;
   hipEvent_t e2 = prof_mark(ctx);
-  if (ring->kind == GH_RING_Q32)
-    hipLaunchKernelGGL((k_finalize<FQ32>), dim3(rows), dim3(1024), 0, ctx->stream,
-                       FQ32{(uint32_t)ring->q, ring->qinv32}, (const uint32_t*)ring->dev_tab,
-                       (const uint32_t*)partial, nparts, rows, (int)kTabUntwistMac,
-                       (uint32_t*)t_dev);
-  else
-    hipLaunchKernelGGL((k_finalize<FGold>), dim3(rows), dim3(1024), 0, ctx->stream, FGold{},
-                       (const uint64_t*)ring->dev_tab, (const uint64_t*)partial, nparts, rows,
-                       (int)kTabUntwist, (uint64_t*)t_dev);
+  // a 16-wave block per row fills a CU: beyond one row per CU (amortization's
+  // n rows), 4-wave blocks finish the rows in one round
+  const bool many = rows > (uint32_t)ctx->num_cus;
+  if (ring->kind == GH_RING_Q32) {
+    const FQ32 f{(uint32_t)ring->q, ring->qinv32};
+    if (many)
+      hipLaunchKernelGGL((k_finalize4<FQ32>), dim3(rows), dim3(256), 0, ctx->stream, f,
+                         (const uint32_t*)ring->dev_tab, (const uint32_t*)partial, nparts, rows,
+                         (int)kTabUntwistMac, (uint32_t*)t_dev);
+    else
+      hipLaunchKernelGGL((k_finalize<FQ32>), dim3(rows), dim3(1024), 0, ctx->stream, f,
+                         (const uint32_t*)ring->dev_tab, (const uint32_t*)partial, nparts, rows,
+                         (int)kTabUntwistMac, (uint32_t*)t_dev);
+  } else {
+    if (many)
+      hipLaunchKernelGGL((k_finalize4<FGold>), dim3(rows), dim3(256), 0, ctx->stream, FGold{},
+                         (const uint64_t*)ring->dev_tab, (const uint64_t*)partial, nparts, rows,
+                         (int)kTabUntwist, (uint64_t*)t_dev);
+    else
+      hipLaunchKernelGGL((k_finalize<FGold>), dim3(rows), dim3(1024), 0, ctx->stream, FGold{},
+                         (const uint64_t*)ring->dev_tab, (const uint64_t*)partial, nparts, rows,
+                         (int)kTabUntwist, (uint64_t*)t_dev);
+  }
   GH_HIP(hipGetLastError());
   hipEvent_t e3 = prof_mark(ctx);
   prof_span(ctx, 0, e0, e1);
