@@ -273,34 +273,39 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch_gold(
 }
 
 // grid (count, rows): item j's row `row` = INTT(sum over its chunks)
+// Waves per (item, row) block of k_finalize_batch: batches have thousands of
+// rows, and a 16-wave block would fill a CU on its own (one round of 256
+// blocks at a time); 4-wave blocks let several run per CU.
+constexpr int kFinBatchWaves = 4;
+
 template <class F>
-__global__ __launch_bounds__(1024) void k_finalize_batch(F f, const typename F::W* __restrict__ tab,
-                                                         const BatchDesc* __restrict__ desc,
-                                                         uint32_t rows, uint32_t chunk_cols,
-                                                         int post) {
+__global__ __launch_bounds__(64 * kFinBatchWaves) void k_finalize_batch(
+    F f, const typename F::W* __restrict__ tab, const BatchDesc* __restrict__ desc, uint32_t rows,
+    uint32_t chunk_cols, int post) {
   using W = typename F::W;
+  constexpr int G = kFinBatchWaves;
   const int k = threadIdx.x & 63, g = threadIdx.x >> 6;
   const uint32_t j = blockIdx.x, row = blockIdx.y;
   const uint32_t nch = (desc[j].len + chunk_cols - 1) / chunk_cols;
   const W* partial = reinterpret_cast<const W*>(desc[j].partial);
   W acc = 0;
-  for (uint32_t c0 = g; c0 < nch; c0 += 16 * 8) {
+  for (uint32_t c0 = g; c0 < nch; c0 += G * 8) {
     W v[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) {
-      const uint32_t c = c0 + 16 * u;
+      const uint32_t c = c0 + G * u;
       v[u] = c < nch ? partial[((size_t)c * rows + row) * 64 + k] : W(0);
     }
 #pragma unroll
     for (int u = 0; u < 8; u++) acc = f.add(acc, v[u]);
   }
-  __shared__ W red[16][64];
+  __shared__ W red[G][64];
   red[g][k] = acc;
   __syncthreads();
   if (g == 0) {
     W s = red[0][k];
 #pragma unroll
-    for (int i = 1; i < 16; i++) s = f.add(s, red[i][k]);
+    for (int i = 1; i < G; i++) s = f.add(s, red[i][k]);
     reinterpret_cast<W*>(desc[j].dst)[(size_t)row * 64 + k] = wave_inv64(f, s, tab, post, k);
   }
 }
@@ -373,11 +378,11 @@ int enqueue_batch_finalize(gh_ring* ring, hipStream_t stream, const BatchDesc* d
                            uint32_t count, uint32_t rows, uint32_t chunk_cols) {
   if (count == 0) return GH_OK;
   if (ring->kind == GH_RING_Q32)
-    hipLaunchKernelGGL((k_finalize_batch<FQ32>), dim3(count, rows), dim3(1024), 0, stream,
+    hipLaunchKernelGGL((k_finalize_batch<FQ32>), dim3(count, rows), dim3(64 * kFinBatchWaves), 0, stream,
                        FQ32{(uint32_t)ring->q, ring->qinv32}, (const uint32_t*)ring->dev_tab, desc,
                        rows, chunk_cols, (int)kTabUntwistMac);
   else
-    hipLaunchKernelGGL((k_finalize_batch<FGold>), dim3(count, rows), dim3(1024), 0, stream,
+    hipLaunchKernelGGL((k_finalize_batch<FGold>), dim3(count, rows), dim3(64 * kFinBatchWaves), 0, stream,
                        FGold{}, (const uint64_t*)ring->dev_tab, desc, rows, chunk_cols,
                        (int)kTabUntwist);
   GH_HIP(hipGetLastError());

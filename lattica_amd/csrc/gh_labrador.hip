@@ -159,15 +159,18 @@ __global__ __launch_bounds__(64) void k_gram_batch_gold(FGold f, const GramDesc*
                  reinterpret_cast<uint64_t*>(d.partial));
 }
 
+// one 4-wave block per Gram entry (a batch has hundreds or thousands; 16-wave
+// blocks would run one per CU at a time)
 template <class F>
-__global__ __launch_bounds__(1024) void k_gram_fin_batch(F f, const typename F::W* __restrict__ tab,
-                                                         const GramDesc* __restrict__ desc,
-                                                         const GramFin* __restrict__ fin, int post) {
+__global__ __launch_bounds__(256) void k_gram_fin_batch(F f, const typename F::W* __restrict__ tab,
+                                                        const GramDesc* __restrict__ desc,
+                                                        const GramFin* __restrict__ fin, int post) {
   using W = typename F::W;
+  __shared__ W red[4 * 64];
   const GramFin e = fin[blockIdx.x];
   const GramDesc d = desc[e.g];
-  finalize_row(f, tab, reinterpret_cast<const W*>(d.partial), d.nchunks, d.r * (d.r + 1) / 2,
-               e.pair, post, reinterpret_cast<W*>(d.dst));
+  finalize_row_g<F, 4>(f, tab, reinterpret_cast<const W*>(d.partial), d.nchunks,
+                       d.r * (d.r + 1) / 2, e.pair, post, reinterpret_cast<W*>(d.dst), red);
 }
 
 uint32_t gram_tiles(uint32_t r) {
@@ -192,12 +195,12 @@ int enqueue_gram_batch(gh_ring* ring, hipStream_t stream, const GramDesc* desc,
     FQ32 f{(uint32_t)ring->q, ring->qinv32};
     hipLaunchKernelGGL(k_gram_batch_q32, dim3(nwork), dim3(64), 0, stream, f, desc, work);
     GH_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_gram_fin_batch<FQ32>), dim3(nfin), dim3(1024), 0, stream, f,
+    hipLaunchKernelGGL((k_gram_fin_batch<FQ32>), dim3(nfin), dim3(256), 0, stream, f,
                        (const uint32_t*)ring->dev_tab, desc, fin, (int)kTabUntwistMac);
   } else {
     hipLaunchKernelGGL(k_gram_batch_gold, dim3(nwork), dim3(64), 0, stream, FGold{}, desc, work);
     GH_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_gram_fin_batch<FGold>), dim3(nfin), dim3(1024), 0, stream, FGold{},
+    hipLaunchKernelGGL((k_gram_fin_batch<FGold>), dim3(nfin), dim3(256), 0, stream, FGold{},
                        (const uint64_t*)ring->dev_tab, desc, fin, (int)kTabUntwist);
   }
   GH_HIP(hipGetLastError());
