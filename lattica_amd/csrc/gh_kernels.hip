@@ -945,6 +945,29 @@ static int launch_finalize_q32(gh_ctx* ctx, FQ32 f, const uint32_t* tab, const u
   return GH_OK;
 }
 
+// Where a fused commitment's partial sums (pbytes) go, for a launch over ring
+// kind `kind`: beside a pending finalize of the same kind when it is deferring
+// and the workspace has room (from offset 0 when they fit before it), else
+// after issuing that finalize (and any pending one of the other kind) from
+// offset 0, with room for the next deferring commitment too. On return
+// ctx->fin.active says whether the pending finalize runs inside this launch.
+static int fused_partial_area(gh_ctx* ctx, int kind, size_t pbytes, bool defer, size_t* off) {
+  PendingFin& pf = ctx->fin;
+  *off = 0;
+  if (pf.active && pf.kind != kind)
+    if (int st = ctx_flush(ctx)) return st;
+  if (pf.active) {
+    if (defer && pbytes <= pf.off) return GH_OK;
+    if (defer && pf.off + pf.bytes + pbytes <= ctx->work_bytes) {
+      *off = pf.off + pf.bytes;
+      return GH_OK;
+    }
+    if (int st = ctx_flush(ctx)) return st;
+  }
+  void* ws;
+  return ctx_workspace(ctx, (defer ? 2 : 1) * pbytes, &ws);
+}
+
 // q32 commitment: the fused transform+MAC kernel, then the finalize. With
 // `defer` the finalize is left pending and runs as the first blocks of the
 // next deferred commitment's MAC launch, so back-to-back commitments cost one
@@ -975,24 +998,13 @@ static int launch_commit_fused(gh_key* key, const void* s_dev, uint32_t cols, vo
   // this commitment's partial sums: beside a pending finalize's, in the
   // workspace as it is (growing it would free them), else from offset 0
   PendingFin& pf = ctx->fin;
-  size_t off = 0;
-  if (pf.active && pf.kind != GH_RING_Q32)
-    if (int st = ctx_flush(ctx)) return st;
-  if (pf.active) {
-    if (defer && pbytes <= pf.off) off = 0;
-    else if (defer && pf.off + pf.bytes + pbytes <= ctx->work_bytes) off = pf.off + pf.bytes;
-    else if (int st = ctx_flush(ctx)) return st;
-  }
+  size_t off;
+  if (int st = fused_partial_area(ctx, GH_RING_Q32, pbytes, defer, &off)) return st;
   FinArgs fin{};
-  if (pf.active) {
+  if (pf.active)
     fin = FinArgs{FQ32{pf.q, pf.qinv}, (const uint32_t*)pf.tab,
                   (const uint32_t*)((const char*)ctx->work + pf.off), (uint32_t*)pf.t_out,
                   pf.nparts, pf.rows, pf.rows};
-  } else {
-    void* ws;
-    int st = ctx_workspace(ctx, (defer ? 2 : 1) * pbytes, &ws);  // room for the next one too
-    if (st) return st;
-  }
 
   FQ32 f{(uint32_t)ring->q, ring->qinv32};
   const uint32_t* tab = (const uint32_t*)ring->dev_tab;
@@ -1056,24 +1068,13 @@ static int launch_commit_fused_gold(gh_key* key, const void* s_dev, uint32_t col
   const size_t pbytes = ((size_t)nparts * rows * 64 * 8 + 255) & ~(size_t)255;
 
   PendingFin& pf = ctx->fin;
-  size_t off = 0;
-  if (pf.active && pf.kind != GH_RING_GOLDILOCKS)
-    if (int st = ctx_flush(ctx)) return st;
-  if (pf.active) {
-    if (defer && pbytes <= pf.off) off = 0;
-    else if (defer && pf.off + pf.bytes + pbytes <= ctx->work_bytes) off = pf.off + pf.bytes;
-    else if (int st = ctx_flush(ctx)) return st;
-  }
+  size_t off;
+  if (int st = fused_partial_area(ctx, GH_RING_GOLDILOCKS, pbytes, defer, &off)) return st;
   FinArgsGold fin{};
-  if (pf.active) {
+  if (pf.active)
     fin = FinArgsGold{(const uint64_t*)pf.tab,
                       (const uint64_t*)((const char*)ctx->work + pf.off), (uint64_t*)pf.t_out,
                       pf.nparts, pf.rows, pf.rows};
-  } else {
-    void* ws;
-    int st = ctx_workspace(ctx, (defer ? 2 : 1) * pbytes, &ws);  // room for the next one too
-    if (st) return st;
-  }
   const uint64_t* tab = (const uint64_t*)ring->dev_tab;
   const uint64_t* A = (const uint64_t*)key->dev_a;
   const uint64_t* S = (const uint64_t*)s_dev;
