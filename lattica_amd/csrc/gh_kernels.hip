@@ -592,9 +592,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_gold(
     for (int v = 0; v < 16; v++) dp[v] = x[v];
   }
   __syncthreads();
-  uint64_t acc[RB];
-#pragma unroll
-  for (int r = 0; r < RB; r++) acc[r] = 0;
+  GoldAcc acc[RB];  // exact lazy sums, reduced once per row at the end
   const size_t col_stride = (size_t)rows * 64;
   const uint64_t* akey = A + (size_t)row0 * 64 + lane;
   // this wave's columns wave + j * WAVES, j < nw; the next column's key loads
@@ -610,7 +608,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_gold(
   auto mac_col = [&](const uint64_t (&v)[RB], uint32_t j) {
     const uint64_t s0 = lds64[(wave + j * WAVES) * kFuseStrideGold + lane];
 #pragma unroll
-    for (int r = 0; r < RB; r++) acc[r] = f.add(acc[r], f.mul(v[r], s0));
+    for (int r = 0; r < RB; r++) gold_mac(acc[r], v[r], s0);
   };
   uint64_t va[RB], vb[RB];
   if (nw > 0) load_col(va, 0);
@@ -624,7 +622,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_gold(
   __syncthreads();  // the witness copy is no longer read
   uint64_t* red = lds64;  // [WAVES][RB][64]
 #pragma unroll
-  for (int r = 0; r < RB; r++) red[(wave * RB + r) * 64 + lane] = acc[r];
+  for (int r = 0; r < RB; r++) red[(wave * RB + r) * 64 + lane] = gold_reduce(f, acc[r]);
   __syncthreads();
   for (int o = threadIdx.x; o < RB * 64; o += 64 * WAVES) {
     uint64_t a = red[o];
@@ -1054,10 +1052,15 @@ static int launch_commit_fused_gold(gh_key* key, const void* s_dev, uint32_t col
   gh_ctx* ctx = ring->ctx;
   const uint32_t rows = key->rows;
   constexpr int kWaves = 4;
-  const int rb = rows % 16 == 0 ? 16 : rows % 8 == 0 ? 8 : 4;
-  // blocks: the tuned count per CU, at least one per kFuseChunkGold columns,
-  // at most one per wave's column; every block owns at least one column
-  uint32_t nparts = (uint32_t)(ctx->num_cus * ctx->tune.mac_blocks_per_cu);
+  // 8-row blocks at most: a block's lazy sums take 9 registers per row, and
+  // two row blocks transforming the same witness columns cost less than
+  // 16-row blocks at two waves per SIMD
+  const int rb = rows % 8 == 0 ? 8 : 4;
+  // blocks: the tuned count per CU over all row blocks, at least one per
+  // kFuseChunkGold columns, at most one per wave's column; every block owns at
+  // least one column
+  uint32_t nparts = (uint32_t)(ctx->num_cus * ctx->tune.mac_blocks_per_cu) / (rows / rb);
+  if (nparts == 0) nparts = 1;
   const uint32_t min_parts = (cols + kFuseChunkGold - 1) / kFuseChunkGold;
   if (nparts < min_parts) nparts = min_parts;
   const uint32_t max_parts = (cols + kWaves - 1) / kWaves;
@@ -1088,8 +1091,7 @@ static int launch_commit_fused_gold(gh_key* key, const void* s_dev, uint32_t col
 #define GH_FUSED_GOLD(R)                                                                        \
   hipExtLaunchKernelGGL((k_commit_fused_gold<R, kWaves>), grid, dim3(64 * kWaves), lds_bytes,   \
                         ctx->stream, e0, e1, 0, FGold{}, tab, A, S, P, rows, cols, cpb, fin)
-  if (rb == 16) GH_FUSED_GOLD(16);
-  else if (rb == 8) GH_FUSED_GOLD(8);
+  if (rb == 8) GH_FUSED_GOLD(8);
   else GH_FUSED_GOLD(4);
 #undef GH_FUSED_GOLD
   GH_HIP(hipGetLastError());
