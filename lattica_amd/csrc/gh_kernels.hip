@@ -543,10 +543,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_q32(
 
 // Fused witness transform + MAC (Goldilocks), the per-call form: a block
 // transforms its contiguous range of at most kFuseChunkGold witness
-// polynomials into LDS (four lanes per polynomial), then each wave runs the
-// modular multiply-add over its columns (wave, wave + WAVES, ...) against
-// that copy, two columns of key loads in flight, and the waves' sums meet in
-// the same LDS. The first fin.nblk blocks run the previous commitment's
+// polynomials into LDS (four lanes per polynomial), then each wave sums its
+// columns' (wave, wave + WAVES, ...) products against that copy exactly and
+// lazily (GoldAcc, one reduction per row at the end), two columns of key loads
+// in flight, and the waves' sums meet in the same LDS. The first fin.nblk blocks run the previous commitment's
 // finalize instead. Results equal k_mac_gold on a separately transformed
 // witness.
 constexpr int kFuseChunkGold = 64;
