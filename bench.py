@@ -722,10 +722,6 @@ def main():
         key = keys[i % len(keys) if key_idx is None else key_idx]
         key.commit_device(s_dev[i % nw], N, t_dev[i % nw])
 
-    for i in range(args.warmup):
-        step(i)
-    ctx.synchronize()
-
     if not args.no_check:
         from oracle import gridref
         step(0, 0)
@@ -738,6 +734,11 @@ def main():
             log("PARITY FAILURE: GPU commitment differs from the oracle")
             sys.exit(1)
         log("parity: commitment 0 bit-exact vs oracle")
+
+    # warm-up steps directly before the timed ones
+    for i in range(args.warmup):
+        step(i)
+    ctx.synchronize()
 
     # ---- timed region: plain per-call commitments, keys in rotation ----
     ctx.synchronize()

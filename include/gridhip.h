@@ -30,7 +30,9 @@
  *    gh_ctx_synchronize() returns, and visible to every later call on the
  *    context; a device-memory gh_commit (or single-item gh_commit_batch) may
  *    hold its last step back until that point, so hipDeviceSynchronize()
- *    alone does not finish it.
+ *    alone does not finish it (tuning key "defer_finalize" = 0 turns this
+ *    off). A later commitment whose witness is that held-back output (a
+ *    commitment of a commitment) completes it first.
  *  - Return value: GH_OK (0) or a negative gh_status. Errors that the
  *    reference reports map 1:1 onto NttError (arith/ntt.rs:40-49) and
  *    CommitmentError (commit/src/error.rs:7-18).
@@ -117,7 +119,13 @@ GH_API int gh_ctx_profile_read(gh_ctx *ctx, int phase, double *total_ms, uint64_
  * "batch_chunk_cols" and "batch_chunk_cols_gold" (16..4096, key columns per
  * block of gh_commit_batch and of the server's waves, for q32 and Goldilocks
  * keys; halved for batches too small to fill the device), "batch_group" (4, 8,
- * 16: witnesses that share each loaded key column). Values are checked;
+ * 16: witnesses that share each loaded key column), "defer_finalize" (1,
+ * the default: a device-memory gh_commit or single-item gh_commit_batch
+ * leaves its last step to the next call on the context, so consecutive
+ * commitments cost one launch each; 0: that step is enqueued before the call
+ * returns, so the result is complete in stream order and a plain
+ * hipStreamSynchronize or hipDeviceSynchronize finishes it; setting 0 also
+ * enqueues a step held back until then). Values are checked;
  * an unknown key or value gives GH_E_INVALID_ARGUMENT. Every setting yields
  * identical results; only the speed changes. */
 GH_API int gh_ctx_set_tuning(gh_ctx *ctx, const char *key, int64_t value);
@@ -311,12 +319,31 @@ GH_API int gh_amortize(gh_ring *ring, const void *c, const void *s, uint32_t r, 
  * parts: num_parts x cols residues (col = poly*d + coeff).
  * p_modq: rows ring words, the reference's p = sum mod q.
  * p_exact (nullable): rows int64, the same sum over centred lifts
- *   (x > q/2 -> x - q), exact in 64-bit integers.
+ *   (x > q/2 -> x - q), exact in 64-bit integers; q < 2^32 only, and fewer
+ *   than 2^26 polynomials (num_parts * cols / d), so that
+ *   |p| <= total * d * (q - 1) / 2 < 2^63 (GH_E_INVALID_ARGUMENT otherwise).
  * flat (nullable): rows*num_parts*cols int8 entries in the reference's flat
  *   layout [m][part][coeff][poly] (jl.rs:262-264). */
 GH_API int gh_jl_project(gh_ring *ring, const uint8_t seed[32], uint32_t rows, const void *parts,
                          uint32_t num_parts, uint64_t cols, void *p_modq, int64_t *p_exact,
                          int8_t *flat, int mem);
+
+/* Launch shape of the int8 matrix-core JL kernels for `total_polys` witness
+ * polynomials on a device of `num_cus` compute units (host arithmetic only;
+ * no device or context needed). Their int32 sums are exact: every product is
+ * a JL entry in {-1, 0, 1} times a signed base-256 digit in [-128, 127], and
+ * each int32 sum takes at most `terms` of them, so |sum| <= bound = terms *
+ * 128 <= 2^31 - 1. kernel: GH_JL_KERNEL_PROJECT (gh_jl_project at 253..256
+ * rows; per_unit = polynomials per wave, four waves per block) or
+ * GH_JL_KERNEL_EVALS (gh_jl_evals; per_unit = polynomials per block; the
+ * witness runs in `chunks` launches of at most chunk_polys polynomials,
+ * `blocks` blocks in the first). total_polys <= 2^36. No reference
+ * counterpart: it states the bound the matrix-core path relies on. */
+enum { GH_JL_KERNEL_PROJECT = 0, GH_JL_KERNEL_EVALS = 1 };
+typedef struct gh_jl_shape {
+  uint64_t chunk_polys, per_unit, blocks, chunks, terms, bound;
+} gh_jl_shape;
+GH_API int gh_jl_launch_shape(int kernel, uint64_t total_polys, int num_cus, gh_jl_shape *out);
 
 /* ---- JL rows for aggregation --------------------------------------------------
  * gh_jl_extract_flat: JLMatrix::extract_jl_rows_flat (labrador/src/jl.rs:311-383):

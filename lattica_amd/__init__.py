@@ -57,7 +57,7 @@ EXPORTS = (
     "gh_batch_plan_destroy", "gh_bdlop_commit", "gh_decompose_balanced", "gh_garbage_h",
     "gh_amortize", "gh_wire_size", "gh_wire_encode", "gh_wire_peek", "gh_wire_decode",
     "gh_key_save", "gh_key_load", "gh_jl_extract_flat", "gh_jl_conjugate", "gh_jl_evals",
-    "gh_jl_aggregate",
+    "gh_jl_aggregate", "gh_jl_launch_shape",
 )
 
 
@@ -95,6 +95,23 @@ class Request(ctypes.Structure):
                 ("g_out", ctypes.c_void_p), ("wire", ctypes.c_uint32),
                 ("reserved", ctypes.c_uint32), ("witness_len", ctypes.c_uint64),
                 ("t_cap", ctypes.c_uint64), ("g_cap", ctypes.c_uint64)]
+
+
+class JlShape(ctypes.Structure):
+    _fields_ = [("chunk_polys", ctypes.c_uint64), ("per_unit", ctypes.c_uint64),
+                ("blocks", ctypes.c_uint64), ("chunks", ctypes.c_uint64),
+                ("terms", ctypes.c_uint64), ("bound", ctypes.c_uint64)]
+
+
+JL_KERNEL_PROJECT, JL_KERNEL_EVALS = 0, 1
+
+
+def jl_launch_shape(kernel: int, total_polys: int, num_cus: int) -> dict:
+    """Launch shape and int32 bound of a matrix-core JL kernel (host arithmetic)."""
+    s = JlShape()
+    _check(lib().gh_jl_launch_shape(kernel, total_polys, num_cus, ctypes.byref(s)),
+           "gh_jl_launch_shape")
+    return {k: getattr(s, k) for k, _ in JlShape._fields_}
 
 
 class ServerStats(ctypes.Structure):
@@ -186,6 +203,7 @@ def lib():
         "gh_jl_conjugate": (i32, [vp, vp, vp, u32, u32, u64, vp, i32]),
         "gh_jl_evals": (i32, [vp, vp, u32, vp, u32, u64, vp, i32]),
         "gh_jl_aggregate": (i32, [vp, vp, u32, vp, u32, u64, vp, i32]),
+        "gh_jl_launch_shape": (i32, [i32, u64, i32, ctypes.POINTER(JlShape)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -245,7 +263,7 @@ class Context:
         _check(lib().gh_ctx_synchronize(self.h), "gh_ctx_synchronize")
 
     TUNING_KEYS = ("mac_variant", "mac_waves", "mac_blocks_per_cu", "quad_max", "fused", "fused_depth",
-                   "batch_chunk_cols", "batch_chunk_cols_gold", "batch_group")
+                   "batch_chunk_cols", "batch_chunk_cols_gold", "batch_group", "defer_finalize")
 
     def set_tuning(self, **kv):
         for k, v in kv.items():

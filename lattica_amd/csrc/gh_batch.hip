@@ -76,7 +76,7 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch_q32(
   const uint32_t row0 = blockIdx.z * RB + 4 * wave;
   const uint32_t c0 = ch * chunk_cols;
   const uint32_t maxlen = desc[g0].len;  // sorted: the group's first item is its longest
-  if (c0 >= maxlen) return;
+  if (c0 >= maxlen || row0 >= rows) return;  // wave-uniform; the last row block may be ragged
   const uint32_t c1 = c0 + chunk_cols < maxlen ? c0 + chunk_cols : maxlen;
   uint32_t len[G];
   const uint32_t* sp[G];
@@ -94,10 +94,15 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch_q32(
     for (int r = 0; r < 4; r++) { lo[i][r] = 0; hi[i][r] = 0; }
   const size_t col_stride = (size_t)rows * 64;
   const uint32_t* ap = A + (size_t)row0 * 64 + lane;
+  // rows past the key's last (a row count not a multiple of 4) load its last
+  // row and are not stored
+  uint32_t roff[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) roff[r] = (row0 + r < rows ? r : rows - 1 - row0) * 64;
   for (uint32_t c = c0; c < c1; c++) {
     uint32_t a[4];
 #pragma unroll
-    for (int r = 0; r < 4; r++) a[r] = ap[(size_t)c * col_stride + r * 64];
+    for (int r = 0; r < 4; r++) a[r] = ap[(size_t)c * col_stride + roff[r]];
 #pragma unroll
     for (int i = 0; i < G; i++) {
       if (c < len[i]) {  // uniform: lengths are block-wide values
@@ -114,8 +119,9 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch_q32(
       uint32_t* pp = reinterpret_cast<uint32_t*>(desc[j].partial);
 #pragma unroll
       for (int r = 0; r < 4; r++)
-        pp[((size_t)ch * rows + row0 + r) * 64 + lane] =
-            f.reduce96_r2((uint32_t)lo[i][r], (uint32_t)(lo[i][r] >> 32), hi[i][r]);
+        if (row0 + r < rows)
+          pp[((size_t)ch * rows + row0 + r) * 64 + lane] =
+              f.reduce96_r2((uint32_t)lo[i][r], (uint32_t)(lo[i][r] >> 32), hi[i][r]);
     }
   }
 }
@@ -135,7 +141,7 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch4_q32(
   const uint32_t row_w = blockIdx.z * RB + 4 * wave;  // this wave's 4 rows
   const uint32_t c0 = ch * chunk_cols;
   const uint32_t maxlen = desc[g0].len;
-  if (c0 >= maxlen) return;
+  if (c0 >= maxlen || row_w >= rows) return;  // wave-uniform; the last row block may be ragged
   const uint32_t c1 = c0 + chunk_cols < maxlen ? c0 + chunk_cols : maxlen;
   uint32_t len[G];
   const u32x4* sp[G];
@@ -147,7 +153,11 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch4_q32(
         reinterpret_cast<const uint32_t*>(desc[j < count ? j : g0].s_ntt) + k0);
   }
   const size_t col_stride4 = (size_t)rows * 16;  // u32x4 per column
-  const u32x4* ap = reinterpret_cast<const u32x4*>(A + (size_t)row_w * 64) + lane;
+  // a lane past the key's last row (a row count not a multiple of 4) loads
+  // that row and stores nothing
+  const uint32_t row = row_w + (lane >> 4);
+  const uint32_t row_ld = row < rows ? row : rows - 1;
+  const u32x4* ap = reinterpret_cast<const u32x4*>(A + (size_t)row_ld * 64) + (lane & 15);
   uint64_t lo[G][4];
   uint32_t hi[G][4];
 #pragma unroll
@@ -179,11 +189,10 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch4_q32(
       }
     }
   }
-  const uint32_t row = row_w + (lane >> 4);
 #pragma unroll
   for (int i = 0; i < G; i++) {
     const uint32_t j = g0 + i;
-    if (j < count && c0 < len[i]) {
+    if (j < count && c0 < len[i] && row < rows) {
       u32x4 o;
       o.x = f.reduce96_r2((uint32_t)lo[i][0], (uint32_t)(lo[i][0] >> 32), hi[i][0]);
       o.y = f.reduce96_r2((uint32_t)lo[i][1], (uint32_t)(lo[i][1] >> 32), hi[i][1]);
@@ -205,7 +214,7 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch_gold(
   const uint32_t row0 = blockIdx.z * RB + 4 * wave;
   const uint32_t c0 = ch * chunk_cols;
   const uint32_t maxlen = desc[g0].len;
-  if (c0 >= maxlen) return;
+  if (c0 >= maxlen || row0 >= rows) return;  // wave-uniform; the last row block may be ragged
   const uint32_t c1 = c0 + chunk_cols < maxlen ? c0 + chunk_cols : maxlen;
   uint32_t len[G];
   const uint64_t* sp[G];
@@ -227,10 +236,14 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch_gold(
     for (int r = 0; r < 4; r++) acc[i][r] = 0;
   const size_t col_stride = (size_t)rows * 64;
   const uint64_t* ap = A + (size_t)row0 * 64 + lane;
+  // rows past the key's last load its last row and are not stored
+  uint32_t roff[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) roff[r] = (row0 + r < rows ? r : rows - 1 - row0) * 64;
   // one-column software pipeline: the next column's loads precede this one's math
   uint64_t an[4], sn[G];
 #pragma unroll
-  for (int r = 0; r < 4; r++) an[r] = ap[(size_t)c0 * col_stride + r * 64];
+  for (int r = 0; r < 4; r++) an[r] = ap[(size_t)c0 * col_stride + roff[r]];
 #pragma unroll
   for (int i = 0; i < G; i++) sn[i] = c0 < len[i] ? sp[i][(size_t)c0 * 64] : 0;
   for (uint32_t c = c0; c < c1; c++) {
@@ -241,7 +254,7 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch_gold(
     for (int i = 0; i < G; i++) s[i] = sn[i];
     if (c + 1 < c1) {
 #pragma unroll
-      for (int r = 0; r < 4; r++) an[r] = ap[(size_t)(c + 1) * col_stride + r * 64];
+      for (int r = 0; r < 4; r++) an[r] = ap[(size_t)(c + 1) * col_stride + roff[r]];
 #pragma unroll
       for (int i = 0; i < G; i++) sn[i] = c + 1 < len[i] ? sp[i][(size_t)(c + 1) * 64] : 0;
     }
@@ -263,6 +276,7 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch_gold(
       uint64_t* pp = reinterpret_cast<uint64_t*>(desc[j].partial);
 #pragma unroll
       for (int r = 0; r < 4; r++) {
+        if (row0 + r >= rows) continue;
         uint64_t v;
         if constexpr (kLazy) v = gold_reduce(f, lz[i][r]);
         else v = acc[i][r];
@@ -332,7 +346,7 @@ template <int G, int RB>
 static void mac_batch_launch(gh_key* key, hipStream_t stream, const BatchDesc* desc,
                              uint32_t count, uint32_t chunk_cols, uint32_t nchunks, int variant) {
   gh_ring* ring = key->ring;
-  dim3 grid(nchunks, (count + G - 1) / G, key->rows / RB);
+  dim3 grid(nchunks, (count + G - 1) / G, (key->rows + RB - 1) / RB);
   if constexpr (G <= 8) {
     if (ring->kind == GH_RING_Q32 && variant == 1) {
       hipLaunchKernelGGL((k_mac_batch4_q32<G, RB>), grid, dim3(16 * RB), 0, stream,
@@ -365,7 +379,6 @@ static void mac_batch_rb(gh_key* key, hipStream_t stream, const BatchDesc* desc,
 int enqueue_batch_mac(gh_key* key, hipStream_t stream, const BatchDesc* desc, uint32_t count,
                       uint32_t maxlen, uint32_t chunk_cols, int group, int variant) {
   if (count == 0 || maxlen == 0) return GH_OK;
-  if (key->rows % 4 != 0) return GH_E_INVALID_ARGUMENT;
   const uint32_t nchunks = (maxlen + chunk_cols - 1) / chunk_cols;
   if (group == 4) mac_batch_rb<4>(key, stream, desc, count, chunk_cols, nchunks, variant);
   else if (group == 16) mac_batch_rb<16>(key, stream, desc, count, chunk_cols, nchunks, variant);
@@ -460,8 +473,8 @@ int launch_commit_batch(gh_key* key, const gh_commit_item* items, size_t count) 
   gh_ring* ring = key->ring;
   gh_ctx* ctx = ring->ctx;
   if (count == 0) return GH_OK;
-  // one item, or a key the grouped kernels do not tile: the plain path per item
-  if (count == 1 || key->rows % 4 != 0 || count > 0x7fffffff) {
+  // one item: the plain path
+  if (count == 1 || count > 0x7fffffff) {
     for (size_t i = 0; i < count; i++) {
       int st = launch_commit(key, items[i].witness, items[i].len, GH_DOMAIN_COEFF, items[i].t_out);
       if (st) return st;
@@ -516,7 +529,6 @@ GH_API int gh_batch_plan_create(gh_key* key, const gh_commit_item* items, size_t
                                 gh_batch_plan** out) {
   if (!key || !out || !items || count == 0 || count > 0x7fffffff) return GH_E_INVALID_ARGUMENT;
   *out = nullptr;
-  if (key->rows % 4 != 0) return GH_E_INVALID_ARGUMENT;
   const uint32_t cols = key->msg_len + key->open_len;
   for (size_t i = 0; i < count; i++) {
     if (!items[i].witness || !items[i].t_out) return GH_E_INVALID_ARGUMENT;

@@ -5,6 +5,7 @@
 // enqueue. Validation mirrors the reference: dimension checks of
 // commit/src/linear.rs:256-327 and the canonical-value check of
 // PrimeField's Valid impl (algebra/src/arith/prime.rs:723-728).
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -131,7 +132,17 @@ int d2h(gh_ctx* ctx, void* dst, const void* src, size_t bytes) {
   if (bytes) GH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
   return GH_OK;
 }
+// Waits for the context stream: polls first, so a short wait ends within
+// about a microsecond of the work (a blocking wait wakes tens of microseconds
+// late), then blocks in the runtime once the wait has lasted 2 ms.
 int sync(gh_ctx* ctx) {
+  const auto t0 = std::chrono::steady_clock::now();
+  for (;;) {
+    const hipError_t e = hipStreamQuery(ctx->stream);
+    if (e == hipSuccess) return GH_OK;
+    if (e != hipErrorNotReady) return e == hipErrorOutOfMemory ? GH_E_OOM : GH_E_DEVICE;
+    if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) break;
+  }
   GH_HIP(hipStreamSynchronize(ctx->stream));
   return GH_OK;
 }
@@ -291,6 +302,11 @@ GH_API int gh_ctx_set_tuning(gh_ctx* ctx, const char* key, int64_t value) {
   } else if (!std::strcmp(key, "batch_group")) {
     if (value != 4 && value != 8 && value != 16) return GH_E_INVALID_ARGUMENT;
     t.batch_group = (int)value;
+  } else if (!std::strcmp(key, "defer_finalize")) {
+    if (value != 0 && value != 1) return GH_E_INVALID_ARGUMENT;
+    // turning deferral off completes a finalize held back until now
+    if (!value) TRY(set_device(ctx));
+    t.defer_finalize = (int)value;
   } else {
     return GH_E_INVALID_ARGUMENT;
   }
@@ -310,6 +326,7 @@ GH_API int gh_ctx_get_tuning(gh_ctx* ctx, const char* key, int64_t* value) {
   else if (!std::strcmp(key, "batch_chunk_cols")) *value = t.batch_chunk_cols;
   else if (!std::strcmp(key, "batch_chunk_cols_gold")) *value = t.batch_chunk_cols_gold;
   else if (!std::strcmp(key, "batch_group")) *value = t.batch_group;
+  else if (!std::strcmp(key, "defer_finalize")) *value = t.defer_finalize;
   else return GH_E_INVALID_ARGUMENT;
   return GH_OK;
 }
@@ -627,7 +644,7 @@ GH_API int gh_commit(gh_key* key, const void* message, uint32_t msg_len, const v
   // back until the next call on the context (the next such commitment runs it
   // inside its own launch)
   if (mem == GH_MEM_DEVICE && open_len == 0)
-    return launch_commit(key, message, cols, domain, t_out, true);
+    return launch_commit(key, message, cols, domain, t_out, ctx->tune.defer_finalize != 0);
   TRY(ctx_flush(ctx));
   void* dev;
   TRY(ctx_staging(ctx, msg_bytes + open_bytes + t_bytes, &dev));
@@ -859,7 +876,7 @@ GH_API int gh_commit_batch(gh_key* key, const gh_commit_item* items, size_t coun
     // flush here: the launch issues a pending finalize itself)
     GH_HIP(hipSetDevice(ctx->device));
     return launch_commit(key, items[0].witness, items[0].len, GH_DOMAIN_COEFF, items[0].t_out,
-                         true);
+                         ctx->tune.defer_finalize != 0);
   }
   TRY(set_device(ctx));
   const size_t wb = ring->word_bytes, d = ring->d;
@@ -926,6 +943,9 @@ GH_API int gh_jl_project(gh_ring* ring, const uint8_t seed[32], uint32_t rows, c
   if (rows == 0 || rows > 256) return GH_E_INVALID_ARGUMENT;
   if (cols % ring->d != 0) return GH_E_INVALID_ARGUMENT;
   if (p_exact && ring->kind != GH_RING_Q32) return GH_E_INVALID_ARGUMENT;
+  // the int64 centred sum is exact below 2^26 polynomials
+  if (p_exact && (uint64_t)num_parts * (cols / ring->d) >= kJlExactMaxPolys)
+    return GH_E_INVALID_ARGUMENT;
   if (cols > 0xffffffffull / ((rows + 3) / 4)) return GH_E_INVALID_ARGUMENT;  // 32-bit block counter
   if (num_parts > 65535) return GH_E_INVALID_ARGUMENT;  // one grid row per part
   gh_ctx* ctx = ring->ctx;
@@ -960,6 +980,23 @@ GH_API int gh_jl_project(gh_ring* ring, const uint8_t seed[32], uint32_t rows, c
   if (p_exact) TRY(d2h(ctx, p_exact, de, e_bytes));
   if (flat) TRY(d2h(ctx, flat, df, f_bytes));
   return sync(ctx);
+}
+
+GH_API int gh_jl_launch_shape(int kernel, uint64_t total_polys, int num_cus,
+                              gh_jl_shape* out) {
+  if (!out || num_cus < 1) return GH_E_INVALID_ARGUMENT;
+  if (total_polys > kJlMaxPolys) return GH_E_INVALID_ARGUMENT;
+  JlShape s;
+  if (kernel == GH_JL_KERNEL_PROJECT) jl_proj_shape(total_polys, num_cus, &s);
+  else if (kernel == GH_JL_KERNEL_EVALS) jl_evals_shape(total_polys, num_cus, &s);
+  else return GH_E_INVALID_ARGUMENT;
+  out->chunk_polys = s.chunk_polys;
+  out->per_unit = s.per_unit;
+  out->blocks = s.blocks;
+  out->chunks = s.chunks;
+  out->terms = s.terms;
+  out->bound = s.bound;
+  return GH_OK;
 }
 
 }  // extern "C"

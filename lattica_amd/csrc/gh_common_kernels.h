@@ -73,6 +73,144 @@ __device__ __forceinline__ void finalize_row_g(F f, const typename F::W* __restr
   }
 }
 
+// ---- held-back finalize of a device-memory commitment, in two stages ----
+// A fused commitment writes its partial sums [row][part][slot]. Stage 1 runs
+// inside the next commitment's launch: every MAC block sums one slice of
+// `per` consecutive partials of one row (one 16-byte or 32-byte load per
+// thread, 16 threads per partial) into a group sum [row][group][slot]. Stage 2
+// runs inside the launch after: one wave per row sums the row's group sums
+// and applies the inverse transform. A standalone finalize (k_finalize_rows)
+// does both at once. Everything is field arithmetic on canonical residues, so
+// results do not depend on the grouping.
+struct FinStage1 {
+  const void* src;  // partial sums [rows][n][64]; never null (a dummy when ntasks = 0)
+  void* dst;        // group sums [rows][groups][64]
+  uint32_t n, rows, groups, per;  // per <= threads / 16
+  uint32_t ntasks;                // rows * groups, at most the launch's MAC blocks
+};
+struct FinStage2 {
+  const void* src;  // group sums [rows][n][64]
+  void* out;        // t_out, rows polynomials
+  uint32_t n, rows, nblk;  // nblk leading blocks of the launch, one wave per row
+};
+
+// four consecutive words at p (16-byte aligned)
+template <class W>
+__device__ __forceinline__ void load4(const W* p, W (&v)[4]) {
+  if constexpr (sizeof(W) == 4) {
+    const u32x4 x = *reinterpret_cast<const u32x4*>(p);
+    v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+  } else {
+    const u32x4 x = *reinterpret_cast<const u32x4*>(p);
+    const u32x4 y = *reinterpret_cast<const u32x4*>(p + 2);
+    v[0] = x.x | ((uint64_t)x.y << 32); v[1] = x.z | ((uint64_t)x.w << 32);
+    v[2] = y.x | ((uint64_t)y.y << 32); v[3] = y.z | ((uint64_t)y.w << 32);
+  }
+}
+__device__ __forceinline__ uint32_t shfl_xor_w(uint32_t v, int m) {
+  return (uint32_t)__shfl_xor((int)v, m, 64);
+}
+__device__ __forceinline__ uint64_t shfl_xor_w(uint64_t v, int m) {
+  const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m, 64);
+  const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Stage 1, issue: this thread's four words of MAC block `bid`'s task (zero
+// when the block has none or the partial lies past the group).
+template <class W>
+__device__ __forceinline__ void stage1_load(const FinStage1& s, uint32_t bid, W (&v)[4]) {
+  const bool has = bid < s.ntasks;  // block-uniform
+  const uint32_t task = has ? bid : 0;
+  const uint32_t r = task % s.rows, g = task / s.rows;
+  const uint32_t j = g * s.per + (threadIdx.x >> 4);
+  const bool live = has && (threadIdx.x >> 4) < s.per && j < s.n;
+  const W* p = reinterpret_cast<const W*>(s.src) +
+               ((size_t)r * s.n + (live ? j : 0)) * 64 + 4 * (threadIdx.x & 15);
+  load4(p, v);
+  if (!live) v[0] = v[1] = v[2] = v[3] = 0;
+}
+// Stage 1, sum within each wave (lanes l, l^16, l^32, l^48 hold the same
+// slots); lanes 0..15 leave their sums in lds[wave][64]
+template <class F>
+__device__ __forceinline__ void stage1_wave_sum(const F& f, typename F::W (&v)[4],
+                                                typename F::W* lds) {
+#pragma unroll
+  for (int m = 16; m <= 32; m *= 2)
+#pragma unroll
+    for (int i = 0; i < 4; i++) v[i] = f.add(v[i], shfl_xor_w(v[i], m));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane < 16)
+#pragma unroll
+    for (int i = 0; i < 4; i++) lds[wave * 64 + 4 * lane + i] = v[i];
+}
+// Stage 1, after a block barrier: the first 64 threads add the waves' sums
+// and store the group sum
+template <class F>
+__device__ __forceinline__ void stage1_store(const F& f, const FinStage1& s, uint32_t bid,
+                                             const typename F::W* lds, int waves) {
+  using W = typename F::W;
+  if (bid >= s.ntasks || threadIdx.x >= 64) return;
+  const uint32_t r = bid % s.rows, g = bid / s.rows;
+  W acc = lds[threadIdx.x];
+  for (int w = 1; w < waves; w++) acc = f.add(acc, lds[w * 64 + threadIdx.x]);
+  reinterpret_cast<W*>(s.dst)[((size_t)r * s.groups + g) * 64 + threadIdx.x] = acc;
+}
+
+// Sum of the n polynomials [row][0..n)[64] of `src` by G waves (lane = slot),
+// then the inverse transform with the untwist table `post` into out[row];
+// `red` holds G * 64 words of LDS (unused for G = 1).
+template <class F, int G>
+__device__ __forceinline__ void sum_rows_inv(const F& f, const typename F::W* __restrict__ tab,
+                                             int post, const typename F::W* __restrict__ src,
+                                             uint32_t n, uint32_t row, int wave,
+                                             typename F::W* __restrict__ out,
+                                             typename F::W* red) {
+  using W = typename F::W;
+  const int k = threadIdx.x & 63;
+  const W* p = src + (size_t)row * n * 64 + k;
+  W acc = 0;
+  constexpr int U = G == 1 ? 16 : 32;  // loads in flight per thread
+  for (uint32_t j0 = wave; j0 < n; j0 += G * U) {
+    W v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint32_t j = j0 + G * u;
+      v[u] = j < n ? p[(size_t)j * 64] : W(0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) acc = f.add(acc, v[u]);
+  }
+  if constexpr (G > 1) {
+    red[wave * 64 + k] = acc;
+    __syncthreads();
+    if (wave != 0) return;
+#pragma unroll
+    for (int i = 1; i < G; i++) acc = f.add(acc, red[i * 64 + k]);
+  }
+  out[(size_t)row * 64 + k] = wave_inv64(f, acc, tab, post, k);
+}
+
+// Standalone finalize of up to two held-back commitments: block b < rows_a
+// finalizes row b of segment a, the next rows_b blocks segment b. Each
+// segment's sums are laid out [row][n][64].
+struct FinSeg {
+  const void* src;
+  void* out;
+  uint32_t n, rows;
+};
+template <class F, int G>
+__global__ __launch_bounds__(64 * G) void k_finalize_rows(F f, const typename F::W* __restrict__ tab,
+                                                          int post, FinSeg a, FinSeg b) {
+  using W = typename F::W;
+  __shared__ W red[G * 64];
+  const bool in_a = blockIdx.x < a.rows;
+  const FinSeg& s = in_a ? a : b;
+  const uint32_t row = in_a ? blockIdx.x : blockIdx.x - a.rows;
+  sum_rows_inv<F, G>(f, tab, post, reinterpret_cast<const W*>(s.src), s.n, row,
+                     (int)(threadIdx.x >> 6), reinterpret_cast<W*>(s.out), red);
+}
+
 template <class F>
 __device__ __forceinline__ void finalize_row(F f, const typename F::W* __restrict__ tab,
                                              const typename F::W* __restrict__ partial,

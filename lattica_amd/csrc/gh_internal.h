@@ -77,6 +77,10 @@ struct gh_tuning {
   int batch_chunk_cols = 256; // batched MAC: key columns per block (q32 keys)
   int batch_chunk_cols_gold = 256; // batched MAC: key columns per block (Goldilocks keys)
   int batch_group = 8;        // batched MAC: witnesses sharing each loaded key column (4, 8, 16)
+  // device-memory gh_commit: 1 = hold the finalize back into the next
+  // commitment's launch (one launch per commitment; results complete at the
+  // next call on the context), 0 = finalize on the stream before returning
+  int defer_finalize = 1;
 };
 
 // Tuned key columns per block of the batched MAC for keys over `ring`.
@@ -84,19 +88,27 @@ inline uint32_t tuned_batch_chunk(const gh_tuning& t, int ring_kind) {
   return (uint32_t)(ring_kind == GH_RING_Q32 ? t.batch_chunk_cols : t.batch_chunk_cols_gold);
 }
 
-// The finalize of a device-memory commitment (fused path, q32 or Goldilocks),
-// held back so that it runs as extra blocks of the next such commitment's
-// launch over the same field instead of as a launch of its own. Its partial
-// sums stay at work + off (bytes long) until it has run; every other call on
-// the context issues it first (ctx_flush).
-struct PendingFin {
+// Finalizes of device-memory commitments (fused path) held back so that they
+// run inside the next commitments' launches instead of as launches of their
+// own: stage s1 is the last commitment, whose partial sums the next launch
+// reduces to group sums (stage 1), and s2 the one before, whose group sums
+// the next launch reduces and inverse-transforms into its t_out (stage 2).
+// Sums of a stage lie at work + off, n polynomials per row, [row][n][slot].
+// Every other call on the context issues both first (ctx_flush); a pipeline
+// holds commitments of one ring. The workspace is carved into four areas of
+// `cap` bytes: partial sums of alternate commitments in areas 0 and 1, their
+// group sums in areas 2 and 3 (commitment c uses areas c % 2 and 2 + c % 2).
+struct PendingStage {
   bool active = false;
-  int kind = GH_RING_Q32;
-  uint32_t q = 0, qinv = 0;
-  const void* tab = nullptr;
-  size_t off = 0, bytes = 0;
-  uint32_t nparts = 0, rows = 0;
+  size_t off = 0;
+  uint32_t n = 0, rows = 0;
   void* t_out = nullptr;
+};
+struct PendingFin {
+  const gh_ring* ring = nullptr;
+  PendingStage s1, s2;
+  uint32_t flip = 0;  // area pair of the next commitment
+  bool any() const { return s1.active || s2.active; }
 };
 
 struct gh_ctx {
@@ -168,8 +180,8 @@ int launch_negacyclic_mul(gh_ring* ring, const void* a, const void* b, void* out
 int launch_pointwise_mul(gh_ring* ring, const void* a, const void* b, void* out, size_t count);
 int launch_key_layout(gh_ring* ring, const void* src_rowmajor, uint32_t rows, uint32_t cols,
                       uint32_t col_offset, uint32_t total_cols, void* dst, int domain);
-// defer: a q32 commitment on the fused path leaves its finalize pending (see
-// PendingFin); other paths flush a pending one and finalize in place
+// defer: a commitment on the fused path leaves its finalize pending (see
+// PendingFin); other paths flush pending ones and finalize in place
 int launch_commit(gh_key* key, const void* s_dev, uint32_t cols, int domain, void* t_dev,
                   bool defer = false);
 // issues a pending finalize on the context stream (context lock held)
@@ -218,6 +230,21 @@ int launch_jl_conj_seed(gh_ring* ring, const uint8_t seed[32], uint32_t rows, ui
                         uint64_t n, void* out);
 int launch_jl_conj_flat(gh_ring* ring, const int8_t* flat, uint32_t rows, uint32_t num_parts,
                         uint64_t n, void* out);
+// Launch shape of an int8 matrix-core JL kernel over `total` witness
+// polynomials: polynomials per chunk launch (chunks of them), blocks of the
+// first (largest) chunk, polynomials per wave (projection) or per block
+// (evaluations), and the int32 bound: each sum adds at most `terms` products
+// of magnitude <= 128, bound = terms * 128 <= 2^31 - 1.
+struct JlShape {
+  uint64_t chunk_polys, per_unit, blocks, chunks, terms, bound;
+};
+void jl_proj_shape(uint64_t total, int num_cus, JlShape* s);
+void jl_evals_shape(uint64_t total, int num_cus, JlShape* s);
+// the int64 p_exact is exact below this many polynomials:
+// |p| <= total * 64 * (q - 1) / 2 < 2^26 * 2^6 * 2^31 = 2^63
+constexpr uint64_t kJlExactMaxPolys = 1ull << 26;
+// witness polynomials the JL entry points accept: 65535 parts x 2^20
+constexpr uint64_t kJlMaxPolys = 1ull << 36;
 // p = Pi * s on the matrix cores; rows in [253, 256] (one keystream block per
 // column); p_exact (q32 only, nullable): the exact centred sum
 int launch_jl_proj(gh_ring* ring, const uint8_t seed[32], uint32_t rows, const void* parts,

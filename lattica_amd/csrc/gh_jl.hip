@@ -72,6 +72,33 @@ __device__ void jl_column_lds(const ChaKey& key, uint64_t col, uint32_t pk, uint
 }
 
 constexpr int kEvPolys = 8;       // witness polynomials per LDS round
+
+// Exactness of the int32 matrix-core sums. Every A operand entry is a JL
+// entry in {-1, 0, 1} and every B operand entry a signed base-256 digit in
+// [-128, 127], so an int32 sum of K such products, and every partial sum on
+// the way, lies within K * 128 in magnitude. The launch shapes keep
+// K * 128 <= 2^31 - 1:
+//   k_jl_proj   a block's sum for one (row, digit) adds 64 products per
+//               polynomial over the polynomials of its 4 waves (each wave's
+//               MFMA accumulator, then the int32 sum of the four):
+//               K = 4 * 64 * ppw; ppw <= kProjMaxPpw = 65535 gives
+//               K * 128 = 2147450880;
+//   k_jl_evals  a block's sum for one (row, digit column) adds 64 products per
+//               polynomial of the block, in one wave's accumulator:
+//               K = 64 * ppb; ppb <= kEvMaxPpb = 262136 (a multiple of 8)
+//               gives K * 128 = 2147418112. Witnesses longer than
+//               (blocks) * kEvMaxPpb polynomials run in chunks, each reduced
+//               mod q and added to the output;
+//   k_jl_agg    each output sums 64 coefficients x 256 rows of one polynomial
+//               (K = 2^14, K * 128 = 2^21), independent of the size.
+// The reductions sum block partials in int64: at most 2^36 polynomials are
+// accepted, so a digit sum stays below 2^49.
+// jl_proj_shape and jl_evals_shape compute these shapes (gh_jl_launch_shape).
+constexpr uint64_t kProjMaxPpw = 65535;
+constexpr uint64_t kEvMaxPpb = 262136;
+static_assert(4 * 64 * kProjMaxPpw * 128 <= 0x7fffffffull, "projection int32 bound");
+static_assert(64 * kEvMaxPpb * 128 <= 0x7fffffffull, "evaluation int32 bound");
+static_assert(kEvMaxPpb % kEvPolys == 0, "evaluation blocks take whole rounds");
 constexpr int kEvRowBytes = 64;   // A tile bytes per row of a polynomial (256 rows x 64 coefficients)
 
 // Decodes this lane's keystream column (16 words, 2 bits per row, 256 rows)
@@ -356,7 +383,8 @@ static_assert(2 * kEvStep == kEvPolys, "the two step buffers fill the LDS tiles"
 
 template <class W, int DT, int ND>
 __device__ __forceinline__ void evals_pass(ChaKey key0, const W* __restrict__ parts, uint64_t n,
-                                           uint64_t total_polys, uint64_t polys_per_block,
+                                           uint64_t g_base, uint64_t total_polys,
+                                           uint64_t polys_per_block,
                                            uint64_t q, uint32_t dbase, int32_t* __restrict__ partial,
                                            int8_t* a_tile, int8_t* e_tile, uint32_t (&blk)[16],
                                            uint64_t& xw) {
@@ -371,7 +399,7 @@ __device__ __forceinline__ void evals_pass(ChaKey key0, const W* __restrict__ pa
   const uint32_t half = t >> 8, ps_own = (t >> 6) & 3;
   const uint32_t u = jl_tcoeff(lane);  // this lane's coefficient
   const v4i tperm = jl_tperm(lane);
-  const uint64_t g_begin = (uint64_t)blockIdx.x * polys_per_block;
+  const uint64_t g_begin = g_base + (uint64_t)blockIdx.x * polys_per_block;
   const uint64_t g_end = std::min<uint64_t>(g_begin + polys_per_block, total_polys);
   const uint32_t nsteps = (uint32_t)((g_end - g_begin + kEvStep - 1) / kEvStep);
   v16i acc[ND][2];
@@ -490,7 +518,8 @@ __device__ __forceinline__ void evals_pass(ChaKey key0, const W* __restrict__ pa
 // records it in span[block] for the later passes and the reduction.
 template <class W, int DT>
 __global__ __launch_bounds__(512) void k_jl_evals(ChaKey key0, const W* __restrict__ parts,
-                                                  uint64_t n, uint64_t total_polys,
+                                                  uint64_t n, uint64_t g_base,
+                                                  uint64_t total_polys,
                                                   uint64_t polys_per_block, uint64_t q,
                                                   uint32_t dbase, uint32_t* __restrict__ span,
                                                   int32_t* __restrict__ partial) {
@@ -498,7 +527,7 @@ __global__ __launch_bounds__(512) void k_jl_evals(ChaKey key0, const W* __restri
   __shared__ __attribute__((aligned(16))) int8_t e_tile[kEvPolys * kEvDigits * 128];
   __shared__ uint32_t wmax[8];
   const uint32_t t = threadIdx.x;
-  const uint64_t g_begin = (uint64_t)blockIdx.x * polys_per_block;
+  const uint64_t g_begin = g_base + (uint64_t)blockIdx.x * polys_per_block;
   const uint64_t g_end = std::min<uint64_t>(g_begin + polys_per_block, total_polys);
   uint32_t deff;
   uint32_t blk[16];
@@ -548,13 +577,13 @@ __global__ __launch_bounds__(512) void k_jl_evals(ChaKey key0, const W* __restri
   }
   const uint32_t nd = deff - dbase;
   if (nd == 1)
-    evals_pass<W, DT, 1>(key0, parts, n, total_polys, polys_per_block, q, dbase, partial, a_tile,
+    evals_pass<W, DT, 1>(key0, parts, n, g_base, total_polys, polys_per_block, q, dbase, partial, a_tile,
                          e_tile, blk, xw);
   else if (nd == 2)
-    evals_pass<W, DT, 2>(key0, parts, n, total_polys, polys_per_block, q, dbase, partial, a_tile,
+    evals_pass<W, DT, 2>(key0, parts, n, g_base, total_polys, polys_per_block, q, dbase, partial, a_tile,
                          e_tile, blk, xw);
   else
-    evals_pass<W, DT, 4>(key0, parts, n, total_polys, polys_per_block, q, dbase, partial, a_tile,
+    evals_pass<W, DT, 4>(key0, parts, n, g_base, total_polys, polys_per_block, q, dbase, partial, a_tile,
                          e_tile, blk, xw);
 }
 
@@ -570,15 +599,16 @@ __device__ __forceinline__ uint64_t gold_shift8(uint64_t r) {
 // evals[m][t] = sum_b 256^b sum_blocks partial[block][m][b * 64 + t] mod q.
 // Block m, one thread per digit column n = b * 64 + t summing its column over
 // the blocks (eight loads in flight), then 64 threads recombine the digits
-// from the top one (Horner). Digit sums are below 2^40 in magnitude (at most
-// 2^31 per block partial, 256 blocks).
+// from the top one (Horner). Digit sums are below 2^41 in magnitude (below
+// 2^31 per block partial, at most kEvMaxBlocks blocks). With `accumulate` the
+// result is added mod q to what `out` holds (the chunks of a long witness).
 constexpr uint32_t kEvMaxBlocks = 1024;  // evaluation blocks (the reduction lists them)
 
 template <class W, int D>
 __global__ __launch_bounds__(512) void k_jl_evals_fin(const int32_t* __restrict__ partial,
                                                       uint32_t nblk, uint64_t q,
                                                       const uint32_t* __restrict__ span,
-                                                      W* __restrict__ out) {
+                                                      int accumulate, W* __restrict__ out) {
   constexpr int kN = 64 * D, kWaves = 8, kU = 16;
   __shared__ uint16_t blocks[D][kEvMaxBlocks];
   __shared__ uint32_t counts[D];
@@ -642,6 +672,11 @@ __global__ __launch_bounds__(512) void k_jl_evals_fin(const int32_t* __restrict_
         r = r >= a ? r - a : r + (q - a);
       }
     }
+  }
+  if (accumulate) {  // a later chunk of the witness: add to the earlier chunks' sum
+    const uint64_t o = (uint64_t)out[(uint64_t)m * 64 + nn];
+    const uint64_t x = r + o;  // both below q; wraps 2^64 only for Goldilocks
+    r = (x < r || x >= q) ? x - q : x;
   }
   out[(uint64_t)m * 64 + nn] = (W)r;
 }
@@ -922,13 +957,23 @@ __global__ __launch_bounds__(256) void k_jl_proj_fin(const int32_t* __restrict__
   }
   if (threadIdx.x != 0) return;
   if (!GOLD) {
-    int64_t e = 0;
+    // p mod q by Horner over the digit sums (each below 2^49 in magnitude for
+    // any accepted size); the int64 p itself only when the caller asks for it,
+    // which the entry point allows for fewer than 2^26 polynomials
+    // (|p| <= total * 64 * (q - 1) / 2 < 2^63)
+    int64_t r = 0;
 #pragma unroll
-    for (int d = kProjDigits - 1; d >= 0; d--) e = e * 256 + red[d][0];
-    if (p_exact) p_exact[m] = e;
-    int64_t r = e % (int64_t)q;
-    if (r < 0) r += (int64_t)q;
+    for (int d = kProjDigits - 1; d >= 0; d--) {
+      r = (r * 256 + red[d][0] % (int64_t)q) % (int64_t)q;
+      if (r < 0) r += (int64_t)q;
+    }
     ((uint32_t*)p_modq)[m] = (uint32_t)r;
+    if (p_exact) {
+      int64_t e = 0;
+#pragma unroll
+      for (int d = kProjDigits - 1; d >= 0; d--) e = e * 256 + red[d][0];
+      p_exact[m] = e;
+    }
   } else {
     uint64_t r = 0;
 #pragma unroll
@@ -949,16 +994,59 @@ __global__ __launch_bounds__(256) void k_jl_proj_fin(const int32_t* __restrict__
 
 }  // namespace
 
+void jl_proj_shape(uint64_t total, int num_cus, JlShape* s) {
+  // two waves per SIMD, whole polynomials per wave, at most kProjMaxPpw each
+  const uint64_t waves = (uint64_t)std::max(num_cus, 1) * 2 * kProjWaves;
+  uint64_t ppw = std::max<uint64_t>(1, (total + waves - 1) / waves);
+  ppw = std::min<uint64_t>(ppw, kProjMaxPpw);
+  uint64_t nblk = (total + ppw * kProjWaves - 1) / (ppw * kProjWaves);
+  if (nblk == 0) nblk = 1;
+  s->chunk_polys = total;
+  s->per_unit = ppw;
+  s->blocks = nblk;
+  s->chunks = 1;
+  s->terms = (uint64_t)kProjWaves * 64 * ppw;
+  s->bound = s->terms * 128;
+}
+
+// one chunk of the evaluations: blocks (at most one per CU) and polynomials
+// per block (a multiple of kEvPolys); polys <= kEvMaxPpb * the block cap
+static void jl_evals_chunk(uint64_t polys, int num_cus, uint64_t* nblk, uint64_t* ppb) {
+  const uint64_t nmax = std::min<uint64_t>((uint64_t)std::max(num_cus, 1), kEvMaxBlocks);
+  uint64_t nb = std::min<uint64_t>(nmax, (polys + kEvPolys - 1) / kEvPolys);
+  if (nb == 0) nb = 1;
+  uint64_t p = (polys + nb - 1) / nb;
+  p = (p + kEvPolys - 1) / kEvPolys * kEvPolys;
+  if (p == 0) p = kEvPolys;
+  nb = (polys + p - 1) / p;
+  if (nb == 0) nb = 1;
+  *nblk = nb;
+  *ppb = p;
+}
+
+void jl_evals_shape(uint64_t total, int num_cus, JlShape* s) {
+  const uint64_t nmax = std::min<uint64_t>((uint64_t)std::max(num_cus, 1), kEvMaxBlocks);
+  const uint64_t chunk = nmax * kEvMaxPpb;
+  const uint64_t first = std::min<uint64_t>(total, chunk);
+  uint64_t nblk, ppb;
+  jl_evals_chunk(first, num_cus, &nblk, &ppb);
+  s->chunk_polys = chunk;
+  s->per_unit = ppb;
+  s->blocks = nblk;
+  s->chunks = total == 0 ? 1 : (total + chunk - 1) / chunk;
+  s->terms = 64 * ppb;
+  s->bound = s->terms * 128;
+}
+
 int launch_jl_proj(gh_ring* ring, const uint8_t seed[32], uint32_t rows, const void* parts,
                    uint32_t num_parts, uint64_t n, void* p_modq, int64_t* p_exact) {
   gh_ctx* ctx = ring->ctx;
   const bool gold = ring->kind == GH_RING_GOLDILOCKS;
   const uint64_t total = (uint64_t)num_parts * n;
-  // two waves per SIMD, whole polynomials per wave
-  const uint64_t waves = (uint64_t)ctx->num_cus * 2 * kProjWaves;
-  const uint64_t ppw = std::max<uint64_t>(1, (total + waves - 1) / waves);
-  uint64_t nblk = (total + ppw * kProjWaves - 1) / (ppw * kProjWaves);
-  if (nblk == 0) nblk = 1;
+  if (p_exact && total >= kJlExactMaxPolys) return GH_E_INVALID_ARGUMENT;
+  JlShape shape;
+  jl_proj_shape(total, ctx->num_cus, &shape);
+  const uint64_t ppw = shape.per_unit, nblk = shape.blocks;
   if (nblk > 0x7fffffffull) return GH_E_INVALID_ARGUMENT;
   void* ws;
   int st = ctx_workspace(ctx, (size_t)nblk * 256 * kProjDigits * sizeof(int32_t), &ws);
@@ -1055,40 +1143,43 @@ int launch_jl_evals(gh_ring* ring, const uint8_t seed[32], const void* parts, ui
   const int D = gold ? 8 : 4;  // digits of a lifted witness word
   const uint64_t total = (uint64_t)num_parts * n;
   // one block per CU (the accumulators fill half the register file), polys
-  // split evenly in rounds of 8
-  uint64_t nblk = std::min<uint64_t>(std::min<uint64_t>(ctx->num_cus, kEvMaxBlocks),
-                                     (total + kEvPolys - 1) / kEvPolys);
-  if (nblk == 0) nblk = 1;
-  uint64_t ppb = (total + nblk - 1) / nblk;
-  ppb = (ppb + kEvPolys - 1) / kEvPolys * kEvPolys;
-  nblk = (total + ppb - 1) / ppb;
-  if (nblk == 0) nblk = 1;
-  const size_t p_bytes = (size_t)nblk * 256 * 64 * D * sizeof(int32_t);
+  // split evenly in rounds of 8, at most kEvMaxPpb per block: longer
+  // witnesses run in chunks added up mod q
+  JlShape shape;
+  jl_evals_shape(total, ctx->num_cus, &shape);
+  const size_t p_bytes = (size_t)shape.blocks * 256 * 64 * D * sizeof(int32_t);
   void* ws;
-  int st = ctx_workspace(ctx, p_bytes + nblk * sizeof(uint32_t), &ws);
+  int st = ctx_workspace(ctx, p_bytes + shape.blocks * sizeof(uint32_t), &ws);
   if (st) return st;
   // per-block digit spans after the partial sums: small-norm witnesses (the
   // LaBRADOR case) need one digit of four, and the passes skip the zero planes
   uint32_t* span = (uint32_t*)((char*)ws + p_bytes);
   const ChaKey key = chacha_key(seed);
-  if (!gold) {
-    hipLaunchKernelGGL((k_jl_evals<uint32_t, 4>), dim3((unsigned)nblk), dim3(512), 0, ctx->stream,
-                       key, (const uint32_t*)parts, n, total, ppb, ring->q, 0u, span,
-                       (int32_t*)ws);
-    GH_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_jl_evals_fin<uint32_t, 4>), dim3(256), dim3(512), 0, ctx->stream,
-                       (const int32_t*)ws, (uint32_t)nblk, ring->q, span, (uint32_t*)out);
-  } else {
-    for (uint32_t dbase = 0; dbase < 8; dbase += 4) {
-      hipLaunchKernelGGL((k_jl_evals<uint64_t, 8>), dim3((unsigned)nblk), dim3(512), 0,
-                         ctx->stream, key, (const uint64_t*)parts, n, total, ppb, ring->q, dbase,
+  for (uint64_t c0 = 0; c0 < total; c0 += shape.chunk_polys) {
+    const uint64_t polys = std::min<uint64_t>(total - c0, shape.chunk_polys);
+    uint64_t nblk, ppb;
+    jl_evals_chunk(polys, ctx->num_cus, &nblk, &ppb);
+    const uint64_t end = c0 + polys;
+    const int acc = c0 > 0;
+    if (!gold) {
+      hipLaunchKernelGGL((k_jl_evals<uint32_t, 4>), dim3((unsigned)nblk), dim3(512), 0,
+                         ctx->stream, key, (const uint32_t*)parts, n, c0, end, ppb, ring->q, 0u,
                          span, (int32_t*)ws);
       GH_HIP(hipGetLastError());
+      hipLaunchKernelGGL((k_jl_evals_fin<uint32_t, 4>), dim3(256), dim3(512), 0, ctx->stream,
+                         (const int32_t*)ws, (uint32_t)nblk, ring->q, span, acc, (uint32_t*)out);
+    } else {
+      for (uint32_t dbase = 0; dbase < 8; dbase += 4) {
+        hipLaunchKernelGGL((k_jl_evals<uint64_t, 8>), dim3((unsigned)nblk), dim3(512), 0,
+                           ctx->stream, key, (const uint64_t*)parts, n, c0, end, ppb, ring->q,
+                           dbase, span, (int32_t*)ws);
+        GH_HIP(hipGetLastError());
+      }
+      hipLaunchKernelGGL((k_jl_evals_fin<uint64_t, 8>), dim3(256), dim3(512), 0, ctx->stream,
+                         (const int32_t*)ws, (uint32_t)nblk, ring->q, span, acc, (uint64_t*)out);
     }
-    hipLaunchKernelGGL((k_jl_evals_fin<uint64_t, 8>), dim3(256), dim3(512), 0, ctx->stream,
-                       (const int32_t*)ws, (uint32_t)nblk, ring->q, span, (uint64_t*)out);
+    GH_HIP(hipGetLastError());
   }
-  GH_HIP(hipGetLastError());
   return GH_OK;
 }
 

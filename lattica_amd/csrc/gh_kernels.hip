@@ -389,33 +389,32 @@ __device__ __forceinline__ void mac_column(uint64_t (&lo)[NI][4], uint32_t (&hi)
   }
 }
 
-// The previous commitment's finalize, run by the first `nblk` blocks (one
-// per row) of a fused launch; nblk = 0: none.
-struct FinArgs {
-  FQ32 f;
-  const uint32_t* tab;
-  const uint32_t* partial;
-  uint32_t* out;
-  uint32_t nparts, rows, nblk;
-};
-
+// Held-back finalizes (FinStage1/FinStage2): the first s2.nblk blocks finish
+// the commitment before the previous one (one wave per row), and every MAC
+// block runs one stage-1 task of the previous commitment (its loads issued
+// with the witness loads, summed after the transform). Partial sums are
+// written [row][part][slot].
 template <int RB, int WAVES, int DEPTH, int NW>
 __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_q32(
     FQ32 f, const uint32_t* __restrict__ tab, const uint32_t* __restrict__ A,
     const uint32_t* __restrict__ s_coeff, uint32_t* __restrict__ partial, uint32_t rows,
-    uint32_t cols, uint32_t cols_per_block, FinArgs fin) {
+    uint32_t cols, uint32_t cols_per_block, FinStage1 s1, FinStage2 s2) {
   constexpr int NI = RB / 4;
   constexpr int PF = DEPTH * 8 / WAVES > 0 ? DEPTH * 8 / WAVES : 1;
-  extern __shared__ uint32_t lds[];  // max(cols_per_block * kFuseStride, WAVES * RB * 64) words
-  if (blockIdx.x < fin.nblk) {
-    // row blockIdx.x of the previous commitment: its partial sums were
-    // complete when this launch began
-    if (blockIdx.y == 0)
-      finalize_row_g<FQ32, WAVES>(fin.f, fin.tab, fin.partial, fin.nparts, fin.rows, blockIdx.x,
-                                  (int)kTabUntwistMac, fin.out, lds);
+  // cols_per_block * kFuseStride witness words, then WAVES * 64 stage-1 words;
+  // WAVES * RB * 64 words for the block reduction
+  extern __shared__ uint32_t lds[];
+  if (blockIdx.x < s2.nblk) {
+    // stage 2: the group sums were complete when this launch began
+    const uint32_t row = blockIdx.x * WAVES + (threadIdx.x >> 6);
+    if (blockIdx.y == 0 && row < s2.rows)
+      sum_rows_inv<FQ32, 1>(f, tab, (int)kTabUntwistMac, (const uint32_t*)s2.src, s2.n, row, 0,
+                            (uint32_t*)s2.out, nullptr);
     return;
   }
-  const uint32_t bx = blockIdx.x - fin.nblk;
+  const uint32_t bx = blockIdx.x - s2.nblk;
+  const uint32_t nparts = gridDim.x - s2.nblk;
+  const uint32_t bid = blockIdx.y * nparts + bx;  // stage-1 task
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int rsub = lane >> 4, k0 = 4 * (lane & 15);
@@ -456,6 +455,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_q32(
 #pragma unroll
     for (int i = 0; i < 16; i++) x[i] = sp[4 * i];
   }
+  uint32_t l1v[4];
+  stage1_load(s1, bid, l1v);
   // (b) key prefetch: this wave's first PF columns
   u32x4 pa[PF][NI];
 #pragma unroll
@@ -484,7 +485,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_q32(
 #pragma unroll
     for (int v = 0; v < 4; v++) dp[v] = u32x4{x[4 * v], x[4 * v + 1], x[4 * v + 2], x[4 * v + 3]};
   }
+  uint32_t* l1lds = lds + cols_per_block * kFuseStride;
+  stage1_wave_sum(f, l1v, l1lds);
   __syncthreads();
+  stage1_store(f, s1, bid, l1lds, WAVES);
   // (d) MAC over the wave's columns, PF in flight; a clamped column meets a
   // zero witness
   auto witness = [&](uint32_t j) -> u32x4 {
@@ -537,7 +541,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_q32(
     uint32_t acc = red[o];
 #pragma unroll
     for (int w = 1; w < WAVES; w++) acc = f.add(acc, red[w * RB * 64 + o]);
-    partial[((size_t)bx * rows + row0) * 64 + o] = acc;
+    partial[((size_t)(row0 + o / 64) * nparts + bx) * 64 + o % 64] = acc;
   }
 }
 
@@ -552,26 +556,24 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_q32(
 constexpr int kFuseChunkGold = 64;
 constexpr int kFuseStrideGold = 66;  // LDS row stride of the transformed witness, words
 
-struct FinArgsGold {
-  const uint64_t* tab;
-  const uint64_t* partial;
-  uint64_t* out;
-  uint32_t nparts, rows, nblk;
-};
-
 template <int RB, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_gold(
     FGold f, const uint64_t* __restrict__ tab, const uint64_t* __restrict__ A,
     const uint64_t* __restrict__ s_coeff, uint64_t* __restrict__ partial, uint32_t rows,
-    uint32_t cols, uint32_t cols_per_block, FinArgsGold fin) {
-  extern __shared__ uint64_t lds64[];  // max(cols_per_block * kFuseStrideGold, WAVES * RB * 64)
-  if (blockIdx.x < fin.nblk) {
-    if (blockIdx.y == 0)
-      finalize_row_g<FGold, WAVES>(f, fin.tab, fin.partial, fin.nparts, fin.rows, blockIdx.x,
-                                   (int)kTabUntwist, fin.out, lds64);
+    uint32_t cols, uint32_t cols_per_block, FinStage1 s1, FinStage2 s2) {
+  // cols_per_block * kFuseStrideGold witness words, then WAVES * 64 stage-1
+  // words; WAVES * RB * 64 words for the block reduction
+  extern __shared__ uint64_t lds64[];
+  if (blockIdx.x < s2.nblk) {
+    const uint32_t row = blockIdx.x * WAVES + (threadIdx.x >> 6);
+    if (blockIdx.y == 0 && row < s2.rows)
+      sum_rows_inv<FGold, 1>(f, tab, (int)kTabUntwist, (const uint64_t*)s2.src, s2.n, row, 0,
+                             (uint64_t*)s2.out, nullptr);
     return;
   }
-  const uint32_t bx = blockIdx.x - fin.nblk;
+  const uint32_t bx = blockIdx.x - s2.nblk;
+  const uint32_t nparts = gridDim.x - s2.nblk;
+  const uint32_t bid = blockIdx.y * nparts + bx;  // stage-1 task
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, t = threadIdx.x & 3;
   const uint32_t row0 = blockIdx.y * RB;
   const uint32_t ch = bx * cols_per_block;  // < cols: every block owns a column
@@ -579,6 +581,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_gold(
   uint64_t ltw[16][1];
 #pragma unroll
   for (int k = 0; k < 16; k++) ltw[k][0] = tab[kTabQuad + 16 * t + k];
+  uint64_t l1v[4];
+  stage1_load(s1, bid, l1v);
   // whole quads share p, so they drop out together (the transform exchanges
   // values within a quad)
   for (uint32_t p = threadIdx.x >> 2; p < ncol; p += 16 * WAVES) {
@@ -591,7 +595,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_gold(
 #pragma unroll
     for (int v = 0; v < 16; v++) dp[v] = x[v];
   }
+  uint64_t* l1lds = lds64 + cols_per_block * kFuseStrideGold;
+  stage1_wave_sum(f, l1v, l1lds);
   __syncthreads();
+  stage1_store(f, s1, bid, l1lds, WAVES);
   GoldAcc acc[RB];  // exact lazy sums, reduced once per row at the end
   const size_t col_stride = (size_t)rows * 64;
   const uint64_t* akey = A + (size_t)row0 * 64 + lane;
@@ -628,7 +635,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_gold(
     uint64_t a = red[o];
 #pragma unroll
     for (int w = 1; w < WAVES; w++) a = f.add(a, red[w * RB * 64 + o]);
-    partial[((size_t)bx * rows + row0) * 64 + o] = a;
+    partial[((size_t)(row0 + o / 64) * nparts + bx) * 64 + o % 64] = a;
   }
 }
 
@@ -896,14 +903,14 @@ template <int RB, int WAVES, int DEPTH>
 static void fused_launch_nw(gh_ctx* ctx, dim3 grid, size_t lds_bytes, hipEvent_t e0,
                             hipEvent_t e1, FQ32 f, const uint32_t* tab, const uint32_t* A,
                             const uint32_t* S, uint32_t* P, uint32_t rows, uint32_t cols,
-                            uint32_t cpb, const FinArgs& fin) {
+                            uint32_t cpb, const FinStage1& s1, const FinStage2& s2) {
   // the unrolled sweep when every full block has the same columns per wave
   // (instantiated for 16-row blocks, kappa a multiple of 16)
   const uint32_t nw = RB == 16 && cpb % WAVES == 0 ? cpb / WAVES : 0;
 #define GH_FUSED_NW(N)                                                                          \
   hipExtLaunchKernelGGL((k_commit_fused_q32<RB, WAVES, DEPTH, N>), grid, dim3(64 * WAVES),      \
                         lds_bytes, ctx->stream, e0, e1, 0, f, tab, A, S, P, rows, cols, cpb,    \
-                        fin)
+                        s1, s2)
   if constexpr (RB == 16) {
     if (nw == 8) GH_FUSED_NW(8);
     else if (nw == 16) GH_FUSED_NW(16);
@@ -919,60 +926,143 @@ template <int RB, int WAVES>
 static void fused_launch(gh_ctx* ctx, dim3 grid, size_t lds_bytes, hipEvent_t e0, hipEvent_t e1,
                          FQ32 f, const uint32_t* tab, const uint32_t* A, const uint32_t* S,
                          uint32_t* P, uint32_t rows, uint32_t cols, uint32_t cpb,
-                         const FinArgs& fin) {
+                         const FinStage1& s1, const FinStage2& s2) {
   switch (ctx->tune.fused_depth) {
-    case 1: fused_launch_nw<RB, WAVES, 1>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, fin); break;
-    case 3: fused_launch_nw<RB, WAVES, 3>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, fin); break;
-    case 4: fused_launch_nw<RB, WAVES, 4>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, fin); break;
-    default: fused_launch_nw<RB, WAVES, 2>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, fin); break;
+    case 1: fused_launch_nw<RB, WAVES, 1>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, s1, s2); break;
+    case 3: fused_launch_nw<RB, WAVES, 3>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, s1, s2); break;
+    case 4: fused_launch_nw<RB, WAVES, 4>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, s1, s2); break;
+    default: fused_launch_nw<RB, WAVES, 2>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, s1, s2); break;
   }
 }
 
-// The standalone finalize of a q32 commitment: 16 waves per row.
-static int launch_finalize_q32(gh_ctx* ctx, FQ32 f, const uint32_t* tab, const uint32_t* P,
-                               uint32_t nparts, uint32_t rows, uint32_t* t_dev) {
+// Standalone finalize of up to two commitments of one ring whose sums are
+// laid out [row][n][slot]: 16 waves per row for up to one row per CU, 4
+// waves per row beyond (all rows in one round).
+static int launch_finalize_rows(gh_ctx* ctx, const gh_ring* ring, const FinSeg& a,
+                                const FinSeg& b) {
+  const uint32_t nrows = a.rows + b.rows;
+  if (nrows == 0) return GH_OK;
   hipEvent_t f0, f1;
   prof_pair(ctx, 2, &f0, &f1);
-  if (rows > (uint32_t)ctx->num_cus)  // 4-wave blocks: all rows in one round
-    hipExtLaunchKernelGGL((k_finalize4<FQ32>), dim3(rows), dim3(256), 0, ctx->stream, f0, f1, 0,
-                          f, tab, P, nparts, rows, (int)kTabUntwistMac, t_dev);
-  else
-    hipExtLaunchKernelGGL((k_finalize<FQ32>), dim3(rows), dim3(1024), 0, ctx->stream, f0, f1, 0,
-                          f, tab, P, nparts, rows, (int)kTabUntwistMac, t_dev);
+  const bool many = nrows > (uint32_t)ctx->num_cus;
+  if (ring->kind == GH_RING_Q32) {
+    const FQ32 f{(uint32_t)ring->q, ring->qinv32};
+    const uint32_t* tab = (const uint32_t*)ring->dev_tab;
+    if (many)
+      hipExtLaunchKernelGGL((k_finalize_rows<FQ32, 4>), dim3(nrows), dim3(256), 0, ctx->stream,
+                            f0, f1, 0, f, tab, (int)kTabUntwistMac, a, b);
+    else
+      hipExtLaunchKernelGGL((k_finalize_rows<FQ32, 16>), dim3(nrows), dim3(1024), 0,
+                            ctx->stream, f0, f1, 0, f, tab, (int)kTabUntwistMac, a, b);
+  } else {
+    const uint64_t* tab = (const uint64_t*)ring->dev_tab;
+    if (many)
+      hipExtLaunchKernelGGL((k_finalize_rows<FGold, 4>), dim3(nrows), dim3(256), 0, ctx->stream,
+                            f0, f1, 0, FGold{}, tab, (int)kTabUntwist, a, b);
+    else
+      hipExtLaunchKernelGGL((k_finalize_rows<FGold, 16>), dim3(nrows), dim3(1024), 0,
+                            ctx->stream, f0, f1, 0, FGold{}, tab, (int)kTabUntwist, a, b);
+  }
   GH_HIP(hipGetLastError());
   return GH_OK;
 }
 
-// Where a fused commitment's partial sums (pbytes) go, for a launch over ring
-// kind `kind`: beside a pending finalize of the same kind when it is deferring
-// and the workspace has room (from offset 0 when they fit before it), else
-// after issuing that finalize (and any pending one of the other kind) from
-// offset 0, with room for the next deferring commitment too. On return
-// ctx->fin.active says whether the pending finalize runs inside this launch.
-static int fused_partial_area(gh_ctx* ctx, int kind, size_t pbytes, bool defer, size_t* off) {
+int ctx_flush(gh_ctx* ctx) {
   PendingFin& pf = ctx->fin;
-  *off = 0;
-  if (pf.active && pf.kind != kind)
-    if (int st = ctx_flush(ctx)) return st;
-  if (pf.active) {
-    if (defer && pbytes <= pf.off) return GH_OK;
-    if (defer && pf.off + pf.bytes + pbytes <= ctx->work_bytes) {
-      *off = pf.off + pf.bytes;
-      return GH_OK;
-    }
-    if (int st = ctx_flush(ctx)) return st;
+  if (!pf.any()) return GH_OK;
+  const char* w = (const char*)ctx->work;
+  FinSeg a{}, b{};
+  if (pf.s2.active) a = FinSeg{w + pf.s2.off, pf.s2.t_out, pf.s2.n, pf.s2.rows};
+  if (pf.s1.active) b = FinSeg{w + pf.s1.off, pf.s1.t_out, pf.s1.n, pf.s1.rows};
+  pf.s1.active = pf.s2.active = false;
+  return launch_finalize_rows(ctx, pf.ring, a, b);
+}
+
+static bool spans_overlap(const void* a, size_t na, const void* b, size_t nb) {
+  const uintptr_t a0 = (uintptr_t)a, b0 = (uintptr_t)b;
+  return a0 < b0 + nb && b0 < a0 + na;
+}
+
+// Shape of a fused launch and the stage work it carries.
+struct FusedPlan {
+  size_t off = 0;        // this commitment's partial sums in the workspace
+  FinStage1 s1{};
+  FinStage2 s2{};
+};
+
+// Places a fused commitment's partial sums (pbytes; nmac MAC blocks of
+// `threads` threads) and, when it defers, the held-back stages its launch
+// runs. Pending finalizes are issued first when the launch cannot carry them:
+// another ring, no deferral, a witness overlapping a pending t_out (inside one
+// launch nothing orders the MAC blocks' witness reads after the stage-2
+// writes), a stage-1 task list longer than the launch's MAC blocks, or
+// partial sums larger than the workspace's areas.
+static int fused_plan(gh_ctx* ctx, const gh_ring* ring, const void* s_dev, size_t s_bytes,
+                      size_t pbytes, uint32_t nmac, uint32_t threads, uint32_t waves,
+                      const void* dummy, bool defer, FusedPlan* plan) {
+  PendingFin& pf = ctx->fin;
+  const size_t wb = ring->word_bytes;
+  const uint32_t per_pass = threads / 16;  // partials a block sums in one stage-1 task
+  const size_t cap = (ctx->work_bytes / 4) & ~(size_t)255;
+  uint32_t groups = 0;
+  bool keep = defer && pf.ring == ring && pbytes <= cap;
+  if (keep && pf.s1.active) {
+    groups = (pf.s1.n + per_pass - 1) / per_pass;
+    if ((uint64_t)pf.s1.rows * groups > nmac) keep = false;
   }
-  void* ws;
-  return ctx_workspace(ctx, (defer ? 2 : 1) * pbytes, &ws);
+  if (keep) {
+    for (const PendingStage* st : {&pf.s1, &pf.s2})
+      if (st->active && spans_overlap(s_dev, s_bytes, st->t_out, (size_t)st->rows * 64 * wb))
+        keep = false;
+  }
+  if (!keep) {
+    if (int st = ctx_flush(ctx)) return st;
+    void* ws;
+    // room for the four areas of a pipeline (or one area without deferral)
+    if (int st = ctx_workspace(ctx, (defer ? 4 : 1) * pbytes, &ws)) return st;
+    pf.flip = 0;
+  }
+  pf.ring = ring;
+  const size_t area = defer ? (ctx->work_bytes / 4) & ~(size_t)255 : 0;
+  char* w = (char*)ctx->work;
+  plan->off = pf.flip * area;
+  plan->s1 = FinStage1{dummy, nullptr, 0, 1, 0, 0, 0};
+  plan->s2 = FinStage2{nullptr, nullptr, 0, 0, 0};
+  if (pf.s2.active)
+    plan->s2 = FinStage2{w + pf.s2.off, pf.s2.t_out, pf.s2.n, pf.s2.rows,
+                         (pf.s2.rows + waves - 1) / waves};
+  if (pf.s1.active) {
+    const uint32_t per = (pf.s1.n + groups - 1) / groups;
+    const size_t goff = (2 + (pf.flip ^ 1)) * area;  // the previous commitment's group sums
+    plan->s1 = FinStage1{w + pf.s1.off, w + goff, pf.s1.n, pf.s1.rows, groups, per,
+                         pf.s1.rows * groups};
+    pf.s2 = PendingStage{true, goff, groups, pf.s1.rows, pf.s1.t_out};
+  } else {
+    pf.s2.active = false;
+  }
+  pf.s1.active = false;
+  return GH_OK;
+}
+
+// After a fused launch: with deferral its finalize is held back as stage s1,
+// else it runs now.
+static int fused_finish(gh_ctx* ctx, const gh_ring* ring, const FusedPlan& plan, uint32_t nparts,
+                        uint32_t rows, void* t_dev, bool defer) {
+  PendingFin& pf = ctx->fin;
+  if (!defer)
+    return launch_finalize_rows(ctx, ring, FinSeg{(const char*)ctx->work + plan.off, t_dev, nparts,
+                                                  rows},
+                                FinSeg{nullptr, nullptr, 0, 0});
+  pf.s1 = PendingStage{true, plan.off, nparts, rows, t_dev};
+  pf.flip ^= 1;
+  return GH_OK;
 }
 
 // q32 commitment: the fused transform+MAC kernel, then the finalize. With
-// `defer` the finalize is left pending and runs as the first blocks of the
-// next deferred commitment's MAC launch, so back-to-back commitments cost one
-// launch each: the next MAC's partial sums go beside the pending ones, and the
-// finalize blocks, dispatched first, read them while the MAC blocks stream
-// the key. (A second stream for the finalize was measured: the cross-queue
-// event hand-off costs more than it overlaps.)
+// `defer` the finalize is held back and runs in two stages inside the next
+// two deferred launches over the same ring (see PendingFin), so back-to-back
+// commitments cost one launch each. (A second stream for the finalize was
+// measured: the cross-queue event hand-off costs more than it overlaps.)
 static int launch_commit_fused(gh_key* key, const void* s_dev, uint32_t cols, void* t_dev,
                                bool defer) {
   gh_ring* ring = key->ring;
@@ -992,60 +1082,34 @@ static int launch_commit_fused(gh_key* key, const void* s_dev, uint32_t cols, vo
   const uint32_t cpb = (cols + nparts - 1) / nparts;
   nparts = (cols + cpb - 1) / cpb;
   const size_t pbytes = ((size_t)nparts * rows * 64 * 4 + 255) & ~(size_t)255;
+  const uint32_t nmac = nparts * (rows / rb);
 
-  // this commitment's partial sums: beside a pending finalize's, in the
-  // workspace as it is (growing it would free them), else from offset 0
-  PendingFin& pf = ctx->fin;
-  size_t off;
-  if (int st = fused_partial_area(ctx, GH_RING_Q32, pbytes, defer, &off)) return st;
-  FinArgs fin{};
-  if (pf.active)
-    fin = FinArgs{FQ32{pf.q, pf.qinv}, (const uint32_t*)pf.tab,
-                  (const uint32_t*)((const char*)ctx->work + pf.off), (uint32_t*)pf.t_out,
-                  pf.nparts, pf.rows, pf.rows};
-
+  FusedPlan plan;
+  if (int st = fused_plan(ctx, ring, s_dev, (size_t)cols * 64 * 4, pbytes, nmac, 64 * waves,
+                          waves, key->dev_a, defer, &plan))
+    return st;
   FQ32 f{(uint32_t)ring->q, ring->qinv32};
   const uint32_t* tab = (const uint32_t*)ring->dev_tab;
   const uint32_t* A = (const uint32_t*)key->dev_a;
   const uint32_t* S = (const uint32_t*)s_dev;
-  uint32_t* P = (uint32_t*)((char*)ctx->work + off);
-  dim3 grid(fin.nblk + nparts, rows / rb);
-  size_t lds_words = (size_t)cpb * kFuseStride;
+  uint32_t* P = (uint32_t*)((char*)ctx->work + plan.off);
+  dim3 grid(plan.s2.nblk + nparts, rows / rb);
+  size_t lds_words = (size_t)cpb * kFuseStride + (size_t)waves * 64;
   if ((size_t)waves * rb * 64 > lds_words) lds_words = (size_t)waves * rb * 64;
   const size_t lds_bytes = lds_words * 4;
   hipEvent_t e0, e1;
   prof_pair(ctx, 1, &e0, &e1);
-#define GH_FUSED(R, W) fused_launch<R, W>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, fin)
+#define GH_FUSED(R, W) fused_launch<R, W>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, plan.s1, plan.s2)
   if (rb == 16) { if (waves == 4) GH_FUSED(16, 4); else if (waves == 8) GH_FUSED(16, 8); else GH_FUSED(16, 16); }
   else if (rb == 8) { if (waves == 4) GH_FUSED(8, 4); else if (waves == 8) GH_FUSED(8, 8); else GH_FUSED(8, 16); }
   else { if (waves == 4) GH_FUSED(4, 4); else if (waves == 8) GH_FUSED(4, 8); else GH_FUSED(4, 16); }
 #undef GH_FUSED
   GH_HIP(hipGetLastError());
-  pf.active = false;  // a pending finalize ran inside this launch
-  if (!defer) return launch_finalize_q32(ctx, f, tab, P, nparts, rows, (uint32_t*)t_dev);
-  pf = PendingFin{true, GH_RING_Q32, (uint32_t)ring->q, ring->qinv32, ring->dev_tab, off, pbytes,
-                  nparts, rows, t_dev};
-  return GH_OK;
-}
-
-// The standalone finalize of a Goldilocks commitment: 16 waves per row.
-static int launch_finalize_gold(gh_ctx* ctx, const uint64_t* tab, const uint64_t* P,
-                                uint32_t nparts, uint32_t rows, uint64_t* t_dev) {
-  hipEvent_t f0, f1;
-  prof_pair(ctx, 2, &f0, &f1);
-  if (rows > (uint32_t)ctx->num_cus)  // 4-wave blocks: all rows in one round
-    hipExtLaunchKernelGGL((k_finalize4<FGold>), dim3(rows), dim3(256), 0, ctx->stream, f0, f1, 0,
-                          FGold{}, tab, P, nparts, rows, (int)kTabUntwist, t_dev);
-  else
-    hipExtLaunchKernelGGL((k_finalize<FGold>), dim3(rows), dim3(1024), 0, ctx->stream, f0, f1, 0,
-                          FGold{}, tab, P, nparts, rows, (int)kTabUntwist, t_dev);
-  GH_HIP(hipGetLastError());
-  return GH_OK;
+  return fused_finish(ctx, ring, plan, nparts, rows, t_dev, defer);
 }
 
 // Goldilocks commitment, per-call form: k_commit_fused_gold, then the
-// finalize, held back with `defer` exactly as for q32 (the partial sums of
-// consecutive commitments alternate between two areas of the workspace).
+// finalize, held back with `defer` exactly as for q32.
 static int launch_commit_fused_gold(gh_key* key, const void* s_dev, uint32_t cols, void* t_dev,
                                     bool defer) {
   gh_ring* ring = key->ring;
@@ -1069,49 +1133,31 @@ static int launch_commit_fused_gold(gh_key* key, const void* s_dev, uint32_t col
   const uint32_t cpb = (cols + nparts - 1) / nparts;
   nparts = (cols + cpb - 1) / cpb;
   const size_t pbytes = ((size_t)nparts * rows * 64 * 8 + 255) & ~(size_t)255;
+  const uint32_t nmac = nparts * (rows / rb);
 
-  PendingFin& pf = ctx->fin;
-  size_t off;
-  if (int st = fused_partial_area(ctx, GH_RING_GOLDILOCKS, pbytes, defer, &off)) return st;
-  FinArgsGold fin{};
-  if (pf.active)
-    fin = FinArgsGold{(const uint64_t*)pf.tab,
-                      (const uint64_t*)((const char*)ctx->work + pf.off), (uint64_t*)pf.t_out,
-                      pf.nparts, pf.rows, pf.rows};
+  FusedPlan plan;
+  if (int st = fused_plan(ctx, ring, s_dev, (size_t)cols * 64 * 8, pbytes, nmac, 64 * kWaves,
+                          kWaves, key->dev_a, defer, &plan))
+    return st;
   const uint64_t* tab = (const uint64_t*)ring->dev_tab;
   const uint64_t* A = (const uint64_t*)key->dev_a;
   const uint64_t* S = (const uint64_t*)s_dev;
-  uint64_t* P = (uint64_t*)((char*)ctx->work + off);
-  dim3 grid(fin.nblk + nparts, rows / rb);
-  size_t lds_words = (size_t)cpb * kFuseStrideGold;
+  uint64_t* P = (uint64_t*)((char*)ctx->work + plan.off);
+  dim3 grid(plan.s2.nblk + nparts, rows / rb);
+  size_t lds_words = (size_t)cpb * kFuseStrideGold + (size_t)kWaves * 64;
   if ((size_t)kWaves * rb * 64 > lds_words) lds_words = (size_t)kWaves * rb * 64;
   const size_t lds_bytes = lds_words * 8;
   hipEvent_t e0, e1;
   prof_pair(ctx, 1, &e0, &e1);
 #define GH_FUSED_GOLD(R)                                                                        \
   hipExtLaunchKernelGGL((k_commit_fused_gold<R, kWaves>), grid, dim3(64 * kWaves), lds_bytes,   \
-                        ctx->stream, e0, e1, 0, FGold{}, tab, A, S, P, rows, cols, cpb, fin)
+                        ctx->stream, e0, e1, 0, FGold{}, tab, A, S, P, rows, cols, cpb, plan.s1, \
+                        plan.s2)
   if (rb == 8) GH_FUSED_GOLD(8);
   else GH_FUSED_GOLD(4);
 #undef GH_FUSED_GOLD
   GH_HIP(hipGetLastError());
-  pf.active = false;  // a pending finalize ran inside this launch
-  if (!defer) return launch_finalize_gold(ctx, tab, P, nparts, rows, (uint64_t*)t_dev);
-  pf = PendingFin{true, GH_RING_GOLDILOCKS, 0, 0, ring->dev_tab, off, pbytes, nparts, rows, t_dev};
-  return GH_OK;
-}
-
-int ctx_flush(gh_ctx* ctx) {
-  PendingFin& pf = ctx->fin;
-  if (!pf.active) return GH_OK;
-  pf.active = false;
-  if (pf.kind == GH_RING_GOLDILOCKS)
-    return launch_finalize_gold(ctx, (const uint64_t*)pf.tab,
-                                (const uint64_t*)((const char*)ctx->work + pf.off), pf.nparts,
-                                pf.rows, (uint64_t*)pf.t_out);
-  return launch_finalize_q32(ctx, FQ32{pf.q, pf.qinv}, (const uint32_t*)pf.tab,
-                             (const uint32_t*)((const char*)ctx->work + pf.off), pf.nparts,
-                             pf.rows, (uint32_t*)pf.t_out);
+  return fused_finish(ctx, ring, plan, nparts, rows, t_dev, defer);
 }
 
 // t = A[:, :cols] * s; s (cols polys) and t (rows polys) are device pointers.

@@ -553,16 +553,21 @@ __global__ __launch_bounds__(256) void k_jl(ChaKey key0, uint32_t rows, uint64_t
     part_acc[((size_t)part * gridDim.x + blockIdx.x) * rows + t] = GOLD ? (int64_t)accm : acc;
 }
 
-// p[m] = sum over blocks; q32: exact int64 then mod q; Goldilocks: modular.
-// One block per row: 256 threads sum strided partials, then a tree in LDS.
+// p[m] = sum over blocks. Goldilocks: modular. q32: each block partial is the
+// exact int64 sum of its columns (below 2^26 * 2^31 in magnitude); p mod q
+// sums them reduced mod q, and the exact p (requested only below 2^26
+// polynomials, |p| < 2^63) sums them in int64. One block per row: 256
+// threads sum strided partials, then a tree in LDS.
 template <bool GOLD>
 __global__ __launch_bounds__(256) void k_jl_reduce(const int64_t* __restrict__ part_acc,
                                                    uint32_t nblk, uint32_t rows, uint64_t q,
                                                    int64_t* __restrict__ p_exact,
                                                    void* __restrict__ p_modq) {
   __shared__ int64_t red[256];
+  __shared__ uint64_t redm[256];
   const uint32_t m = blockIdx.x, t = threadIdx.x;
   int64_t a = 0;
+  uint64_t am = 0;  // q32: the sum mod q
   for (uint32_t b = t; b < nblk; b += 256) {
     const int64_t v = part_acc[(size_t)b * rows + m];
     if (GOLD) {
@@ -570,9 +575,13 @@ __global__ __launch_bounds__(256) void k_jl_reduce(const int64_t* __restrict__ p
       a = (int64_t)(s < x ? s + kGoldEps : (s >= kGoldilocks ? s - kGoldilocks : s));
     } else {
       a += v;
+      int64_t r = v % (int64_t)q;
+      if (r < 0) r += (int64_t)q;
+      am = (am + (uint64_t)r) % q;
     }
   }
   red[t] = a;
+  redm[t] = am;
   __syncthreads();
   for (uint32_t h = 128; h > 0; h >>= 1) {
     if (t < h) {
@@ -581,6 +590,7 @@ __global__ __launch_bounds__(256) void k_jl_reduce(const int64_t* __restrict__ p
         red[t] = (int64_t)(s < x ? s + kGoldEps : (s >= kGoldilocks ? s - kGoldilocks : s));
       } else {
         red[t] += red[t + h];
+        redm[t] = (redm[t] + redm[t + h]) % q;
       }
     }
     __syncthreads();
@@ -589,11 +599,8 @@ __global__ __launch_bounds__(256) void k_jl_reduce(const int64_t* __restrict__ p
   if (GOLD) {
     ((uint64_t*)p_modq)[m] = (uint64_t)red[0];
   } else {
-    const int64_t sum = red[0];
-    if (p_exact) p_exact[m] = sum;
-    int64_t r = sum % (int64_t)q;
-    if (r < 0) r += (int64_t)q;
-    ((uint32_t*)p_modq)[m] = (uint32_t)r;
+    if (p_exact) p_exact[m] = red[0];
+    ((uint32_t*)p_modq)[m] = (uint32_t)redm[0];
   }
 }
 

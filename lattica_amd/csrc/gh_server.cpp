@@ -244,7 +244,6 @@ int validate(gh_server* s, gh_request* r) {
   if ((uint64_t)r->r * r->n > 0xffffffffull || r->r > 0xffff) return GH_E_INVALID_ARGUMENT;
   if (r->ops & GH_OP_COMMIT) {
     if (!r->key || r->key->ring != r->ring || !r->t_out) return GH_E_INVALID_ARGUMENT;
-    if (r->key->rows % 4 != 0) return GH_E_INVALID_ARGUMENT;
     if (r->n > r->key->msg_len + r->key->open_len) return GH_E_DIMENSION_MISMATCH;
   }
   if ((r->ops & GH_OP_GRAM) && !r->g_out) return GH_E_INVALID_ARGUMENT;

@@ -191,3 +191,175 @@ def test_single_item_device_batches_back_to_back(ctx, ring_q32, ring_gold, name)
         for b in bufs:
             b.free()
         key.close()
+
+
+def _commit_raw(la, key, s_ptr, n, t_ptr):
+    assert la.lib().gh_commit(key.h, s_ptr, n, None, 0, la.DOMAIN_COEFF, t_ptr,
+                              la.MEM_DEVICE) == 0
+
+
+@pytest.mark.parametrize("name", ["q32", "goldilocks"])
+def test_commitment_of_a_commitment(ctx, ring_q32, ring_gold, name):
+    """The witness of a device commitment is the previous commitment's output
+    (whole, and as the tail of a longer witness), with no synchronize in
+    between: the held-back finalize that writes it must complete before the
+    next launch reads it."""
+    import lattica_amd as la
+    ring = ring_q32 if name == "q32" else ring_gold
+    wb = ring.info.word_bytes
+    rng = np.random.default_rng(66)
+    k1 = _key(la, ring, rng, 16, 600)
+    k2 = _key(la, ring, rng, 16, 16)       # commits t1 itself
+    k3 = _key(la, ring, rng, 16, 40 + 16)  # commits [w3 || t2]
+    w1 = small_signed(rng, ring.q, (600, 64), -128, 128).astype(ring.dtype)
+    w3 = rand_residues(rng, ring.q, (40, 64)).astype(ring.dtype)
+    s1 = ctx.to_device(w1)
+    t1 = ctx.alloc(16 * 64 * wb)
+    s3 = ctx.alloc(56 * 64 * wb)
+    s3.upload(w3)
+    t2_in_s3 = s3.offset(40 * 64 * wb)
+    t3 = ctx.alloc(16 * 64 * wb)
+    t4 = ctx.alloc(16 * 64 * wb)
+    try:
+        for _ in range(2):
+            _commit_raw(la, k1, s1.ptr, 600, t1.ptr)
+            _commit_raw(la, k2, t1.ptr, 16, t2_in_s3)   # witness = t1 (pending)
+            _commit_raw(la, k3, s3.ptr, 56, t3.ptr)     # witness tail = t2 (pending)
+            _commit_raw(la, k2, t3.ptr, 16, t4.ptr)     # witness = t3 (pending)
+            ctx.synchronize()
+            e1 = k1.commit(w1)
+            e2 = k2.commit(e1)
+            e3 = k3.commit(np.concatenate([w3, e2]))
+            e4 = k2.commit(e3)
+            assert np.array_equal(t1.download(ring.dtype, (16, 64)), e1)
+            assert np.array_equal(s3.download(ring.dtype, (56, 64))[40:], e2)
+            assert np.array_equal(t3.download(ring.dtype, (16, 64)), e3)
+            assert np.array_equal(t4.download(ring.dtype, (16, 64)), e4)
+    finally:
+        for b in (s1, t1, s3, t3, t4):
+            b.free()
+        for k in (k1, k2, k3):
+            k.close()
+
+
+def _hip():
+    import ctypes
+    import glob
+    for cand in ["libamdhip64.so", *sorted(glob.glob("/opt/rocm*/lib/libamdhip64.so*"))]:
+        try:
+            return ctypes.CDLL(cand)
+        except OSError:
+            continue
+    pytest.skip("libamdhip64 not loadable")
+
+
+@pytest.mark.parametrize("name", ["q32", "goldilocks"])
+def test_defer_off_completes_in_stream_order(ctx, ring_q32, ring_gold, name):
+    """With defer_finalize = 0 a device commitment is complete once the device
+    is synchronized through HIP itself (no library call in between)."""
+    import ctypes
+    import lattica_amd as la
+    ring = ring_q32 if name == "q32" else ring_gold
+    rng = np.random.default_rng(67)
+    key = _key(la, ring, rng, 16, 1000)
+    hip = _hip()
+    bufs = []
+    ctx.set_tuning(defer_finalize=0)
+    try:
+        assert ctx.get_tuning()["defer_finalize"] == 0
+        cases = [_case(ctx, ring, key, rng) for _ in range(3)]
+        for s, t, _ in cases:
+            bufs += [s, t]
+        for s, t, _ in cases:
+            key.commit_device(s, key.msg_len, t)
+        assert hip.hipDeviceSynchronize() == 0
+        for s, t, want in cases:
+            out = np.empty((16, 64), dtype=ring.dtype)
+            assert hip.hipMemcpy(ctypes.c_void_p(out.ctypes.data), t.ptr,
+                                 ctypes.c_size_t(out.nbytes), 2) == 0  # DeviceToHost
+            assert np.array_equal(out, want)
+    finally:
+        ctx.set_tuning(defer_finalize=1)
+        for b in bufs:
+            b.free()
+        key.close()
+
+
+def test_turning_defer_off_completes_a_held_back_step(ctx, ring_q32):
+    import ctypes
+    import lattica_amd as la
+    rng = np.random.default_rng(68)
+    key = _key(la, ring_q32, rng, 16, 300)
+    hip = _hip()
+    s, t, want = _case(ctx, ring_q32, key, rng)
+    try:
+        key.commit_device(s, key.msg_len, t)  # held back
+        ctx.set_tuning(defer_finalize=0)      # enqueues it
+        assert hip.hipDeviceSynchronize() == 0
+        out = np.empty((16, 64), dtype=np.uint32)
+        assert hip.hipMemcpy(ctypes.c_void_p(out.ctypes.data), t.ptr,
+                             ctypes.c_size_t(out.nbytes), 2) == 0
+        assert np.array_equal(out, want)
+    finally:
+        ctx.set_tuning(defer_finalize=1)
+        s.free()
+        t.free()
+        key.close()
+
+
+@pytest.mark.parametrize("name", ["q32", "goldilocks"])
+def test_long_pipeline_of_varied_commitments(oracle, ctx, ring_q32, ring_gold, name):
+    """Forty device commitments back to back on one ring (one-item device
+    batches, which take the per-call path with its finalize held back), no
+    synchronize in between: lengths from 1 to 4096 columns and row counts from
+    4 to 64, so the held-back stages of one commitment sometimes fit the next
+    launches and sometimes are issued on their own. One output buffer is
+    reused by three commitments in a row (the last one must win)."""
+    import lattica_amd as la
+    ring = ring_q32 if name == "q32" else ring_gold
+    wb = ring.info.word_bytes
+    rng = np.random.default_rng(69)
+    shapes = [(16, 4096), (16, 512), (4, 300), (64, 1000), (16, 1), (8, 2048), (32, 4096),
+              (12, 777), (16, 4096)]
+    keys, mats = [], []
+    for r, c in shapes:
+        a = rand_residues(rng, ring.q, (r, c, 64)).astype(ring.dtype)
+        keys.append(la.Key(ring, a, domain=la.DOMAIN_NTT))
+        mats.append(a)
+    shared = ctx.alloc(64 * 64 * wb)
+    cases, bufs = [], [shared]
+    try:
+        for i in range(40):
+            ki = int(rng.integers(0, len(keys)))
+            key = keys[ki]
+            n = int(rng.integers(1, key.msg_len + 1)) if i % 3 else key.msg_len
+            w = small_signed(rng, ring.q, (n, 64), -128, 128).astype(ring.dtype)
+            s = ctx.to_device(w)
+            bufs.append(s)
+            if i in (20, 21, 22):
+                t = shared
+            else:
+                t = ctx.alloc(key.rows * 64 * wb)
+                bufs.append(t)
+            cases.append((ki, n, w, s, t, i))
+        for ki, n, w, s, t, _ in cases:
+            keys[ki].commit_batch_device([s], [n], [t])
+        ctx.synchronize()
+        for ki, n, w, s, t, i in cases:
+            if t is shared and i != 22:
+                continue
+            key = keys[ki]
+            got = t.download(ring.dtype, (key.rows, 64))
+            if n <= 64:
+                z = np.zeros((key.rows, 1, 64), dtype=np.uint64)
+                want = oracle.commit(ring.q, 64, mats[ki][:, :n].astype(np.uint64), z,
+                                     w.astype(np.uint64), np.zeros((1, 64), dtype=np.uint64),
+                                     key_domain=1)
+                assert np.array_equal(got.astype(np.uint64), want), (i, key.rows, n)
+            else:
+                assert np.array_equal(got, key.commit_batch([w])[0]), (i, key.rows, n)
+    finally:
+        for b in bufs:
+            b.free()
+        for k in keys:
+            k.close()
