@@ -128,17 +128,48 @@ def goldilocks_batch(ctx, la, log, count=2000, kappa=16, max_cols=1 << 14):
         key.commit_batch_device([wptr[i]], [lens[i]], [optr[i]])
     ctx.synchronize()
     plain = count / (time.perf_counter() - t0)
+    res = {"batched_commits_per_s": round(batched, 1),
+           "graph_replay_commits_per_s": round(graph, 1), "graph_equals_batched": graph_same,
+           "per_call_commits_per_s": round(plain, 1),
+           "mean_len": float(lens.mean()), "oracle_spot_check": ok}
     log(f"Goldilocks batch x{count}: batched {batched:,.0f} commits/s, graph replay {graph:,.0f} "
         f"commits/s (== batched: {graph_same}), per-call {plain:,.0f} commits/s; "
         f"oracle spot check {ok}")
+    if CPU_ON:
+        # the reference's AVX2 prepared commit on the wide Goldilocks path
+        # (L = u64, u64_arith.rs:193-357), restated: a random sample of the
+        # items on one core, a larger one over all cores (whole commitments per
+        # thread), scaled to the batch by witness length (the work is linear in it)
+        res_all = out.download(np.uint64, (count, kappa, D))
+        c = gridref.Avx2Commit(p, 64, a, D)
+        pick = rng.choice(count, 40, replace=False)
+        c.load_items(pool, offs[pick], lens[pick])
+        t1 = once_seconds(lambda: c.run_items(1))
+        cpu_ok = bool(np.array_equal(c.items_result(), res_all[pick]))
+        thr = cpu_threads()
+        pick_all = rng.choice(count, min(count, 40 * thr), replace=False)
+        c.load_items(pool, offs[pick_all], lens[pick_all])
+        t_all = once_seconds(lambda: c.run_items(thr))
+        c.close()
+        cpu_1 = count / (t1 * float(lens.sum()) / float(lens[pick].sum()))
+        cpu_all = count / (t_all * float(lens.sum()) / float(lens[pick_all].sum()))
+        log(f"  CPU AVX2 {cpu_1:,.1f} commits/s on 1 core, {cpu_all:,.1f} on {thr} "
+            f"(sample == GPU: {cpu_ok})")
+        res["cpu_1core"] = {"commits_per_s": round(cpu_1, 2), "kind": "port",
+                            "path": "prepared Ajtai commit, AVX2 L = u64 wide Montgomery "
+                                    "(oracle/gridref_avx2.c)",
+                            "sample": f"40 random items ({int(lens[pick].sum())} witness "
+                                      f"polynomials, {t1:.2f} s), scaled to the batch's "
+                                      f"{int(lens.sum())} by witness length",
+                            "equals_gpu": cpu_ok}
+        res["cpu_all_cores"] = {"commits_per_s": round(cpu_all, 2), "threads": thr,
+                                "sample": f"{len(pick_all)} random items, scaled the same way",
+                                "note": "one commitment per thread at a time"}
     key.close()
     pool_dev.free()
     out.free()
     ring.close()
-    return {"batched_commits_per_s": round(batched, 1),
-            "graph_replay_commits_per_s": round(graph, 1), "graph_equals_batched": graph_same,
-            "per_call_commits_per_s": round(plain, 1),
-            "mean_len": float(lens.mean()), "oracle_spot_check": ok}
+    return res
 
 
 def ct_pair_sum(a, b, q):
@@ -166,6 +197,22 @@ def host_cpu():
         pass
     return {"model": model, "logical_cpus": os.cpu_count(),
             "usable_cpus": len(os.sched_getaffinity(0))}
+
+
+# CPU baselines inside the side workloads (off with --no-cpu, e.g. for profiling)
+CPU_ON = True
+
+
+def cpu_threads():
+    """Host threads for the all-core CPU figures (at most 16: the GPU box's CPU share)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def once_seconds(fn):
+    """Seconds of one call of fn (after the caller's own warm-up)."""
+    a = time.perf_counter()
+    fn()
+    return time.perf_counter() - a
 
 
 def cpu_seconds(fn, min_s=0.5):
@@ -204,16 +251,37 @@ def negacyclic_batch(ctx, la, ring, log, pairs=1 << 18, reps=20):
                                                          hb[idx].astype(np.uint64))))
     for b in (pa, pb, po):
         b.free()
-    # the reference's CPU path (poly_mul_ntt) as restated by the oracle, one core
-    sa, sb = ha[:4096].astype(np.uint64), hb[:4096].astype(np.uint64)
-    cpu_pairs_s = 4096 / cpu_seconds(lambda: gridref.negacyclic_mul(Q, D, sa, sb))
+    res = {"pairs_per_s": round(pairs / dt, 1), "us_per_batch": round(dt * 1e6, 1),
+           "algorithmic_GBps": round(3 * pairs * D * 4 / dt / 1e9, 1), "oracle_sample": ok}
     log(f"negacyclic x{pairs}: {dt * 1e6:,.1f} us ({pairs / dt / 1e9:.2f} G pairs/s, "
-        f"{3 * pairs * D * 4 / dt / 1e9:,.0f} GB/s algorithmic); oracle sample {ok}; "
-        f"CPU 1 core {cpu_pairs_s / 1e6:.2f} M pairs/s")
-    return {"pairs_per_s": round(pairs / dt, 1), "us_per_batch": round(dt * 1e6, 1),
-            "algorithmic_GBps": round(3 * pairs * D * 4 / dt / 1e9, 1), "oracle_sample": ok,
-            "cpu_1core": {"pairs_per_s": round(cpu_pairs_s, 1), "kind": "port",
-                          "sample": "4096 pairs through the oracle's NTT product"}}
+        f"{3 * pairs * D * 4 / dt / 1e9:,.0f} GB/s algorithmic); oracle sample {ok}")
+    if CPU_ON:
+        # the reference's CPU path: negacyclic_mul_coeffs through its AVX2
+        # twisted NTT (poly/ring.rs:83-101, simd/montgomery.rs:688-890),
+        # restated, on a sample of the pairs; the scalar oracle beside it
+        cpu_n = 1 << 16
+        m = gridref.Avx2Negacyclic(Q, 64, D)
+        m.load(ha[:cpu_n].astype(np.uint64), hb[:cpu_n].astype(np.uint64))
+        m.run(1)
+        cpu_1 = cpu_n / once_seconds(lambda: m.run(1))
+        thr = cpu_threads()
+        cpu_all = cpu_n / once_seconds(lambda: m.run(thr))
+        sel = idx < cpu_n
+        cpu_ok = bool(np.array_equal(m.result()[idx[sel]], got[sel])) if np.any(sel) else None
+        m.close()
+        sa, sb = ha[:4096].astype(np.uint64), hb[:4096].astype(np.uint64)
+        oracle_pairs_s = 4096 / cpu_seconds(lambda: gridref.negacyclic_mul(Q, D, sa, sb))
+        log(f"  CPU AVX2 path {cpu_1 / 1e6:.3f} M pairs/s on 1 core, {cpu_all / 1e6:.3f} M on "
+            f"{thr}; scalar oracle {oracle_pairs_s / 1e6:.3f} M on 1 core")
+        res["cpu_1core"] = {"pairs_per_s": round(cpu_1, 1), "kind": "port",
+                            "path": "negacyclic_mul_coeffs via the AVX2 twisted NTT, L = u64 "
+                                    "(oracle/gridref_avx2.c)",
+                            "sample": f"{cpu_n} of the pairs", "equals_gpu": cpu_ok}
+        res["cpu_all_cores"] = {"pairs_per_s": round(cpu_all, 1), "threads": thr,
+                                "note": "pairs split over threads (this build's parallelisation)"}
+        res["cpu_oracle_scalar_1core"] = {"pairs_per_s": round(oracle_pairs_s, 1),
+                                          "sample": "4096 pairs through the scalar oracle"}
+    return res
 
 
 def commit_small(ctx, la, ring, log, batch=1024, reps=200):
@@ -244,13 +312,36 @@ def commit_small(ctx, la, ring, log, batch=1024, reps=200):
         key.commit_batch_device(wptr, [n] * batch, optr)
     ctx.synchronize()
     batched = batch * 10 / (time.perf_counter() - t0)
+    res = {"per_call_commits_per_s": round(per_call, 1), "batched_commits_per_s": round(batched, 1),
+           "batch": batch}
     log(f"commit kappa 8 x n 256: per-call {per_call:,.0f} commits/s, batched ({batch}/call) "
         f"{batched:,.0f} commits/s")
+    if CPU_ON:
+        # the reference's AVX2 prepared commit (L = u64), restated: 256 of the
+        # commitments, one core and all cores (independent commitments per thread)
+        from oracle import gridref
+        gpu = o_dev.download(np.uint32, (batch, kappa, D))
+        pool = np.where(t < 0, Q - 1, t).astype(np.uint64).reshape(batch * n, D)
+        cpu_n = 256
+        c = gridref.Avx2Commit(Q, 64, a.astype(np.uint64), D)
+        c.load_items(pool[:cpu_n * n], [i * n for i in range(cpu_n)], [n] * cpu_n)
+        c.run_items(1)
+        cpu_1 = cpu_n / once_seconds(lambda: c.run_items(1))
+        thr = cpu_threads()
+        cpu_all = cpu_n / once_seconds(lambda: c.run_items(thr))
+        cpu_ok = bool(np.array_equal(c.items_result(), gpu[:cpu_n].astype(np.uint64)))
+        c.close()
+        log(f"  CPU AVX2 {cpu_1:,.0f} commits/s on 1 core, {cpu_all:,.0f} on {thr} "
+            f"(== GPU: {cpu_ok})")
+        res["cpu_1core"] = {"commits_per_s": round(cpu_1, 1), "kind": "port",
+                            "path": "prepared Ajtai commit, AVX2 L = u64 (oracle/gridref_avx2.c)",
+                            "sample": f"{cpu_n} of the commitments", "equals_gpu": cpu_ok}
+        res["cpu_all_cores"] = {"commits_per_s": round(cpu_all, 1), "threads": thr,
+                                "note": "one commitment per thread at a time"}
     key.close()
     s_dev.free()
     o_dev.free()
-    return {"per_call_commits_per_s": round(per_call, 1), "batched_commits_per_s": round(batched, 1),
-            "batch": batch}
+    return res
 
 
 def labrador_calls(ctx, la, ring, log, reps=10):
@@ -355,62 +446,92 @@ def labrador_calls(ctx, la, ring, log, reps=10):
     phi = phi_dev.download(np.uint32, (r_ * n_, D))
     res["jl_aggregate_pairs_with_evals"] = bool(
         ct_pair_sum(phi, s.reshape(-1, D), Q) == ct_pair_sum(scale, ev, Q))
-    # the reference's CPU path as restated by the oracle, one core, on bounded
-    # samples scaled to the full call (Gram per pair, JL per part, evaluations
-    # per witness polynomial, inner commitments on the AVX2 restatement)
-    s64 = s.astype(np.uint64)
-    seed_b = bytes(range(32))
-    g_s = cpu_seconds(lambda: gridref.gram(Q, D, s64[:4])) / 10 * npairs
-    jl_s = cpu_seconds(lambda: gridref.jl_project(Q, seed_b, rows, s64[:1].reshape(1, -1), D)) * r_
-    ev_s = cpu_seconds(lambda: gridref.jl_evals(Q, D, seed_b, rows, s64[:1, :4])) / 4 * r_ * n_
-    # the aggregation's JL term is rows x r x n ring products (aggregation.rs:237-245);
-    # timed as 16384 of the oracle's NTT products, conjugation and sums not counted
-    pa = rng.integers(0, Q, size=(16384, D), dtype=np.uint64)
-    pb = rng.integers(0, Q, size=(16384, D), dtype=np.uint64)
-    ag_s = cpu_seconds(lambda: gridref.negacyclic_mul(Q, D, pa, pb)) / 16384 * rows * r_ * n_
-    c = gridref.Avx2Commit(Q, 64, a.astype(np.uint64), D)
-    c.load(s64[0])
-    ic_s = cpu_seconds(lambda: c.run(1)) * r_
-    c.close()
-    # the reference's own algorithm for these two calls multiplies in the
-    # coefficient domain, a full NTT product per ring product (crs.rs:211-256,
-    # garbage.rs:76-93); timed on 1 commitment over 128 columns and on 2 parts
-    # of 128 polynomials (3 pairs), scaled to the call
-    zk = np.zeros((KAPPA, 1, D), dtype=np.uint64)
-    zr = np.zeros((1, D), dtype=np.uint64)
-    a128 = a[:, :128].astype(np.uint64)   # the key's words taken as coefficients
-    icc_s = cpu_seconds(lambda: gridref.commit(Q, D, a128, zk, s64[0, :128], zr,
-                                               path="coeff")) * r_ * (n_ / 128)
-    gc_s = cpu_seconds(lambda: gridref.gram(Q, D, s64[:2, :128], ntt_once=False)) / 3 * npairs * (n_ / 128)
-    res["cpu_1core_us"] = {"inner_commit_32x1024": round(ic_s * 1e6, 1),
-                           "gram_32x1024": round(g_s * 1e6, 1),
-                           "jl_256x2^21": round(jl_s * 1e6, 1),
-                           "jl_evals_256x32x1024": round(ev_s * 1e6, 1),
-                           "jl_aggregate_256x32x1024": round(ag_s * 1e6, 1),
-                           "coefficient_domain": {
-                               "inner_commit_32x1024": round(icc_s * 1e6, 1),
-                               "gram_32x1024": round(gc_s * 1e6, 1),
-                               "note": "the reference's algorithm: every ring product a full "
-                                       "NTT multiply; 1 commitment over 128 columns and 3 pairs "
-                                       "of 128 polynomials, scaled"},
-                           "kind": "port",
-                           "sample": "NTT once: Gram over 4 of the 32 parts (10 pairs), JL over "
-                                     "1 part, evaluations over 4 polynomials of 1 part, "
-                                     "aggregation as 16384 NTT products scaled to its "
-                                     "256 x 32768 products, 1 inner commitment on "
-                                     "the AVX2 restatement; scaled to the call"}
+    if CPU_ON:
+        # The reference's CPU path on this host. Inner commitments and the Gram
+        # matrix in full: the reference's own algorithm multiplies in the
+        # coefficient domain, one negacyclic_mul_coeffs per ring product
+        # (crs.rs:211-216 -> RingMat::mul_slice; garbage.rs:76-93 ->
+        # Ring::dot_product), through its AVX2 twisted NTT (restated in
+        # oracle/gridref_avx2.c), and, labelled apart, the prepared (NTT-domain
+        # key) commitments and the NTT-once Gram of the scalar oracle. The JL
+        # projection in full through the oracle (project_and_extract_seq is scalar
+        # code, jl.rs:388-508). Evaluations and aggregation on bounded samples.
+        s64 = s.astype(np.uint64)
+        seed_b = bytes(range(32))
+        thr = cpu_threads()
+        t_gpu = t_dev.download(np.uint32, (r_, KAPPA, D)).astype(np.uint64)
+        dots = gridref.Avx2Dots(Q, 64, D)
+        # inner commitments, coefficient-domain key (its words taken as coefficients)
+        a_coef = a.astype(np.uint64)
+        dots.load(a_coef, s64, [row for p_ in range(r_) for row in range(KAPPA)],
+                  [p_ for p_ in range(r_) for row in range(KAPPA)])
+        icc_s = once_seconds(lambda: dots.run(1))
+        icc_all = once_seconds(lambda: dots.run(thr))
+        zk = np.zeros((KAPPA, 1, D), dtype=np.uint64)
+        zr = np.zeros((1, D), dtype=np.uint64)
+        icc_ok = bool(np.array_equal(dots.result()[:KAPPA],
+                                     gridref.commit(Q, D, a_coef, zk, s64[0], zr)))
+        pairs = [(i, j) for i in range(r_) for j in range(i, r_)]
+        dots.load(s64, s64, [i for i, _ in pairs], [j for _, j in pairs])
+        gc_s = once_seconds(lambda: dots.run(1))
+        gc_all = once_seconds(lambda: dots.run(thr))
+        gc_ok = bool(np.array_equal(dots.result(), g.astype(np.uint64)))
+        dots.close()
+        # prepared commitments (NTT-domain key), AVX2 L = u64, all 32 parts
+        c = gridref.Avx2Commit(Q, 64, a.astype(np.uint64), D)
+        c.load_items(s64.reshape(r_ * n_, D), [i * n_ for i in range(r_)], [n_] * r_)
+        ic_s = once_seconds(lambda: c.run_items(1))
+        ic_all = once_seconds(lambda: c.run_items(thr))
+        ic_ok = bool(np.array_equal(c.items_result(), t_gpu))
+        c.close()
+        g_s = once_seconds(lambda: gridref.gram(Q, D, s64))
+        jl_s = once_seconds(lambda: gridref.jl_project(Q, seed_b, rows, s64.reshape(r_, -1), D))
+        ev_s = cpu_seconds(lambda: gridref.jl_evals(Q, D, seed_b, rows, s64[:1, :4])) / 4 * r_ * n_
+        # the aggregation's JL term is rows x r x n ring products (aggregation.rs:237-245);
+        # timed as 16384 of the oracle's NTT products, conjugation and sums not counted
+        pa = rng.integers(0, Q, size=(16384, D), dtype=np.uint64)
+        pb = rng.integers(0, Q, size=(16384, D), dtype=np.uint64)
+        ag_s = cpu_seconds(lambda: gridref.negacyclic_mul(Q, D, pa, pb)) / 16384 * rows * r_ * n_
+        res["cpu_1core_us"] = {
+            "inner_commit_32x1024": round(icc_s * 1e6, 1),
+            "gram_32x1024": round(gc_s * 1e6, 1),
+            "jl_256x2^21": round(jl_s * 1e6, 1),
+            "path": "the reference's algorithm: inner commitments and Gram with every ring "
+                    "product a negacyclic_mul_coeffs through the AVX2 twisted NTT (L = u64), "
+                    "in full (524288 and 540672 products); JL projection in full through "
+                    "the scalar oracle",
+            "checks": {"inner_commit_part0_equals_oracle": icc_ok, "gram_equals_gpu": gc_ok},
+            "prepared_key_ntt_once": {"inner_commit_32x1024": round(ic_s * 1e6, 1),
+                                      "gram_32x1024": round(g_s * 1e6, 1),
+                                      "inner_commit_equals_gpu": ic_ok,
+                                      "note": "NTT-domain key: AVX2 prepared commit of all 32 "
+                                              "parts; Gram with each witness transformed once "
+                                              "(scalar oracle)"},
+            "jl_evals_256x32x1024": round(ev_s * 1e6, 1),
+            "jl_aggregate_256x32x1024": round(ag_s * 1e6, 1),
+            "kind": "port",
+            "sample": "inner commitments, Gram, prepared commitments and JL projection: the "
+                      "whole call; evaluations over 4 polynomials of 1 part and aggregation as "
+                      "16384 NTT products, scaled to the call"}
+        res["cpu_all_cores_us"] = {"threads": thr,
+                                   "inner_commit_32x1024": round(icc_all * 1e6, 1),
+                                   "gram_32x1024": round(gc_all * 1e6, 1),
+                                   "prepared_inner_commit_32x1024": round(ic_all * 1e6, 1),
+                                   "note": "products / parts split over threads (this build's "
+                                           "parallelisation)"}
     log(f"LaBRADOR r 32 x 1024: inner commits {res['inner_commit_32x1024']['us_per_call']:,.0f} us "
         f"({res['inner_commit_32x1024']['commits_per_s']:,.0f} commits/s), Gram "
         f"{res['gram_32x1024']['us_per_call']:,.0f} us, JL {res['jl_256x2^21']['us_per_call']:,.0f} us; "
         f"oracle Gram entry {res['oracle_gram_entry']}, JL mod q == exact mod q "
         f"{res['jl_modq_matches_exact']}")
-    cpu1 = res["cpu_1core_us"]
-    log(f"  CPU 1 core (oracle, scaled samples): inner commits {cpu1['inner_commit_32x1024'] / 1e3:,.1f} "
-        f"ms, Gram {cpu1['gram_32x1024'] / 1e3:,.1f} ms, JL {cpu1['jl_256x2^21'] / 1e3:,.1f} ms, "
-        f"JL evals {cpu1['jl_evals_256x32x1024'] / 1e6:,.1f} s, JL aggregation "
-        f"{cpu1['jl_aggregate_256x32x1024'] / 1e6:,.1f} s; coefficient-domain products: "
-        f"inner commits {cpu1['coefficient_domain']['inner_commit_32x1024'] / 1e3:,.1f} ms, "
-        f"Gram {cpu1['coefficient_domain']['gram_32x1024'] / 1e3:,.1f} ms")
+    cpu1 = res.get("cpu_1core_us")
+    if cpu1:
+        log(f"  CPU 1 core (reference algorithm, AVX2): inner commits "
+            f"{cpu1['inner_commit_32x1024'] / 1e3:,.1f} ms, Gram {cpu1['gram_32x1024'] / 1e3:,.1f} ms, "
+            f"JL {cpu1['jl_256x2^21'] / 1e3:,.1f} ms; prepared key / NTT once: inner commits "
+            f"{cpu1['prepared_key_ntt_once']['inner_commit_32x1024'] / 1e3:,.1f} ms, Gram "
+            f"{cpu1['prepared_key_ntt_once']['gram_32x1024'] / 1e3:,.1f} ms; checks {cpu1['checks']}, "
+            f"prepared == GPU {cpu1['prepared_key_ntt_once']['inner_commit_equals_gpu']}")
     log(f"  JL evals {res['jl_evals_256x32x1024']['us_per_call']:,.0f} us (ct == p "
         f"{res['jl_evals_constant_terms_equal_p']}; uniform residues "
         f"{res['jl_evals_256x32x1024_uniform']['us_per_call']:,.0f} us), JL aggregation "
@@ -493,6 +614,63 @@ def wire_formats(ctx, la, ring, log, reps=5):
     return res
 
 
+def serving_cpu(reqs, host_keys, host_pools, plain_bytes, served_bytes, la, log, sample=64,
+                seed=56):
+    """The same trace through the reference's CPU path, one request at a time
+    on one core: commit requests as AVX2 prepared commitments of their parts
+    (L = u64; the wide Montgomery path for Goldilocks), Gram requests as the
+    reference's coefficient-domain dot products (every ring product a
+    negacyclic_mul_coeffs through the AVX2 twisted NTT). Timed on a random
+    sample of the requests, scaled to the trace by request count; each CPU
+    result is compared with the plain GPU result's bytes."""
+    from oracle import gridref
+    rng = np.random.default_rng(seed)
+    pick = sorted(rng.choice(len(reqs), min(sample, len(reqs)), replace=False))
+    qs = {"q32": Q, "gold": la.GOLDILOCKS}
+    commit_h = {n: gridref.Avx2Commit(qs[n], 64, host_keys[n].astype(np.uint64), D)
+                for n in ("q32", "gold")}
+    dots = {n: gridref.Avx2Dots(qs[n], 64, D) for n in ("q32", "gold")}
+    times, ok, commits = [], True, 0
+    for i in pick:
+        q = reqs[i]
+        w = host_pools[q["name"]][q["off"]:q["off"] + q["r"] * q["n"]]
+        if q["op"] == la.OP_COMMIT:
+            h = commit_h[q["name"]]
+            h.load_items(w, [k * q["n"] for k in range(q["r"])], [q["n"]] * q["r"])
+            times.append(once_seconds(lambda: h.run_items(1)))
+            out = h.items_result()
+            commits += q["r"]
+        else:
+            parts = w.reshape(q["r"], q["n"], D)
+            prs = [(a_, b_) for a_ in range(q["r"]) for b_ in range(a_, q["r"])]
+            h = dots[q["name"]]
+            h.load(parts, parts, [a_ for a_, _ in prs], [b_ for _, b_ in prs])
+            times.append(once_seconds(lambda: h.run(1)))
+            out = h.result()
+        dt = np.uint32 if q["wb"] == 4 else np.uint64
+        gpu = plain_bytes[q["o"]:q["o"] + q["ob"]].view(dt).astype(np.uint64)
+        ok = ok and bool(np.array_equal(out.reshape(-1), gpu))
+    for h in list(commit_h.values()) + list(dots.values()):
+        h.close()
+    t = np.array(times)
+    scale = len(reqs) / len(pick)
+    total_commits = sum(q["r"] for q in reqs if q["op"] == la.OP_COMMIT)
+    est = float(t.sum()) * scale
+    log(f"serving CPU 1 core (sample of {len(pick)}): {len(reqs) / est:,.1f} req/s, "
+        f"{total_commits / est:,.1f} commits/s; per-request service p50 "
+        f"{np.percentile(t, 50) * 1e3:,.2f} ms p99 {np.percentile(t, 99) * 1e3:,.2f} ms; "
+        f"== GPU {ok}")
+    return {"requests_per_s": round(len(reqs) / est, 2),
+            "commits_per_s": round(total_commits / est, 2), "kind": "port",
+            "service_p50_us": round(float(np.percentile(t, 50)) * 1e6, 1),
+            "service_p99_us": round(float(np.percentile(t, 99)) * 1e6, 1),
+            "path": "AVX2 prepared commitments (L = u64) and the reference's "
+                    "coefficient-domain Gram (oracle/gridref_avx2.c), one request at a time",
+            "sample": f"{len(pick)} random requests of the trace ({commits} commitments), "
+                      f"scaled to its {len(reqs)} requests",
+            "equals_gpu": ok}
+
+
 def serving(ctx, la, log, count=1000, seed=55):
     """Serving: a Poisson arrival stream of mixed commit and Gram requests, half
     q32 and half Goldilocks (d = 64, kappa = 16 keys of 2^15 columns, both
@@ -509,7 +687,7 @@ def serving(ctx, la, log, count=1000, seed=55):
     rng = np.random.default_rng(seed)
     kappa, cols, pool_polys = 16, 1 << 15, 1 << 15
     rings = {"q32": la.Ring(ctx, Q, D), "gold": la.Ring(ctx, la.GOLDILOCKS, D)}
-    keys, pools = {}, {}
+    keys, pools, host_keys, host_pools = {}, {}, {}, {}
     for name, ring in rings.items():
         if name == "q32":
             a = rng.integers(0, Q, size=(kappa, cols, D), dtype=np.uint32)
@@ -519,10 +697,11 @@ def serving(ctx, la, log, count=1000, seed=55):
                                                        dtype=np.uint64)) % np.uint64(la.GOLDILOCKS)
             del hi
         keys[name] = la.Key(ring, a, domain=la.DOMAIN_NTT)
-        del a
+        host_keys[name] = a
         x = rng.integers(-128, 128, size=(pool_polys, D), dtype=np.int64)
         lift = np.where(x < 0, np.uint64(ring.q) - (-x).astype(np.uint64), x.astype(np.uint64))
         pools[name] = ctx.to_device(lift.astype(ring.dtype))
+        host_pools[name] = lift
     reqs, out_bytes = [], 0
     for i in range(count):
         name = "q32" if rng.random() < 0.5 else "gold"
@@ -535,7 +714,7 @@ def serving(ctx, la, log, count=1000, seed=55):
         off = int(rng.integers(0, pool_polys - r * n + 1))
         ob = r * kappa * D * wb if op == la.OP_COMMIT else r * (r + 1) // 2 * D * wb
         reqs.append({"name": name, "op": op, "r": r, "n": n, "w": pools[name].ptr.value + off * D * wb,
-                     "o": out_bytes, "ob": ob, "wb": wb})
+                     "off": off, "o": out_bytes, "ob": ob, "wb": wb})
         out_bytes += (ob + 255) // 256 * 256
     outs = {"plain": ctx.alloc(out_bytes), "served": ctx.alloc(out_bytes)}
     n_commits = sum(q["r"] for q in reqs if q["op"] == la.OP_COMMIT)
@@ -651,6 +830,8 @@ def serving(ctx, la, log, count=1000, seed=55):
     a = outs["plain"].download(np.uint8, (out_bytes,))
     b = outs["served"].download(np.uint8, (out_bytes,))
     res["served_equals_plain"] = bool(np.array_equal(a, b))
+    if CPU_ON:
+        res["cpu_1core"] = serving_cpu(reqs, host_keys, host_pools, a, b, la, log)
     log(f"serving saturated: plain {sat_plain['commits_per_s']:,.0f} commits/s "
         f"({sat_plain['requests_per_s']:,.0f} req/s), served {sat_served['commits_per_s']:,.0f} "
         f"commits/s ({sat_served['requests_per_s']:,.0f} req/s, p99 {sat_served['p99_us']:,.0f} us); "
@@ -676,9 +857,11 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--only", choices=["small", "negacyclic", "labrador", "goldilocks", "serving",
-                                       "wire"],
+                                       "wire", "hbm"],
                     help="run one side workload alone (for profiling) and print its JSON")
     args = ap.parse_args()
+    global CPU_ON
+    CPU_ON = not args.no_cpu
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}, "
@@ -698,7 +881,11 @@ def main():
                 "labrador": lambda: labrador_calls(ctx, la, ring, log),
                 "goldilocks": lambda: goldilocks_batch(ctx, la, log),
                 "serving": lambda: serving(ctx, la, log),
-                "wire": lambda: wire_formats(ctx, la, ring, log)}
+                "wire": lambda: wire_formats(ctx, la, ring, log),
+                # streaming read and copy probes of known size (1 GiB): the
+                # calibration of rocprofv3's FETCH_SIZE / WRITE_SIZE
+                "hbm": lambda: dict(zip(("read_GBps", "copy_GBps"),
+                                        ctx.measure_bandwidth(1 << 30, 3)))}
         print(json.dumps({"workload": args.only, "result": legs[args.only]()}), flush=True)
         ring.close()
         ctx.close()
@@ -794,6 +981,32 @@ def main():
             step(i, 0)
         ctx.synchronize()
         extra["single_key_commits_per_s"] = round(args.steps / (time.perf_counter() - a), 1)
+        # the same with 5 keys in rotation (670 MB of keys against the 256 MB
+        # Infinity Cache): equal to the 3-key rate when 3 already defeat the cache
+        more = [la.Key(ring, rng.integers(0, Q, size=(KAPPA, N, D), dtype=np.uint32),
+                       domain=la.DOMAIN_NTT) for _ in range(max(0, 5 - len(keys)))]
+        five = (keys + more)[:5]
+        nrot = max(args.steps, 200)
+        for i in range(50):
+            five[i % 5].commit_device(s_dev[i % nw], N, t_dev[i % nw])
+        ctx.synchronize()
+        a = time.perf_counter()
+        for i in range(nrot):
+            five[i % 5].commit_device(s_dev[i % nw], N, t_dev[i % nw])
+        ctx.synchronize()
+        r5 = nrot / (time.perf_counter() - a)
+        a = time.perf_counter()
+        for i in range(nrot):
+            step(i)
+        ctx.synchronize()
+        r3 = nrot / (time.perf_counter() - a)
+        extra["keys_in_rotation_commits_per_s"] = {"1": extra["single_key_commits_per_s"],
+                                                   str(len(keys)): round(r3, 1),
+                                                   "5": round(r5, 1), "commits_timed": nrot}
+        for k in more:
+            k.close()
+        log(f"keys in rotation: 1 -> {extra['single_key_commits_per_s']:,.0f}, {len(keys)} -> "
+            f"{r3:,.0f}, 5 -> {r5:,.0f} commits/s")
         # measured HBM ceiling: streaming read / copy over 1 GiB, and a streaming
         # read of one commitment's bytes (three buffers in turn, so none is
         # served from the Infinity Cache), launch ramp and tail included

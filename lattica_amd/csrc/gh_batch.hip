@@ -18,6 +18,7 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <cstring>
 #include <new>
 #include <numeric>
 #include <vector>
@@ -488,13 +489,29 @@ int launch_commit_batch(gh_key* key, const gh_commit_item* items, size_t count) 
   int st = ctx_workspace(ctx, sizes.total(), &ws);
   if (st) return st;
   char* base = (char*)ws;
-  // the host images stay in the context until the next batch: the uploads are asynchronous
   BatchLayout& L = ctx->batch_layout;
   batch_layout(key, items, (uint32_t)count, cc, base, L);
-  GH_HIP(hipMemcpyAsync(base + L.s_bytes + L.p_bytes, L.desc.data(),
-                        L.desc.size() * sizeof(BatchDesc), hipMemcpyHostToDevice, ctx->stream));
-  GH_HIP(hipMemcpyAsync(base + L.s_bytes + L.p_bytes + L.d_bytes, L.seg.data(),
-                        L.seg.size() * sizeof(BatchSeg), hipMemcpyHostToDevice, ctx->stream));
+  // descriptors and segments go up in one asynchronous copy from pinned
+  // staging; the previous batch's copy must have read the staging first
+  const size_t dbytes = L.desc.size() * sizeof(BatchDesc), sbytes = L.seg.size() * sizeof(BatchSeg);
+  if (ctx->upload_pending) GH_HIP(hipEventSynchronize(ctx->upload_done));
+  ctx->upload_pending = false;
+  if (!ctx->upload_done) GH_HIP(hipEventCreateWithFlags(&ctx->upload_done, hipEventDisableTiming));
+  if (ctx->upload_bytes < L.d_bytes + sbytes) {
+    if (ctx->upload_host) GH_HIP(hipHostFree(ctx->upload_host));
+    ctx->upload_host = nullptr;
+    ctx->upload_bytes = 0;
+    const size_t want = std::max<size_t>(L.d_bytes + sbytes, (size_t)1 << 16);
+    GH_HIP(hipHostMalloc(&ctx->upload_host, want, hipHostMallocDefault));
+    ctx->upload_bytes = want;
+  }
+  char* h = (char*)ctx->upload_host;
+  std::memcpy(h, L.desc.data(), dbytes);
+  std::memcpy(h + L.d_bytes, L.seg.data(), sbytes);
+  GH_HIP(hipMemcpyAsync(base + L.s_bytes + L.p_bytes, h, L.d_bytes + sbytes,
+                        hipMemcpyHostToDevice, ctx->stream));
+  GH_HIP(hipEventRecord(ctx->upload_done, ctx->stream));
+  ctx->upload_pending = true;
   return batch_enqueue(key, ctx->stream, L, base, cc, ctx->tune.batch_group,
                        ctx->tune.mac_variant);
 }

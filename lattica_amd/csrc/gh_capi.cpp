@@ -228,6 +228,8 @@ GH_API int gh_ctx_destroy(gh_ctx* ctx) {
   if (ctx->stage) hipFree(ctx->stage);
   if (ctx->misc) hipFree(ctx->misc);
   for (hipEvent_t e : ctx->prof_pool) hipEventDestroy(e);
+  if (ctx->upload_done) hipEventDestroy(ctx->upload_done);
+  if (ctx->upload_host) hipHostFree(ctx->upload_host);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
   return GH_OK;

@@ -86,7 +86,12 @@ struct FinStage1 {
   const void* src;  // partial sums [rows][n][64]; never null (a dummy when ntasks = 0)
   void* dst;        // group sums [rows][groups][64]
   uint32_t n, rows, groups, per;  // per <= threads / 16
-  uint32_t ntasks;                // rows * groups, at most the launch's MAC blocks
+  uint32_t ntasks;                // rows * groups
+  // blocks per row block, after the stage-2 blocks and before the MAC blocks,
+  // that run only stage-1 tasks (when there are more tasks than MAC blocks);
+  // MAC block (x, y) takes task y * nparts + x, extra block (x, y) task
+  // nparts * row blocks + y * extra + x
+  uint32_t extra;
 };
 struct FinStage2 {
   const void* src;  // group sums [rows][n][64]
@@ -155,6 +160,18 @@ __device__ __forceinline__ void stage1_store(const F& f, const FinStage1& s, uin
   W acc = lds[threadIdx.x];
   for (int w = 1; w < waves; w++) acc = f.add(acc, lds[w * 64 + threadIdx.x]);
   reinterpret_cast<W*>(s.dst)[((size_t)r * s.groups + g) * 64 + threadIdx.x] = acc;
+}
+
+// A block that only runs a stage-1 task (a launch with more tasks than MAC
+// blocks); `lds` holds waves * 64 words.
+template <class F>
+__device__ __forceinline__ void stage1_only(const F& f, const FinStage1& s, uint32_t bid,
+                                            typename F::W* lds, int waves) {
+  typename F::W v[4];
+  stage1_load(s, bid, v);
+  stage1_wave_sum(f, v, lds);
+  __syncthreads();
+  stage1_store(f, s, bid, lds, waves);
 }
 
 // Sum of the n polynomials [row][0..n)[64] of `src` by G waves (lane = slot),

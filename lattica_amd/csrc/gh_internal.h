@@ -124,8 +124,13 @@ struct gh_ctx {
   size_t stage_bytes = 0;
   // small fixed device buffer (flags)
   void* misc = nullptr;
-  // host image of the last batch's descriptors (kept alive across the async upload)
+  // pinned host staging of batch descriptors: the upload of one batch reads it
+  // asynchronously, so the next batch waits for `upload_done` before reuse
   gh::BatchLayout batch_layout;
+  void* upload_host = nullptr;
+  size_t upload_bytes = 0;
+  hipEvent_t upload_done = nullptr;
+  bool upload_pending = false;
   // phase timing (gh_ctx_profile_*): event pairs recorded per commit
   bool prof_on = false;
   std::vector<hipEvent_t> prof_pool;

@@ -404,16 +404,22 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_q32(
   // cols_per_block * kFuseStride witness words, then WAVES * 64 stage-1 words;
   // WAVES * RB * 64 words for the block reduction
   extern __shared__ uint32_t lds[];
-  if (blockIdx.x < s2.nblk) {
-    // stage 2: the group sums were complete when this launch began
-    const uint32_t row = blockIdx.x * WAVES + (threadIdx.x >> 6);
-    if (blockIdx.y == 0 && row < s2.rows)
-      sum_rows_inv<FQ32, 1>(f, tab, (int)kTabUntwistMac, (const uint32_t*)s2.src, s2.n, row, 0,
-                            (uint32_t*)s2.out, nullptr);
+  const uint32_t lead = s2.nblk + s1.extra;
+  const uint32_t nparts = gridDim.x - lead;
+  if (blockIdx.x < lead) {
+    if (blockIdx.x < s2.nblk) {
+      // stage 2: the group sums were complete when this launch began
+      const uint32_t row = blockIdx.x * WAVES + (threadIdx.x >> 6);
+      if (blockIdx.y == 0 && row < s2.rows)
+        sum_rows_inv<FQ32, 1>(f, tab, (int)kTabUntwistMac, (const uint32_t*)s2.src, s2.n, row,
+                              0, (uint32_t*)s2.out, nullptr);
+    } else {
+      stage1_only(f, s1, nparts * gridDim.y + blockIdx.y * s1.extra + (blockIdx.x - s2.nblk),
+                  lds, WAVES);
+    }
     return;
   }
-  const uint32_t bx = blockIdx.x - s2.nblk;
-  const uint32_t nparts = gridDim.x - s2.nblk;
+  const uint32_t bx = blockIdx.x - lead;
   const uint32_t bid = blockIdx.y * nparts + bx;  // stage-1 task
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -564,15 +570,21 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_gold(
   // cols_per_block * kFuseStrideGold witness words, then WAVES * 64 stage-1
   // words; WAVES * RB * 64 words for the block reduction
   extern __shared__ uint64_t lds64[];
-  if (blockIdx.x < s2.nblk) {
-    const uint32_t row = blockIdx.x * WAVES + (threadIdx.x >> 6);
-    if (blockIdx.y == 0 && row < s2.rows)
-      sum_rows_inv<FGold, 1>(f, tab, (int)kTabUntwist, (const uint64_t*)s2.src, s2.n, row, 0,
-                             (uint64_t*)s2.out, nullptr);
+  const uint32_t lead = s2.nblk + s1.extra;
+  const uint32_t nparts = gridDim.x - lead;
+  if (blockIdx.x < lead) {
+    if (blockIdx.x < s2.nblk) {
+      const uint32_t row = blockIdx.x * WAVES + (threadIdx.x >> 6);
+      if (blockIdx.y == 0 && row < s2.rows)
+        sum_rows_inv<FGold, 1>(f, tab, (int)kTabUntwist, (const uint64_t*)s2.src, s2.n, row, 0,
+                               (uint64_t*)s2.out, nullptr);
+    } else {
+      stage1_only(f, s1, nparts * gridDim.y + blockIdx.y * s1.extra + (blockIdx.x - s2.nblk),
+                  lds64, WAVES);
+    }
     return;
   }
-  const uint32_t bx = blockIdx.x - s2.nblk;
-  const uint32_t nparts = gridDim.x - s2.nblk;
+  const uint32_t bx = blockIdx.x - lead;
   const uint32_t bid = blockIdx.y * nparts + bx;  // stage-1 task
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, t = threadIdx.x & 3;
   const uint32_t row0 = blockIdx.y * RB;
@@ -990,26 +1002,23 @@ struct FusedPlan {
   FinStage2 s2{};
 };
 
-// Places a fused commitment's partial sums (pbytes; nmac MAC blocks of
-// `threads` threads) and, when it defers, the held-back stages its launch
-// runs. Pending finalizes are issued first when the launch cannot carry them:
-// another ring, no deferral, a witness overlapping a pending t_out (inside one
-// launch nothing orders the MAC blocks' witness reads after the stage-2
-// writes), a stage-1 task list longer than the launch's MAC blocks, or
-// partial sums larger than the workspace's areas.
+// Places a fused commitment's partial sums (pbytes; a grid of nparts MAC
+// blocks per row block, nrb row blocks, `threads` threads each) and, when it
+// defers, the held-back stages its launch runs; a stage-1 task list longer
+// than the MAC blocks gets extra blocks. Pending finalizes are issued first
+// when the launch cannot carry them: another ring, no deferral, a witness
+// overlapping a pending t_out (inside one launch nothing orders the MAC
+// blocks' witness reads after the stage-2 writes), or partial sums larger than
+// the workspace's areas.
 static int fused_plan(gh_ctx* ctx, const gh_ring* ring, const void* s_dev, size_t s_bytes,
-                      size_t pbytes, uint32_t nmac, uint32_t threads, uint32_t waves,
-                      const void* dummy, bool defer, FusedPlan* plan) {
+                      size_t pbytes, uint32_t nparts, uint32_t nrb, uint32_t threads,
+                      uint32_t waves, const void* dummy, bool defer, FusedPlan* plan) {
   PendingFin& pf = ctx->fin;
   const size_t wb = ring->word_bytes;
   const uint32_t per_pass = threads / 16;  // partials a block sums in one stage-1 task
   const size_t cap = (ctx->work_bytes / 4) & ~(size_t)255;
   uint32_t groups = 0;
   bool keep = defer && pf.ring == ring && pbytes <= cap;
-  if (keep && pf.s1.active) {
-    groups = (pf.s1.n + per_pass - 1) / per_pass;
-    if ((uint64_t)pf.s1.rows * groups > nmac) keep = false;
-  }
   if (keep) {
     for (const PendingStage* st : {&pf.s1, &pf.s2})
       if (st->active && spans_overlap(s_dev, s_bytes, st->t_out, (size_t)st->rows * 64 * wb))
@@ -1026,16 +1035,20 @@ static int fused_plan(gh_ctx* ctx, const gh_ring* ring, const void* s_dev, size_
   const size_t area = defer ? (ctx->work_bytes / 4) & ~(size_t)255 : 0;
   char* w = (char*)ctx->work;
   plan->off = pf.flip * area;
-  plan->s1 = FinStage1{dummy, nullptr, 0, 1, 0, 0, 0};
+  plan->s1 = FinStage1{dummy, nullptr, 0, 1, 0, 0, 0, 0};
   plan->s2 = FinStage2{nullptr, nullptr, 0, 0, 0};
   if (pf.s2.active)
     plan->s2 = FinStage2{w + pf.s2.off, pf.s2.t_out, pf.s2.n, pf.s2.rows,
                          (pf.s2.rows + waves - 1) / waves};
   if (pf.s1.active) {
+    groups = (pf.s1.n + per_pass - 1) / per_pass;
+    const uint64_t tasks = (uint64_t)pf.s1.rows * groups;
+    const uint64_t per_rb = (tasks + nrb - 1) / nrb;  // blocks needed per row block
+    const uint32_t extra = per_rb > nparts ? (uint32_t)(per_rb - nparts) : 0;
     const uint32_t per = (pf.s1.n + groups - 1) / groups;
     const size_t goff = (2 + (pf.flip ^ 1)) * area;  // the previous commitment's group sums
     plan->s1 = FinStage1{w + pf.s1.off, w + goff, pf.s1.n, pf.s1.rows, groups, per,
-                         pf.s1.rows * groups};
+                         pf.s1.rows * groups, extra};
     pf.s2 = PendingStage{true, goff, groups, pf.s1.rows, pf.s1.t_out};
   } else {
     pf.s2.active = false;
@@ -1082,18 +1095,17 @@ static int launch_commit_fused(gh_key* key, const void* s_dev, uint32_t cols, vo
   const uint32_t cpb = (cols + nparts - 1) / nparts;
   nparts = (cols + cpb - 1) / cpb;
   const size_t pbytes = ((size_t)nparts * rows * 64 * 4 + 255) & ~(size_t)255;
-  const uint32_t nmac = nparts * (rows / rb);
 
   FusedPlan plan;
-  if (int st = fused_plan(ctx, ring, s_dev, (size_t)cols * 64 * 4, pbytes, nmac, 64 * waves,
-                          waves, key->dev_a, defer, &plan))
+  if (int st = fused_plan(ctx, ring, s_dev, (size_t)cols * 64 * 4, pbytes, nparts, rows / rb,
+                          64 * waves, waves, key->dev_a, defer, &plan))
     return st;
   FQ32 f{(uint32_t)ring->q, ring->qinv32};
   const uint32_t* tab = (const uint32_t*)ring->dev_tab;
   const uint32_t* A = (const uint32_t*)key->dev_a;
   const uint32_t* S = (const uint32_t*)s_dev;
   uint32_t* P = (uint32_t*)((char*)ctx->work + plan.off);
-  dim3 grid(plan.s2.nblk + nparts, rows / rb);
+  dim3 grid(plan.s2.nblk + plan.s1.extra + nparts, rows / rb);
   size_t lds_words = (size_t)cpb * kFuseStride + (size_t)waves * 64;
   if ((size_t)waves * rb * 64 > lds_words) lds_words = (size_t)waves * rb * 64;
   const size_t lds_bytes = lds_words * 4;
@@ -1133,17 +1145,16 @@ static int launch_commit_fused_gold(gh_key* key, const void* s_dev, uint32_t col
   const uint32_t cpb = (cols + nparts - 1) / nparts;
   nparts = (cols + cpb - 1) / cpb;
   const size_t pbytes = ((size_t)nparts * rows * 64 * 8 + 255) & ~(size_t)255;
-  const uint32_t nmac = nparts * (rows / rb);
 
   FusedPlan plan;
-  if (int st = fused_plan(ctx, ring, s_dev, (size_t)cols * 64 * 8, pbytes, nmac, 64 * kWaves,
-                          kWaves, key->dev_a, defer, &plan))
+  if (int st = fused_plan(ctx, ring, s_dev, (size_t)cols * 64 * 8, pbytes, nparts, rows / rb,
+                          64 * kWaves, kWaves, key->dev_a, defer, &plan))
     return st;
   const uint64_t* tab = (const uint64_t*)ring->dev_tab;
   const uint64_t* A = (const uint64_t*)key->dev_a;
   const uint64_t* S = (const uint64_t*)s_dev;
   uint64_t* P = (uint64_t*)((char*)ctx->work + plan.off);
-  dim3 grid(plan.s2.nblk + nparts, rows / rb);
+  dim3 grid(plan.s2.nblk + plan.s1.extra + nparts, rows / rb);
   size_t lds_words = (size_t)cpb * kFuseStrideGold + (size_t)kWaves * 64;
   if ((size_t)kWaves * rb * 64 > lds_words) lds_words = (size_t)kWaves * rb * 64;
   const size_t lds_bytes = lds_words * 8;

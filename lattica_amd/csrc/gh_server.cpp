@@ -508,6 +508,13 @@ void finish(gh_server* s, std::vector<SrvReq*>& reqs, int status) {
 void run(gh_server* s) {
   hipSetDevice(s->ctx->device);
   std::unique_lock<std::mutex> lk(s->mu);
+  // Polling a wave's completion event gives microsecond latency but keeps a
+  // core busy, so the worker polls only within kSpinWindow of its last
+  // progress (a wave launched or completed) and past that sleeps in steps of
+  // kSleepStep, woken at once by a new submission.
+  constexpr auto kSpinWindow = std::chrono::microseconds(200);
+  constexpr auto kSleepStep = std::chrono::microseconds(500);
+  auto last_progress = std::chrono::steady_clock::now();
   for (;;) {
     bool progress = false;
     for (Slot& sl : s->slots) {
@@ -577,16 +584,21 @@ void run(gh_server* s) {
     bool busy = false;
     for (Slot& sl : s->slots) busy |= sl.busy;
     if (s->stop && s->pending.empty() && !busy) break;
-    if (!progress) {
-      if (!busy) {
-        s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty(); });
-      } else {
-        // a wave is on the device: poll its completion, staying responsive to
-        // new submissions
-        lk.unlock();
-        for (int i = 0; i < 32; i++) std::this_thread::yield();
-        lk.lock();
-      }
+    const auto now = std::chrono::steady_clock::now();
+    if (progress) {
+      last_progress = now;
+    } else if (!busy) {
+      s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty(); });
+      last_progress = std::chrono::steady_clock::now();
+    } else if (now - last_progress < kSpinWindow) {
+      // a wave is on the device: poll its completion, staying responsive to
+      // new submissions
+      lk.unlock();
+      for (int i = 0; i < 32; i++) std::this_thread::yield();
+      lk.lock();
+    } else {
+      // a long wave: sleep between polls; a submission wakes the worker
+      s->cv_work.wait_for(lk, kSleepStep);
     }
   }
 }

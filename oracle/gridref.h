@@ -118,10 +118,12 @@ void gr_chacha20_block(const uint8_t key[32], uint32_t counter, const uint8_t no
                        uint8_t out[64]);
 
 /* ---- AVX2 restatement of the reference's SIMD CPU path (CPU baseline) -----
- * Prepared Ajtai commit t = A_ntt * s for q < 2^32 with s in the coefficient
- * domain, using the reference's AVX2 kernel structure:
- *   limb 64: PrimeField<Q, u64> (the default type) — u64_arith.rs:360-539
- *   limb 32: PrimeField<Q, u32>                    — u32_arith.rs:28-315
+ * Prepared Ajtai commit t = A_ntt * s with s in the coefficient domain, and
+ * negacyclic products, using the reference's AVX2 kernel structure:
+ *   limb 64: PrimeField<Q, u64> (the default type) — u64_arith.rs:360-539 for
+ *            q < 2^32, the wide Montgomery path u64_arith.rs:193-357 for
+ *            larger q (Goldilocks)
+ *   limb 32: PrimeField<Q, u32>, q < 2^32          — u32_arith.rs:28-315
  * a_ntt: rows x n polys (NTT domain, canonical), s: n polys (coefficients).
  * threads > 1 splits the rows over OpenMP-free pthreads (the build's own
  * parallelisation; the reference is single-threaded).                       */
@@ -137,6 +139,28 @@ void gr_avx2_load_witness(void *h, const uint64_t *s);
 void gr_avx2_commit_run(void *h, int threads);
 void gr_avx2_result(void *h, uint64_t *t_out);
 void gr_avx2_free(void *h);
+/* commits of a witness of n_use <= n polynomials against the key's first
+ * n_use columns (rows = 0 and n = 0 make a handle for products only) */
+int gr_avx2_set_len(void *h, uint32_t n_use);
+void gr_avx2_load_witness_len(void *h, const uint64_t *s, uint32_t n_use);
+/* negacyclic products of count pairs of d-coefficient polynomials (canonical
+ * in and out; load converts once, run is the timed part) */
+int gr_avx2_negacyclic_load(void *h, const uint64_t *a, const uint64_t *b, size_t count);
+void gr_avx2_negacyclic_run(void *h, int threads);
+void gr_avx2_negacyclic_result(void *h, uint64_t *out);
+/* sums over k of coefficient-domain negacyclic products x[tx[t]][k] * y[ty[t]][k]
+ * (Ring::dot_product / RingMat::mul_slice of CyclotomicPolyRing) */
+int gr_avx2_dots_load(void *h, const uint64_t *x, uint32_t nx, const uint64_t *y, uint32_t ny,
+                      uint32_t len, const uint32_t *tx, const uint32_t *ty, uint32_t ntasks);
+void gr_avx2_dots_run(void *h, int threads);
+void gr_avx2_dots_result(void *h, uint64_t *out);
+/* independent commitments, one per item: witness = pool polys [offs[i],
+ * offs[i] + lens[i]) against the key's first lens[i] columns; run splits the
+ * items over threads (the build's own parallelisation) */
+int gr_avx2_items_load(void *h, const uint64_t *pool, size_t pool_polys, const uint64_t *offs,
+                       const uint32_t *lens, uint32_t count);
+void gr_avx2_items_run(void *h, int threads);
+void gr_avx2_items_result(void *h, uint64_t *out);
 
 #ifdef __cplusplus
 }

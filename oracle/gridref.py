@@ -87,6 +87,32 @@ def lib():
         L.gr_avx2_result.argtypes = [ctypes.c_void_p, U64P]
         L.gr_avx2_free.restype = None
         L.gr_avx2_free.argtypes = [ctypes.c_void_p]
+        L.gr_avx2_set_len.restype = i32
+        L.gr_avx2_set_len.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+        L.gr_avx2_load_witness_len.restype = None
+        L.gr_avx2_load_witness_len.argtypes = [ctypes.c_void_p, U64P, ctypes.c_uint32]
+        L.gr_avx2_negacyclic_load.restype = i32
+        L.gr_avx2_negacyclic_load.argtypes = [ctypes.c_void_p, U64P, U64P, ctypes.c_size_t]
+        L.gr_avx2_negacyclic_run.restype = None
+        L.gr_avx2_negacyclic_run.argtypes = [ctypes.c_void_p, i32]
+        L.gr_avx2_negacyclic_result.restype = None
+        L.gr_avx2_negacyclic_result.argtypes = [ctypes.c_void_p, U64P]
+        U32P = ctypes.POINTER(ctypes.c_uint32)
+        L.gr_avx2_dots_load.restype = i32
+        L.gr_avx2_dots_load.argtypes = [ctypes.c_void_p, U64P, ctypes.c_uint32, U64P,
+                                        ctypes.c_uint32, ctypes.c_uint32, U32P, U32P,
+                                        ctypes.c_uint32]
+        L.gr_avx2_dots_run.restype = None
+        L.gr_avx2_dots_run.argtypes = [ctypes.c_void_p, i32]
+        L.gr_avx2_dots_result.restype = None
+        L.gr_avx2_dots_result.argtypes = [ctypes.c_void_p, U64P]
+        L.gr_avx2_items_load.restype = i32
+        L.gr_avx2_items_load.argtypes = [ctypes.c_void_p, U64P, ctypes.c_size_t, U64P, U32P,
+                                         ctypes.c_uint32]
+        L.gr_avx2_items_run.restype = None
+        L.gr_avx2_items_run.argtypes = [ctypes.c_void_p, i32]
+        L.gr_avx2_items_result.restype = None
+        L.gr_avx2_items_result.argtypes = [ctypes.c_void_p, U64P]
         _lib = L
     return _lib
 
@@ -350,8 +376,11 @@ class Avx2Commit:
             raise OracleError(-3, "avx2 prepare")
 
     def load(self, s):
+        """Witness of n polynomials (the key's width), or fewer: a prefix commit."""
         s = _u64(s)
-        lib().gr_avx2_load_witness(self.h, _p(s))
+        n_use = s.shape[0]
+        _check(lib().gr_avx2_set_len(self.h, n_use), "avx2 set_len")
+        lib().gr_avx2_load_witness_len(self.h, _p(s), n_use)
 
     def run(self, threads=1):
         lib().gr_avx2_commit_run(self.h, threads)
@@ -359,6 +388,102 @@ class Avx2Commit:
     def result(self):
         out = np.zeros((self.rows, self.d), dtype=np.uint64)
         lib().gr_avx2_result(self.h, _p(out))
+        return out
+
+    def load_items(self, pool, offs, lens):
+        """Independent commitments: item i = pool[offs[i] : offs[i] + lens[i]]."""
+        pool = _u64(pool)
+        offs = np.ascontiguousarray(offs, dtype=np.uint64)
+        lens = np.ascontiguousarray(lens, dtype=np.uint32)
+        self._items = (offs, lens)
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        _check(lib().gr_avx2_items_load(self.h, _p(pool), pool.shape[0], _p(offs),
+                                        lens.ctypes.data_as(u32p), len(lens)), "avx2 items load")
+
+    def run_items(self, threads=1):
+        lib().gr_avx2_items_run(self.h, threads)
+
+    def items_result(self):
+        out = np.zeros((len(self._items[1]), self.rows, self.d), dtype=np.uint64)
+        lib().gr_avx2_items_result(self.h, _p(out))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().gr_avx2_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Avx2Dots:
+    """Sums of coefficient-domain negacyclic products through the AVX2 twisted
+    NTT: the reference's Ring::dot_product / RingMat::mul_slice over
+    CyclotomicPolyRing, each product a full negacyclic_mul_coeffs (CPU
+    baseline of the Gram matrix, garbage.rs:76-93, and of the inner
+    commitments, crs.rs:211-216)."""
+
+    def __init__(self, q, limb_bits, d=64):
+        self.d = d
+        empty = np.zeros(1, dtype=np.uint64)
+        self.h = lib().gr_avx2_prepare(q, limb_bits, 0, 0, d, _p(empty))
+        if not self.h:
+            raise OracleError(-3, "avx2 prepare")
+        self.ntasks = 0
+
+    def load(self, x, y, tx, ty):
+        """x: (nx, len, d), y: (ny, len, d); output t = sum_k x[tx[t]][k] * y[ty[t]][k]."""
+        x, y = _u64(x), _u64(y)
+        tx = np.ascontiguousarray(tx, dtype=np.uint32)
+        ty = np.ascontiguousarray(ty, dtype=np.uint32)
+        self.ntasks = len(tx)
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        self._keep = (tx, ty)
+        _check(lib().gr_avx2_dots_load(self.h, _p(x), x.shape[0], _p(y), y.shape[0], x.shape[1],
+                                       tx.ctypes.data_as(u32p), ty.ctypes.data_as(u32p),
+                                       self.ntasks), "avx2 dots load")
+
+    def run(self, threads=1):
+        lib().gr_avx2_dots_run(self.h, threads)
+
+    def result(self):
+        out = np.zeros((self.ntasks, self.d), dtype=np.uint64)
+        lib().gr_avx2_dots_result(self.h, _p(out))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().gr_avx2_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Avx2Negacyclic:
+    """The reference's negacyclic product through its AVX2 twisted NTT
+    (poly/ring.rs:83-101), restated (CPU baseline)."""
+
+    def __init__(self, q, limb_bits, d=64):
+        self.d = d
+        empty = np.zeros(1, dtype=np.uint64)
+        self.h = lib().gr_avx2_prepare(q, limb_bits, 0, 0, d, _p(empty))
+        if not self.h:
+            raise OracleError(-3, "avx2 prepare")
+        self.count = 0
+
+    def load(self, a, b):
+        a, b = _u64(a), _u64(b)
+        self.count = a.shape[0]
+        _check(lib().gr_avx2_negacyclic_load(self.h, _p(a), _p(b), self.count), "avx2 load")
+
+    def run(self, threads=1):
+        lib().gr_avx2_negacyclic_run(self.h, threads)
+
+    def result(self):
+        out = np.zeros((self.count, self.d), dtype=np.uint64)
+        lib().gr_avx2_negacyclic_result(self.h, _p(out))
         return out
 
     def close(self):

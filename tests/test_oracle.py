@@ -274,6 +274,117 @@ def test_avx2_commit_matches_oracle(oracle, limb):
     c.close()
 
 
+@pytest.mark.parametrize("q", [Q32, GOLDILOCKS])
+def test_avx2_prefix_commits_match_oracle(oracle, q):
+    """Commits of witnesses shorter than the key (its first n columns), on
+    the q < 2^32 path and the wide Goldilocks path (u64_arith.rs:193-357)."""
+    if not oracle.lib().gr_avx2_available():
+        pytest.skip("no AVX2 on this host")
+    rng = np.random.default_rng(7)
+    a = rand_residues(rng, q, (5, 23, 64))
+    c = oracle.Avx2Commit(q, 64, a)
+    z = np.zeros((5, 1, 64), dtype=np.uint64)
+    zr = np.zeros((1, 64), dtype=np.uint64)
+    for n in (23, 1, 7, 16):
+        s = rand_residues(rng, q, (n, 64))
+        c.load(s)
+        c.run(threads=1 + n % 3)
+        ref = oracle.commit(q, 64, a[:, :n], z, s, zr, key_domain=1)
+        assert np.array_equal(c.result(), ref), n
+    c.close()
+
+
+def test_avx2_wide_words_at_the_extremes(oracle):
+    """Goldilocks words next to 0, 2^32 and p, where the 64 x 64 products'
+    carries (mul_epu64_wide_avx2) and the wide reduction's overflow branch
+    (montgomery_reduce_wide) are taken."""
+    if not oracle.lib().gr_avx2_available():
+        pytest.skip("no AVX2 on this host")
+    q = GOLDILOCKS
+    edge = np.array([0, 1, 2, (1 << 32) - 1, 1 << 32, (1 << 32) + 1, (1 << 63) - 1, 1 << 63,
+                     q - 2, q - 1, q // 2, q // 2 + 1, 0xFFFFFFFF00000000, 0xFFFFFFFE00000001,
+                     (1 << 64) - (1 << 33), 12345678901234567890 % q], dtype=np.uint64)
+    rng = np.random.default_rng(8)
+    a = edge[rng.integers(0, len(edge), size=(3, 4, 64))]
+    s = edge[rng.integers(0, len(edge), size=(4, 64))]
+    c = oracle.Avx2Commit(q, 64, a)
+    c.load(s)
+    c.run()
+    z = np.zeros((3, 1, 64), dtype=np.uint64)
+    ref = oracle.commit(q, 64, a, z, s, np.zeros((1, 64), dtype=np.uint64), key_domain=1)
+    assert np.array_equal(c.result(), ref)
+    c.close()
+
+
+@pytest.mark.parametrize("q,limb", [(Q32, 64), (Q32, 32), (GOLDILOCKS, 64), (12289, 64)])
+def test_avx2_negacyclic_matches_schoolbook(oracle, q, limb):
+    """negacyclic_mul_coeffs through the AVX2 twisted NTT against the
+    definition (and the scalar oracle on more pairs)."""
+    if not oracle.lib().gr_avx2_available():
+        pytest.skip("no AVX2 on this host")
+    rng = np.random.default_rng(9)
+    a = rand_residues(rng, q, (40, 64))
+    b = rand_residues(rng, q, (40, 64))
+    m = oracle.Avx2Negacyclic(q, limb)
+    m.load(a, b)
+    m.run(threads=3)
+    got = m.result()
+    m.close()
+    for i in range(3):
+        want = _schoolbook(q, [int(x) for x in a[i]], [int(x) for x in b[i]])
+        assert [int(x) for x in got[i]] == want, i
+    assert np.array_equal(got, oracle.negacyclic_mul(q, 64, a, b))
+
+
+@pytest.mark.parametrize("q", [Q32, GOLDILOCKS])
+def test_avx2_independent_items_match_oracle(oracle, q):
+    """Independent prefix commitments drawn from a pool, several threads."""
+    if not oracle.lib().gr_avx2_available():
+        pytest.skip("no AVX2 on this host")
+    rng = np.random.default_rng(11)
+    a = rand_residues(rng, q, (4, 30, 64))
+    pool = small_signed(rng, q, (50, 64), -1, 2)
+    lens = [30, 1, 17, 8, 29, 2]
+    offs = [0, 49, 10, 40, 21, 3]
+    c = oracle.Avx2Commit(q, 64, a)
+    c.load_items(pool, offs, lens)
+    c.run_items(threads=3)
+    got = c.items_result()
+    c.close()
+    z = np.zeros((4, 1, 64), dtype=np.uint64)
+    zr = np.zeros((1, 64), dtype=np.uint64)
+    for i, (o, n) in enumerate(zip(offs, lens)):
+        exp = oracle.commit(q, 64, a[:, :n], z, pool[o:o + n], zr, key_domain=1)
+        assert np.array_equal(got[i], exp), i
+
+
+@pytest.mark.parametrize("q", [Q32, GOLDILOCKS])
+def test_avx2_coefficient_domain_gram_and_inner_commit(oracle, q):
+    """The reference-faithful coefficient-domain Gram matrix and inner
+    commitments (one NTT product per term) through the AVX2 restatement equal
+    the scalar oracle's Gram and commitment."""
+    if not oracle.lib().gr_avx2_available():
+        pytest.skip("no AVX2 on this host")
+    rng = np.random.default_rng(10)
+    r, n, rows = 4, 6, 3
+    s = small_signed(rng, q, (r, n, 64), -128, 128)
+    pairs = [(i, j) for i in range(r) for j in range(i, r)]
+    m = oracle.Avx2Dots(q, 64)
+    m.load(s, s, [i for i, _ in pairs], [j for _, j in pairs])
+    m.run(threads=2)
+    assert np.array_equal(m.result(), oracle.gram(q, 64, s))
+    a = rand_residues(rng, q, (rows, n, 64))
+    m.load(a, s, [row for part in range(r) for row in range(rows)],
+           [part for part in range(r) for row in range(rows)])
+    m.run()
+    got = m.result().reshape(r, rows, 64)
+    m.close()
+    z = np.zeros((rows, 1, 64), dtype=np.uint64)
+    for part in range(r):
+        exp = oracle.commit(q, 64, a, z, s[part], np.zeros((1, 64), dtype=np.uint64))
+        assert np.array_equal(got[part], exp), part
+
+
 # ---- LaBRADOR step around the commitment ---------------------------------------
 
 def _schoolbook(q, a, b):
