@@ -227,6 +227,28 @@ GH_API int gh_key_dims(const gh_key *key, uint32_t *rows, uint32_t *msg_len, uin
 GH_API int gh_commit(gh_key *key, const void *message, uint32_t msg_len, const void *randomness,
                      uint32_t open_len, int domain, void *t_out, int mem);
 
+/* ---- commitment that allocates nothing --------------------------------------
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc). Same result as gh_commit(key, message, msg_len, NULL, 0,
+ * GH_DOMAIN_COEFF, t_out, GH_MEM_DEVICE): message and t_out are device
+ * pointers, the message in the coefficient domain, and the key has open_len 0
+ * and rows % 4 == 0. The partial sums go to one 4 MiB array per device
+ * (static device memory the library holds until the process ends) instead of
+ * the context workspace; nothing is allocated and the result is complete in
+ * stream order (the finalize is never held back). A held-back finalize of an
+ * earlier gh_commit is issued first. The array holds blocks x rows x 64 words
+ * of partial sums, so the grid shrinks to fit it; the fewest blocks a launch
+ * can have are ceil(msg_len / 128) for q32 and ceil(msg_len / 64) for
+ * Goldilocks, and a key for which even those exceed 4 MiB gets
+ * GH_E_INVALID_ARGUMENT (rows 16: msg_len above 131072 for q32, above 32768
+ * for Goldilocks).
+ * Every call on one device shares the array and each call overwrites the part
+ * it uses: calls that overlap on one device (two streams, two contexts, two
+ * threads without ordering) race; calls in one stream do not.
+ * DIMENSION_MISMATCH when msg_len differs from the key. */
+GH_API int gh_commit_noalloc(gh_key *key, const void *message, uint32_t msg_len, void *t_out);
+
 /* ---- BDLOP commitment ------------------------------------------------------
  * BdlopCommitmentScheme::commit / verify recompute (commit/src/bdlop/mod.rs:34-54)
  * through PreparedLinearOps::recompute_bdlop_parts_runtime

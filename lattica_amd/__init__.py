@@ -57,7 +57,7 @@ EXPORTS = (
     "gh_batch_plan_destroy", "gh_bdlop_commit", "gh_decompose_balanced", "gh_garbage_h",
     "gh_amortize", "gh_wire_size", "gh_wire_encode", "gh_wire_peek", "gh_wire_decode",
     "gh_key_save", "gh_key_load", "gh_jl_extract_flat", "gh_jl_conjugate", "gh_jl_evals",
-    "gh_jl_aggregate", "gh_jl_launch_shape",
+    "gh_jl_aggregate", "gh_jl_launch_shape", "gh_commit_noalloc",
 )
 
 
@@ -174,6 +174,7 @@ def lib():
         "gh_key_destroy": (i32, [vp]),
         "gh_key_dims": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
         "gh_commit": (i32, [vp, vp, u32, vp, u32, i32, vp, i32]),
+        "gh_commit_noalloc": (i32, [vp, vp, u32, vp]),
         "gh_commit_batch": (i32, [vp, ctypes.POINTER(CommitItem), sz, i32]),
         "gh_gram": (i32, [vp, vp, u32, u32, i32, vp, i32]),
         "gh_inner_commit": (i32, [vp, vp, u32, u32, i32, vp, i32]),
@@ -584,6 +585,14 @@ class Key:
     def commit_device(self, s_dev: DeviceBuffer, n: int, t_dev: DeviceBuffer, domain=DOMAIN_COEFF):
         _check(lib().gh_commit(self.h, s_dev.ptr, n, None, 0, domain, t_dev.ptr, MEM_DEVICE),
                "gh_commit")
+
+    def commit_noalloc(self, s_dev, n: int, t_dev):
+        """gh_commit_noalloc: the device commitment with its partial sums in the
+        library's static per-device array; s_dev and t_dev are DeviceBuffers
+        or raw device addresses."""
+        sp = s_dev.ptr if isinstance(s_dev, DeviceBuffer) else ctypes.c_void_p(s_dev)
+        tp = t_dev.ptr if isinstance(t_dev, DeviceBuffer) else ctypes.c_void_p(t_dev)
+        _check(lib().gh_commit_noalloc(self.h, sp, n, tp), "gh_commit_noalloc")
 
     def commit_batch(self, witnesses):
         ring = self.ring

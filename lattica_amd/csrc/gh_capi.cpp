@@ -212,6 +212,11 @@ GH_API int gh_ctx_create(int device, gh_ctx** out) {
     gh_ctx_destroy(ctx);
     return GH_E_OOM;
   }
+  // the device's own copy of the no-allocation scratch array (device current)
+  if (int st = scratch_address(&ctx->scratch)) {
+    gh_ctx_destroy(ctx);
+    return st;
+  }
   *out = ctx;
   return GH_OK;
 }
@@ -668,6 +673,16 @@ GH_API int gh_commit(gh_key* key, const void* message, uint32_t msg_len, const v
     return sync(ctx);
   }
   return GH_OK;
+}
+
+GH_API int gh_commit_noalloc(gh_key* key, const void* message, uint32_t msg_len, void* t_out) {
+  if (!key || !message || !t_out) return GH_E_INVALID_ARGUMENT;
+  if (msg_len != key->msg_len) return GH_E_DIMENSION_MISMATCH;
+  if (key->open_len != 0) return GH_E_INVALID_ARGUMENT;
+  gh_ctx* ctx = key->ring->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  GH_HIP(hipSetDevice(ctx->device));
+  return launch_commit_noalloc(key, message, msg_len, t_out, ctx->scratch);
 }
 
 GH_API int gh_decompose_balanced(gh_ring* ring, const void* values, size_t count, uint64_t base,

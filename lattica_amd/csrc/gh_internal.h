@@ -124,6 +124,8 @@ struct gh_ctx {
   size_t stage_bytes = 0;
   // small fixed device buffer (flags)
   void* misc = nullptr;
+  // this device's address of the no-allocation scratch array
+  void* scratch = nullptr;
   // pinned host staging of batch descriptors: the upload of one batch reads it
   // asynchronously, so the next batch waits for `upload_done` before reuse
   gh::BatchLayout batch_layout;
@@ -191,6 +193,14 @@ int launch_commit(gh_key* key, const void* s_dev, uint32_t cols, int domain, voi
                   bool defer = false);
 // issues a pending finalize on the context stream (context lock held)
 int ctx_flush(gh_ctx* ctx);
+// Bytes of the device array behind gh_commit_noalloc (partial sums of one
+// commitment: blocks x rows x 64 words), and its address on the current device.
+constexpr size_t kScratchBytes = (size_t)4 << 20;
+int scratch_address(void** out);
+// gh_commit_noalloc: fused path only (rows % 4 == 0, coefficient-domain
+// witness), partial sums in `scratch`; GH_E_INVALID_ARGUMENT beyond its size
+int launch_commit_noalloc(gh_key* key, const void* s_dev, uint32_t cols, void* t_dev,
+                          void* scratch);
 // items hold device pointers; every len in [1, key columns]
 int launch_commit_batch(gh_key* key, const gh_commit_item* items, size_t count);
 

@@ -17,6 +17,8 @@
 //      with one slot per lane.
 #include <hip/hip_ext.h>
 
+#include <algorithm>
+
 #include "gh_internal.h"
 #include "gh_ntt.h"
 #include "gh_common_kernels.h"
@@ -951,11 +953,11 @@ static void fused_launch(gh_ctx* ctx, dim3 grid, size_t lds_bytes, hipEvent_t e0
 // laid out [row][n][slot]: 16 waves per row for up to one row per CU, 4
 // waves per row beyond (all rows in one round).
 static int launch_finalize_rows(gh_ctx* ctx, const gh_ring* ring, const FinSeg& a,
-                                const FinSeg& b) {
+                                const FinSeg& b, bool prof = true) {
   const uint32_t nrows = a.rows + b.rows;
   if (nrows == 0) return GH_OK;
-  hipEvent_t f0, f1;
-  prof_pair(ctx, 2, &f0, &f1);
+  hipEvent_t f0 = nullptr, f1 = nullptr;
+  if (prof) prof_pair(ctx, 2, &f0, &f1);
   const bool many = nrows > (uint32_t)ctx->num_cus;
   if (ring->kind == GH_RING_Q32) {
     const FQ32 f{(uint32_t)ring->q, ring->qinv32};
@@ -993,6 +995,17 @@ int ctx_flush(gh_ctx* ctx) {
 static bool spans_overlap(const void* a, size_t na, const void* b, size_t nb) {
   const uintptr_t a0 = (uintptr_t)a, b0 = (uintptr_t)b;
   return a0 < b0 + nb && b0 < a0 + na;
+}
+
+// The no-allocation form's partial sums (gh_commit_noalloc): one array per
+// device, held until the process ends; host code uses the address that
+// hipGetSymbolAddress reports for the current device, never the name.
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+int scratch_address(void** out) {
+  *out = nullptr;
+  if (hipGetSymbolAddress(out, HIP_SYMBOL(scratch)) != hipSuccess || !*out) return GH_E_DEVICE;
+  return GH_OK;
 }
 
 // Shape of a fused launch and the stage work it carries.
@@ -1057,11 +1070,27 @@ static int fused_plan(gh_ctx* ctx, const gh_ring* ring, const void* s_dev, size_
   return GH_OK;
 }
 
+// Plan of a fused launch whose partial sums go to `scratch` (the
+// no-allocation form): no stage work, nothing allocated. A held-back finalize
+// is issued first (its launch allocates nothing), so an earlier commitment's
+// output is complete before this one reads or overwrites it.
+static int scratch_plan(gh_ctx* ctx, size_t pbytes, const void* dummy, FusedPlan* plan) {
+  if (pbytes > kScratchBytes) return GH_E_INVALID_ARGUMENT;
+  if (int e = ctx_flush(ctx)) return e;
+  plan->off = 0;
+  plan->s1 = FinStage1{dummy, nullptr, 0, 1, 0, 0, 0, 0};
+  plan->s2 = FinStage2{nullptr, nullptr, 0, 0, 0};
+  return GH_OK;
+}
+
 // After a fused launch: with deferral its finalize is held back as stage s1,
-// else it runs now.
+// else it runs now (from `scratch` when that holds the partial sums).
 static int fused_finish(gh_ctx* ctx, const gh_ring* ring, const FusedPlan& plan, uint32_t nparts,
-                        uint32_t rows, void* t_dev, bool defer) {
+                        uint32_t rows, void* t_dev, bool defer, void* scratch = nullptr) {
   PendingFin& pf = ctx->fin;
+  if (scratch)
+    return launch_finalize_rows(ctx, ring, FinSeg{scratch, t_dev, nparts, rows},
+                                FinSeg{nullptr, nullptr, 0, 0}, false);
   if (!defer)
     return launch_finalize_rows(ctx, ring, FinSeg{(const char*)ctx->work + plan.off, t_dev, nparts,
                                                   rows},
@@ -1077,7 +1106,7 @@ static int fused_finish(gh_ctx* ctx, const gh_ring* ring, const FusedPlan& plan,
 // commitments cost one launch each. (A second stream for the finalize was
 // measured: the cross-queue event hand-off costs more than it overlaps.)
 static int launch_commit_fused(gh_key* key, const void* s_dev, uint32_t cols, void* t_dev,
-                               bool defer) {
+                               bool defer, void* scratch = nullptr) {
   gh_ring* ring = key->ring;
   gh_ctx* ctx = ring->ctx;
   const uint32_t rows = key->rows;
@@ -1087,6 +1116,7 @@ static int launch_commit_fused(gh_key* key, const void* s_dev, uint32_t cols, vo
   // block's witness fits its LDS), at most one per `waves` columns; every block
   // owns at least one column
   uint32_t nparts = (uint32_t)(ctx->num_cus * ctx->tune.mac_blocks_per_cu);
+  if (scratch) nparts = std::min<uint32_t>(nparts, kScratchBytes / ((size_t)rows * 64 * 4));
   const uint32_t min_parts = (cols + kFuseChunk - 1) / kFuseChunk;
   if (nparts < min_parts) nparts = min_parts;
   const uint32_t max_parts = (cols + waves - 1) / waves;
@@ -1097,33 +1127,34 @@ static int launch_commit_fused(gh_key* key, const void* s_dev, uint32_t cols, vo
   const size_t pbytes = ((size_t)nparts * rows * 64 * 4 + 255) & ~(size_t)255;
 
   FusedPlan plan;
-  if (int st = fused_plan(ctx, ring, s_dev, (size_t)cols * 64 * 4, pbytes, nparts, rows / rb,
-                          64 * waves, waves, key->dev_a, defer, &plan))
+  if (int st = scratch ? scratch_plan(ctx, pbytes, key->dev_a, &plan)
+                       : fused_plan(ctx, ring, s_dev, (size_t)cols * 64 * 4, pbytes, nparts,
+                                    rows / rb, 64 * waves, waves, key->dev_a, defer, &plan))
     return st;
   FQ32 f{(uint32_t)ring->q, ring->qinv32};
   const uint32_t* tab = (const uint32_t*)ring->dev_tab;
   const uint32_t* A = (const uint32_t*)key->dev_a;
   const uint32_t* S = (const uint32_t*)s_dev;
-  uint32_t* P = (uint32_t*)((char*)ctx->work + plan.off);
+  uint32_t* P = scratch ? (uint32_t*)scratch : (uint32_t*)((char*)ctx->work + plan.off);
   dim3 grid(plan.s2.nblk + plan.s1.extra + nparts, rows / rb);
   size_t lds_words = (size_t)cpb * kFuseStride + (size_t)waves * 64;
   if ((size_t)waves * rb * 64 > lds_words) lds_words = (size_t)waves * rb * 64;
   const size_t lds_bytes = lds_words * 4;
-  hipEvent_t e0, e1;
-  prof_pair(ctx, 1, &e0, &e1);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!scratch) prof_pair(ctx, 1, &e0, &e1);
 #define GH_FUSED(R, W) fused_launch<R, W>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, plan.s1, plan.s2)
   if (rb == 16) { if (waves == 4) GH_FUSED(16, 4); else if (waves == 8) GH_FUSED(16, 8); else GH_FUSED(16, 16); }
   else if (rb == 8) { if (waves == 4) GH_FUSED(8, 4); else if (waves == 8) GH_FUSED(8, 8); else GH_FUSED(8, 16); }
   else { if (waves == 4) GH_FUSED(4, 4); else if (waves == 8) GH_FUSED(4, 8); else GH_FUSED(4, 16); }
 #undef GH_FUSED
   GH_HIP(hipGetLastError());
-  return fused_finish(ctx, ring, plan, nparts, rows, t_dev, defer);
+  return fused_finish(ctx, ring, plan, nparts, rows, t_dev, defer, scratch);
 }
 
 // Goldilocks commitment, per-call form: k_commit_fused_gold, then the
 // finalize, held back with `defer` exactly as for q32.
 static int launch_commit_fused_gold(gh_key* key, const void* s_dev, uint32_t cols, void* t_dev,
-                                    bool defer) {
+                                    bool defer, void* scratch = nullptr) {
   gh_ring* ring = key->ring;
   gh_ctx* ctx = ring->ctx;
   const uint32_t rows = key->rows;
@@ -1136,6 +1167,7 @@ static int launch_commit_fused_gold(gh_key* key, const void* s_dev, uint32_t col
   // kFuseChunkGold columns, at most one per wave's column; every block owns at
   // least one column
   uint32_t nparts = (uint32_t)(ctx->num_cus * ctx->tune.mac_blocks_per_cu) / (rows / rb);
+  if (scratch) nparts = std::min<uint32_t>(nparts, kScratchBytes / ((size_t)rows * 64 * 8));
   if (nparts == 0) nparts = 1;
   const uint32_t min_parts = (cols + kFuseChunkGold - 1) / kFuseChunkGold;
   if (nparts < min_parts) nparts = min_parts;
@@ -1147,19 +1179,20 @@ static int launch_commit_fused_gold(gh_key* key, const void* s_dev, uint32_t col
   const size_t pbytes = ((size_t)nparts * rows * 64 * 8 + 255) & ~(size_t)255;
 
   FusedPlan plan;
-  if (int st = fused_plan(ctx, ring, s_dev, (size_t)cols * 64 * 8, pbytes, nparts, rows / rb,
-                          64 * kWaves, kWaves, key->dev_a, defer, &plan))
+  if (int st = scratch ? scratch_plan(ctx, pbytes, key->dev_a, &plan)
+                       : fused_plan(ctx, ring, s_dev, (size_t)cols * 64 * 8, pbytes, nparts,
+                                    rows / rb, 64 * kWaves, kWaves, key->dev_a, defer, &plan))
     return st;
   const uint64_t* tab = (const uint64_t*)ring->dev_tab;
   const uint64_t* A = (const uint64_t*)key->dev_a;
   const uint64_t* S = (const uint64_t*)s_dev;
-  uint64_t* P = (uint64_t*)((char*)ctx->work + plan.off);
+  uint64_t* P = scratch ? (uint64_t*)scratch : (uint64_t*)((char*)ctx->work + plan.off);
   dim3 grid(plan.s2.nblk + plan.s1.extra + nparts, rows / rb);
   size_t lds_words = (size_t)cpb * kFuseStrideGold + (size_t)kWaves * 64;
   if ((size_t)kWaves * rb * 64 > lds_words) lds_words = (size_t)kWaves * rb * 64;
   const size_t lds_bytes = lds_words * 8;
-  hipEvent_t e0, e1;
-  prof_pair(ctx, 1, &e0, &e1);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!scratch) prof_pair(ctx, 1, &e0, &e1);
 #define GH_FUSED_GOLD(R)                                                                        \
   hipExtLaunchKernelGGL((k_commit_fused_gold<R, kWaves>), grid, dim3(64 * kWaves), lds_bytes,   \
                         ctx->stream, e0, e1, 0, FGold{}, tab, A, S, P, rows, cols, cpb, plan.s1, \
@@ -1168,7 +1201,18 @@ static int launch_commit_fused_gold(gh_key* key, const void* s_dev, uint32_t col
   else GH_FUSED_GOLD(4);
 #undef GH_FUSED_GOLD
   GH_HIP(hipGetLastError());
-  return fused_finish(ctx, ring, plan, nparts, rows, t_dev, defer);
+  return fused_finish(ctx, ring, plan, nparts, rows, t_dev, defer, scratch);
+}
+
+// The no-allocation commitment: the fused kernel with its partial sums in
+// `scratch` (this device's address of the scratch array), then the finalize
+// from there; both enqueued, nothing allocated.
+int launch_commit_noalloc(gh_key* key, const void* s_dev, uint32_t cols, void* t_dev,
+                          void* scratch) {
+  if (!scratch || key->rows % 4 != 0) return GH_E_INVALID_ARGUMENT;
+  if (key->ring->kind == GH_RING_Q32)
+    return launch_commit_fused(key, s_dev, cols, t_dev, false, scratch);
+  return launch_commit_fused_gold(key, s_dev, cols, t_dev, false, scratch);
 }
 
 // t = A[:, :cols] * s; s (cols polys) and t (rows polys) are device pointers.
