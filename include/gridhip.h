@@ -125,7 +125,10 @@ GH_API int gh_ctx_profile_read(gh_ctx *ctx, int phase, double *total_ms, uint64_
  * commitments cost one launch each; 0: that step is enqueued before the call
  * returns, so the result is complete in stream order and a plain
  * hipStreamSynchronize or hipDeviceSynchronize finishes it; setting 0 also
- * enqueues a step held back until then). Values are checked;
+ * enqueues a step held back until then), "stage1_tasks_per_block" (-1, the
+ * default: the first half of a held-back step runs in about num_cus / 2
+ * leading blocks of the next launch; 0: in its MAC blocks; 1..8: in leading
+ * blocks of that many tasks). Values are checked;
  * an unknown key or value gives GH_E_INVALID_ARGUMENT. Every setting yields
  * identical results; only the speed changes. */
 GH_API int gh_ctx_set_tuning(gh_ctx *ctx, const char *key, int64_t value);

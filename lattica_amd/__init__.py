@@ -264,7 +264,8 @@ class Context:
         _check(lib().gh_ctx_synchronize(self.h), "gh_ctx_synchronize")
 
     TUNING_KEYS = ("mac_variant", "mac_waves", "mac_blocks_per_cu", "quad_max", "fused", "fused_depth",
-                   "batch_chunk_cols", "batch_chunk_cols_gold", "batch_group", "defer_finalize")
+                   "batch_chunk_cols", "batch_chunk_cols_gold", "batch_group", "defer_finalize",
+                   "stage1_tasks_per_block")
 
     def set_tuning(self, **kv):
         for k, v in kv.items():

@@ -314,6 +314,9 @@ GH_API int gh_ctx_set_tuning(gh_ctx* ctx, const char* key, int64_t value) {
     // turning deferral off completes a finalize held back until now
     if (!value) TRY(set_device(ctx));
     t.defer_finalize = (int)value;
+  } else if (!std::strcmp(key, "stage1_tasks_per_block")) {
+    if (value < -1 || value > (int64_t)kStage1MaxTpb) return GH_E_INVALID_ARGUMENT;
+    t.stage1_tasks_per_block = (int)value;
   } else {
     return GH_E_INVALID_ARGUMENT;
   }
@@ -334,6 +337,7 @@ GH_API int gh_ctx_get_tuning(gh_ctx* ctx, const char* key, int64_t* value) {
   else if (!std::strcmp(key, "batch_chunk_cols_gold")) *value = t.batch_chunk_cols_gold;
   else if (!std::strcmp(key, "batch_group")) *value = t.batch_group;
   else if (!std::strcmp(key, "defer_finalize")) *value = t.defer_finalize;
+  else if (!std::strcmp(key, "stage1_tasks_per_block")) *value = t.stage1_tasks_per_block;
   else return GH_E_INVALID_ARGUMENT;
   return GH_OK;
 }

@@ -87,11 +87,13 @@ struct FinStage1 {
   void* dst;        // group sums [rows][groups][64]
   uint32_t n, rows, groups, per;  // per <= threads / 16
   uint32_t ntasks;                // rows * groups
+  // tasks [0, mac_tasks) run in MAC blocks: MAC block (x, y) takes task
+  // y * nparts + x
+  uint32_t mac_tasks;
   // blocks per row block, after the stage-2 blocks and before the MAC blocks,
-  // that run only stage-1 tasks (when there are more tasks than MAC blocks);
-  // MAC block (x, y) takes task y * nparts + x, extra block (x, y) task
-  // nparts * row blocks + y * extra + x
-  uint32_t extra;
+  // that run only stage-1 tasks, `tpb` each (at most kStage1MaxTpb): block
+  // (x, y) takes tasks mac_tasks + (y * extra + x) * tpb + i
+  uint32_t extra, tpb;
 };
 struct FinStage2 {
   const void* src;  // group sums [rows][n][64]
@@ -121,15 +123,16 @@ __device__ __forceinline__ uint64_t shfl_xor_w(uint64_t v, int m) {
   return ((uint64_t)hi << 32) | lo;
 }
 
-// Stage 1, issue: this thread's four words of MAC block `bid`'s task (zero
-// when the block has none or the partial lies past the group).
+// Stage 1, issue: this thread's four words of task `bid` (zero when bid >=
+// limit, a block-uniform test, or the partial lies past the group).
 template <class W>
-__device__ __forceinline__ void stage1_load(const FinStage1& s, uint32_t bid, W (&v)[4]) {
-  const bool has = bid < s.ntasks;  // block-uniform
-  const uint32_t task = has ? bid : 0;
-  const uint32_t r = task % s.rows, g = task / s.rows;
+__device__ __forceinline__ void stage1_load(const FinStage1& s, uint32_t bid, uint32_t limit,
+                                            W (&v)[4]) {
+  v[0] = v[1] = v[2] = v[3] = 0;
+  if (bid >= limit) return;
+  const uint32_t r = bid % s.rows, g = bid / s.rows;
   const uint32_t j = g * s.per + (threadIdx.x >> 4);
-  const bool live = has && (threadIdx.x >> 4) < s.per && j < s.n;
+  const bool live = (threadIdx.x >> 4) < s.per && j < s.n;
   const W* p = reinterpret_cast<const W*>(s.src) +
                ((size_t)r * s.n + (live ? j : 0)) * 64 + 4 * (threadIdx.x & 15);
   load4(p, v);
@@ -150,28 +153,36 @@ __device__ __forceinline__ void stage1_wave_sum(const F& f, typename F::W (&v)[4
     for (int i = 0; i < 4; i++) lds[wave * 64 + 4 * lane + i] = v[i];
 }
 // Stage 1, after a block barrier: the first 64 threads add the waves' sums
-// and store the group sum
+// and store the group sum of task bid (< limit)
 template <class F>
 __device__ __forceinline__ void stage1_store(const F& f, const FinStage1& s, uint32_t bid,
-                                             const typename F::W* lds, int waves) {
+                                             uint32_t limit, const typename F::W* lds,
+                                             int waves) {
   using W = typename F::W;
-  if (bid >= s.ntasks || threadIdx.x >= 64) return;
+  if (bid >= limit || threadIdx.x >= 64) return;
   const uint32_t r = bid % s.rows, g = bid / s.rows;
   W acc = lds[threadIdx.x];
   for (int w = 1; w < waves; w++) acc = f.add(acc, lds[w * 64 + threadIdx.x]);
   reinterpret_cast<W*>(s.dst)[((size_t)r * s.groups + g) * 64 + threadIdx.x] = acc;
 }
 
-// A block that only runs a stage-1 task (a launch with more tasks than MAC
-// blocks); `lds` holds waves * 64 words.
+// A block that only runs stage-1 tasks: tasks first .. first + tpb - 1 (those
+// below ntasks), every load issued before the first sum; `lds` holds
+// waves * 64 * tpb words.
 template <class F>
-__device__ __forceinline__ void stage1_only(const F& f, const FinStage1& s, uint32_t bid,
+__device__ __forceinline__ void stage1_only(const F& f, const FinStage1& s, uint32_t first,
                                             typename F::W* lds, int waves) {
-  typename F::W v[4];
-  stage1_load(s, bid, v);
-  stage1_wave_sum(f, v, lds);
+  using W = typename F::W;
+  W v[kStage1MaxTpb][4];
+#pragma unroll
+  for (uint32_t i = 0; i < kStage1MaxTpb; i++)
+    if (i < s.tpb) stage1_load(s, first + i, s.ntasks, v[i]);
+#pragma unroll
+  for (uint32_t i = 0; i < kStage1MaxTpb; i++)
+    if (i < s.tpb) stage1_wave_sum(f, v[i], lds + i * waves * 64);
   __syncthreads();
-  stage1_store(f, s, bid, lds, waves);
+  for (uint32_t i = 0; i < s.tpb; i++)
+    stage1_store(f, s, first + i, s.ntasks, lds + i * waves * 64, waves);
 }
 
 // Sum of the n polynomials [row][0..n)[64] of `src` by G waves (lane = slot),

@@ -81,7 +81,13 @@ struct gh_tuning {
   // commitment's launch (one launch per commitment; results complete at the
   // next call on the context), 0 = finalize on the stream before returning
   int defer_finalize = 1;
+  // held-back finalize, first stage (partial sums -> group sums), run in the
+  // next launch: 0 = one task in each MAC block, n (1..8) = every task in
+  // dedicated leading blocks of n tasks, -1 = such blocks, about num_cus / 2
+  int stage1_tasks_per_block = -1;
 };
+// most stage-1 tasks one dedicated block runs
+constexpr uint32_t kStage1MaxTpb = 8;
 
 // Tuned key columns per block of the batched MAC for keys over `ring`.
 inline uint32_t tuned_batch_chunk(const gh_tuning& t, int ring_kind) {

@@ -416,7 +416,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_q32(
         sum_rows_inv<FQ32, 1>(f, tab, (int)kTabUntwistMac, (const uint32_t*)s2.src, s2.n, row,
                               0, (uint32_t*)s2.out, nullptr);
     } else {
-      stage1_only(f, s1, nparts * gridDim.y + blockIdx.y * s1.extra + (blockIdx.x - s2.nblk),
+      stage1_only(f, s1, s1.mac_tasks + (blockIdx.y * s1.extra + (blockIdx.x - s2.nblk)) * s1.tpb,
                   lds, WAVES);
     }
     return;
@@ -464,7 +464,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_q32(
     for (int i = 0; i < 16; i++) x[i] = sp[4 * i];
   }
   uint32_t l1v[4];
-  stage1_load(s1, bid, l1v);
+  stage1_load(s1, bid, s1.mac_tasks, l1v);
   // (b) key prefetch: this wave's first PF columns
   u32x4 pa[PF][NI];
 #pragma unroll
@@ -496,7 +496,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_q32(
   uint32_t* l1lds = lds + cols_per_block * kFuseStride;
   stage1_wave_sum(f, l1v, l1lds);
   __syncthreads();
-  stage1_store(f, s1, bid, l1lds, WAVES);
+  stage1_store(f, s1, bid, s1.mac_tasks, l1lds, WAVES);
   // (d) MAC over the wave's columns, PF in flight; a clamped column meets a
   // zero witness
   auto witness = [&](uint32_t j) -> u32x4 {
@@ -581,7 +581,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_gold(
         sum_rows_inv<FGold, 1>(f, tab, (int)kTabUntwist, (const uint64_t*)s2.src, s2.n, row, 0,
                                (uint64_t*)s2.out, nullptr);
     } else {
-      stage1_only(f, s1, nparts * gridDim.y + blockIdx.y * s1.extra + (blockIdx.x - s2.nblk),
+      stage1_only(f, s1, s1.mac_tasks + (blockIdx.y * s1.extra + (blockIdx.x - s2.nblk)) * s1.tpb,
                   lds64, WAVES);
     }
     return;
@@ -596,7 +596,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_gold(
 #pragma unroll
   for (int k = 0; k < 16; k++) ltw[k][0] = tab[kTabQuad + 16 * t + k];
   uint64_t l1v[4];
-  stage1_load(s1, bid, l1v);
+  stage1_load(s1, bid, s1.mac_tasks, l1v);
   // whole quads share p, so they drop out together (the transform exchanges
   // values within a quad)
   for (uint32_t p = threadIdx.x >> 2; p < ncol; p += 16 * WAVES) {
@@ -612,7 +612,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_gold(
   uint64_t* l1lds = lds64 + cols_per_block * kFuseStrideGold;
   stage1_wave_sum(f, l1v, l1lds);
   __syncthreads();
-  stage1_store(f, s1, bid, l1lds, WAVES);
+  stage1_store(f, s1, bid, s1.mac_tasks, l1lds, WAVES);
   GoldAcc acc[RB];  // exact lazy sums, reduced once per row at the end
   const size_t col_stride = (size_t)rows * 64;
   const uint64_t* akey = A + (size_t)row0 * 64 + lane;
@@ -1048,20 +1048,33 @@ static int fused_plan(gh_ctx* ctx, const gh_ring* ring, const void* s_dev, size_
   const size_t area = defer ? (ctx->work_bytes / 4) & ~(size_t)255 : 0;
   char* w = (char*)ctx->work;
   plan->off = pf.flip * area;
-  plan->s1 = FinStage1{dummy, nullptr, 0, 1, 0, 0, 0, 0};
+  plan->s1 = FinStage1{dummy, nullptr, 0, 1, 0, 0, 0, 0, 0, 1};
   plan->s2 = FinStage2{nullptr, nullptr, 0, 0, 0};
   if (pf.s2.active)
     plan->s2 = FinStage2{w + pf.s2.off, pf.s2.t_out, pf.s2.n, pf.s2.rows,
                          (pf.s2.rows + waves - 1) / waves};
   if (pf.s1.active) {
     groups = (pf.s1.n + per_pass - 1) / per_pass;
-    const uint64_t tasks = (uint64_t)pf.s1.rows * groups;
-    const uint64_t per_rb = (tasks + nrb - 1) / nrb;  // blocks needed per row block
-    const uint32_t extra = per_rb > nparts ? (uint32_t)(per_rb - nparts) : 0;
+    const uint32_t tasks = pf.s1.rows * groups;
+    // stage1_tasks_per_block 0: one task in each MAC block, the rest in
+    // stage-1-only blocks of one task; n > 0: every task in stage-1-only
+    // blocks of n; -1: such blocks, about one per two CUs. (The kernels fit
+    // three blocks per CU; about num_cus / 2 leading blocks leave the MAC
+    // blocks spread evenly and measure fastest, while num_cus leading blocks
+    // put three MAC blocks on some CUs.)
+    const int tune_tpb = ctx->tune.stage1_tasks_per_block;
+    const uint32_t half_cus = std::max(1, ctx->num_cus / 2);
+    const uint32_t tpb =
+        tune_tpb >= 0 ? (uint32_t)tune_tpb
+                      : std::min(kStage1MaxTpb, std::max(1u, (tasks + half_cus - 1) / half_cus));
+    const uint32_t mac_tasks = tpb ? 0 : std::min(tasks, nparts * nrb);
+    const uint32_t eb = tpb ? tpb : 1;
+    const uint32_t nextra = (tasks - mac_tasks + eb - 1) / eb;  // stage-1-only blocks
+    const uint32_t extra = (nextra + nrb - 1) / nrb;
     const uint32_t per = (pf.s1.n + groups - 1) / groups;
     const size_t goff = (2 + (pf.flip ^ 1)) * area;  // the previous commitment's group sums
-    plan->s1 = FinStage1{w + pf.s1.off, w + goff, pf.s1.n, pf.s1.rows, groups, per,
-                         pf.s1.rows * groups, extra};
+    plan->s1 = FinStage1{w + pf.s1.off, w + goff, pf.s1.n, pf.s1.rows, groups, per, tasks,
+                         mac_tasks, extra, eb};
     pf.s2 = PendingStage{true, goff, groups, pf.s1.rows, pf.s1.t_out};
   } else {
     pf.s2.active = false;
@@ -1078,7 +1091,7 @@ static int scratch_plan(gh_ctx* ctx, size_t pbytes, const void* dummy, FusedPlan
   if (pbytes > kScratchBytes) return GH_E_INVALID_ARGUMENT;
   if (int e = ctx_flush(ctx)) return e;
   plan->off = 0;
-  plan->s1 = FinStage1{dummy, nullptr, 0, 1, 0, 0, 0, 0};
+  plan->s1 = FinStage1{dummy, nullptr, 0, 1, 0, 0, 0, 0, 0, 1};
   plan->s2 = FinStage2{nullptr, nullptr, 0, 0, 0};
   return GH_OK;
 }
@@ -1138,7 +1151,8 @@ static int launch_commit_fused(gh_key* key, const void* s_dev, uint32_t cols, vo
   uint32_t* P = scratch ? (uint32_t*)scratch : (uint32_t*)((char*)ctx->work + plan.off);
   dim3 grid(plan.s2.nblk + plan.s1.extra + nparts, rows / rb);
   size_t lds_words = (size_t)cpb * kFuseStride + (size_t)waves * 64;
-  if ((size_t)waves * rb * 64 > lds_words) lds_words = (size_t)waves * rb * 64;
+  lds_words = std::max(lds_words, (size_t)waves * rb * 64);
+  lds_words = std::max(lds_words, (size_t)waves * 64 * plan.s1.tpb);
   const size_t lds_bytes = lds_words * 4;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (!scratch) prof_pair(ctx, 1, &e0, &e1);
@@ -1189,7 +1203,8 @@ static int launch_commit_fused_gold(gh_key* key, const void* s_dev, uint32_t col
   uint64_t* P = scratch ? (uint64_t*)scratch : (uint64_t*)((char*)ctx->work + plan.off);
   dim3 grid(plan.s2.nblk + plan.s1.extra + nparts, rows / rb);
   size_t lds_words = (size_t)cpb * kFuseStrideGold + (size_t)kWaves * 64;
-  if ((size_t)kWaves * rb * 64 > lds_words) lds_words = (size_t)kWaves * rb * 64;
+  lds_words = std::max(lds_words, (size_t)kWaves * rb * 64);
+  lds_words = std::max(lds_words, (size_t)kWaves * 64 * plan.s1.tpb);
   const size_t lds_bytes = lds_words * 8;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (!scratch) prof_pair(ctx, 1, &e0, &e1);

@@ -307,16 +307,25 @@ def test_turning_defer_off_completes_a_held_back_step(ctx, ring_q32):
         key.close()
 
 
+@pytest.mark.parametrize("stage1", [-1, 0, 1, 3, 8])
 @pytest.mark.parametrize("name", ["q32", "goldilocks"])
-def test_long_pipeline_of_varied_commitments(oracle, ctx, ring_q32, ring_gold, name):
+def test_long_pipeline_of_varied_commitments(oracle, ctx, ring_q32, ring_gold, name, stage1):
     """Forty device commitments back to back on one ring (one-item device
     batches, which take the per-call path with its finalize held back), no
     synchronize in between: lengths from 1 to 4096 columns and row counts from
     4 to 64, so the held-back stages of one commitment sometimes fit the next
     launches and sometimes are issued on their own. One output buffer is
-    reused by three commitments in a row (the last one must win)."""
+    reused by three commitments in a row (the last one must win). Every
+    placement of the first held-back stage (stage1_tasks_per_block)."""
+    ctx.set_tuning(stage1_tasks_per_block=stage1)
+    try:
+        _long_pipeline(oracle, ctx, ring_q32 if name == "q32" else ring_gold)
+    finally:
+        ctx.set_tuning(stage1_tasks_per_block=-1)
+
+
+def _long_pipeline(oracle, ctx, ring):
     import lattica_amd as la
-    ring = ring_q32 if name == "q32" else ring_gold
     wb = ring.info.word_bytes
     rng = np.random.default_rng(69)
     shapes = [(16, 4096), (16, 512), (4, 300), (64, 1000), (16, 1), (8, 2048), (32, 4096),
@@ -363,3 +372,52 @@ def test_long_pipeline_of_varied_commitments(oracle, ctx, ring_q32, ring_gold, n
             b.free()
         for k in keys:
             k.close()
+
+
+@pytest.fixture(scope="module")
+def rotation(ctx, ring_q32, ring_gold):
+    """Per ring: three kappa-16 NTT-domain keys over the full key width (2^15
+    q32 columns, 2^14 Goldilocks) and four resident witnesses."""
+    import lattica_amd as la
+    made = {}
+    for name, ring, cols in (("q32", ring_q32, 1 << 15), ("goldilocks", ring_gold, 1 << 14)):
+        rng = np.random.default_rng(4040 + cols)
+        keys = [la.Key(ring, rand_residues(rng, ring.q, (16, cols, 64)).astype(ring.dtype),
+                       domain=la.DOMAIN_NTT) for _ in range(3)]
+        ws = [ctx.to_device(small_signed(rng, ring.q, (cols, 64), -128, 128).astype(ring.dtype))
+              for _ in range(4)]
+        made[name] = (ring, cols, keys, ws)
+    yield made
+    for ring, cols, keys, ws in made.values():
+        for b in ws:
+            b.free()
+        for k in keys:
+            k.close()
+
+
+@pytest.mark.parametrize("stage1", [-1, 0, 1, 2, 4, 8])
+@pytest.mark.parametrize("name", ["q32", "goldilocks"])
+def test_headline_shape_back_to_back_every_stage1_placement(ctx, rotation, name, stage1):
+    """Twelve device commitments at kappa 16 over the full key width, three
+    keys in rotation, no synchronize in between, for each
+    stage1_tasks_per_block: every output equals the one of the same
+    commitment made alone with deferral off."""
+    ring, cols, keys, ws = rotation[name]
+    wb = ring.info.word_bytes
+    outs = [ctx.alloc(16 * 64 * wb) for _ in range(12)]
+    ref = ctx.alloc(16 * 64 * wb)
+    try:
+        ctx.set_tuning(stage1_tasks_per_block=stage1)
+        for i in range(12):
+            keys[i % 3].commit_device(ws[i % 4], cols, outs[i])
+        ctx.synchronize()
+        got = [o.download(ring.dtype, (16, 64)) for o in outs]
+        ctx.set_tuning(defer_finalize=0)
+        for i in range(12):
+            keys[i % 3].commit_device(ws[i % 4], cols, ref)
+            ctx.synchronize()
+            assert np.array_equal(got[i], ref.download(ring.dtype, (16, 64))), (i, stage1)
+    finally:
+        ctx.set_tuning(defer_finalize=1, stage1_tasks_per_block=-1)
+        for b in outs + [ref]:
+            b.free()
