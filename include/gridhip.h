@@ -128,7 +128,11 @@ GH_API int gh_ctx_profile_read(gh_ctx *ctx, int phase, double *total_ms, uint64_
  * enqueues a step held back until then), "stage1_tasks_per_block" (-1, the
  * default: the first half of a held-back step runs in about num_cus / 2
  * leading blocks of the next launch; 0: in its MAC blocks; 1..8: in leading
- * blocks of that many tasks). Values are checked;
+ * blocks of that many tasks), "batch_fused" (0, the default: a Goldilocks
+ * gh_commit_batch or batch plan runs a transform, a grouped MAC and a
+ * finalize kernel; 1: one launch that transforms, multiplies and finalizes,
+ * slower on the batches measured) and "batch_fused_cols" (32..4096, a
+ * multiple of 32: key columns per block of that launch). Values are checked;
  * an unknown key or value gives GH_E_INVALID_ARGUMENT. Every setting yields
  * identical results; only the speed changes. */
 GH_API int gh_ctx_set_tuning(gh_ctx *ctx, const char *key, int64_t value);

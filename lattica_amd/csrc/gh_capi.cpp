@@ -232,6 +232,7 @@ GH_API int gh_ctx_destroy(gh_ctx* ctx) {
   if (ctx->work) hipFree(ctx->work);
   if (ctx->stage) hipFree(ctx->stage);
   if (ctx->misc) hipFree(ctx->misc);
+  if (ctx->batch_done) hipFree(ctx->batch_done);
   for (hipEvent_t e : ctx->prof_pool) hipEventDestroy(e);
   if (ctx->upload_done) hipEventDestroy(ctx->upload_done);
   if (ctx->upload_host) hipHostFree(ctx->upload_host);
@@ -314,6 +315,12 @@ GH_API int gh_ctx_set_tuning(gh_ctx* ctx, const char* key, int64_t value) {
     // turning deferral off completes a finalize held back until now
     if (!value) TRY(set_device(ctx));
     t.defer_finalize = (int)value;
+  } else if (!std::strcmp(key, "batch_fused")) {
+    if (value != 0 && value != 1) return GH_E_INVALID_ARGUMENT;
+    t.batch_fused = (int)value;
+  } else if (!std::strcmp(key, "batch_fused_cols")) {
+    if (value < 32 || value > 4096 || value % 32) return GH_E_INVALID_ARGUMENT;
+    t.batch_fused_cols = (int)value;
   } else if (!std::strcmp(key, "stage1_tasks_per_block")) {
     if (value < -1 || value > (int64_t)kStage1MaxTpb) return GH_E_INVALID_ARGUMENT;
     t.stage1_tasks_per_block = (int)value;
@@ -338,6 +345,8 @@ GH_API int gh_ctx_get_tuning(gh_ctx* ctx, const char* key, int64_t* value) {
   else if (!std::strcmp(key, "batch_group")) *value = t.batch_group;
   else if (!std::strcmp(key, "defer_finalize")) *value = t.defer_finalize;
   else if (!std::strcmp(key, "stage1_tasks_per_block")) *value = t.stage1_tasks_per_block;
+  else if (!std::strcmp(key, "batch_fused")) *value = t.batch_fused;
+  else if (!std::strcmp(key, "batch_fused_cols")) *value = t.batch_fused_cols;
   else return GH_E_INVALID_ARGUMENT;
   return GH_OK;
 }

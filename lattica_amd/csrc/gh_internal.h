@@ -85,6 +85,11 @@ struct gh_tuning {
   // next launch: 0 = one task in each MAC block, n (1..8) = every task in
   // dedicated leading blocks of n tasks, -1 = such blocks, about num_cus / 2
   int stage1_tasks_per_block = -1;
+  // Goldilocks gh_commit_batch and batch plans: 0 = transform, grouped MAC
+  // and finalize kernels (faster), 1 = one launch (k_batch_fused_gold)
+  int batch_fused = 0;
+  // key columns per block of that launch (a multiple of 16)
+  int batch_fused_cols = 256;
 };
 // most stage-1 tasks one dedicated block runs
 constexpr uint32_t kStage1MaxTpb = 8;
@@ -139,6 +144,10 @@ struct gh_ctx {
   size_t upload_bytes = 0;
   hipEvent_t upload_done = nullptr;
   bool upload_pending = false;
+  // per-item completion counters of the one-launch Goldilocks batch; zero
+  // between launches (the kernel resets what it counts)
+  uint32_t* batch_done = nullptr;
+  size_t batch_done_items = 0;
   // phase timing (gh_ctx_profile_*): event pairs recorded per commit
   bool prof_on = false;
   std::vector<hipEvent_t> prof_pool;
@@ -209,6 +218,11 @@ int launch_commit_noalloc(gh_key* key, const void* s_dev, uint32_t cols, void* t
                           void* scratch);
 // items hold device pointers; every len in [1, key columns]
 int launch_commit_batch(gh_key* key, const gh_commit_item* items, size_t count);
+// one-launch Goldilocks batch over sorted descriptors (ranges of range_cols
+// key columns); done: count zeroed counters, zero again when it completes
+int enqueue_batch_fused_gold(gh_key* key, hipStream_t stream, const BatchDesc* desc,
+                             uint32_t count, uint32_t maxlen, uint32_t range_cols,
+                             uint32_t* done);
 
 // Batch building blocks on an explicit stream. Descriptor arrays are device
 // copies; MAC descriptors are sorted by len, longest first, and the host
