@@ -857,7 +857,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--only", choices=["small", "negacyclic", "labrador", "goldilocks", "serving",
-                                       "wire", "hbm"],
+                                       "wire", "hbm", "issue"],
                     help="run one side workload alone (for profiling) and print its JSON")
     args = ap.parse_args()
     global CPU_ON
@@ -885,7 +885,10 @@ def main():
                 # streaming read and copy probes of known size (1 GiB): the
                 # calibration of rocprofv3's FETCH_SIZE / WRITE_SIZE
                 "hbm": lambda: dict(zip(("read_GBps", "copy_GBps"),
-                                        ctx.measure_bandwidth(1 << 30, 3)))}
+                                        ctx.measure_bandwidth(1 << 30, 3))),
+                # int8 MFMA and VALU issue rates, alone and interleaved: what
+                # bounds the JL projection
+                "issue": lambda: {str(w): ctx.measure_issue(w) for w in (1, 2, 3, 4)}}
         print(json.dumps({"workload": args.only, "result": legs[args.only]()}), flush=True)
         ring.close()
         ctx.close()

@@ -147,6 +147,20 @@ GH_API int gh_ctx_get_tuning(gh_ctx *ctx, const char *key, int64_t *value);
 GH_API int gh_measure_bandwidth(gh_ctx *ctx, size_t bytes, int reps, double *read_gbps,
                                 double *copy_gbps);
 
+/* Issue rates of the context's device: the best of three timed launches of
+ * four probe kernels, in microseconds, on num_cus * waves_per_simd blocks of
+ * four waves, each wave running 4096 iterations of
+ *   out_us[0]: four independent v_mfma_i32_32x32x32_i8,
+ *   out_us[1]: eight independent v_mfma_i32_16x16x64_i8 (the same int8
+ *              multiply-accumulates as [0]),
+ *   out_us[2]: four independent ChaCha20 quarter rounds (48 VALU
+ *              instructions),
+ *   out_us[3]: the work of [0] and [2] interleaved in one loop, which shows
+ *              how much of the matrix-core time a wave's VALU work hides.
+ * waves_per_simd 1..8. Measurement support for the JL projection's roofline;
+ * no reference counterpart. */
+GH_API int gh_measure_issue(gh_ctx *ctx, int waves_per_simd, double out_us[4]);
+
 /* device memory, so callers without a HIP runtime of their own can keep
  * inputs resident in HBM */
 GH_API int gh_alloc(gh_ctx *ctx, size_t bytes, void **dev_ptr);

@@ -57,7 +57,7 @@ EXPORTS = (
     "gh_batch_plan_destroy", "gh_bdlop_commit", "gh_decompose_balanced", "gh_garbage_h",
     "gh_amortize", "gh_wire_size", "gh_wire_encode", "gh_wire_peek", "gh_wire_decode",
     "gh_key_save", "gh_key_load", "gh_jl_extract_flat", "gh_jl_conjugate", "gh_jl_evals",
-    "gh_jl_aggregate", "gh_jl_launch_shape", "gh_commit_noalloc",
+    "gh_jl_aggregate", "gh_jl_launch_shape", "gh_commit_noalloc", "gh_measure_issue",
 )
 
 
@@ -152,6 +152,7 @@ def lib():
         "gh_ctx_num_cus": (i32, [vp, ctypes.POINTER(i32)]),
         "gh_ctx_profile_enable": (i32, [vp, i32]),
         "gh_ctx_set_tuning": (i32, [vp, ctypes.c_char_p, ctypes.c_int64]),
+        "gh_measure_issue": (i32, [vp, i32, ctypes.POINTER(ctypes.c_double)]),
         "gh_measure_bandwidth": (i32, [vp, sz, i32, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_double)]),
         "gh_ctx_get_tuning": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]),
@@ -285,6 +286,26 @@ class Context:
         _check(lib().gh_measure_bandwidth(self.h, nbytes, reps, ctypes.byref(r), ctypes.byref(c)),
                "gh_measure_bandwidth")
         return r.value, c.value
+
+    def measure_issue(self, waves_per_simd: int = 2) -> dict:
+        """gh_measure_issue: microseconds per probe launch, the int8 MFMA
+        rates (TOP/s, two ops per multiply-add), the VALU instruction rate
+        (wave instructions per second) and the mixed loop's time against the
+        sum and the larger of its two parts."""
+        us = (ctypes.c_double * 4)()
+        _check(lib().gh_measure_issue(self.h, waves_per_simd, us), "gh_measure_issue")
+        waves = self.num_cus * waves_per_simd * 4
+        iters = 4096
+        ops_mfma = waves * iters * 4 * 2 * 32 * 32 * 32  # int8 ops (2 per multiply-add)
+        valu = waves * iters * 48                           # wave-level VALU instructions
+        return {"waves_per_simd": waves_per_simd,
+                "us": {"mfma_32x32x32": us[0], "mfma_16x16x64": us[1], "valu_chacha": us[2],
+                       "mixed": us[3]},
+                "mfma_32x32x32_tops": ops_mfma / (us[0] * 1e-6) / 1e12,
+                "mfma_16x16x64_tops": ops_mfma / (us[1] * 1e-6) / 1e12,
+                "valu_ginst_per_s": valu / (us[2] * 1e-6) / 1e9,
+                "mixed_over_sum": us[3] / (us[0] + us[2]),
+                "mixed_over_max": us[3] / max(us[0], us[2])}
 
     def profile(self, on: bool):
         _check(lib().gh_ctx_profile_enable(self.h, 1 if on else 0), "gh_ctx_profile_enable")

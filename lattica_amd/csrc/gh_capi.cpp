@@ -395,6 +395,37 @@ GH_API int gh_measure_bandwidth(gh_ctx* ctx, size_t bytes, int reps, double* rea
   return st;
 }
 
+GH_API int gh_measure_issue(gh_ctx* ctx, int waves_per_simd, double out_us[4]) {
+  if (!ctx || !out_us || waves_per_simd < 1 || waves_per_simd > 8) return GH_E_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  // 4-wave blocks: waves_per_simd blocks per CU
+  const uint32_t blocks = (uint32_t)ctx->num_cus * (uint32_t)waves_per_simd;
+  int* out = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int st = GH_OK;
+  if (hipMalloc(&out, (size_t)blocks * 256 * sizeof(int)) != hipSuccess) st = GH_E_OOM;
+  if (st == GH_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    st = GH_E_DEVICE;
+  for (int kind = 0; kind < 4 && st == GH_OK; kind++) {
+    double best = 0;
+    for (int r = 0; r < 4 && st == GH_OK; r++) {  // the first run warms up
+      st = launch_issue_probe(ctx, kind, blocks, out, e0, e1);
+      float ms = 0;
+      if (st == GH_OK && (hipEventSynchronize(e1) != hipSuccess ||
+                          hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        st = GH_E_DEVICE;
+      if (st == GH_OK && r > 0 && (best == 0 || ms * 1e3 < best)) best = ms * 1e3;
+    }
+    out_us[kind] = best;
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  hipStreamSynchronize(ctx->stream);
+  if (out) hipFree(out);
+  return st;
+}
+
 GH_API int gh_ctx_num_cus(gh_ctx* ctx, int* out) {
   if (!ctx || !out) return GH_E_INVALID_ARGUMENT;
   *out = ctx->num_cus;

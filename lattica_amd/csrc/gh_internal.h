@@ -247,6 +247,9 @@ int launch_garbage_h(gh_ring* ring, const void* phi, const void* s, uint32_t r, 
 // s_scratch: r x n polys of device memory for the NTT-domain witness (domain COEFF)
 int launch_amortize(gh_ring* ring, const void* c, const void* s, uint32_t r, uint32_t n,
                     int domain, void* z_dev, void* s_scratch);
+// gh_measure_issue's kernels: kind 0..3, kIssueIters iterations per wave
+int launch_issue_probe(gh_ctx* ctx, int kind, uint32_t blocks, int* out, hipEvent_t e0,
+                       hipEvent_t e1);
 int launch_stream_probe(gh_ctx* ctx, const void* src, void* dst, size_t bytes, int copy,
                         hipEvent_t e0, hipEvent_t e1);
 // canonical encodings (gh_wire.hip); chunk lists follow the segments

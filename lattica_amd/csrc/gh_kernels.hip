@@ -724,6 +724,70 @@ __global__ __launch_bounds__(512) void k_stream_copy(const u32x4* __restrict__ s
     __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
+// Issue-rate probes (gh_measure_issue): each wave runs kIssueIters loop
+// iterations of independent work and writes one value per thread (so the
+// work stays live) to out[blockIdx.x * 256 + threadIdx.x]:
+//   0: four independent v_mfma_i32_32x32x32_i8 per iteration;
+//   1: eight independent v_mfma_i32_16x16x64_i8 per iteration (the same
+//      int8 multiply-accumulates per iteration as 0);
+//   2: four independent ChaCha20 quarter rounds (48 VALU instructions);
+//   3: the MFMAs of 0 and the quarter rounds of 2 interleaved in one loop.
+constexpr int kIssueIters = 4096;
+typedef int pv4i __attribute__((ext_vector_type(4)));
+typedef int pv16i __attribute__((ext_vector_type(16)));
+#define GH_PQR(a, b, c, d)                                                                   \
+  a += b; d ^= a; d = __builtin_amdgcn_alignbit(d, d, 16); c += d; b ^= c;                   \
+  b = __builtin_amdgcn_alignbit(b, b, 20); a += b; d ^= a; d = __builtin_amdgcn_alignbit(d, d, 24); \
+  c += d; b ^= c; b = __builtin_amdgcn_alignbit(b, b, 25);
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_issue_probe(int* __restrict__ out, int seed) {
+  const pv4i a = {seed + (int)threadIdx.x, 1, 2, 3}, b = {3, 2, 1, seed};
+  pv16i c0{}, c1{}, c2{}, c3{};
+  pv4i d[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) d[j] = pv4i{};
+  uint32_t x0 = seed + threadIdx.x, x1 = 1, x2 = 2, x3 = 3, x4 = 4, x5 = 5, x6 = 6, x7 = 7;
+  uint32_t x8 = 8, x9 = 9, x10 = 10, x11 = 11, x12 = 12, x13 = 13, x14 = 14, x15 = 15;
+  for (int i = 0; i < kIssueIters; i++) {
+    if constexpr (KIND == 0 || KIND == 3) c0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c0, 0, 0, 0);
+    if constexpr (KIND >= 2) { GH_PQR(x0, x4, x8, x12) }
+    if constexpr (KIND == 0 || KIND == 3) c1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c1, 0, 0, 0);
+    if constexpr (KIND >= 2) { GH_PQR(x1, x5, x9, x13) }
+    if constexpr (KIND == 0 || KIND == 3) c2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c2, 0, 0, 0);
+    if constexpr (KIND >= 2) { GH_PQR(x2, x6, x10, x14) }
+    if constexpr (KIND == 0 || KIND == 3) c3 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c3, 0, 0, 0);
+    if constexpr (KIND >= 2) { GH_PQR(x3, x7, x11, x15) }
+    if constexpr (KIND == 1) {
+#pragma unroll
+      for (int j = 0; j < 8; j++) d[j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, d[j], 0, 0, 0);
+    }
+  }
+  int s = (int)(x0 ^ x1 ^ x2 ^ x3 ^ x4 ^ x5 ^ x6 ^ x7 ^ x8 ^ x9 ^ x10 ^ x11 ^ x12 ^ x13 ^ x14 ^ x15);
+#pragma unroll
+  for (int r = 0; r < 16; r++) s += c0[r] + c1[r] + c2[r] + c3[r];
+#pragma unroll
+  for (int j = 0; j < 8; j++) s += d[j][0] + d[j][1] + d[j][2] + d[j][3];
+  out[blockIdx.x * 256 + threadIdx.x] = s;
+}
+#undef GH_PQR
+
+// kind 0..3 on `blocks` blocks of 256 threads; out holds blocks * 256 ints
+int launch_issue_probe(gh_ctx* ctx, int kind, uint32_t blocks, int* out, hipEvent_t e0,
+                       hipEvent_t e1) {
+  const dim3 grid(blocks), block(256);
+  if (kind == 0)
+    hipExtLaunchKernelGGL((k_issue_probe<0>), grid, block, 0, ctx->stream, e0, e1, 0, out, 1);
+  else if (kind == 1)
+    hipExtLaunchKernelGGL((k_issue_probe<1>), grid, block, 0, ctx->stream, e0, e1, 0, out, 1);
+  else if (kind == 2)
+    hipExtLaunchKernelGGL((k_issue_probe<2>), grid, block, 0, ctx->stream, e0, e1, 0, out, 1);
+  else
+    hipExtLaunchKernelGGL((k_issue_probe<3>), grid, block, 0, ctx->stream, e0, e1, 0, out, 1);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
 int launch_stream_probe(gh_ctx* ctx, const void* src, void* dst, size_t bytes, int copy,
                         hipEvent_t e0, hipEvent_t e1) {
   const size_t n = bytes / 16;
