@@ -123,6 +123,19 @@ def goldilocks_batch(ctx, la, log, count=2000, kappa=16, max_cols=1 << 14):
     ctx.synchronize()
     graph = count * reps / (time.perf_counter() - t0)
     plan.close()
+    # the one-launch form (tuning batch_fused = 1), graph-replayed the same way
+    ctx.set_tuning(batch_fused=1, batch_fused_cols=128)
+    plan = key.batch_plan(wptr, lens, optr)
+    plan.launch()
+    ctx.synchronize()
+    fused_same = bool(np.array_equal(out.download(np.uint64, (count, kappa, D)), res))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plan.launch()
+    ctx.synchronize()
+    fused = count * reps / (time.perf_counter() - t0)
+    plan.close()
+    ctx.set_tuning(batch_fused=0)
     t0 = time.perf_counter()
     for i in range(count):
         key.commit_batch_device([wptr[i]], [lens[i]], [optr[i]])
@@ -130,11 +143,13 @@ def goldilocks_batch(ctx, la, log, count=2000, kappa=16, max_cols=1 << 14):
     plain = count / (time.perf_counter() - t0)
     res = {"batched_commits_per_s": round(batched, 1),
            "graph_replay_commits_per_s": round(graph, 1), "graph_equals_batched": graph_same,
+           "one_launch_graph_commits_per_s": round(fused, 1),
+           "one_launch_equals_batched": fused_same,
            "per_call_commits_per_s": round(plain, 1),
            "mean_len": float(lens.mean()), "oracle_spot_check": ok}
     log(f"Goldilocks batch x{count}: batched {batched:,.0f} commits/s, graph replay {graph:,.0f} "
-        f"commits/s (== batched: {graph_same}), per-call {plain:,.0f} commits/s; "
-        f"oracle spot check {ok}")
+        f"commits/s (== batched: {graph_same}), one-launch graph {fused:,.0f} commits/s "
+        f"(== batched: {fused_same}), per-call {plain:,.0f} commits/s; oracle spot check {ok}")
     if CPU_ON:
         # the reference's AVX2 prepared commit on the wide Goldilocks path
         # (L = u64, u64_arith.rs:193-357), restated: a random sample of the
@@ -1020,11 +1035,22 @@ def main():
                                      "stream_read_commit_size": round(rd_at, 1)}
         log(f"measured HBM ceiling: read {rd:,.0f} GB/s, copy {cp:,.0f} GB/s (1 GiB); "
             f"read {rd_at:,.0f} GB/s at {ALGO_BYTES / 1e6:.1f} MB")
+        # VALU counters of the C2 and C4 kernels from the committed rocprofv3
+        # summary (not measured in this run)
+        vpath = os.path.join(ROOT, "profiles", "valu_c2_c4.json")
+        valu = {}
+        if os.path.exists(vpath):
+            with open(vpath) as fh:
+                valu = json.load(fh)
         extra["negacyclic_q32_2^18_pairs"] = negacyclic_batch(ctx, la, ring, log)
+        if "negacyclic_q32_2^18_pairs" in valu:
+            extra["negacyclic_q32_2^18_pairs"]["valu_profile"] = valu["negacyclic_q32_2^18_pairs"]
         extra["batched_same_shape"] = batched_same_shape(ctx, keys, s_dev, log)
         extra["commit_q32_k8_n256"] = commit_small(ctx, la, ring, log)
         extra["labrador_q32_r32_n1024"] = labrador_calls(ctx, la, ring, log)
         extra["goldilocks_batch_2000"] = goldilocks_batch(ctx, la, log)
+        if "goldilocks_batch_2000" in valu:
+            extra["goldilocks_batch_2000"]["valu_profile"] = valu["goldilocks_batch_2000"]
         extra["serving_mixed_1000"] = serving(ctx, la, log)
         extra["wire_formats"] = wire_formats(ctx, la, ring, log)
 
