@@ -221,6 +221,27 @@ GH_API int gh_negacyclic_mul(gh_ring *ring, const void *a, const void *b, void *
 GH_API int gh_pointwise_mul(gh_ring *ring, const void *a, const void *b, void *out, size_t count,
                             int mem);
 
+/* ---- Ring slice hooks ------------------------------------------------------
+ * The remaining `Ring` slice hooks (arith/ring.rs:75-135), overridden for
+ * PrimeField at arith/prime.rs:433-491 (simd/montgomery.rs add_assign :125,
+ * sub_assign :257, scalar_mul :389). They act on `len` field elements (any
+ * length, not a number of polynomials), canonical words of the ring. `src`
+ * may equal `dst`; other overlaps are undefined. `scalar` is a canonical
+ * residue (GH_E_NOT_CANONICAL otherwise).
+ *   gh_add_assign:        dst[i] = dst[i] + src[i]           (add_assign_slice)
+ *   gh_sub_assign:        dst[i] = dst[i] - src[i]           (sub_assign_slice)
+ *   gh_scalar_mul:        dst[i] = dst[i] * scalar           (scalar_mul_slice)
+ *   gh_add_assign_scaled: dst[i] = dst[i] + src[i] * scalar  (add_assign_scaled_slice)
+ *   gh_dot_product:       out[0] = sum_i a[i] * b[i]         (dot_product), one
+ *     word in `mem` (a device pointer for GH_MEM_DEVICE); 0 for len 0. */
+GH_API int gh_add_assign(gh_ring *ring, void *dst, const void *src, size_t len, int mem);
+GH_API int gh_sub_assign(gh_ring *ring, void *dst, const void *src, size_t len, int mem);
+GH_API int gh_scalar_mul(gh_ring *ring, void *dst, uint64_t scalar, size_t len, int mem);
+GH_API int gh_add_assign_scaled(gh_ring *ring, void *dst, const void *src, uint64_t scalar,
+                                size_t len, int mem);
+GH_API int gh_dot_product(gh_ring *ring, const void *a, const void *b, size_t len, void *out,
+                          int mem);
+
 /* ---- resident commitment key ---------------------------------------------
  * PreparedLinearOps::build_linear_key_cache (commit/src/linear.rs:701-708,
  * prepare_linear_key :377-395): A = [A_msg | A_open] kept NTT-domain in HBM in

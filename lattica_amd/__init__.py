@@ -58,6 +58,7 @@ EXPORTS = (
     "gh_amortize", "gh_wire_size", "gh_wire_encode", "gh_wire_peek", "gh_wire_decode",
     "gh_key_save", "gh_key_load", "gh_jl_extract_flat", "gh_jl_conjugate", "gh_jl_evals",
     "gh_jl_aggregate", "gh_jl_launch_shape", "gh_commit_noalloc", "gh_measure_issue",
+    "gh_add_assign", "gh_sub_assign", "gh_scalar_mul", "gh_add_assign_scaled", "gh_dot_product",
 )
 
 
@@ -171,6 +172,11 @@ def lib():
         "gh_twisted_inverse": (i32, [vp, vp, sz, i32]),
         "gh_negacyclic_mul": (i32, [vp, vp, vp, vp, sz, i32]),
         "gh_pointwise_mul": (i32, [vp, vp, vp, vp, sz, i32]),
+        "gh_add_assign": (i32, [vp, vp, vp, sz, i32]),
+        "gh_sub_assign": (i32, [vp, vp, vp, sz, i32]),
+        "gh_scalar_mul": (i32, [vp, vp, u64, sz, i32]),
+        "gh_add_assign_scaled": (i32, [vp, vp, vp, u64, sz, i32]),
+        "gh_dot_product": (i32, [vp, vp, vp, sz, vp, i32]),
         "gh_key_create": (i32, [vp, u32, vp, u32, vp, u32, i32, i32, ctypes.POINTER(vp)]),
         "gh_key_destroy": (i32, [vp]),
         "gh_key_dims": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
@@ -416,6 +422,35 @@ class Ring:
         _check(lib().gh_pointwise_mul(self.h, _ptr(a), _ptr(b), _ptr(out), a.size // self.d,
                                       MEM_HOST), "gh_pointwise_mul")
         return out
+
+    # Ring slice hooks on flat word arrays (any length); each returns a new array
+    def add_assign(self, dst, src):
+        d, s = self.words(dst).copy(), self.words(src)
+        _check(lib().gh_add_assign(self.h, _ptr(d), _ptr(s), d.size, MEM_HOST), "gh_add_assign")
+        return d
+
+    def sub_assign(self, dst, src):
+        d, s = self.words(dst).copy(), self.words(src)
+        _check(lib().gh_sub_assign(self.h, _ptr(d), _ptr(s), d.size, MEM_HOST), "gh_sub_assign")
+        return d
+
+    def scalar_mul(self, dst, scalar: int):
+        d = self.words(dst).copy()
+        _check(lib().gh_scalar_mul(self.h, _ptr(d), int(scalar), d.size, MEM_HOST), "gh_scalar_mul")
+        return d
+
+    def add_assign_scaled(self, dst, src, scalar: int):
+        d, s = self.words(dst).copy(), self.words(src)
+        _check(lib().gh_add_assign_scaled(self.h, _ptr(d), _ptr(s), int(scalar), d.size, MEM_HOST),
+               "gh_add_assign_scaled")
+        return d
+
+    def dot_product(self, a, b) -> int:
+        a, b = self.words(a), self.words(b)
+        out = np.zeros(1, dtype=self.dtype)
+        _check(lib().gh_dot_product(self.h, _ptr(a), _ptr(b), a.size, _ptr(out), MEM_HOST),
+               "gh_dot_product")
+        return int(out[0])
 
     def gram(self, s, domain=DOMAIN_COEFF):
         s = self.words(s)

@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libgridhip.so")
 ARCH = os.environ.get("GRIDHIP_ARCH", "gfx950")
 
 SOURCES = ["gh_plan.cpp", "gh_capi.cpp", "gh_kernels.hip", "gh_labrador.hip", "gh_batch.hip",
-           "gh_server.cpp", "gh_wire.hip", "gh_jl.hip"]
+           "gh_server.cpp", "gh_wire.hip", "gh_jl.hip", "gh_slice.hip"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",

@@ -585,6 +585,81 @@ GH_API int gh_pointwise_mul(gh_ring* ring, const void* a, const void* b, void* o
 
 namespace {
 
+// Ring slice hooks: dst op= src (src unused by kSliceScale) on `len` words.
+int slice_entry(gh_ring* ring, int op, void* dst, const void* src, uint64_t scalar, size_t len,
+                int mem) {
+  const bool has_src = op != kSliceScale;
+  if (!ring || (len && (!dst || (has_src && !src)))) return GH_E_INVALID_ARGUMENT;
+  TRY(check_mem(mem));
+  if ((op == kSliceScale || op == kSliceAxpy) && scalar >= ring->q) return GH_E_NOT_CANONICAL;
+  if (len == 0) return GH_OK;
+  gh_ctx* ctx = ring->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  if (mem == GH_MEM_DEVICE) return launch_slice(ring, op, dst, src, scalar, len);
+  TRY(check_canonical_host(ring, dst, len));
+  if (has_src) TRY(check_canonical_host(ring, src, len));
+  const size_t bytes = len * ring->word_bytes;
+  const size_t bpad = (bytes + 255) & ~(size_t)255;  // both operands 16-byte aligned
+  void* dev;
+  TRY(ctx_staging(ctx, (has_src ? 2 : 1) * bpad, &dev));
+  void* dsrc = has_src ? (char*)dev + bpad : nullptr;
+  TRY(h2d(ctx, dev, dst, bytes));
+  if (has_src) TRY(h2d(ctx, dsrc, src, bytes));
+  TRY(launch_slice(ring, op, dev, dsrc, scalar, len));
+  TRY(d2h(ctx, dst, dev, bytes));
+  return sync(ctx);
+}
+
+}  // namespace
+
+extern "C" {
+
+GH_API int gh_add_assign(gh_ring* ring, void* dst, const void* src, size_t len, int mem) {
+  return slice_entry(ring, kSliceAdd, dst, src, 0, len, mem);
+}
+GH_API int gh_sub_assign(gh_ring* ring, void* dst, const void* src, size_t len, int mem) {
+  return slice_entry(ring, kSliceSub, dst, src, 0, len, mem);
+}
+GH_API int gh_scalar_mul(gh_ring* ring, void* dst, uint64_t scalar, size_t len, int mem) {
+  return slice_entry(ring, kSliceScale, dst, nullptr, scalar, len, mem);
+}
+GH_API int gh_add_assign_scaled(gh_ring* ring, void* dst, const void* src, uint64_t scalar,
+                                size_t len, int mem) {
+  return slice_entry(ring, kSliceAxpy, dst, src, scalar, len, mem);
+}
+
+GH_API int gh_dot_product(gh_ring* ring, const void* a, const void* b, size_t len, void* out,
+                          int mem) {
+  if (!ring || !out || (len && (!a || !b))) return GH_E_INVALID_ARGUMENT;
+  TRY(check_mem(mem));
+  gh_ctx* ctx = ring->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  const size_t wb = ring->word_bytes;
+  void* partial;
+  TRY(ctx_workspace(ctx, dot_partial_words(ring, len) * wb, &partial));
+  if (mem == GH_MEM_DEVICE) return launch_dot(ring, a, b, len, partial, out);
+  TRY(check_canonical_host(ring, a, len));
+  TRY(check_canonical_host(ring, b, len));
+  const size_t bytes = len * wb;
+  const size_t bpad = (bytes + 255) & ~(size_t)255;
+  void* dev;
+  TRY(ctx_staging(ctx, 2 * bpad + 256, &dev));
+  void* da = dev;
+  void* db = (char*)dev + bpad;
+  void* dout = (char*)dev + 2 * bpad;
+  TRY(h2d(ctx, da, a, bytes));
+  TRY(h2d(ctx, db, b, bytes));
+  TRY(launch_dot(ring, da, db, len, partial, dout));
+  TRY(d2h(ctx, out, dout, wb));
+  return sync(ctx);
+}
+
+}  // extern "C"
+
+namespace {
+
 // Builds a resident key from device sources (context lock held): the
 // [col][row][slot] layout of both blocks, then, for a coefficient-domain key,
 // prepare_poly_matrix (linear.rs:345-359): one forward transform; the layout

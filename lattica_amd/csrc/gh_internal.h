@@ -252,6 +252,14 @@ int launch_issue_probe(gh_ctx* ctx, int kind, uint32_t blocks, int* out, hipEven
                        hipEvent_t e1);
 int launch_stream_probe(gh_ctx* ctx, const void* src, void* dst, size_t bytes, int copy,
                         hipEvent_t e0, hipEvent_t e1);
+// Ring slice hooks (gh_slice.hip) on `len` device words: dst op= src, the
+// scalar a canonical residue (kSliceScale: dst *= scalar, src unused;
+// kSliceAxpy: dst += scalar * src)
+enum SliceOp { kSliceAdd = 0, kSliceSub = 1, kSliceScale = 2, kSliceAxpy = 3 };
+int launch_slice(gh_ring* ring, int op, void* dst, const void* src, uint64_t scalar, size_t len);
+// out[0] = sum a[i] * b[i] mod q; partial: dot_partial_words(len) device words
+size_t dot_partial_words(gh_ring* ring, size_t len);
+int launch_dot(gh_ring* ring, const void* a, const void* b, size_t len, void* partial, void* out);
 // canonical encodings (gh_wire.hip); chunk lists follow the segments
 void wire_chunks(const std::vector<WireSeg>& segs, std::vector<WireChunk>* out);
 int enqueue_wire_unpack(gh_ring* ring, hipStream_t stream, const WireSeg* segs,

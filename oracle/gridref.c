@@ -127,6 +127,50 @@ uint64_t gr_to_montgomery(uint64_t q, uint64_t a) { field_t f; field_init(&f, q)
 uint64_t gr_from_montgomery(uint64_t q, uint64_t a) { field_t f; field_init(&f, q); return f_from(&f, a); }
 uint64_t gr_q_inv_neg(uint64_t q) { field_t f; field_init(&f, q); return f.q_inv; }
 
+/* ---- Ring slice hooks of PrimeField, prime.rs:433-491 -------------------- */
+/* Canonical words in and out; in between they live in Montgomery form as the
+ * reference stores them. op 0: add_assign_slice (montgomery.rs add_assign),
+ * 1: sub_assign_slice, 2: scalar_mul_slice (dst *= scalar), 3:
+ * add_assign_scaled_slice: slices longer than CHUNK = 64 go chunk by chunk,
+ * the chunk's products scaled first and then added (prime.rs:449-465). */
+int gr_slice_op(uint64_t q, int op, uint64_t *dst, const uint64_t *src, uint64_t scalar,
+                size_t len) {
+    if (op < 0 || op > 3) return GR_E_INVALID_ARGUMENT;
+    field_t f; field_init(&f, q);
+    const uint64_t c = f_to(&f, scalar);
+    enum { CHUNK = 64 };
+    for (size_t base = 0; base < len; base += CHUNK) {
+        const size_t n = len - base < CHUNK ? len - base : CHUNK;
+        uint64_t prod[CHUNK];
+        for (size_t i = 0; i < n; i++) {
+            const uint64_t d = f_to(&f, dst[base + i]);
+            uint64_t r;
+            if (op == 0) {
+                r = f_add(&f, d, f_to(&f, src[base + i]));
+            } else if (op == 1) {
+                r = f_sub(&f, d, f_to(&f, src[base + i]));
+            } else if (op == 2) {
+                r = f_mul(&f, d, c);
+            } else {
+                prod[i] = f_mul(&f, f_to(&f, src[base + i]), c);
+                r = f_add(&f, d, prod[i]);
+            }
+            dst[base + i] = f_from(&f, r);
+        }
+    }
+    return GR_OK;
+}
+
+/* dot_product, prime.rs:467-491: chunks of 64 pointwise products, each added
+ * into one running sum in order. */
+uint64_t gr_dot_product(uint64_t q, const uint64_t *a, const uint64_t *b, size_t len) {
+    field_t f; field_init(&f, q);
+    uint64_t acc = 0; /* Montgomery zero */
+    for (size_t i = 0; i < len; i++)
+        acc = f_add(&f, acc, f_mul(&f, f_to(&f, a[i]), f_to(&f, b[i])));
+    return f_from(&f, acc);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Generator and roots of unity                                              */
 /* ------------------------------------------------------------------------ */

@@ -37,6 +37,11 @@ extern "C" {
 uint64_t gr_add(uint64_t q, uint64_t a, uint64_t b);
 uint64_t gr_sub(uint64_t q, uint64_t a, uint64_t b);
 uint64_t gr_mul(uint64_t q, uint64_t a, uint64_t b);
+/* Ring slice hooks (prime.rs:433-491): op 0 add, 1 sub, 2 scalar mul,
+ * 3 add scaled; canonical words */
+int gr_slice_op(uint64_t q, int op, uint64_t *dst, const uint64_t *src, uint64_t scalar,
+                size_t len);
+uint64_t gr_dot_product(uint64_t q, const uint64_t *a, const uint64_t *b, size_t len);
 uint64_t gr_pow(uint64_t q, uint64_t a, uint64_t e);
 uint64_t gr_inv(uint64_t q, uint64_t a);
 /* raw Montgomery hooks, so tests can pin the R = 2^64 words themselves */

@@ -43,6 +43,10 @@ def lib():
             getattr(L, name).argtypes = [u64, u64]
         L.gr_q_inv_neg.restype = u64
         L.gr_q_inv_neg.argtypes = [u64]
+        L.gr_slice_op.restype = i32
+        L.gr_slice_op.argtypes = [u64, i32, U64P, U64P, u64, sz]
+        L.gr_dot_product.restype = u64
+        L.gr_dot_product.argtypes = [u64, U64P, U64P, sz]
         L.gr_max_ntt_size.restype = u64
         L.gr_max_ntt_size.argtypes = [u64]
         L.gr_find_generator.restype = i32
@@ -146,6 +150,25 @@ def sub(q, a, b):
 
 def mul(q, a, b):
     return int(lib().gr_mul(q, a, b))
+
+
+SLICE_OPS = {"add": 0, "sub": 1, "scalar_mul": 2, "add_scaled": 3}
+
+
+def slice_op(q, op: str, dst, src=None, scalar: int = 0):
+    """Ring slice hooks of PrimeField (prime.rs:433-491) on flat word arrays:
+    add / sub (dst op src), scalar_mul (dst * scalar), add_scaled (dst + src * scalar)."""
+    d = _u64(dst).copy().reshape(-1)
+    s = _u64(src).reshape(-1) if src is not None else np.zeros(1, dtype=np.uint64)
+    _check(lib().gr_slice_op(q, SLICE_OPS[op], _p(d), _p(s), int(scalar), d.size), "slice_op")
+    return d
+
+
+def dot_product(q, a, b) -> int:
+    """Ring::dot_product of PrimeField (prime.rs:467-491)."""
+    a, b = _u64(a).reshape(-1), _u64(b).reshape(-1)
+    assert a.size == b.size
+    return int(lib().gr_dot_product(q, _p(a), _p(b), a.size))
 
 
 def pow_(q, a, e):

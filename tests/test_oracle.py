@@ -482,3 +482,55 @@ def test_amortize_pins(oracle):
             acc = [(a + b) % q for a, b in zip(acc, t)]
         exp.append(acc)
     assert oracle.amortize(q, d, c, s).tolist() == exp
+
+
+# ---- Ring slice hooks (prime.rs:433-491) ------------------------------------
+
+# The inputs of the reference's slice-hook tests; each asserts hook == the
+# scalar operation, whose value is the closed form below.
+SLICE_KATS = [
+    # prime.rs:945-972 (add/sub) and :1163-1190 (mul hooks, scalar 19)
+    (F12289, [1, 7, 12_000, 3, 11], [2, 9, 1_111, 14, 6], 19),
+    # prime.rs:1055-1095: every hook, scalar 19
+    (F3329, [1, 7, 3_000, 3, 11], [2, 9, 1_111, 14, 6], 19),
+    # prime.rs:1097-1124 (add/sub above 2^32)
+    (4294967311, [1, 7, 4_000_000_000, 3, 11], [2, 9, 111_111_111, 14, 6], 5),
+    # prime.rs:1126-1160 (Goldilocks, scalar 2^32 + 13)
+    (GOLDILOCKS, [1, (1 << 32) - 1, GOLDILOCKS - 2, 123_456_789, 9_876_543_210],
+     [2, (1 << 32) + 7, GOLDILOCKS - 3, 987_654_321, 77_777_777], (1 << 32) + 13),
+    # prime.rs:1192-1224 (q = 8380417, scalar 19)
+    (8380417, [1, 7, 8_000_000, 3, 11], [2, 9, 111_111, 14, 6], 19),
+]
+
+
+def _closed(q, op, d, s, c):
+    if op == "add":
+        return [(x + y) % q for x, y in zip(d, s)]
+    if op == "sub":
+        return [(x - y) % q for x, y in zip(d, s)]
+    if op == "scalar_mul":
+        return [(x * c) % q for x in d]
+    return [(x + y * c) % q for x, y in zip(d, s)]
+
+
+@pytest.mark.parametrize("q,lhs,rhs,scalar", SLICE_KATS)
+def test_slice_hooks_reference_inputs(oracle, q, lhs, rhs, scalar):
+    for op in ("add", "sub", "scalar_mul", "add_scaled"):
+        got = oracle.slice_op(q, op, lhs, rhs, scalar)
+        assert [int(v) for v in got] == _closed(q, op, lhs, rhs, scalar), op
+    assert oracle.dot_product(q, lhs, rhs) == sum(x * y for x, y in zip(lhs, rhs)) % q
+
+
+@pytest.mark.parametrize("q", [F17, F12289, Q32, GOLDILOCKS])
+@pytest.mark.parametrize("n", [0, 1, 63, 64, 65, 130])
+def test_slice_hooks_random_against_integers(oracle, q, n):
+    # lengths around the 64-word chunk of add_assign_scaled_slice / dot_product
+    rng = np.random.default_rng(n + q % 1000)
+    d = rand_residues(rng, q, (n,))
+    s = rand_residues(rng, q, (n,))
+    dl, sl = [int(x) for x in d], [int(x) for x in s]
+    for c in (0, 1, q - 1, int(rand_residues(rng, q, (1,))[0])):
+        for op in ("add", "sub", "scalar_mul", "add_scaled"):
+            got = oracle.slice_op(q, op, d, s, c)
+            assert [int(v) for v in got] == _closed(q, op, dl, sl, c), (op, c)
+    assert oracle.dot_product(q, d, s) == sum(x * y for x, y in zip(dl, sl)) % q
