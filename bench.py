@@ -956,8 +956,10 @@ def main():
     log(f"plain: {value:,.0f} commits/s ({1e6 / value:.1f} us/commit)")
 
     tuning = ctx.get_tuning()
-    if tuning["fused"]:
-        mac_kernel = "k_commit_fused_q32"  # witness transform + MAC in one kernel
+    if tuning["fused"] == 2 and KAPPA % 16 == 0:
+        mac_kernel = "k_commit_rows_q32"  # witness transform + MAC, waves split the rows
+    elif tuning["fused"]:
+        mac_kernel = "k_commit_fused_q32"  # witness transform + MAC, waves split the columns
     else:
         mac_kernel = "k_mac4_q32" if tuning["mac_variant"] == 1 else "k_mac_q32"
     # HBM bytes per dispatch of that kernel from the committed rocprofv3 --pmc summary

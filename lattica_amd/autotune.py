@@ -32,7 +32,7 @@ def _witness(rng, q, n, d=64):
     return np.where(x < 0, q + x, x).astype(np.uint32)
 
 
-def tune_commit(ctx, kappa=16, n=1 << 15, reps=60, quick=False, log=print):
+def tune_commit(ctx, kappa=16, n=1 << 15, reps=200, quick=False, log=print):
     import lattica_amd as la
     q = la.Q32
     ring = la.Ring(ctx, q, 64)
@@ -44,7 +44,12 @@ def tune_commit(ctx, kappa=16, n=1 << 15, reps=60, quick=False, log=print):
 
     bpcs = [1, 2] if quick else [1, 2, 3, 4]
     configs = []
-    # fused transform+MAC (16-byte loads): block shape and key prefetch depth
+    # fused transform+MAC, row split (16-row blocks of four waves): key
+    # prefetch depth and blocks per CU
+    for bpc, depth in itertools.product(bpcs, [1, 2, 3, 4]):
+        configs.append({"fused": 2, "fused_depth": depth, "mac_variant": 1, "mac_waves": 4,
+                        "mac_blocks_per_cu": bpc, "quad_max": 1 << 17})
+    # fused transform+MAC, column split (16-byte loads): block shape and depth
     for waves, bpc, depth in itertools.product([16, 8, 4], bpcs, [1, 2, 3, 4]):
         configs.append({"fused": 1, "fused_depth": depth, "mac_variant": 1, "mac_waves": waves,
                         "mac_blocks_per_cu": bpc, "quad_max": 1 << 17})

@@ -296,7 +296,7 @@ GH_API int gh_ctx_set_tuning(gh_ctx* ctx, const char* key, int64_t value) {
     if (value < 0) return GH_E_INVALID_ARGUMENT;
     t.quad_max = value;
   } else if (!std::strcmp(key, "fused")) {
-    if (value != 0 && value != 1) return GH_E_INVALID_ARGUMENT;
+    if (value < 0 || value > 2) return GH_E_INVALID_ARGUMENT;
     t.fused = (int)value;
   } else if (!std::strcmp(key, "fused_depth")) {
     if (value < 1 || value > 4) return GH_E_INVALID_ARGUMENT;

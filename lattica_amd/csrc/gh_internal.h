@@ -69,10 +69,13 @@ struct BatchLayout {
 
 struct gh_tuning {
   int mac_variant = 1;        // q32 MAC: 0 = dword loads, 1 = 16-byte loads
-  int mac_waves = 16;         // waves per MAC block (4, 8, 16)
-  int mac_blocks_per_cu = 1;  // MAC blocks (= partial sums) per compute unit
+  int mac_waves = 4;          // waves per MAC block (4, 8, 16)
+  int mac_blocks_per_cu = 2;  // MAC blocks (= partial sums) per compute unit
   int64_t quad_max = 1 << 17; // largest batch transformed four lanes per polynomial
-  int fused = 1;              // q32 commit: witness transform fused into the MAC kernel
+  // q32 commit: 0 = transform and MAC kernels, 1 = one fused kernel whose
+  // waves split a block's columns, 2 = one fused kernel whose waves split a
+  // 16-row block's rows (rows % 16 == 0; four waves per block; else as 1)
+  int fused = 2;
   int fused_depth = 2;        // fused kernel: key columns in flight per wave, x 8 / mac_waves
   int batch_chunk_cols = 256; // batched MAC: key columns per block (q32 keys)
   int batch_chunk_cols_gold = 256; // batched MAC: key columns per block (Goldilocks keys)

@@ -553,6 +553,134 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_q32(
   }
 }
 
+// Fused witness transform + MAC (q32), row-split form for 16-row blocks of
+// four waves: wave w owns rows 4w..4w+3 of the block for every column the
+// block owns (lane l accumulates row 4w + l / 16, slots 4 (l % 16) .. +3). A
+// wave's sums are complete when its sweep ends, so there is no cross-wave
+// reduction through LDS and no barrier after the sweep: each lane reduces its
+// four 96-bit sums and stores them as one 16-byte word. Per column a wave
+// loads its four rows (1 KB, contiguous in the [col][row][slot] key) and
+// reads the column's transformed witness from LDS; PF = 8 * DEPTH columns are
+// kept in flight per wave. The transform, the held-back stages and the
+// clamped prefetch are those of k_commit_fused_q32, as are the results.
+// NC > 0: every block owns exactly NC columns (fully unrolled sweep).
+template <int DEPTH, int NC>
+__global__ __launch_bounds__(256) void k_commit_rows_q32(
+    FQ32 f, const uint32_t* __restrict__ tab, const uint32_t* __restrict__ A,
+    const uint32_t* __restrict__ s_coeff, uint32_t* __restrict__ partial, uint32_t rows,
+    uint32_t cols, uint32_t cols_per_block, FinStage1 s1, FinStage2 s2) {
+  constexpr int WAVES = 4;
+  constexpr int PF = 8 * DEPTH;
+  // cols_per_block * kFuseStride witness words, then WAVES * 64 stage-1 words
+  extern __shared__ uint32_t lds[];
+  const uint32_t lead = s2.nblk + s1.extra;
+  const uint32_t nparts = gridDim.x - lead;
+  if (blockIdx.x < lead) {
+    if (blockIdx.x < s2.nblk) {
+      const uint32_t row = blockIdx.x * WAVES + (threadIdx.x >> 6);
+      if (blockIdx.y == 0 && row < s2.rows)
+        sum_rows_inv<FQ32, 1>(f, tab, (int)kTabUntwistMac, (const uint32_t*)s2.src, s2.n, row,
+                              0, (uint32_t*)s2.out, nullptr);
+    } else {
+      stage1_only(f, s1, s1.mac_tasks + (blockIdx.y * s1.extra + (blockIdx.x - s2.nblk)) * s1.tpb,
+                  lds, WAVES);
+    }
+    return;
+  }
+  const uint32_t bx = blockIdx.x - lead;
+  const uint32_t bid = blockIdx.y * nparts + bx;  // stage-1 task
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int k0 = 4 * (lane & 15);
+  const int t = threadIdx.x & 3;
+  const uint32_t row = blockIdx.y * 16 + 4 * wave + (lane >> 4);
+  const uint32_t ch = bx * cols_per_block;  // < cols: every block owns a column
+  const uint32_t ncol =
+      NC > 0 ? (uint32_t)NC : (ch + cols_per_block < cols ? cols_per_block : cols - ch);
+  // u32x4 of (column c, this lane): c * rows * 16 + (16 * blockIdx.y + 4 * wave) * 16 + lane
+  const u32x4* akey = reinterpret_cast<const u32x4*>(A + (size_t)blockIdx.y * 16 * 64) + 64 * wave + lane;
+  const size_t col4 = (size_t)rows * 16;  // one key column, in u32x4
+  auto col_of = [&](uint32_t c) -> uint32_t { return c < ncol ? c : ncol - 1; };
+
+  uint64_t lo[4] = {0, 0, 0, 0};
+  uint32_t hi[4] = {0, 0, 0, 0};
+  // (a) lane twiddles and this thread's first witness polynomial
+  uint32_t ltw[16][2];
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    ltw[k][0] = tab[(kTabQuad + 16 * t + k) * 2];
+    ltw[k][1] = tab[(kTabQuad + 16 * t + k) * 2 + 1];
+  }
+  const uint32_t p0 = threadIdx.x >> 2;
+  uint32_t x[16];
+  {
+    const uint32_t* sp = s_coeff + (size_t)(ch + col_of(p0)) * 64 + t;
+#pragma unroll
+    for (int i = 0; i < 16; i++) x[i] = sp[4 * i];
+  }
+  uint32_t l1v[4];
+  stage1_load(s1, bid, s1.mac_tasks, l1v);
+  // (b) key prefetch: the first PF columns
+  u32x4 pa[PF];
+#pragma unroll
+  for (int q = 0; q < PF; q++) pa[q] = __builtin_nontemporal_load(akey + (size_t)(ch + col_of(q)) * col4);
+  // (c) witness transform into LDS while (b) is in flight
+  quad_fwd64(f, x, tab, ltw, t);
+  if (p0 < ncol) {
+    u32x4* dp = reinterpret_cast<u32x4*>(lds + p0 * kFuseStride + 16 * bitrev2(t));
+#pragma unroll
+    for (int v = 0; v < 4; v++) dp[v] = u32x4{x[4 * v], x[4 * v + 1], x[4 * v + 2], x[4 * v + 3]};
+  }
+  for (uint32_t p = p0 + 16 * WAVES; p < ncol; p += 16 * WAVES) {
+    const uint32_t* sp = s_coeff + (size_t)(ch + p) * 64 + t;
+#pragma unroll
+    for (int i = 0; i < 16; i++) x[i] = sp[4 * i];
+    quad_fwd64(f, x, tab, ltw, t);
+    u32x4* dp = reinterpret_cast<u32x4*>(lds + p * kFuseStride + 16 * bitrev2(t));
+#pragma unroll
+    for (int v = 0; v < 4; v++) dp[v] = u32x4{x[4 * v], x[4 * v + 1], x[4 * v + 2], x[4 * v + 3]};
+  }
+  uint32_t* l1lds = lds + cols_per_block * kFuseStride;
+  stage1_wave_sum(f, l1v, l1lds);
+  __syncthreads();
+  stage1_store(f, s1, bid, s1.mac_tasks, l1lds, WAVES);
+  // (d) sweep: every column of the block, PF in flight
+  auto mac = [&](const u32x4& a, uint32_t c) {
+    const u32x4 s0 = *reinterpret_cast<const u32x4*>(lds + c * kFuseStride + k0);
+    mac96(lo[0], hi[0], a.x, s0.x);
+    mac96(lo[1], hi[1], a.y, s0.y);
+    mac96(lo[2], hi[2], a.z, s0.z);
+    mac96(lo[3], hi[3], a.w, s0.w);
+  };
+  if constexpr (NC > 0) {
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      constexpr int kPF = PF;
+      const int q = c % kPF;
+      mac(pa[q], (uint32_t)c);
+      if (c + kPF < NC) pa[q] = __builtin_nontemporal_load(akey + (size_t)(ch + c + kPF) * col4);
+    }
+  } else {
+    for (uint32_t c0 = 0; c0 < ncol; c0 += PF) {
+#pragma unroll
+      for (int q = 0; q < PF; q++) {
+        const uint32_t c = c0 + q;
+        if (c < ncol) {
+          mac(pa[q], c);
+          if (c + PF < ncol) pa[q] = __builtin_nontemporal_load(akey + (size_t)(ch + c + PF) * col4);
+        }
+      }
+    }
+  }
+  // (e) this lane's four sums, reduced and stored: [row][part][slot]
+  u32x4 out;
+  out.x = f.reduce96_r2((uint32_t)lo[0], (uint32_t)(lo[0] >> 32), hi[0]);
+  out.y = f.reduce96_r2((uint32_t)lo[1], (uint32_t)(lo[1] >> 32), hi[1]);
+  out.z = f.reduce96_r2((uint32_t)lo[2], (uint32_t)(lo[2] >> 32), hi[2]);
+  out.w = f.reduce96_r2((uint32_t)lo[3], (uint32_t)(lo[3] >> 32), hi[3]);
+  *reinterpret_cast<u32x4*>(partial + ((size_t)row * nparts + bx) * 64 + k0) = out;
+}
+
 // Fused witness transform + MAC (Goldilocks), the per-call form: a block
 // transforms its contiguous range of at most kFuseChunkGold witness
 // polynomials into LDS (four lanes per polynomial), then each wave sums its
@@ -1013,6 +1141,32 @@ static void fused_launch(gh_ctx* ctx, dim3 grid, size_t lds_bytes, hipEvent_t e0
   }
 }
 
+template <int DEPTH>
+static void rows_launch_nc(gh_ctx* ctx, dim3 grid, size_t lds_bytes, hipEvent_t e0, hipEvent_t e1,
+                           FQ32 f, const uint32_t* tab, const uint32_t* A, const uint32_t* S,
+                           uint32_t* P, uint32_t rows, uint32_t cols, uint32_t cpb,
+                           const FinStage1& s1, const FinStage2& s2) {
+  // the unrolled sweep when every block owns 64 columns
+  if (cpb == 64 && cols % 64 == 0)
+    hipExtLaunchKernelGGL((k_commit_rows_q32<DEPTH, 64>), grid, dim3(256), lds_bytes, ctx->stream,
+                          e0, e1, 0, f, tab, A, S, P, rows, cols, cpb, s1, s2);
+  else
+    hipExtLaunchKernelGGL((k_commit_rows_q32<DEPTH, 0>), grid, dim3(256), lds_bytes, ctx->stream,
+                          e0, e1, 0, f, tab, A, S, P, rows, cols, cpb, s1, s2);
+}
+
+static void rows_launch(gh_ctx* ctx, dim3 grid, size_t lds_bytes, hipEvent_t e0, hipEvent_t e1,
+                        FQ32 f, const uint32_t* tab, const uint32_t* A, const uint32_t* S,
+                        uint32_t* P, uint32_t rows, uint32_t cols, uint32_t cpb,
+                        const FinStage1& s1, const FinStage2& s2) {
+  switch (ctx->tune.fused_depth) {
+    case 1: rows_launch_nc<1>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, s1, s2); break;
+    case 3: rows_launch_nc<3>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, s1, s2); break;
+    case 4: rows_launch_nc<4>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, s1, s2); break;
+    default: rows_launch_nc<2>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, s1, s2); break;
+  }
+}
+
 // Standalone finalize of up to two commitments of one ring whose sums are
 // laid out [row][n][slot]: 16 waves per row for up to one row per CU, 4
 // waves per row beyond (all rows in one round).
@@ -1187,7 +1341,9 @@ static int launch_commit_fused(gh_key* key, const void* s_dev, uint32_t cols, vo
   gh_ring* ring = key->ring;
   gh_ctx* ctx = ring->ctx;
   const uint32_t rows = key->rows;
-  const int waves = ctx->tune.mac_waves;
+  // fused 2: the row-split kernel (16-row blocks of four waves)
+  const bool rowsplit = ctx->tune.fused == 2 && rows % 16 == 0;
+  const int waves = rowsplit ? 4 : ctx->tune.mac_waves;
   const int rb = rows % 16 == 0 ? 16 : rows % 8 == 0 ? 8 : 4;
   // blocks: the tuned count per CU, at least one per kFuseChunk columns (a
   // block's witness fits its LDS), at most one per `waves` columns; every block
@@ -1215,13 +1371,14 @@ static int launch_commit_fused(gh_key* key, const void* s_dev, uint32_t cols, vo
   uint32_t* P = scratch ? (uint32_t*)scratch : (uint32_t*)((char*)ctx->work + plan.off);
   dim3 grid(plan.s2.nblk + plan.s1.extra + nparts, rows / rb);
   size_t lds_words = (size_t)cpb * kFuseStride + (size_t)waves * 64;
-  lds_words = std::max(lds_words, (size_t)waves * rb * 64);
+  if (!rowsplit) lds_words = std::max(lds_words, (size_t)waves * rb * 64);
   lds_words = std::max(lds_words, (size_t)waves * 64 * plan.s1.tpb);
   const size_t lds_bytes = lds_words * 4;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (!scratch) prof_pair(ctx, 1, &e0, &e1);
 #define GH_FUSED(R, W) fused_launch<R, W>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, plan.s1, plan.s2)
-  if (rb == 16) { if (waves == 4) GH_FUSED(16, 4); else if (waves == 8) GH_FUSED(16, 8); else GH_FUSED(16, 16); }
+  if (rowsplit) rows_launch(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, plan.s1, plan.s2);
+  else if (rb == 16) { if (waves == 4) GH_FUSED(16, 4); else if (waves == 8) GH_FUSED(16, 8); else GH_FUSED(16, 16); }
   else if (rb == 8) { if (waves == 4) GH_FUSED(8, 4); else if (waves == 8) GH_FUSED(8, 8); else GH_FUSED(8, 16); }
   else { if (waves == 4) GH_FUSED(4, 4); else if (waves == 8) GH_FUSED(4, 8); else GH_FUSED(4, 16); }
 #undef GH_FUSED

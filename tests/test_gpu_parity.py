@@ -272,15 +272,17 @@ def full_key(oracle, ring_q32):
     key.close()
 
 
-@pytest.mark.parametrize("waves,bpc", [(4, 1), (4, 2), (4, 4), (8, 1), (8, 2)])
-def test_unrolled_sweeps_at_the_headline_shape(ctx, full_key, waves, bpc):
-    """The fused kernel's unrolled sweeps (8, 16 and 32 columns per wave) at every
+@pytest.mark.parametrize("fused,waves,bpc", [(1, 4, 1), (1, 4, 2), (1, 4, 4), (1, 8, 1),
+                                             (1, 8, 2), (2, 4, 1), (2, 4, 2), (2, 4, 3)])
+def test_unrolled_sweeps_at_the_headline_shape(ctx, full_key, fused, waves, bpc):
+    """The fused kernels' unrolled sweeps (column split: 8, 16 and 32 columns per
+    wave; row split: 64 or 128 columns per block, unrolled for 64) at every
     depth, on the full 2^15 columns and on 32763 (a last block 5 columns short)."""
     key, s, exp, exp_tail = full_key
     saved = ctx.get_tuning()
     try:
         for depth in (1, 2, 3, 4):
-            ctx.set_tuning(fused=1, mac_waves=waves, mac_blocks_per_cu=bpc, fused_depth=depth)
+            ctx.set_tuning(fused=fused, mac_waves=waves, mac_blocks_per_cu=bpc, fused_depth=depth)
             assert np.array_equal(key.commit(s).astype(np.uint64), exp), depth
             # a one-item batch takes the plain path over the key's first 32763 columns
             got = key.commit_batch([s[:32763]])[0]
@@ -307,7 +309,8 @@ def test_every_tuning_setting_gives_the_same_commitment(oracle, ctx, ring_q32):
                 (1, 1, 8, 2, 0, 1), (1, 1, 16, 1, 0, 1), (1, 1, 16, 8, 0, 1), (1, 1, 4, 3, 0, 1),
                 (1, 1, 8, 3, 0, 1), (1, 1, 4, 1, 0, 2), (1, 1, 4, 2, 0, 2), (1, 1, 4, 3, 0, 2),
                 (1, 1, 8, 1, 0, 2), (1, 1, 16, 2, 0, 2), (1, 1, 4, 2, 0, 3), (1, 1, 4, 2, 0, 4),
-                (1, 1, 16, 1, 0, 4)]:
+                (1, 1, 16, 1, 0, 4), (2, 1, 4, 1, 0, 1), (2, 1, 4, 2, 0, 2), (2, 1, 4, 3, 0, 3),
+                (2, 1, 4, 2, 0, 4), (2, 1, 16, 8, 0, 2), (2, 1, 8, 4, 0, 1)]:
             ctx.set_tuning(fused=fused, mac_variant=variant, mac_waves=waves,
                            mac_blocks_per_cu=bpc, quad_max=quad, fused_depth=depth)
             got = key.commit(s).astype(np.uint64)
