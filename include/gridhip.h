@@ -112,10 +112,13 @@ GH_API int gh_ctx_profile_read(gh_ctx *ctx, int phase, double *total_ms, uint64_
 /* Launch-shape tuning of a context (written by the autotuner, read back for
  * reports). Keys: "mac_variant" (0 dword / 1 16-byte loads, q32 MAC),
  * "mac_waves" (4, 8, 16), "mac_blocks_per_cu" (1..8), "quad_max" (largest
- * witness batch transformed four lanes per polynomial), "fused" (1: q32
- * commitments transform the witness inside the MAC kernel; 0: separate
- * transform and MAC kernels), "fused_depth" (1..4: the fused
- * kernel keeps max(1, depth * 8 / mac_waves) key columns in flight per wave),
+ * witness batch transformed four lanes per polynomial), "fused" (2, the
+ * default: q32 commitments transform the witness inside the MAC kernel, whose
+ * waves each own four rows of a 16-row block — row counts not divisible by 16
+ * take 1; 1: the same kernel with the waves splitting the block's columns; 0:
+ * separate transform and MAC kernels), "fused_depth" (1..4: key columns in
+ * flight per wave, 8 * depth for fused 2 and max(1, depth * 8 / mac_waves)
+ * for fused 1),
  * "batch_chunk_cols" and "batch_chunk_cols_gold" (16..4096, key columns per
  * block of gh_commit_batch and of the server's waves, for q32 and Goldilocks
  * keys; halved for batches too small to fill the device), "batch_group" (4, 8,
