@@ -872,7 +872,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--only", choices=["small", "negacyclic", "labrador", "goldilocks", "serving",
-                                       "wire", "hbm", "issue"],
+                                       "wire", "hbm", "issue", "valu"],
                     help="run one side workload alone (for profiling) and print its JSON")
     args = ap.parse_args()
     global CPU_ON
@@ -903,7 +903,10 @@ def main():
                                         ctx.measure_bandwidth(1 << 30, 3))),
                 # int8 MFMA and VALU issue rates, alone and interleaved: what
                 # bounds the JL projection
-                "issue": lambda: {str(w): ctx.measure_issue(w) for w in (1, 2, 3, 4)}}
+                "issue": lambda: {str(w): ctx.measure_issue(w) for w in (1, 2, 3, 4)},
+                # single VALU instruction kinds: what the multiply-heavy
+                # transforms and multiply-accumulates are bounded by
+                "valu": lambda: {str(w): ctx.measure_valu(w) for w in (1, 2, 4)}}
         print(json.dumps({"workload": args.only, "result": legs[args.only]()}), flush=True)
         ring.close()
         ctx.close()

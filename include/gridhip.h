@@ -164,6 +164,16 @@ GH_API int gh_measure_bandwidth(gh_ctx *ctx, size_t bytes, int reps, double *rea
  * no reference counterpart. */
 GH_API int gh_measure_issue(gh_ctx *ctx, int waves_per_simd, double out_us[4]);
 
+/* Issue rate of single VALU instruction kinds on the context's device, in G
+ * wave instructions per second (best of three timed launches on num_cus *
+ * waves_per_simd blocks of four waves, each lane running eight independent
+ * chains of one instruction): out_gips[0] v_add_u32, [1] v_mul_hi_u32,
+ * [2] v_mul_lo_u32, [3] v_mad_u64_u32, [4] v_lshl_add_u64 (64-bit add),
+ * [5] v_fma_f64. waves_per_simd 1..8. Measurement support for the VALU
+ * rooflines of the transforms and multiply-accumulates; no reference
+ * counterpart. */
+GH_API int gh_measure_valu(gh_ctx *ctx, int waves_per_simd, double out_gips[6]);
+
 /* device memory, so callers without a HIP runtime of their own can keep
  * inputs resident in HBM */
 GH_API int gh_alloc(gh_ctx *ctx, size_t bytes, void **dev_ptr);

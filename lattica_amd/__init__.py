@@ -59,6 +59,7 @@ EXPORTS = (
     "gh_key_save", "gh_key_load", "gh_jl_extract_flat", "gh_jl_conjugate", "gh_jl_evals",
     "gh_jl_aggregate", "gh_jl_launch_shape", "gh_commit_noalloc", "gh_measure_issue",
     "gh_add_assign", "gh_sub_assign", "gh_scalar_mul", "gh_add_assign_scaled", "gh_dot_product",
+    "gh_measure_valu",
 )
 
 
@@ -154,6 +155,7 @@ def lib():
         "gh_ctx_profile_enable": (i32, [vp, i32]),
         "gh_ctx_set_tuning": (i32, [vp, ctypes.c_char_p, ctypes.c_int64]),
         "gh_measure_issue": (i32, [vp, i32, ctypes.POINTER(ctypes.c_double)]),
+        "gh_measure_valu": (i32, [vp, i32, ctypes.POINTER(ctypes.c_double)]),
         "gh_measure_bandwidth": (i32, [vp, sz, i32, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_double)]),
         "gh_ctx_get_tuning": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]),
@@ -292,6 +294,19 @@ class Context:
         _check(lib().gh_measure_bandwidth(self.h, nbytes, reps, ctypes.byref(r), ctypes.byref(c)),
                "gh_measure_bandwidth")
         return r.value, c.value
+
+    VALU_KINDS = ("v_add_u32", "v_mul_hi_u32", "v_mul_lo_u32", "v_mad_u64_u32",
+                  "v_lshl_add_u64", "v_fma_f64")
+
+    def measure_valu(self, waves_per_simd: int = 4) -> dict:
+        """gh_measure_valu: G wave instructions per second of single VALU
+        instruction kinds, and each against v_add_u32."""
+        g = (ctypes.c_double * len(self.VALU_KINDS))()
+        _check(lib().gh_measure_valu(self.h, waves_per_simd, g), "gh_measure_valu")
+        rates = {k: round(g[i], 1) for i, k in enumerate(self.VALU_KINDS)}
+        return {"waves_per_simd": waves_per_simd, "G_wave_inst_per_s": rates,
+                "slots_vs_add": {k: round(g[0] / g[i], 2) for i, k in enumerate(self.VALU_KINDS)
+                                 if g[i] > 0}}
 
     def measure_issue(self, waves_per_simd: int = 2) -> dict:
         """gh_measure_issue: microseconds per probe launch, the int8 MFMA

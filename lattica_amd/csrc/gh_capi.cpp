@@ -426,6 +426,39 @@ GH_API int gh_measure_issue(gh_ctx* ctx, int waves_per_simd, double out_us[4]) {
   return st;
 }
 
+GH_API int gh_measure_valu(gh_ctx* ctx, int waves_per_simd, double out_gips[6]) {
+  if (!ctx || !out_gips || waves_per_simd < 1 || waves_per_simd > 8) return GH_E_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  TRY(set_device(ctx));
+  const uint32_t blocks = (uint32_t)ctx->num_cus * (uint32_t)waves_per_simd;
+  uint32_t* out = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int st = GH_OK;
+  if (hipMalloc(&out, (size_t)blocks * 256 * sizeof(uint32_t)) != hipSuccess) st = GH_E_OOM;
+  if (st == GH_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    st = GH_E_DEVICE;
+  // wave instructions of the measured kind per launch: 4 waves per block,
+  // 2048 iterations of eight
+  const double winst = (double)blocks * 4 * 2048 * 8;
+  for (int kind = 0; kind < kValuKinds && st == GH_OK; kind++) {
+    double best = 0;
+    for (int r = 0; r < 4 && st == GH_OK; r++) {  // the first run warms up
+      st = launch_valu_probe(ctx, kind, blocks, out, e0, e1);
+      float ms = 0;
+      if (st == GH_OK && (hipEventSynchronize(e1) != hipSuccess ||
+                          hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        st = GH_E_DEVICE;
+      if (st == GH_OK && r > 0 && ms > 0 && (best == 0 || ms < best)) best = ms;
+    }
+    out_gips[kind] = best > 0 ? winst / (best * 1e-3) / 1e9 : 0;
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  hipStreamSynchronize(ctx->stream);
+  if (out) hipFree(out);
+  return st;
+}
+
 GH_API int gh_ctx_num_cus(gh_ctx* ctx, int* out) {
   if (!ctx || !out) return GH_E_INVALID_ARGUMENT;
   *out = ctx->num_cus;

@@ -916,6 +916,94 @@ int launch_issue_probe(gh_ctx* ctx, int kind, uint32_t blocks, int* out, hipEven
   return GH_OK;
 }
 
+// Issue rate of single VALU instruction kinds (gh_measure_valu): eight
+// independent chains per lane, one instruction per chain and iteration,
+// written as asm so that the compiler neither fuses nor removes them; no
+// memory operation inside the loop. Each thread writes one word to
+// out[blockIdx.x * 256 + threadIdx.x].
+//   0 v_add_u32, 1 v_mul_hi_u32, 2 v_mul_lo_u32, 3 v_mad_u64_u32 (64-bit
+//   accumulate, carry to an SGPR pair), 4 v_lshl_add_u64 (64-bit add),
+//   5 v_fma_f64.
+constexpr int kValuIters = 2048;
+#define GH_CHAINS8(ST) ST(0) ST(1) ST(2) ST(3) ST(4) ST(5) ST(6) ST(7)
+template <int KIND>
+__global__ __launch_bounds__(256) void k_valu_probe(uint32_t* __restrict__ out, uint32_t s) {
+  uint32_t res = 0;
+  if constexpr (KIND <= 2) {
+    uint32_t a[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) a[j] = threadIdx.x + j;
+    for (int i = 0; i < kValuIters; i++) {
+      if constexpr (KIND == 0) {
+#define ST(j) asm volatile("v_add_u32 %0, %0, %1" : "+v"(a[j]) : "v"(s));
+        GH_CHAINS8(ST)
+#undef ST
+      } else if constexpr (KIND == 1) {
+#define ST(j) asm volatile("v_mul_hi_u32 %0, %0, %1" : "+v"(a[j]) : "v"(s));
+        GH_CHAINS8(ST)
+#undef ST
+      } else {
+#define ST(j) asm volatile("v_mul_lo_u32 %0, %0, %1" : "+v"(a[j]) : "v"(s));
+        GH_CHAINS8(ST)
+#undef ST
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) res ^= a[j];
+  } else if constexpr (KIND <= 4) {
+    uint64_t a[8], cy[8];
+    const uint64_t b = s;
+#pragma unroll
+    for (int j = 0; j < 8; j++) a[j] = threadIdx.x + j;
+    for (int i = 0; i < kValuIters; i++) {
+      if constexpr (KIND == 3) {
+#define ST(j) asm volatile("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(a[j]), "=s"(cy[j]) : "v"(s), "v"(s));
+        GH_CHAINS8(ST)
+#undef ST
+      } else {
+#define ST(j) asm volatile("v_lshl_add_u64 %0, %0, 0, %1" : "+v"(a[j]) : "v"(b));
+        GH_CHAINS8(ST)
+#undef ST
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) res ^= (uint32_t)a[j] ^ (uint32_t)(a[j] >> 32);
+  } else {
+    double a[8];
+    const double b = 1.0000001, c = 0.5;
+#pragma unroll
+    for (int j = 0; j < 8; j++) a[j] = threadIdx.x + j;
+    for (int i = 0; i < kValuIters; i++) {
+#define ST(j) asm volatile("v_fma_f64 %0, %0, %1, %2" : "+v"(a[j]) : "v"(b), "v"(c));
+      GH_CHAINS8(ST)
+#undef ST
+    }
+    double r = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) r += a[j];
+    res = (uint32_t)r;
+  }
+  out[blockIdx.x * 256 + threadIdx.x] = res;
+}
+#undef GH_CHAINS8
+
+// kind 0..5 on `blocks` blocks of 256 threads; out holds blocks * 256 words
+int launch_valu_probe(gh_ctx* ctx, int kind, uint32_t blocks, uint32_t* out, hipEvent_t e0,
+                      hipEvent_t e1) {
+  const dim3 grid(blocks), block(256);
+  const uint32_t s = 12345u;
+  switch (kind) {
+    case 0: hipExtLaunchKernelGGL((k_valu_probe<0>), grid, block, 0, ctx->stream, e0, e1, 0, out, s); break;
+    case 1: hipExtLaunchKernelGGL((k_valu_probe<1>), grid, block, 0, ctx->stream, e0, e1, 0, out, s); break;
+    case 2: hipExtLaunchKernelGGL((k_valu_probe<2>), grid, block, 0, ctx->stream, e0, e1, 0, out, s); break;
+    case 3: hipExtLaunchKernelGGL((k_valu_probe<3>), grid, block, 0, ctx->stream, e0, e1, 0, out, s); break;
+    case 4: hipExtLaunchKernelGGL((k_valu_probe<4>), grid, block, 0, ctx->stream, e0, e1, 0, out, s); break;
+    default: hipExtLaunchKernelGGL((k_valu_probe<5>), grid, block, 0, ctx->stream, e0, e1, 0, out, s); break;
+  }
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
 int launch_stream_probe(gh_ctx* ctx, const void* src, void* dst, size_t bytes, int copy,
                         hipEvent_t e0, hipEvent_t e1) {
   const size_t n = bytes / 16;

@@ -1,6 +1,6 @@
 """The measurement entry points behind the roofline figures: gh_measure_bandwidth
-(streaming read and copy) and gh_measure_issue (int8 MFMA and VALU issue
-rates). Their numbers vary with the device, so the checks are sanity bounds:
+(streaming read and copy), gh_measure_issue (int8 MFMA and VALU issue
+rates) and gh_measure_valu (single VALU instruction kinds). Their numbers vary with the device, so the checks are sanity bounds:
 positive, below the published peaks, and the argument checks."""
 import pytest
 
@@ -27,3 +27,17 @@ def test_issue_probe_rates(ctx):
         ctx.measure_issue(0)
     with pytest.raises(la.GridHipError):
         ctx.measure_issue(9)
+
+
+def test_valu_probe_rates(ctx):
+    import lattica_amd as la
+    r = ctx.measure_valu(2)
+    rates = r["G_wave_inst_per_s"]
+    assert set(rates) == set(ctx.VALU_KINDS)
+    # at most one wave instruction per cycle per SIMD (clock below 3 GHz)
+    for k, v in rates.items():
+        assert 0 < v < ctx.num_cus * 4 * 3.0, k
+    with pytest.raises(la.GridHipError):
+        ctx.measure_valu(0)
+    with pytest.raises(la.GridHipError):
+        ctx.measure_valu(9)
