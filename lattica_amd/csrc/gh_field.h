@@ -34,39 +34,34 @@ struct FQ32 {
   uint32_t q;
   uint32_t qinv;  // q^-1 mod 2^32
 
-  __device__ __forceinline__ W add(W a, W b) const {
-    W nb = q - b;
-    W r = a - nb;
-    return a < nb ? r + q : r;
+  // r - s mod q for r, s < 2^32 with r - s in (-q, q): the borrow of the
+  // subtraction (v_sub_co_u32) selects the +q. Written with the overflow
+  // builtin so the compiler keeps the compare in 32 bits (an ordinary
+  // `r < s` after __umulhi operands becomes a 64-bit compare plus moves).
+  __device__ __forceinline__ W sub_fix(W r, W s) const {
+    W d;
+    const bool borrow = __builtin_sub_overflow(r, s, &d);
+    return d + (borrow ? q : 0u);
   }
-  __device__ __forceinline__ W sub(W a, W b) const {
-    W r = a - b;
-    return a < b ? r + q : r;
-  }
+  __device__ __forceinline__ W add(W a, W b) const { return sub_fix(a, q - b); }
+  __device__ __forceinline__ W sub(W a, W b) const { return sub_fix(a, b); }
   // a * b * 2^-32 mod q, for a < 2^32 and b < q
   __device__ __forceinline__ W mont(W a, W b) const {
     W lo = a * b;
     W hi = __umulhi(a, b);
     W m = lo * qinv;
-    W mh = __umulhi(m, q);
-    W r = hi - mh;
-    return hi < mh ? r + q : r;
+    return sub_fix(hi, __umulhi(m, q));
   }
   // a * v mod q where tw = (v*2^32 mod q, (v*2^32 mod q) * q^-1 mod 2^32)
   __device__ __forceinline__ W mul_tw(W a, const W* tw) const {
     W m = a * tw[1];
-    W hi = __umulhi(a, tw[0]);
-    W mh = __umulhi(m, q);
-    W r = hi - mh;
-    return hi < mh ? r + q : r;
+    return sub_fix(__umulhi(a, tw[0]), __umulhi(m, q));
   }
   // REDC of a 64-bit value t < q * 2^32: t * 2^-32 mod q
   __device__ __forceinline__ W redc64(uint64_t t) const {
     W lo = (W)t, hi = (W)(t >> 32);
     W m = lo * qinv;
-    W mh = __umulhi(m, q);
-    W r = hi - mh;
-    return hi < mh ? r + q : r;
+    return sub_fix(hi, __umulhi(m, q));
   }
   // (h2*2^64 + h1*2^32 + h0) * 2^-64 mod q, for h2 < q
   __device__ __forceinline__ W reduce96_r2(W h0, W h1, W h2) const {
@@ -133,7 +128,8 @@ enum TabOffset : int {
   kTabUntwistMac = 384,   // psi^-i * d^-1 * R^2          (after a reduce96_r2 accumulation)
   kTabQuad = 448,         // psi^(t*(2k+1)), t = 0..3, k = 0..15 at kTabQuad + 16t + k (quad NTT)
   kTabUntwistMacHalf = 512,  // psi^-i * d^-1 * R^2 * 2^-1  (accumulated sums halved: garbage h)
-  kTabEntries = 576,
+  kTabQuadInv = 576,      // psi^-(t*(2k+1)), t = 0..3, k = 0..15 at kTabQuadInv + 16t + k (quad inverse)
+  kTabEntries = 640,
 };
 
 }  // namespace gh

@@ -144,7 +144,10 @@ int plan_ring(uint64_t q, uint32_t d, gh_ring* ring) {
     v[kTabUntwistMacHalf + i] = mulmod(mulmod(u, R2, q), inv2, q);
   }
   for (uint32_t t = 0; t < 4; t++)
-    for (uint32_t k = 0; k < 16; k++) v[kTabQuad + 16 * t + k] = powmod(psi, t * (2 * k + 1), q);
+    for (uint32_t k = 0; k < 16; k++) {
+      v[kTabQuad + 16 * t + k] = powmod(psi, t * (2 * k + 1), q);
+      v[kTabQuadInv + 16 * t + k] = powmod(ipsi, t * (2 * k + 1), q);
+    }
   if (kind == GH_RING_Q32) {
     uint32_t q32 = (uint32_t)q;
     uint32_t inv = 1;

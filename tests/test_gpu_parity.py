@@ -133,13 +133,16 @@ def test_negacyclic_matches_oracle(oracle, ring_q32, ring_gold, name, count):
     r = ring_for(name, ring_q32, ring_gold)
     rng = np.random.default_rng(1000 + count)
     a, b = rand_residues(rng, r.q, (count, 64)), rand_residues(rng, r.q, (count, 64))
+    # the largest residues in the first pair (every modular correction taken)
+    a[0], b[0] = r.q - 1, r.q - 1
     assert np.array_equal(r.negacyclic_mul(a, b).astype(np.uint64), oracle.negacyclic_mul(r.q, 64, a, b))
 
 
-def test_negacyclic_wide_tile_c2_shape_sample(oracle, ctx, ring_q32):
-    # the batched product at the C2 scale (2^18 pairs); exact on a sample,
-    # and commutativity a*b == b*a on all pairs
-    r = ring_q32
+@pytest.mark.parametrize("name", ["q32", "goldilocks"])
+def test_negacyclic_c2_shape_sample(oracle, ctx, ring_q32, ring_gold, name):
+    # the batched product at the C2 scale (2^18 pairs: several tiles per
+    # block); exact on a sample, and commutativity a*b == b*a on all pairs
+    r = ring_for(name, ring_q32, ring_gold)
     count = 1 << 18
     rng = np.random.default_rng(7)
     a = r.words(rand_residues(rng, r.q, (count, 64)))
