@@ -1049,7 +1049,12 @@ def main():
                 valu = json.load(fh)
         extra["negacyclic_q32_2^18_pairs"] = negacyclic_batch(ctx, la, ring, log)
         if "negacyclic_q32_2^18_pairs" in valu:
-            extra["negacyclic_q32_2^18_pairs"]["valu_profile"] = valu["negacyclic_q32_2^18_pairs"]
+            vp = valu["negacyclic_q32_2^18_pairs"]
+            leg = extra["negacyclic_q32_2^18_pairs"]
+            leg["valu_profile"] = vp
+            # the profiled VALU floor against this run's time per call
+            if vp.get("valu_roofline_us"):
+                leg["valu_roofline_frac_this_run"] = round(vp["valu_roofline_us"] / leg["us_per_batch"], 3)
         extra["batched_same_shape"] = batched_same_shape(ctx, keys, s_dev, log)
         extra["commit_q32_k8_n256"] = commit_small(ctx, la, ring, log)
         extra["labrador_q32_r32_n1024"] = labrador_calls(ctx, la, ring, log)

@@ -169,10 +169,12 @@ GH_API int gh_measure_issue(gh_ctx *ctx, int waves_per_simd, double out_us[4]);
  * waves_per_simd blocks of four waves, each lane running eight independent
  * chains of one instruction): out_gips[0] v_add_u32, [1] v_mul_hi_u32,
  * [2] v_mul_lo_u32, [3] v_mad_u64_u32, [4] v_lshl_add_u64 (64-bit add),
- * [5] v_fma_f64. waves_per_simd 1..8. Measurement support for the VALU
+ * [5] v_fma_f64, [6] the modular correction v_sub_co_u32 + v_cndmask_b32 +
+ * v_add_u32 (its three instructions counted), [7] v_cndmask_b32, [8] v_xor_b32,
+ * [9] v_alignbit_b32. waves_per_simd 1..8. Measurement support for the VALU
  * rooflines of the transforms and multiply-accumulates; no reference
  * counterpart. */
-GH_API int gh_measure_valu(gh_ctx *ctx, int waves_per_simd, double out_gips[6]);
+GH_API int gh_measure_valu(gh_ctx *ctx, int waves_per_simd, double out_gips[10]);
 
 /* device memory, so callers without a HIP runtime of their own can keep
  * inputs resident in HBM */

@@ -296,7 +296,8 @@ class Context:
         return r.value, c.value
 
     VALU_KINDS = ("v_add_u32", "v_mul_hi_u32", "v_mul_lo_u32", "v_mad_u64_u32",
-                  "v_lshl_add_u64", "v_fma_f64")
+                  "v_lshl_add_u64", "v_fma_f64", "sub_fix_3op", "v_cndmask_b32",
+                  "v_xor_b32", "v_alignbit_b32")
 
     def measure_valu(self, waves_per_simd: int = 4) -> dict:
         """gh_measure_valu: G wave instructions per second of single VALU

@@ -426,7 +426,7 @@ GH_API int gh_measure_issue(gh_ctx* ctx, int waves_per_simd, double out_us[4]) {
   return st;
 }
 
-GH_API int gh_measure_valu(gh_ctx* ctx, int waves_per_simd, double out_gips[6]) {
+GH_API int gh_measure_valu(gh_ctx* ctx, int waves_per_simd, double out_gips[10]) {
   if (!ctx || !out_gips || waves_per_simd < 1 || waves_per_simd > 8) return GH_E_INVALID_ARGUMENT;
   std::lock_guard<std::mutex> lk(ctx->mu);
   TRY(set_device(ctx));
@@ -450,7 +450,9 @@ GH_API int gh_measure_valu(gh_ctx* ctx, int waves_per_simd, double out_gips[6]) 
         st = GH_E_DEVICE;
       if (st == GH_OK && r > 0 && ms > 0 && (best == 0 || ms < best)) best = ms;
     }
-    out_gips[kind] = best > 0 ? winst / (best * 1e-3) / 1e9 : 0;
+    // kind 6 is a three-instruction step
+    const double per_step = kind == 6 ? 3.0 : 1.0;
+    out_gips[kind] = best > 0 ? per_step * winst / (best * 1e-3) / 1e9 : 0;
   }
   if (e0) hipEventDestroy(e0);
   if (e1) hipEventDestroy(e1);

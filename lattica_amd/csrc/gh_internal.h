@@ -253,8 +253,8 @@ int launch_amortize(gh_ring* ring, const void* c, const void* s, uint32_t r, uin
 // gh_measure_issue's kernels: kind 0..3, kIssueIters iterations per wave
 int launch_issue_probe(gh_ctx* ctx, int kind, uint32_t blocks, int* out, hipEvent_t e0,
                        hipEvent_t e1);
-// gh_measure_valu's kernels: kind 0..5, kValuIters iterations of eight chains
-constexpr int kValuKinds = 6;
+// gh_measure_valu's kernels: kind 0..9, kValuIters iterations of eight chains
+constexpr int kValuKinds = 10;
 int launch_valu_probe(gh_ctx* ctx, int kind, uint32_t blocks, uint32_t* out, hipEvent_t e0,
                       hipEvent_t e1);
 int launch_stream_probe(gh_ctx* ctx, const void* src, void* dst, size_t bytes, int copy,

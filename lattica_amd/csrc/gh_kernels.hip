@@ -937,14 +937,21 @@ int launch_issue_probe(gh_ctx* ctx, int kind, uint32_t blocks, int* out, hipEven
 // out[blockIdx.x * 256 + threadIdx.x].
 //   0 v_add_u32, 1 v_mul_hi_u32, 2 v_mul_lo_u32, 3 v_mad_u64_u32 (64-bit
 //   accumulate, carry to an SGPR pair), 4 v_lshl_add_u64 (64-bit add),
-//   5 v_fma_f64.
+//   5 v_fma_f64, 6 FQ32::sub_fix (v_sub_co_u32 with the borrow in an SGPR
+//   pair, v_cndmask_b32 on it, v_add_u32: three instructions per step, with
+//   the hazard waits the compiler puts between them), 7 v_cndmask_b32 (lane
+//   mask in an SGPR pair written before the loop), 8 v_xor_b32,
+//   9 v_alignbit_b32 (32-bit rotate).
 constexpr int kValuIters = 2048;
 #define GH_CHAINS8(ST) ST(0) ST(1) ST(2) ST(3) ST(4) ST(5) ST(6) ST(7)
 template <int KIND>
 __global__ __launch_bounds__(256) void k_valu_probe(uint32_t* __restrict__ out, uint32_t s) {
   uint32_t res = 0;
-  if constexpr (KIND <= 2) {
+  if constexpr (KIND <= 2 || KIND >= 6) {
     uint32_t a[8];
+    const FQ32 fq{s | 0x80000000u, 1};
+    // lane mask for kind 7: every other lane
+    const uint64_t mask = 0x5555555555555555ull ^ (uint64_t)s;
 #pragma unroll
     for (int j = 0; j < 8; j++) a[j] = threadIdx.x + j;
     for (int i = 0; i < kValuIters; i++) {
@@ -956,12 +963,30 @@ __global__ __launch_bounds__(256) void k_valu_probe(uint32_t* __restrict__ out, 
 #define ST(j) asm volatile("v_mul_hi_u32 %0, %0, %1" : "+v"(a[j]) : "v"(s));
         GH_CHAINS8(ST)
 #undef ST
-      } else {
+      } else if constexpr (KIND == 2) {
 #define ST(j) asm volatile("v_mul_lo_u32 %0, %0, %1" : "+v"(a[j]) : "v"(s));
+        GH_CHAINS8(ST)
+#undef ST
+      } else if constexpr (KIND == 6) {
+        // compiler-generated, so that its hazard waits are the ones the
+        // transforms get; q = s | 2^31 keeps every step's result below 2^32
+#pragma unroll
+        for (int j = 0; j < 8; j++) a[j] = fq.sub_fix(a[j], s);
+      } else if constexpr (KIND == 7) {
+#define ST(j) asm volatile("v_cndmask_b32 %0, %0, %1, %2" : "+v"(a[j]) : "v"(s), "s"(mask));
+        GH_CHAINS8(ST)
+#undef ST
+      } else if constexpr (KIND == 8) {
+#define ST(j) asm volatile("v_xor_b32 %0, %0, %1" : "+v"(a[j]) : "v"(s));
+        GH_CHAINS8(ST)
+#undef ST
+      } else {
+#define ST(j) asm volatile("v_alignbit_b32 %0, %0, %0, 25" : "+v"(a[j]));
         GH_CHAINS8(ST)
 #undef ST
       }
     }
+
 #pragma unroll
     for (int j = 0; j < 8; j++) res ^= a[j];
   } else if constexpr (KIND <= 4) {
@@ -1001,7 +1026,7 @@ __global__ __launch_bounds__(256) void k_valu_probe(uint32_t* __restrict__ out, 
 }
 #undef GH_CHAINS8
 
-// kind 0..5 on `blocks` blocks of 256 threads; out holds blocks * 256 words
+// kind 0..kValuKinds-1 on `blocks` blocks of 256 threads; out holds blocks * 256 words
 int launch_valu_probe(gh_ctx* ctx, int kind, uint32_t blocks, uint32_t* out, hipEvent_t e0,
                       hipEvent_t e1) {
   const dim3 grid(blocks), block(256);
@@ -1012,7 +1037,12 @@ int launch_valu_probe(gh_ctx* ctx, int kind, uint32_t blocks, uint32_t* out, hip
     case 2: hipExtLaunchKernelGGL((k_valu_probe<2>), grid, block, 0, ctx->stream, e0, e1, 0, out, s); break;
     case 3: hipExtLaunchKernelGGL((k_valu_probe<3>), grid, block, 0, ctx->stream, e0, e1, 0, out, s); break;
     case 4: hipExtLaunchKernelGGL((k_valu_probe<4>), grid, block, 0, ctx->stream, e0, e1, 0, out, s); break;
-    default: hipExtLaunchKernelGGL((k_valu_probe<5>), grid, block, 0, ctx->stream, e0, e1, 0, out, s); break;
+    case 5: hipExtLaunchKernelGGL((k_valu_probe<5>), grid, block, 0, ctx->stream, e0, e1, 0, out, s); break;
+    case 6: hipExtLaunchKernelGGL((k_valu_probe<6>), grid, block, 0, ctx->stream, e0, e1, 0, out, s); break;
+    case 7: hipExtLaunchKernelGGL((k_valu_probe<7>), grid, block, 0, ctx->stream, e0, e1, 0, out, s); break;
+    case 8: hipExtLaunchKernelGGL((k_valu_probe<8>), grid, block, 0, ctx->stream, e0, e1, 0, out, s); break;
+    case 9: hipExtLaunchKernelGGL((k_valu_probe<9>), grid, block, 0, ctx->stream, e0, e1, 0, out, s); break;
+    default: return GH_E_INVALID_ARGUMENT;
   }
   GH_HIP(hipGetLastError());
   return GH_OK;
