@@ -134,8 +134,13 @@ GH_API int gh_ctx_profile_read(gh_ctx *ctx, int phase, double *total_ms, uint64_
  * blocks of that many tasks), "batch_fused" (0, the default: a Goldilocks
  * gh_commit_batch or batch plan runs a transform, a grouped MAC and a
  * finalize kernel; 1: one launch that transforms, multiplies and finalizes,
- * slower on the batches measured) and "batch_fused_cols" (32..4096, a
- * multiple of 32: key columns per block of that launch). Values are checked;
+ * slower on the batches measured), "batch_fused_cols" (32..4096, a
+ * multiple of 32: key columns per block of that launch) and "gold_mac_ring"
+ * (4, the default, down to 2: the Goldilocks grouped MAC stages that many key
+ * columns per wave in an LDS ring filled by global-to-LDS loads, for groups of
+ * 4 and row counts divisible by 16; 0: one column of loads in flight in
+ * registers).
+ * Values are checked;
  * an unknown key or value gives GH_E_INVALID_ARGUMENT. Every setting yields
  * identical results; only the speed changes. */
 GH_API int gh_ctx_set_tuning(gh_ctx *ctx, const char *key, int64_t value);

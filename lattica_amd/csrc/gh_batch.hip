@@ -111,8 +111,7 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch_q32(
     for (int i = 0; i < G; i++) {
       if (c < len[i]) {  // uniform: lengths are block-wide values
         const uint32_t s = sp[i][(size_t)c * 64];
-#pragma unroll
-        for (int r = 0; r < 4; r++) mac96(lo[i][r], hi[i][r], a[r], s);
+        mac96_rows(lo[i], hi[i], a, s);
       }
     }
   }
@@ -186,10 +185,7 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch4_q32(
 #pragma unroll
     for (int i = 0; i < G; i++) {
       if (c < len[i]) {
-        mac96(lo[i][0], hi[i][0], a.x, s[i].x);
-        mac96(lo[i][1], hi[i][1], a.y, s[i].y);
-        mac96(lo[i][2], hi[i][2], a.z, s[i].z);
-        mac96(lo[i][3], hi[i][3], a.w, s[i].w);
+        mac96x4(lo[i], hi[i], a.x, a.y, a.z, a.w, s[i].x, s[i].y, s[i].z, s[i].w);
       }
     }
   }
@@ -286,6 +282,106 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch_gold(
         else v = acc[i][r];
         pp[((size_t)ch * rows + row0 + r) * 64 + lane] = v;
       }
+    }
+  }
+}
+
+// Grouped Goldilocks MAC with the key and witness columns staged through a
+// per-wave LDS ring: each wave owns D stages of 4 KB (its 4 key rows and the
+// group's 4 witness columns of one key column) and refills a stage with
+// global-to-LDS loads (global_load_lds_dwordx4) as soon as it has read it, so
+// D columns are in flight per wave without holding them in registers (the
+// lazy sums take 144 of them). Same partial sums as k_mac_batch_gold<4, 16>.
+// Wave-private stages need no barrier: a wave waits for its own loads
+// (vmcnt) before reading a stage and for its reads (lgkmcnt) before refilling it.
+// 16 bytes per lane from global memory straight into LDS at lds + 16 * lane
+// (global_load_lds_dwordx4; lds must be the same for every lane of the wave)
+__device__ __forceinline__ void lds_dma16(const uint64_t* g, uint64_t* lds) {
+  __builtin_amdgcn_global_load_lds(g, lds, 16, 0, 0);
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_mac_batch_gold_ring(
+    FGold f, const uint64_t* __restrict__ A, const BatchDesc* __restrict__ desc, uint32_t count,
+    uint32_t rows, uint32_t chunk_cols) {
+  constexpr int G = 4;
+  __shared__ uint64_t ring[4][D][8][64];  // [wave][stage][key row 0..3, item 0..3][slot]
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const uint32_t ch = blockIdx.x, g0 = blockIdx.y * G;
+  const uint32_t row0 = blockIdx.z * 16 + 4 * wave;  // rows % 16 == 0 (launcher)
+  const uint32_t c0 = ch * chunk_cols;
+  const uint32_t maxlen = desc[g0].len;  // sorted: the group's first item is its longest
+  if (c0 >= maxlen) return;              // block-uniform
+  const uint32_t c1 = c0 + chunk_cols < maxlen ? c0 + chunk_cols : maxlen;
+  uint32_t len[G];
+  const uint64_t* sbase[G];
+#pragma unroll
+  for (int i = 0; i < G; i++) {
+    const uint32_t j = g0 + i;
+    len[i] = j < count ? desc[j].len : 0;
+    sbase[i] = reinterpret_cast<const uint64_t*>(desc[j < count ? j : g0].s_ntt);
+  }
+  const uint64_t* s_g0 = reinterpret_cast<const uint64_t*>(desc[g0].s_ntt);
+  // per-lane sources of the four loads of a stage: lane l moves 16 bytes
+  // (words 2l, 2l + 1 of a 128-word span) into the stage at the same offset
+  const size_t col_stride = (size_t)rows * 64;
+  const uint64_t* key_src = A + (size_t)row0 * 64 + 2 * lane;  // + c * col_stride, + 128
+  // witness loads: lanes 0..31 move item 0 (2), lanes 32..63 item 1 (3)
+  const bool upper = lane >= 32;
+  const int wl = 2 * (lane & 31);
+  const uint64_t* wb0 = (upper ? sbase[1] : sbase[0]) + wl;
+  const uint64_t* wb1 = (upper ? sbase[3] : sbase[2]) + wl;
+  const uint32_t wn0 = upper ? len[1] : len[0], wn1 = upper ? len[3] : len[2];
+  const uint64_t* wg = s_g0 + wl;
+  auto issue = [&](int d, uint32_t c) {
+    const uint64_t* k0 = key_src + (size_t)c * col_stride;
+    lds_dma16(k0, &ring[wave][d][0][0]);
+    lds_dma16(k0 + 128, &ring[wave][d][2][0]);
+    // items past their length read the group's first (longest) item instead
+    const uint64_t* w0 = (c < wn0 ? wb0 : wg) + (size_t)c * 64;
+    const uint64_t* w1 = (c < wn1 ? wb1 : wg) + (size_t)c * 64;
+    lds_dma16(w0, &ring[wave][d][4][0]);
+    lds_dma16(w1, &ring[wave][d][6][0]);
+  };
+  GoldAcc lz[G][4];
+  // fill the ring; columns past the range reload the last one, so that every
+  // step has exactly four loads behind the stage it reads
+#pragma unroll
+  for (int d = 0; d < D; d++) issue(d, c0 + d < c1 ? c0 + d : c1 - 1);
+  for (uint32_t cb = c0; cb < c1; cb += D) {
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      const uint32_t c = cb + d;
+      if (c >= c1) break;  // uniform
+      asm volatile("s_waitcnt vmcnt(%0)" ::"n"(4 * (D - 1)) : "memory");
+      uint64_t a[4], sv[G];
+#pragma unroll
+      for (int r = 0; r < 4; r++) a[r] = ring[wave][d][r][lane];
+#pragma unroll
+      for (int i = 0; i < G; i++) sv[i] = ring[wave][d][4 + i][lane];
+      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+      const uint32_t cn = c + D < c1 ? c + D : c1 - 1;
+      issue(d, cn);
+#pragma unroll
+      for (int i = 0; i < G; i++) {
+        if (c < len[i]) {
+#pragma unroll
+          for (int r = 0; r < 4; r++) gold_mac(lz[i][r], a[r], sv[i]);
+        }
+      }
+    }
+  }
+  // the ring's last loads land before the wave ends
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+#pragma unroll
+  for (int i = 0; i < G; i++) {
+    const uint32_t j = g0 + i;
+    if (j < count && c0 < len[i]) {
+      uint64_t* pp = reinterpret_cast<uint64_t*>(desc[j].partial);
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        pp[((size_t)ch * rows + row0 + r) * 64 + lane] = gold_reduce(f, lz[i][r]);
     }
   }
 }
@@ -518,6 +614,21 @@ int enqueue_batch_xform(gh_ring* ring, hipStream_t stream, const BatchDesc* desc
   return GH_OK;
 }
 
+// the LDS-ring Goldilocks MAC (groups of 4, 16-row blocks), `depth` columns per wave
+static void mac_gold_ring_launch(gh_key* key, hipStream_t stream, const BatchDesc* desc,
+                                 uint32_t count, uint32_t chunk_cols, dim3 grid, int depth) {
+  const uint64_t* a = (const uint64_t*)key->dev_a;
+  if (depth == 2)
+    hipLaunchKernelGGL((k_mac_batch_gold_ring<2>), grid, dim3(256), 0, stream, FGold{}, a, desc,
+                       count, key->rows, chunk_cols);
+  else if (depth == 3)
+    hipLaunchKernelGGL((k_mac_batch_gold_ring<3>), grid, dim3(256), 0, stream, FGold{}, a, desc,
+                       count, key->rows, chunk_cols);
+  else
+    hipLaunchKernelGGL((k_mac_batch_gold_ring<4>), grid, dim3(256), 0, stream, FGold{}, a, desc,
+                       count, key->rows, chunk_cols);
+}
+
 template <int G, int RB>
 static void mac_batch_launch(gh_key* key, hipStream_t stream, const BatchDesc* desc,
                              uint32_t count, uint32_t chunk_cols, uint32_t nchunks, int variant) {
@@ -535,6 +646,8 @@ static void mac_batch_launch(gh_key* key, hipStream_t stream, const BatchDesc* d
     hipLaunchKernelGGL((k_mac_batch_q32<G, RB>), grid, dim3(16 * RB), 0, stream,
                        FQ32{(uint32_t)ring->q, ring->qinv32}, (const uint32_t*)key->dev_a, desc,
                        count, key->rows, chunk_cols);
+  else if (G == 4 && RB == 16 && ring->ctx->tune.gold_mac_ring >= 2)
+    mac_gold_ring_launch(key, stream, desc, count, chunk_cols, grid, ring->ctx->tune.gold_mac_ring);
   else
     hipLaunchKernelGGL((k_mac_batch_gold<G, RB>), grid, dim3(16 * RB), 0, stream, FGold{},
                        (const uint64_t*)key->dev_a, desc, count, key->rows, chunk_cols);

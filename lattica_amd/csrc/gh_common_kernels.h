@@ -6,13 +6,55 @@
 namespace gh {
 
 // Exact accumulation of a*b into a 96-bit (lo64, hi32) pair: one
-// v_mad_u64_u32 with carry-out plus one add-with-carry.
+// v_mad_u64_u32 with carry-out plus one add-with-carry. The add reads the
+// carry that the multiply wrote to VCC; the two wait states between them are
+// the ones the compiler itself puts between a VALU write of VCC and a VALU
+// read of it as a carry on gfx950.
 __device__ __forceinline__ void mac96(uint64_t& lo, uint32_t& hi, uint32_t a, uint32_t b) {
   asm("v_mad_u64_u32 %0, vcc, %2, %3, %0\n\t"
+      "s_nop 1\n\t"
       "v_addc_co_u32 %1, vcc, 0, %1, vcc"
       : "+v"(lo), "+v"(hi)
       : "v"(a), "v"(b)
       : "vcc");
+}
+
+// Four independent mac96: the four multiplies first, each carry in its own
+// SGPR pair, then the four carry additions, each three instructions after its
+// carry is written (no wait state needed).
+__device__ __forceinline__ void mac96x4(uint64_t& l0, uint64_t& l1, uint64_t& l2, uint64_t& l3,
+                                        uint32_t& h0, uint32_t& h1, uint32_t& h2, uint32_t& h3,
+                                        uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3,
+                                        uint32_t b0, uint32_t b1, uint32_t b2, uint32_t b3) {
+  uint64_t c0, c1, c2, c3;  // carries (SGPR pairs)
+  asm("v_mad_u64_u32 %0, %8, %12, %16, %0\n\t"
+      "v_mad_u64_u32 %1, %9, %13, %17, %1\n\t"
+      "v_mad_u64_u32 %2, %10, %14, %18, %2\n\t"
+      "v_mad_u64_u32 %3, %11, %15, %19, %3\n\t"
+      "v_addc_co_u32 %4, %8, 0, %4, %8\n\t"
+      "v_addc_co_u32 %5, %9, 0, %5, %9\n\t"
+      "v_addc_co_u32 %6, %10, 0, %6, %10\n\t"
+      "v_addc_co_u32 %7, %11, 0, %7, %11"
+      : "+v"(l0), "+v"(l1), "+v"(l2), "+v"(l3), "+v"(h0), "+v"(h1), "+v"(h2), "+v"(h3),
+        "=&s"(c0), "=&s"(c1), "=&s"(c2), "=&s"(c3)
+      : "v"(a0), "v"(a1), "v"(a2), "v"(a3), "v"(b0), "v"(b1), "v"(b2), "v"(b3));
+}
+// the same on four-element accumulator arrays
+__device__ __forceinline__ void mac96x4(uint64_t (&lo)[4], uint32_t (&hi)[4], uint32_t a0,
+                                        uint32_t a1, uint32_t a2, uint32_t a3, uint32_t b0,
+                                        uint32_t b1, uint32_t b2, uint32_t b3) {
+  mac96x4(lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3], a0, a1, a2, a3, b0, b1, b2, b3);
+}
+// lo[r] += a[r] * b for r < N, four at a time
+template <int N>
+__device__ __forceinline__ void mac96_rows(uint64_t (&lo)[N], uint32_t (&hi)[N],
+                                           const uint32_t (&a)[N], uint32_t b) {
+#pragma unroll
+  for (int r = 0; r + 3 < N; r += 4)
+    mac96x4(lo[r], lo[r + 1], lo[r + 2], lo[r + 3], hi[r], hi[r + 1], hi[r + 2], hi[r + 3], a[r],
+            a[r + 1], a[r + 2], a[r + 3], b, b, b, b);
+#pragma unroll
+  for (int r = N & ~3; r < N; r++) mac96(lo[r], hi[r], a[r], b);
 }
 
 // Exact lazy accumulation of Goldilocks products: x*y split into 32-bit halves,
@@ -24,13 +66,24 @@ struct GoldAcc {
   uint64_t a = 0, b = 0, d = 0;
   uint32_t ah = 0, bh = 0, dh = 0;
 };
+// The four multiply-accumulates are interleaved in one block, each carry in
+// its own SGPR pair and read at least two instructions after it is written,
+// so no wait state is needed between them.
 __device__ __forceinline__ void gold_mac(GoldAcc& s, uint64_t x, uint64_t y) {
   const uint32_t x0 = (uint32_t)x, x1 = (uint32_t)(x >> 32);
   const uint32_t y0 = (uint32_t)y, y1 = (uint32_t)(y >> 32);
-  mac96(s.a, s.ah, x0, y0);
-  mac96(s.b, s.bh, x0, y1);
-  mac96(s.b, s.bh, x1, y0);
-  mac96(s.d, s.dh, x1, y1);
+  uint64_t c0, c1, c2, c3;  // carries (SGPR pairs)
+  asm("v_mad_u64_u32 %0, %6, %10, %12, %0\n\t"   // a += x0 y0
+      "v_mad_u64_u32 %1, %7, %10, %13, %1\n\t"   // b += x0 y1
+      "v_mad_u64_u32 %2, %8, %11, %13, %2\n\t"   // d += x1 y1
+      "v_addc_co_u32 %3, %6, 0, %3, %6\n\t"
+      "v_mad_u64_u32 %1, %9, %11, %12, %1\n\t"   // b += x1 y0
+      "v_addc_co_u32 %5, %8, 0, %5, %8\n\t"
+      "v_addc_co_u32 %4, %7, 0, %4, %7\n\t"
+      "v_addc_co_u32 %4, %9, 0, %4, %9"
+      : "+v"(s.a), "+v"(s.b), "+v"(s.d), "+v"(s.ah), "+v"(s.bh), "+v"(s.dh), "=&s"(c0),
+        "=&s"(c1), "=&s"(c2), "=&s"(c3)
+      : "v"(x0), "v"(x1), "v"(y0), "v"(y1));
 }
 // canonical a + b*2^32 + d*2^64 mod p, with 2^64 = 2^32 - 1 mod p
 __device__ __forceinline__ uint64_t gold_reduce(const FGold& f, const GoldAcc& s) {

@@ -267,16 +267,16 @@ __global__ __launch_bounds__(64 * kMacWaves) void k_mac_q32(
     for (int r = 0; r < RB; r++) va[r] = __builtin_nontemporal_load(a0 + r * 64);
 #pragma unroll
     for (int r = 0; r < RB; r++) vb[r] = __builtin_nontemporal_load(a1 + r * 64);
-#pragma unroll
-    for (int r = 0; r < RB; r++) mac96(lo[r], hi[r], va[r], s0);
-#pragma unroll
-    for (int r = 0; r < RB; r++) mac96(lo[r], hi[r], vb[r], s1);
+    mac96_rows(lo, hi, va, s0);
+    mac96_rows(lo, hi, vb, s1);
   }
   if (c < c_end) {
     const uint32_t* a0 = A + (size_t)c * col_stride + (size_t)row0 * 64 + lane;
     uint32_t s0 = s_ntt[(size_t)c * 64 + lane];
+    uint32_t va[RB];
 #pragma unroll
-    for (int r = 0; r < RB; r++) mac96(lo[r], hi[r], __builtin_nontemporal_load(a0 + r * 64), s0);
+    for (int r = 0; r < RB; r++) va[r] = __builtin_nontemporal_load(a0 + r * 64);
+    mac96_rows(lo, hi, va, s0);
   }
 
   __shared__ uint32_t red[kMacWaves][RB][64];
@@ -328,17 +328,13 @@ __global__ __launch_bounds__(64 * WAVES) void k_mac4_q32(
     for (int i = 0; i < NI; i++) vb[i] = __builtin_nontemporal_load(a1 + i * 64);
 #pragma unroll
     for (int i = 0; i < NI; i++) {
-      mac96(lo[i][0], hi[i][0], va[i].x, s0.x);
-      mac96(lo[i][1], hi[i][1], va[i].y, s0.y);
-      mac96(lo[i][2], hi[i][2], va[i].z, s0.z);
-      mac96(lo[i][3], hi[i][3], va[i].w, s0.w);
+      mac96x4(lo[i], hi[i], va[i].x, va[i].y, va[i].z, va[i].w, s0.x,
+              s0.y, s0.z, s0.w);
     }
 #pragma unroll
     for (int i = 0; i < NI; i++) {
-      mac96(lo[i][0], hi[i][0], vb[i].x, s1.x);
-      mac96(lo[i][1], hi[i][1], vb[i].y, s1.y);
-      mac96(lo[i][2], hi[i][2], vb[i].z, s1.z);
-      mac96(lo[i][3], hi[i][3], vb[i].w, s1.w);
+      mac96x4(lo[i], hi[i], vb[i].x, vb[i].y, vb[i].z, vb[i].w, s1.x,
+              s1.y, s1.z, s1.w);
     }
   }
   if (c < c_end) {
@@ -347,10 +343,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_mac4_q32(
 #pragma unroll
     for (int i = 0; i < NI; i++) {
       const u32x4 v = __builtin_nontemporal_load(a0 + i * 64);
-      mac96(lo[i][0], hi[i][0], v.x, s0.x);
-      mac96(lo[i][1], hi[i][1], v.y, s0.y);
-      mac96(lo[i][2], hi[i][2], v.z, s0.z);
-      mac96(lo[i][3], hi[i][3], v.w, s0.w);
+      mac96x4(lo[i], hi[i], v.x, v.y, v.z, v.w, s0.x,
+              s0.y, s0.z, s0.w);
     }
   }
   __shared__ uint32_t red[WAVES][RB][64];
@@ -398,10 +392,8 @@ __device__ __forceinline__ void mac_column(uint64_t (&lo)[NI][4], uint32_t (&hi)
                                            const u32x4 (&a)[NI], const u32x4 s0) {
 #pragma unroll
   for (int i = 0; i < NI; i++) {
-    mac96(lo[i][0], hi[i][0], a[i].x, s0.x);
-    mac96(lo[i][1], hi[i][1], a[i].y, s0.y);
-    mac96(lo[i][2], hi[i][2], a[i].z, s0.z);
-    mac96(lo[i][3], hi[i][3], a[i].w, s0.w);
+    mac96x4(lo[i], hi[i], a[i].x, a[i].y, a[i].z, a[i].w, s0.x,
+            s0.y, s0.z, s0.w);
   }
 }
 
@@ -661,10 +653,8 @@ __global__ __launch_bounds__(256) void k_commit_rows_q32(
   // (d) sweep: every column of the block, PF in flight
   auto mac = [&](const u32x4& a, uint32_t c) {
     const u32x4 s0 = *reinterpret_cast<const u32x4*>(lds + c * kFuseStride + k0);
-    mac96(lo[0], hi[0], a.x, s0.x);
-    mac96(lo[1], hi[1], a.y, s0.y);
-    mac96(lo[2], hi[2], a.z, s0.z);
-    mac96(lo[3], hi[3], a.w, s0.w);
+    mac96x4(lo, hi, a.x, a.y, a.z, a.w, s0.x, s0.y,
+            s0.z, s0.w);
   };
   if constexpr (NC > 0) {
 #pragma unroll

@@ -72,10 +72,10 @@ __device__ __forceinline__ void gram_tile_q32(FQ32 f, const uint32_t* __restrict
     for (int x = 0; x < kGramTile; x++) si[x] = s_ntt[((size_t)ii[x] * n + c) * 64 + lane];
 #pragma unroll
     for (int x = 0; x < kGramTile; x++) sj[x] = s_ntt[((size_t)jj[x] * n + c) * 64 + lane];
+    static_assert(kGramTile == 4, "four accumulators per mac96x4");
 #pragma unroll
     for (int x = 0; x < kGramTile; x++)
-#pragma unroll
-      for (int y = 0; y < kGramTile; y++) mac96(lo[x][y], hi[x][y], si[x], sj[y]);
+      mac96x4(lo[x], hi[x], si[x], si[x], si[x], si[x], sj[0], sj[1], sj[2], sj[3]);
   }
   const uint32_t npairs = r * (r + 1) / 2;
 #pragma unroll
@@ -316,12 +316,10 @@ __device__ __forceinline__ void garbage_tile_q32(FQ32 f, const uint32_t* __restr
       sj[x] = s[((size_t)jj[x] * n + c) * 64 + lane];
     }
 #pragma unroll
-    for (int x = 0; x < kGramTile; x++)
-#pragma unroll
-      for (int y = 0; y < kGramTile; y++) {
-        mac96(lo[x][y], hi[x][y], pi[x], sj[y]);
-        mac96(lo[x][y], hi[x][y], pj[y], si[x]);
-      }
+    for (int x = 0; x < kGramTile; x++) {
+      mac96x4(lo[x], hi[x], pi[x], pi[x], pi[x], pi[x], sj[0], sj[1], sj[2], sj[3]);
+      mac96x4(lo[x], hi[x], pj[0], pj[1], pj[2], pj[3], si[x], si[x], si[x], si[x]);
+    }
   }
   const uint32_t npairs = r * (r + 1) / 2;
 #pragma unroll

@@ -1,0 +1,67 @@
+"""The Goldilocks grouped MAC with its LDS ring (k_mac_batch_gold_ring, tuning
+gold_mac_ring = 2..4) against the register-pipelined kernel (gold_mac_ring = 0)
+and the oracle.
+
+Full-range residues in key and witness (the lazy sums' largest terms), ragged
+lengths (1, 2, 3, 5 and lengths around the chunk width), a batch whose last
+group is short, chunk widths below, at and above the ring depth, and row
+counts that take the ring (multiples of 16) or fall back (20).
+"""
+import numpy as np
+import pytest
+
+from conftest import GOLDILOCKS, rand_residues
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(ctx, key, ws, lens, ring, cols):
+    ring_field = key.ring
+    saved = ctx.get_tuning()
+    ctx.set_tuning(gold_mac_ring=ring, batch_chunk_cols_gold=cols)
+    bufs = [ctx.to_device(w) for w in ws]
+    outs = [ctx.alloc(key.rows * 64 * 8) for _ in ws]
+    try:
+        key.commit_batch_device(bufs, lens, outs)
+        ctx.synchronize()
+        return [o.download(ring_field.dtype, (key.rows, 64)) for o in outs]
+    finally:
+        ctx.set_tuning(gold_mac_ring=saved["gold_mac_ring"],
+                       batch_chunk_cols_gold=saved["batch_chunk_cols_gold"])
+        for b in bufs + outs:
+            b.free()
+
+
+@pytest.mark.parametrize("rows", [16, 32, 20])
+def test_ring_mac_equals_register_mac(oracle, ctx, ring_gold, rows):
+    import lattica_amd as la
+    p = GOLDILOCKS
+    rng = np.random.default_rng(1300 + rows)
+    cols = 700
+    a = rand_residues(rng, p, (rows, cols, 64))
+    key = la.Key(ring_gold, a, domain=la.DOMAIN_NTT)
+    lens = [700, 1, 2, 3, 5, 63, 64, 65, 129, 300, 17]
+    ws = [rand_residues(rng, p, (n, 64)) for n in lens]
+    try:
+        for cc in (16, 64, 256):
+            base = _batch(ctx, key, ws, lens, 0, cc)
+            for depth in (2, 3, 4):
+                got = _batch(ctx, key, ws, lens, depth, cc)
+                for i in range(len(lens)):
+                    assert np.array_equal(got[i], base[i]), (rows, cc, depth, i, lens[i])
+        z = np.zeros((rows, 1, 64), dtype=np.uint64)
+        zr = np.zeros((1, 64), dtype=np.uint64)
+        got = _batch(ctx, key, ws, lens, 4, 64)
+        for i in (0, 1, 3, 6, 10):
+            exp = oracle.commit(p, 64, a[:, :lens[i]], z, ws[i], zr, key_domain=1)
+            assert np.array_equal(got[i], exp), (rows, i)
+    finally:
+        key.close()
+
+
+def test_ring_tuning_values_checked(ctx):
+    import lattica_amd as la
+    for bad in (1, 5, -1):
+        with pytest.raises(la.GridHipError):
+            ctx.set_tuning(gold_mac_ring=bad)
+    assert ctx.get_tuning()["gold_mac_ring"] in (0, 2, 3, 4)
