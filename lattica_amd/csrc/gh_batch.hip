@@ -291,7 +291,11 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch_gold(
 // group's 4 witness columns of one key column) and refills a stage with
 // global-to-LDS loads (global_load_lds_dwordx4) as soon as it has read it, so
 // D columns are in flight per wave without holding them in registers (the
-// lazy sums take 144 of them). Same partial sums as k_mac_batch_gold<4, 16>.
+// lazy sums take 144 of them). A block runs through a run of consecutive
+// chunks of one group (the launcher picks `split` runs per group, so every
+// block has work and the ring flows across chunk boundaries), storing each
+// chunk's partial sums as it completes: the same partial sums as
+// k_mac_batch_gold<4, 16>.
 // Wave-private stages need no barrier: a wave waits for its own loads
 // (vmcnt) before reading a stage and for its reads (lgkmcnt) before refilling it.
 // 16 bytes per lane from global memory straight into LDS at lds + 16 * lane
@@ -303,24 +307,33 @@ __device__ __forceinline__ void lds_dma16(const uint64_t* g, uint64_t* lds) {
 template <int D>
 __global__ __launch_bounds__(256) void k_mac_batch_gold_ring(
     FGold f, const uint64_t* __restrict__ A, const BatchDesc* __restrict__ desc, uint32_t count,
-    uint32_t rows, uint32_t chunk_cols) {
+    uint32_t rows, uint32_t chunk_cols, uint32_t split) {
   constexpr int G = 4;
   __shared__ uint64_t ring[4][D][8][64];  // [wave][stage][key row 0..3, item 0..3][slot]
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const uint32_t ch = blockIdx.x, g0 = blockIdx.y * G;
+  // block x: group x / split, y = x % split takes the y-th of `split` runs of
+  // consecutive chunks of the group's columns (the ring flows through them)
+  const uint32_t g0 = (blockIdx.x / split) * G, y = blockIdx.x % split;
   const uint32_t row0 = blockIdx.z * 16 + 4 * wave;  // rows % 16 == 0 (launcher)
-  const uint32_t c0 = ch * chunk_cols;
   const uint32_t maxlen = desc[g0].len;  // sorted: the group's first item is its longest
-  if (c0 >= maxlen) return;              // block-uniform
-  const uint32_t c1 = c0 + chunk_cols < maxlen ? c0 + chunk_cols : maxlen;
+  const uint32_t nch = (maxlen + chunk_cols - 1) / chunk_cols;
+  const uint32_t per = (nch + split - 1) / split;
+  uint32_t ch = y * per;
+  if (ch >= nch) return;  // block-uniform
+  const uint32_t c0 = ch * chunk_cols;
+  const uint32_t ch_end = ch + per < nch ? ch + per : nch;
+  const uint32_t c1 = ch_end * chunk_cols < maxlen ? ch_end * chunk_cols : maxlen;
   uint32_t len[G];
   const uint64_t* sbase[G];
+  uint64_t* part[G];
 #pragma unroll
   for (int i = 0; i < G; i++) {
     const uint32_t j = g0 + i;
     len[i] = j < count ? desc[j].len : 0;
     sbase[i] = reinterpret_cast<const uint64_t*>(desc[j < count ? j : g0].s_ntt);
+    part[i] = reinterpret_cast<uint64_t*>(desc[j < count ? j : g0].partial) +
+              (size_t)(row0 * 64 + lane);
   }
   const uint64_t* s_g0 = reinterpret_cast<const uint64_t*>(desc[g0].s_ntt);
   // per-lane sources of the four loads of a stage: lane l moves 16 bytes
@@ -334,7 +347,7 @@ __global__ __launch_bounds__(256) void k_mac_batch_gold_ring(
   const uint64_t* wb1 = (upper ? sbase[3] : sbase[2]) + wl;
   const uint32_t wn0 = upper ? len[1] : len[0], wn1 = upper ? len[3] : len[2];
   const uint64_t* wg = s_g0 + wl;
-  auto issue = [&](int d, uint32_t c) {
+  auto issue = [&](uint32_t d, uint32_t c) {
     const uint64_t* k0 = key_src + (size_t)c * col_stride;
     lds_dma16(k0, &ring[wave][d][0][0]);
     lds_dma16(k0 + 128, &ring[wave][d][2][0]);
@@ -349,41 +362,42 @@ __global__ __launch_bounds__(256) void k_mac_batch_gold_ring(
   // step has exactly four loads behind the stage it reads
 #pragma unroll
   for (int d = 0; d < D; d++) issue(d, c0 + d < c1 ? c0 + d : c1 - 1);
-  for (uint32_t cb = c0; cb < c1; cb += D) {
+  uint32_t flush_at = (ch + 1) * chunk_cols < c1 ? (ch + 1) * chunk_cols : c1;
+  uint32_t d = 0;
+  for (uint32_t c = c0; c < c1; c++) {
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(4 * (D - 1)) : "memory");
+    uint64_t a[4], sv[G];
 #pragma unroll
-    for (int d = 0; d < D; d++) {
-      const uint32_t c = cb + d;
-      if (c >= c1) break;  // uniform
-      asm volatile("s_waitcnt vmcnt(%0)" ::"n"(4 * (D - 1)) : "memory");
-      uint64_t a[4], sv[G];
+    for (int r = 0; r < 4; r++) a[r] = ring[wave][d][r][lane];
 #pragma unroll
-      for (int r = 0; r < 4; r++) a[r] = ring[wave][d][r][lane];
+    for (int i = 0; i < G; i++) sv[i] = ring[wave][d][4 + i][lane];
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    issue(d, c + D < c1 ? c + D : c1 - 1);
+    d = d + 1 == D ? 0 : d + 1;
 #pragma unroll
-      for (int i = 0; i < G; i++) sv[i] = ring[wave][d][4 + i][lane];
-      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-      const uint32_t cn = c + D < c1 ? c + D : c1 - 1;
-      issue(d, cn);
+    for (int i = 0; i < G; i++) {
+      if (c < len[i]) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) gold_mac(lz[i][r], a[r], sv[i]);
+      }
+    }
+    if (c + 1 == flush_at) {  // uniform: the chunk's partial sums, then a fresh start
 #pragma unroll
       for (int i = 0; i < G; i++) {
-        if (c < len[i]) {
+        if (g0 + i < count && ch * chunk_cols < len[i]) {
 #pragma unroll
-          for (int r = 0; r < 4; r++) gold_mac(lz[i][r], a[r], sv[i]);
+          for (int r = 0; r < 4; r++)
+            part[i][((size_t)ch * rows + r) * 64] = gold_reduce(f, lz[i][r]);
         }
+#pragma unroll
+        for (int r = 0; r < 4; r++) lz[i][r] = GoldAcc{};
       }
+      ch++;
+      flush_at = flush_at + chunk_cols < c1 ? flush_at + chunk_cols : c1;
     }
   }
   // the ring's last loads land before the wave ends
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-#pragma unroll
-  for (int i = 0; i < G; i++) {
-    const uint32_t j = g0 + i;
-    if (j < count && c0 < len[i]) {
-      uint64_t* pp = reinterpret_cast<uint64_t*>(desc[j].partial);
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-        pp[((size_t)ch * rows + row0 + r) * 64 + lane] = gold_reduce(f, lz[i][r]);
-    }
-  }
 }
 
 // grid (count, rows): item j's row `row` = INTT(sum over its chunks)
@@ -616,17 +630,27 @@ int enqueue_batch_xform(gh_ring* ring, hipStream_t stream, const BatchDesc* desc
 
 // the LDS-ring Goldilocks MAC (groups of 4, 16-row blocks), `depth` columns per wave
 static void mac_gold_ring_launch(gh_key* key, hipStream_t stream, const BatchDesc* desc,
-                                 uint32_t count, uint32_t chunk_cols, dim3 grid, int depth) {
+                                 uint32_t count, uint32_t chunk_cols, uint32_t nchunks,
+                                 int depth) {
   const uint64_t* a = (const uint64_t*)key->dev_a;
+  const uint32_t groups = (count + 3) / 4, rblocks = key->rows / 16;
+  // runs per group: about eight blocks per block slot of the device (two per
+  // CU), never more than the longest group's chunks
+  const uint64_t want = (uint64_t)key->ring->ctx->num_cus * 2 * 8;
+  uint64_t split = (want + (uint64_t)groups * rblocks - 1) / ((uint64_t)groups * rblocks);
+  if (split > nchunks) split = nchunks;
+  if (split < 1) split = 1;
+  const dim3 grid((uint32_t)(groups * split), 1, rblocks);
+  const uint32_t sp = (uint32_t)split;
   if (depth == 2)
     hipLaunchKernelGGL((k_mac_batch_gold_ring<2>), grid, dim3(256), 0, stream, FGold{}, a, desc,
-                       count, key->rows, chunk_cols);
+                       count, key->rows, chunk_cols, sp);
   else if (depth == 3)
     hipLaunchKernelGGL((k_mac_batch_gold_ring<3>), grid, dim3(256), 0, stream, FGold{}, a, desc,
-                       count, key->rows, chunk_cols);
+                       count, key->rows, chunk_cols, sp);
   else
     hipLaunchKernelGGL((k_mac_batch_gold_ring<4>), grid, dim3(256), 0, stream, FGold{}, a, desc,
-                       count, key->rows, chunk_cols);
+                       count, key->rows, chunk_cols, sp);
 }
 
 template <int G, int RB>
@@ -647,7 +671,8 @@ static void mac_batch_launch(gh_key* key, hipStream_t stream, const BatchDesc* d
                        FQ32{(uint32_t)ring->q, ring->qinv32}, (const uint32_t*)key->dev_a, desc,
                        count, key->rows, chunk_cols);
   else if (G == 4 && RB == 16 && ring->ctx->tune.gold_mac_ring >= 2)
-    mac_gold_ring_launch(key, stream, desc, count, chunk_cols, grid, ring->ctx->tune.gold_mac_ring);
+    mac_gold_ring_launch(key, stream, desc, count, chunk_cols, nchunks,
+                         ring->ctx->tune.gold_mac_ring);
   else
     hipLaunchKernelGGL((k_mac_batch_gold<G, RB>), grid, dim3(16 * RB), 0, stream, FGold{},
                        (const uint64_t*)key->dev_a, desc, count, key->rows, chunk_cols);

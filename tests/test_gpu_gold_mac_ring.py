@@ -65,3 +65,33 @@ def test_ring_tuning_values_checked(ctx):
         with pytest.raises(la.GridHipError):
             ctx.set_tuning(gold_mac_ring=bad)
     assert ctx.get_tuning()["gold_mac_ring"] in (0, 2, 3, 4)
+
+
+def test_ring_mac_chunk_runs(oracle, ctx, ring_gold):
+    # enough groups that each block runs through several chunks of its group
+    # (runs per group = about 8 * 2 * num_cus / groups, below the chunk count)
+    import lattica_amd as la
+    p = GOLDILOCKS
+    rng = np.random.default_rng(1400)
+    rows, cols = 16, 600
+    a = rand_residues(rng, p, (rows, cols, 64))
+    key = la.Key(ring_gold, a, domain=la.DOMAIN_NTT)
+    lens = [int(x) for x in rng.integers(1, cols + 1, size=1200)]
+    lens[0] = cols
+    ws = [rand_residues(rng, p, (n, 64)) for n in lens]
+    try:
+        for cc in (16, 32):
+            groups = (len(lens) + 3) // 4
+            assert -(-8 * 2 * ctx.num_cus // groups) < -(-cols // cc)  # runs of several chunks
+            base = _batch(ctx, key, ws, lens, 0, cc)
+            for depth in (2, 4):
+                got = _batch(ctx, key, ws, lens, depth, cc)
+                for i in range(len(lens)):
+                    assert np.array_equal(got[i], base[i]), (cc, depth, i, lens[i])
+        z = np.zeros((rows, 1, 64), dtype=np.uint64)
+        zr = np.zeros((1, 64), dtype=np.uint64)
+        for i in (0, 1, 777):
+            exp = oracle.commit(p, 64, a[:, :lens[i]], z, ws[i], zr, key_domain=1)
+            assert np.array_equal(got[i], exp), i
+    finally:
+        key.close()
