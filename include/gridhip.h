@@ -135,11 +135,11 @@ GH_API int gh_ctx_profile_read(gh_ctx *ctx, int phase, double *total_ms, uint64_
  * gh_commit_batch or batch plan runs a transform, a grouped MAC and a
  * finalize kernel; 1: one launch that transforms, multiplies and finalizes,
  * slower on the batches measured), "batch_fused_cols" (32..4096, a
- * multiple of 32: key columns per block of that launch) and "gold_mac_ring"
- * (4, the default, down to 2: the Goldilocks grouped MAC stages that many key
- * columns per wave in an LDS ring filled by global-to-LDS loads, for groups of
- * 4 and row counts divisible by 16; 0: one column of loads in flight in
- * registers).
+ * multiple of 32: key columns per block of that launch) and "batch_mac_ring"
+ * (4, the default, down to 2: the grouped MAC of gh_commit_batch and the
+ * server stages that many key columns per wave in an LDS ring filled by
+ * global-to-LDS loads, for groups of 4 and row counts divisible by 16; 0: the
+ * kernels with one column of loads in flight in registers).
  * Values are checked;
  * an unknown key or value gives GH_E_INVALID_ARGUMENT. Every setting yields
  * identical results; only the speed changes. */

@@ -300,7 +300,7 @@ __global__ __launch_bounds__(16 * RB) void k_mac_batch_gold(
 // (vmcnt) before reading a stage and for its reads (lgkmcnt) before refilling it.
 // 16 bytes per lane from global memory straight into LDS at lds + 16 * lane
 // (global_load_lds_dwordx4; lds must be the same for every lane of the wave)
-__device__ __forceinline__ void lds_dma16(const uint64_t* g, uint64_t* lds) {
+__device__ __forceinline__ void lds_dma16(const void* g, void* lds) {
   __builtin_amdgcn_global_load_lds(g, lds, 16, 0, 0);
 }
 
@@ -397,6 +397,95 @@ __global__ __launch_bounds__(256) void k_mac_batch_gold_ring(
     }
   }
   // the ring's last loads land before the wave ends
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+// The q32 grouped MAC in the same form (groups of 4, 16-row blocks): per wave
+// and stage 1 KB of key (its 4 rows, lane l's 16 bytes at 16 l) and the
+// group's 4 witness columns (lanes 16 i .. 16 i + 15 move item i's 256 bytes),
+// read back as lane l's 4 slots of row 4w + l / 16 and of every item; exact
+// 96-bit sums, reduced per chunk as in k_mac_batch4_q32.
+template <int D>
+__global__ __launch_bounds__(256) void k_mac_batch_q32_ring(
+    FQ32 f, const uint32_t* __restrict__ A, const BatchDesc* __restrict__ desc, uint32_t count,
+    uint32_t rows, uint32_t chunk_cols, uint32_t split) {
+  constexpr int G = 4;
+  __shared__ uint32_t ring[4][D][2][256];  // [wave][stage][key | witness][4 rows or items x 64]
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int k0 = 4 * (lane & 15), sub = lane >> 4;
+  const uint32_t g0 = (blockIdx.x / split) * G, y = blockIdx.x % split;
+  const uint32_t row_w = blockIdx.z * 16 + 4 * wave;  // rows % 16 == 0 (launcher)
+  const uint32_t maxlen = desc[g0].len;
+  const uint32_t nch = (maxlen + chunk_cols - 1) / chunk_cols;
+  const uint32_t per = (nch + split - 1) / split;
+  uint32_t ch = y * per;
+  if (ch >= nch) return;  // block-uniform
+  const uint32_t c0 = ch * chunk_cols;
+  const uint32_t ch_end = ch + per < nch ? ch + per : nch;
+  const uint32_t c1 = ch_end * chunk_cols < maxlen ? ch_end * chunk_cols : maxlen;
+  uint32_t len[G];
+  uint32_t* part[G];
+#pragma unroll
+  for (int i = 0; i < G; i++) {
+    const uint32_t j = g0 + i;
+    len[i] = j < count ? desc[j].len : 0;
+    part[i] = reinterpret_cast<uint32_t*>(desc[j < count ? j : g0].partial) +
+              (size_t)(row_w + sub) * 64 + k0;
+  }
+  // lane l's loads: key words 4 l .. 4 l + 3 of the wave's 4 rows; witness
+  // item `sub`, slots k0 .. k0 + 3 (the group's first item past its length)
+  const size_t col_stride = (size_t)rows * 64;
+  const uint32_t* key_src = A + (size_t)row_w * 64 + 4 * lane;
+  const uint32_t my_len = len[0] * (sub == 0) + len[1] * (sub == 1) + len[2] * (sub == 2) +
+                          len[3] * (sub == 3);
+  const uint32_t jl = g0 + sub < count ? g0 + sub : g0;
+  const uint32_t* wmine = reinterpret_cast<const uint32_t*>(desc[jl].s_ntt) + k0;
+  const uint32_t* wg = reinterpret_cast<const uint32_t*>(desc[g0].s_ntt) + k0;
+  auto issue = [&](uint32_t d, uint32_t c) {
+    lds_dma16(key_src + (size_t)c * col_stride, &ring[wave][d][0][0]);
+    lds_dma16((c < my_len ? wmine : wg) + (size_t)c * 64, &ring[wave][d][1][0]);
+  };
+  uint64_t lo[G][4];
+  uint32_t hi[G][4];
+#pragma unroll
+  for (int i = 0; i < G; i++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) { lo[i][q] = 0; hi[i][q] = 0; }
+#pragma unroll
+  for (int d = 0; d < D; d++) issue(d, c0 + d < c1 ? c0 + d : c1 - 1);
+  uint32_t flush_at = (ch + 1) * chunk_cols < c1 ? (ch + 1) * chunk_cols : c1;
+  uint32_t d = 0;
+  for (uint32_t c = c0; c < c1; c++) {
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(2 * (D - 1)) : "memory");
+    const u32x4 a = *reinterpret_cast<const u32x4*>(&ring[wave][d][0][4 * lane]);
+    u32x4 sv[G];
+#pragma unroll
+    for (int i = 0; i < G; i++) sv[i] = *reinterpret_cast<const u32x4*>(&ring[wave][d][1][64 * i + k0]);
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    issue(d, c + D < c1 ? c + D : c1 - 1);
+    d = d + 1 == D ? 0 : d + 1;
+#pragma unroll
+    for (int i = 0; i < G; i++)
+      if (c < len[i]) mac96x4(lo[i], hi[i], a.x, a.y, a.z, a.w, sv[i].x, sv[i].y, sv[i].z, sv[i].w);
+    if (c + 1 == flush_at) {  // uniform: the chunk's partial sums, then a fresh start
+#pragma unroll
+      for (int i = 0; i < G; i++) {
+        if (g0 + i < count && ch * chunk_cols < len[i]) {
+          u32x4 o;
+          o.x = f.reduce96_r2((uint32_t)lo[i][0], (uint32_t)(lo[i][0] >> 32), hi[i][0]);
+          o.y = f.reduce96_r2((uint32_t)lo[i][1], (uint32_t)(lo[i][1] >> 32), hi[i][1]);
+          o.z = f.reduce96_r2((uint32_t)lo[i][2], (uint32_t)(lo[i][2] >> 32), hi[i][2]);
+          o.w = f.reduce96_r2((uint32_t)lo[i][3], (uint32_t)(lo[i][3] >> 32), hi[i][3]);
+          *reinterpret_cast<u32x4*>(part[i] + (size_t)ch * rows * 64) = o;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) { lo[i][q] = 0; hi[i][q] = 0; }
+      }
+      ch++;
+      flush_at = flush_at + chunk_cols < c1 ? flush_at + chunk_cols : c1;
+    }
+  }
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
@@ -628,20 +717,37 @@ int enqueue_batch_xform(gh_ring* ring, hipStream_t stream, const BatchDesc* desc
   return GH_OK;
 }
 
-// the LDS-ring Goldilocks MAC (groups of 4, 16-row blocks), `depth` columns per wave
-static void mac_gold_ring_launch(gh_key* key, hipStream_t stream, const BatchDesc* desc,
-                                 uint32_t count, uint32_t chunk_cols, uint32_t nchunks,
-                                 int depth) {
-  const uint64_t* a = (const uint64_t*)key->dev_a;
-  const uint32_t groups = (count + 3) / 4, rblocks = key->rows / 16;
-  // runs per group: about eight blocks per block slot of the device (two per
-  // CU), never more than the longest group's chunks
+// The LDS-ring grouped MACs (groups of 4, 16-row blocks), `depth` columns per
+// wave: runs per group about eight blocks per block slot of the device (two per
+// CU), never more than the longest group's chunks.
+static uint32_t ring_split(gh_key* key, uint32_t count, uint32_t nchunks) {
+  const uint64_t groups = (count + 3) / 4, rblocks = key->rows / 16;
   const uint64_t want = (uint64_t)key->ring->ctx->num_cus * 2 * 8;
-  uint64_t split = (want + (uint64_t)groups * rblocks - 1) / ((uint64_t)groups * rblocks);
+  uint64_t split = (want + groups * rblocks - 1) / (groups * rblocks);
   if (split > nchunks) split = nchunks;
-  if (split < 1) split = 1;
-  const dim3 grid((uint32_t)(groups * split), 1, rblocks);
-  const uint32_t sp = (uint32_t)split;
+  return split < 1 ? 1u : (uint32_t)split;
+}
+
+static void mac_ring_launch(gh_key* key, hipStream_t stream, const BatchDesc* desc,
+                            uint32_t count, uint32_t chunk_cols, uint32_t nchunks, int depth) {
+  gh_ring* ring = key->ring;
+  const uint32_t sp = ring_split(key, count, nchunks);
+  const dim3 grid(((count + 3) / 4) * sp, 1, key->rows / 16);
+  if (ring->kind == GH_RING_Q32) {
+    const FQ32 f{(uint32_t)ring->q, ring->qinv32};
+    const uint32_t* a = (const uint32_t*)key->dev_a;
+    if (depth == 2)
+      hipLaunchKernelGGL((k_mac_batch_q32_ring<2>), grid, dim3(256), 0, stream, f, a, desc, count,
+                         key->rows, chunk_cols, sp);
+    else if (depth == 3)
+      hipLaunchKernelGGL((k_mac_batch_q32_ring<3>), grid, dim3(256), 0, stream, f, a, desc, count,
+                         key->rows, chunk_cols, sp);
+    else
+      hipLaunchKernelGGL((k_mac_batch_q32_ring<4>), grid, dim3(256), 0, stream, f, a, desc, count,
+                         key->rows, chunk_cols, sp);
+    return;
+  }
+  const uint64_t* a = (const uint64_t*)key->dev_a;
   if (depth == 2)
     hipLaunchKernelGGL((k_mac_batch_gold_ring<2>), grid, dim3(256), 0, stream, FGold{}, a, desc,
                        count, key->rows, chunk_cols, sp);
@@ -657,6 +763,10 @@ template <int G, int RB>
 static void mac_batch_launch(gh_key* key, hipStream_t stream, const BatchDesc* desc,
                              uint32_t count, uint32_t chunk_cols, uint32_t nchunks, int variant) {
   gh_ring* ring = key->ring;
+  if (G == 4 && RB == 16 && ring->ctx->tune.batch_mac_ring >= 2) {
+    mac_ring_launch(key, stream, desc, count, chunk_cols, nchunks, ring->ctx->tune.batch_mac_ring);
+    return;
+  }
   dim3 grid(nchunks, (count + G - 1) / G, (key->rows + RB - 1) / RB);
   if constexpr (G <= 8) {
     if (ring->kind == GH_RING_Q32 && variant == 1) {
@@ -670,9 +780,6 @@ static void mac_batch_launch(gh_key* key, hipStream_t stream, const BatchDesc* d
     hipLaunchKernelGGL((k_mac_batch_q32<G, RB>), grid, dim3(16 * RB), 0, stream,
                        FQ32{(uint32_t)ring->q, ring->qinv32}, (const uint32_t*)key->dev_a, desc,
                        count, key->rows, chunk_cols);
-  else if (G == 4 && RB == 16 && ring->ctx->tune.gold_mac_ring >= 2)
-    mac_gold_ring_launch(key, stream, desc, count, chunk_cols, nchunks,
-                         ring->ctx->tune.gold_mac_ring);
   else
     hipLaunchKernelGGL((k_mac_batch_gold<G, RB>), grid, dim3(16 * RB), 0, stream, FGold{},
                        (const uint64_t*)key->dev_a, desc, count, key->rows, chunk_cols);

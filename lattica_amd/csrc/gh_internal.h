@@ -93,10 +93,11 @@ struct gh_tuning {
   int batch_fused = 0;
   // key columns per block of that launch (a multiple of 16)
   int batch_fused_cols = 256;
-  // Goldilocks grouped MAC (groups of 4, rows % 16 == 0): 0 = one column of
-  // loads in flight in registers, 2..4 = that many columns staged per wave in
-  // an LDS ring filled by global-to-LDS loads
-  int gold_mac_ring = 4;
+  // grouped MAC of gh_commit_batch and the server (groups of 4, rows % 16 ==
+  // 0): 0 = the register-pipelined kernels, 2..4 = that many key columns
+  // staged per wave in an LDS ring filled by global-to-LDS loads, each block
+  // running through consecutive chunks of its group
+  int batch_mac_ring = 4;
 };
 // most stage-1 tasks one dedicated block runs
 constexpr uint32_t kStage1MaxTpb = 8;
