@@ -139,8 +139,10 @@ GH_API int gh_ctx_profile_read(gh_ctx *ctx, int phase, double *total_ms, uint64_
  * (4, the default, down to 2: the grouped MAC of gh_commit_batch and the
  * server stages that many key columns per wave in an LDS ring filled by
  * global-to-LDS loads, for groups of 4 and row counts divisible by 16; 0: the
- * kernels with one column of loads in flight in registers).
- * Values are checked;
+ * kernels with one column of loads in flight in registers) and
+ * "negacyclic_quad" (-1, the default: gh_negacyclic_mul runs one pair per
+ * thread for q < 2^32 and one pair per quad of lanes for Goldilocks; 0 and 1:
+ * per thread or per quad for both). Values are checked;
  * an unknown key or value gives GH_E_INVALID_ARGUMENT. Every setting yields
  * identical results; only the speed changes. */
 GH_API int gh_ctx_set_tuning(gh_ctx *ctx, const char *key, int64_t value);

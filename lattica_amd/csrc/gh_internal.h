@@ -98,6 +98,10 @@ struct gh_tuning {
   // staged per wave in an LDS ring filled by global-to-LDS loads, each block
   // running through consecutive chunks of its group
   int batch_mac_ring = 4;
+  // gh_negacyclic_mul: 0 = one pair per thread (the first operand's transform
+  // parked in LDS), 1 = one pair per quad of lanes, -1 = the faster per field
+  // (per thread for q < 2^32, per quad for Goldilocks)
+  int negacyclic_quad = -1;
 };
 // most stage-1 tasks one dedicated block runs
 constexpr uint32_t kStage1MaxTpb = 8;

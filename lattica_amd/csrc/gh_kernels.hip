@@ -199,6 +199,43 @@ __global__ __launch_bounds__(TILE) void k_negacyclic_mul(F f, const typename F::
   }
 }
 
+// Negacyclic product, one pair per quad of lanes (gh_ntt.h quad_fwd64 and
+// quad_inv64): no LDS and 16 coefficients of each operand per lane, so more
+// waves fit on a SIMD than with a whole polynomial per thread. Loads and
+// stores are stride-4 words; a quad together covers whole 64-byte segments.
+template <class F>
+__global__ __launch_bounds__(256) void k_negacyclic_quad(F f, const typename F::W* __restrict__ tab,
+                                                         const typename F::W* a,
+                                                         const typename F::W* b,
+                                                         typename F::W* out, size_t count) {
+  using W = typename F::W;
+  constexpr int TW = F::kTwWords;
+  const int t = threadIdx.x & 3;
+  W ltw[16][TW];
+#pragma unroll
+  for (int k = 0; k < 16; k++)
+#pragma unroll
+    for (int w = 0; w < TW; w++) ltw[k][w] = tab[(kTabQuad + 16 * t + k) * TW + w];
+  const size_t stride = ((size_t)gridDim.x * blockDim.x) >> 2;
+  for (size_t p = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 2; p < count; p += stride) {
+    W x[16], y[16];
+    const W* ap = a + p * 64 + t;
+    const W* bp = b + p * 64 + t;
+#pragma unroll
+    for (int i = 0; i < 16; i++) x[i] = ap[4 * i];
+#pragma unroll
+    for (int i = 0; i < 16; i++) y[i] = bp[4 * i];
+    quad_fwd64(f, x, tab, ltw, t);
+    quad_fwd64(f, y, tab, ltw, t);
+#pragma unroll
+    for (int k = 0; k < 16; k++) x[k] = f.mont(x[k], y[k]);
+    quad_inv64(f, x, tab, kTabUntwistMul, t);
+    W* op = out + p * 64 + t;
+#pragma unroll
+    for (int i = 0; i < 16; i++) op[4 * i] = x[i];
+  }
+}
+
 // out = a (.) b, canonical: Montgomery product then a Montgomery multiply by
 // R^2 (stored as the kTabUntwistMul-free constant r2m) restores the R factor.
 template <class F>
@@ -1121,6 +1158,15 @@ static int negacyclic_impl(gh_ring* ring, F f, const void* a, const void* b, voi
   using W = typename F::W;
   gh_ctx* ctx = ring->ctx;
   const W* tab = (const W*)ring->dev_tab;
+  const int quad = ctx->tune.negacyclic_quad;
+  if (quad == 1 || (quad < 0 && sizeof(W) == 8)) {
+    // 64 pairs per 256-thread block
+    unsigned grid = grid_for((count + 63) / 64, ctx->num_cus, 8);
+    hipLaunchKernelGGL((k_negacyclic_quad<F>), dim3(grid), dim3(256), 0, ctx->stream, f, tab,
+                       (const W*)a, (const W*)b, (W*)out, count);
+    GH_HIP(hipGetLastError());
+    return GH_OK;
+  }
   // blocks per CU: LDS-bound for q < 2^32 (9 x 16.6 KB), one wave per SIMD
   // for Goldilocks (480 VGPRs)
   constexpr int kPerCu = sizeof(W) == 4 ? 9 : 4;

@@ -224,6 +224,75 @@ __device__ __forceinline__ void quad_fwd64(const F& f, typename F::W (&x)[16],
   }
 }
 
+// Inverse of quad_fwd64 with the per-coefficient factor at `post`, for a
+// negacyclic product: in, lane t holds x[k2] = C[16*bitrev2(t) + k2]; out,
+// x[i2] = c[t + 4*i2] * post[4*i2], where c_i = sum_k C_k psi^(-i(2k+1)).
+// With i = t + 4*i2 and k = k2 + 16*k1,
+//   psi^(-i(2k+1)) = psi^(-t(2k2+1)) * iota^(-t*k1) * (psi^-8)^(i2*k2) * psi^(-4*i2),
+// so the 4-point inverse DFT over k1 runs across the quad first (the
+// forward's two exchanges in reverse order, iota^-1 on lane 3), each lane
+// multiplies by psi^(-t(2k2+1)) (kTabQuadInv), runs the 16-point inverse DFT
+// over k2 (the 64-point DIF stages of length 16..2 share its roots) and
+// multiplies by post[4*i2] (psi^(-4*i2) d^-1, times R where the pointwise
+// product left R^-1).
+template <class F>
+__device__ __forceinline__ void quad_inv64(const F& f, typename F::W (&x)[16],
+                                           const typename F::W* __restrict__ tab, int post,
+                                           int t) {
+  using W = typename F::W;
+  constexpr int TW = F::kTwWords;
+  // 4-point inverse DFT over k1: t, t^1 combine ...
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    W o = quad_dpp<kDppXor1>(x[k]);
+    x[k] = (t & 1) ? f.sub(o, x[k]) : f.add(x[k], o);
+  }
+  // ... lane 3 holds C1 - C3 and takes the factor iota^-1 ...
+  if (t == 3) {
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      if constexpr (F::kPow2Roots)  // psi^-32 = omega^-16 = 2^(-16 * kOmegaLog2)
+        x[k] = f.mul_pow2(x[k], (192 - (16 * F::kOmegaLog2) % 192) % 192);
+      else
+        x[k] = f.mul_tw(x[k], tab + (kTabInv + 16) * TW);  // stage 0 root iomega^16
+    }
+  }
+  // ... and t, t^2 combine: lane t ends with D_t.
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    W o = quad_dpp<kDppXor2>(x[k]);
+    x[k] = (t & 2) ? f.sub(o, x[k]) : f.add(x[k], o);
+  }
+#pragma unroll
+  for (int k = 0; k < 16; k++) x[k] = f.mul_tw(x[k], tab + (kTabQuadInv + 16 * t + k) * TW);
+  // 16-point DIF with the 64-point tables' stages of length 16, 8, 4, 2
+#pragma unroll
+  for (int s = 2; s < 6; s++) {
+    const int len = 64 >> s, half = len / 2, off = 64 - (64 >> s);
+#pragma unroll
+    for (int st = 0; st < 16; st += len) {
+#pragma unroll
+      for (int j = 0; j < half; j++) {
+        W e = x[st + j], o = x[st + half + j];
+        x[st + j] = f.add(e, o);
+        W dlt = f.sub(e, o);
+        if (j != 0) {
+          if constexpr (F::kPow2Roots)  // (omega^-(64 / len))^j
+            dlt = f.mul_pow2(dlt, (192 - (F::kOmegaLog2 * (64 / len) * j) % 192) % 192);
+          else
+            dlt = f.mul_tw(dlt, tab + (kTabInv + off + j) * TW);
+        }
+        x[st + half + j] = dlt;
+      }
+    }
+  }
+  W y[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) y[i] = f.mul_tw(x[bitrev4(i)], tab + (post + 4 * i) * TW);
+#pragma unroll
+  for (int i = 0; i < 16; i++) x[i] = y[i];
+}
+
 // Wave-cooperative forward twisted transform (lane k holds coefficient k).
 template <bool TWIST, class F>
 __device__ __forceinline__ typename F::W wave_fwd64(const F& f, typename F::W x,
