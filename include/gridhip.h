@@ -139,7 +139,9 @@ GH_API int gh_ctx_profile_read(gh_ctx *ctx, int phase, double *total_ms, uint64_
  * (4, the default, down to 2: the grouped MAC of gh_commit_batch and the
  * server stages that many key columns per wave in an LDS ring filled by
  * global-to-LDS loads, for groups of 4 and row counts divisible by 16; 0: the
- * kernels with one column of loads in flight in registers) and
+ * kernels with one column of loads in flight in registers), "q32_ring_group"
+ * (4, the default, or 8: witnesses sharing each key column in that ring MAC
+ * for q32 keys; Goldilocks keys use 4) and
  * "negacyclic_quad" (-1, the default: gh_negacyclic_mul runs one pair per
  * thread for q < 2^32 and one pair per quad of lanes for Goldilocks; 0 and 1:
  * per thread or per quad for both). Values are checked;

@@ -405,12 +405,14 @@ __global__ __launch_bounds__(256) void k_mac_batch_gold_ring(
 // group's 4 witness columns (lanes 16 i .. 16 i + 15 move item i's 256 bytes),
 // read back as lane l's 4 slots of row 4w + l / 16 and of every item; exact
 // 96-bit sums, reduced per chunk as in k_mac_batch4_q32.
-template <int D>
+template <int G, int D>
 __global__ __launch_bounds__(256) void k_mac_batch_q32_ring(
     FQ32 f, const uint32_t* __restrict__ A, const BatchDesc* __restrict__ desc, uint32_t count,
     uint32_t rows, uint32_t chunk_cols, uint32_t split) {
-  constexpr int G = 4;
-  __shared__ uint32_t ring[4][D][2][256];  // [wave][stage][key | witness][4 rows or items x 64]
+  static_assert(G == 4 || G == 8, "witness loads move four items per instruction");
+  constexpr int H = G / 4;  // witness loads per stage
+  // [wave][stage][key: 4 rows x 64 | witness: G items x 64]
+  __shared__ uint32_t ring[4][D][256 * (1 + H)];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int k0 = 4 * (lane & 15), sub = lane >> 4;
@@ -434,17 +436,24 @@ __global__ __launch_bounds__(256) void k_mac_batch_q32_ring(
               (size_t)(row_w + sub) * 64 + k0;
   }
   // lane l's loads: key words 4 l .. 4 l + 3 of the wave's 4 rows; witness
-  // item `sub`, slots k0 .. k0 + 3 (the group's first item past its length)
+  // load h: item 4 h + l / 16, slots k0 .. k0 + 3 (the group's first item
+  // once past its length)
   const size_t col_stride = (size_t)rows * 64;
   const uint32_t* key_src = A + (size_t)row_w * 64 + 4 * lane;
-  const uint32_t my_len = len[0] * (sub == 0) + len[1] * (sub == 1) + len[2] * (sub == 2) +
-                          len[3] * (sub == 3);
-  const uint32_t jl = g0 + sub < count ? g0 + sub : g0;
-  const uint32_t* wmine = reinterpret_cast<const uint32_t*>(desc[jl].s_ntt) + k0;
   const uint32_t* wg = reinterpret_cast<const uint32_t*>(desc[g0].s_ntt) + k0;
+  const uint32_t* wmine[H];
+  uint32_t my_len[H];
+#pragma unroll
+  for (int h = 0; h < H; h++) {
+    const uint32_t j = g0 + 4 * h + sub;
+    my_len[h] = j < count ? desc[j].len : 0;
+    wmine[h] = reinterpret_cast<const uint32_t*>(desc[j < count ? j : g0].s_ntt) + k0;
+  }
   auto issue = [&](uint32_t d, uint32_t c) {
-    lds_dma16(key_src + (size_t)c * col_stride, &ring[wave][d][0][0]);
-    lds_dma16((c < my_len ? wmine : wg) + (size_t)c * 64, &ring[wave][d][1][0]);
+    lds_dma16(key_src + (size_t)c * col_stride, &ring[wave][d][0]);
+#pragma unroll
+    for (int h = 0; h < H; h++)
+      lds_dma16((c < my_len[h] ? wmine[h] : wg) + (size_t)c * 64, &ring[wave][d][256 * (1 + h)]);
   };
   uint64_t lo[G][4];
   uint32_t hi[G][4];
@@ -457,11 +466,11 @@ __global__ __launch_bounds__(256) void k_mac_batch_q32_ring(
   uint32_t flush_at = (ch + 1) * chunk_cols < c1 ? (ch + 1) * chunk_cols : c1;
   uint32_t d = 0;
   for (uint32_t c = c0; c < c1; c++) {
-    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(2 * (D - 1)) : "memory");
-    const u32x4 a = *reinterpret_cast<const u32x4*>(&ring[wave][d][0][4 * lane]);
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"((1 + H) * (D - 1)) : "memory");
+    const u32x4 a = *reinterpret_cast<const u32x4*>(&ring[wave][d][4 * lane]);
     u32x4 sv[G];
 #pragma unroll
-    for (int i = 0; i < G; i++) sv[i] = *reinterpret_cast<const u32x4*>(&ring[wave][d][1][64 * i + k0]);
+    for (int i = 0; i < G; i++) sv[i] = *reinterpret_cast<const u32x4*>(&ring[wave][d][256 + 64 * i + k0]);
     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     issue(d, c + D < c1 ? c + D : c1 - 1);
     d = d + 1 == D ? 0 : d + 1;
@@ -720,8 +729,8 @@ int enqueue_batch_xform(gh_ring* ring, hipStream_t stream, const BatchDesc* desc
 // The LDS-ring grouped MACs (groups of 4, 16-row blocks), `depth` columns per
 // wave: runs per group about eight blocks per block slot of the device (two per
 // CU), never more than the longest group's chunks.
-static uint32_t ring_split(gh_key* key, uint32_t count, uint32_t nchunks) {
-  const uint64_t groups = (count + 3) / 4, rblocks = key->rows / 16;
+static uint32_t ring_split(gh_key* key, uint32_t count, uint32_t nchunks, uint32_t group) {
+  const uint64_t groups = (count + group - 1) / group, rblocks = key->rows / 16;
   const uint64_t want = (uint64_t)key->ring->ctx->num_cus * 2 * 8;
   uint64_t split = (want + groups * rblocks - 1) / (groups * rblocks);
   if (split > nchunks) split = nchunks;
@@ -731,22 +740,32 @@ static uint32_t ring_split(gh_key* key, uint32_t count, uint32_t nchunks) {
 static void mac_ring_launch(gh_key* key, hipStream_t stream, const BatchDesc* desc,
                             uint32_t count, uint32_t chunk_cols, uint32_t nchunks, int depth) {
   gh_ring* ring = key->ring;
-  const uint32_t sp = ring_split(key, count, nchunks);
-  const dim3 grid(((count + 3) / 4) * sp, 1, key->rows / 16);
   if (ring->kind == GH_RING_Q32) {
+    // q32 groups: the tuned width (exact 96-bit sums take 12 registers per
+    // item and row set, so 8 items fit)
+    const uint32_t g = ring->ctx->tune.q32_ring_group == 8 ? 8 : 4;
+    const uint32_t sp = ring_split(key, count, nchunks, g);
+    const dim3 grid(((count + g - 1) / g) * sp, 1, key->rows / 16);
     const FQ32 f{(uint32_t)ring->q, ring->qinv32};
     const uint32_t* a = (const uint32_t*)key->dev_a;
-    if (depth == 2)
-      hipLaunchKernelGGL((k_mac_batch_q32_ring<2>), grid, dim3(256), 0, stream, f, a, desc, count,
-                         key->rows, chunk_cols, sp);
-    else if (depth == 3)
-      hipLaunchKernelGGL((k_mac_batch_q32_ring<3>), grid, dim3(256), 0, stream, f, a, desc, count,
-                         key->rows, chunk_cols, sp);
-    else
-      hipLaunchKernelGGL((k_mac_batch_q32_ring<4>), grid, dim3(256), 0, stream, f, a, desc, count,
-                         key->rows, chunk_cols, sp);
+    const int dep = depth < 3 ? 2 : (depth == 3 ? 3 : 4);
+#define GH_Q32_RING(GG, DD)                                                                   \
+  hipLaunchKernelGGL((k_mac_batch_q32_ring<GG, DD>), grid, dim3(256), 0, stream, f, a, desc, \
+                     count, key->rows, chunk_cols, sp)
+    if (g == 8) {
+      if (dep == 2) GH_Q32_RING(8, 2);
+      else if (dep == 3) GH_Q32_RING(8, 3);
+      else GH_Q32_RING(8, 4);
+    } else {
+      if (dep == 2) GH_Q32_RING(4, 2);
+      else if (dep == 3) GH_Q32_RING(4, 3);
+      else GH_Q32_RING(4, 4);
+    }
+#undef GH_Q32_RING
     return;
   }
+  const uint32_t sp = ring_split(key, count, nchunks, 4);
+  const dim3 grid(((count + 3) / 4) * sp, 1, key->rows / 16);
   const uint64_t* a = (const uint64_t*)key->dev_a;
   if (depth == 2)
     hipLaunchKernelGGL((k_mac_batch_gold_ring<2>), grid, dim3(256), 0, stream, FGold{}, a, desc,
@@ -763,7 +782,7 @@ template <int G, int RB>
 static void mac_batch_launch(gh_key* key, hipStream_t stream, const BatchDesc* desc,
                              uint32_t count, uint32_t chunk_cols, uint32_t nchunks, int variant) {
   gh_ring* ring = key->ring;
-  if (G == 4 && RB == 16 && ring->ctx->tune.batch_mac_ring >= 2) {
+  if (RB == 16 && (G == 4 || ring->kind == GH_RING_Q32) && ring->ctx->tune.batch_mac_ring >= 2) {
     mac_ring_launch(key, stream, desc, count, chunk_cols, nchunks, ring->ctx->tune.batch_mac_ring);
     return;
   }

@@ -98,6 +98,10 @@ struct gh_tuning {
   // staged per wave in an LDS ring filled by global-to-LDS loads, each block
   // running through consecutive chunks of its group
   int batch_mac_ring = 4;
+  // witnesses per group of the q32 ring MAC (4 or 8; Goldilocks rings use 4):
+  // 8 halves the key reads (same-shape batches 94 k against 87 k commits/s)
+  // but halves the blocks of small batches (inner commitments 53 against 41 us)
+  int q32_ring_group = 4;
   // gh_negacyclic_mul: 0 = one pair per thread (the first operand's transform
   // parked in LDS), 1 = one pair per quad of lanes, -1 = the faster per field
   // (per thread for q < 2^32, per quad for Goldilocks)

@@ -21,10 +21,11 @@ def ring_for(name, ring_q32, ring_gold):
     return ring_q32 if name == "q32" else ring_gold
 
 
-def _batch(ctx, key, ws, lens, ring, cols):
+def _batch(ctx, key, ws, lens, ring, cols, group=8):
     ring_field = key.ring
     saved = ctx.get_tuning()
-    ctx.set_tuning(batch_mac_ring=ring, batch_chunk_cols=cols, batch_chunk_cols_gold=cols)
+    ctx.set_tuning(batch_mac_ring=ring, batch_chunk_cols=cols, batch_chunk_cols_gold=cols,
+                   q32_ring_group=group)
     bufs = [ctx.to_device(ring_field.words(w)) for w in ws]
     wb = np.dtype(ring_field.dtype).itemsize
     outs = [ctx.alloc(key.rows * 64 * wb) for _ in ws]
@@ -34,6 +35,7 @@ def _batch(ctx, key, ws, lens, ring, cols):
         return [o.download(ring_field.dtype, (key.rows, 64)) for o in outs]
     finally:
         ctx.set_tuning(batch_mac_ring=saved["batch_mac_ring"],
+                       q32_ring_group=saved["q32_ring_group"],
                        batch_chunk_cols=saved["batch_chunk_cols"],
                        batch_chunk_cols_gold=saved["batch_chunk_cols_gold"])
         for b in bufs + outs:
@@ -53,12 +55,14 @@ def test_ring_mac_equals_register_mac(oracle, ctx, ring_q32, ring_gold, name, ro
     lens = [700, 1, 2, 3, 5, 63, 64, 65, 129, 300, 17]
     ws = [rand_residues(rng, p, (n, 64)) for n in lens]
     try:
+        groups = (4, 8) if name == "q32" else (4,)
         for cc in (16, 64, 256):
             base = _batch(ctx, key, ws, lens, 0, cc)
             for depth in (2, 3, 4):
-                got = _batch(ctx, key, ws, lens, depth, cc)
-                for i in range(len(lens)):
-                    assert np.array_equal(got[i], base[i]), (name, rows, cc, depth, i, lens[i])
+                for g in groups:
+                    got = _batch(ctx, key, ws, lens, depth, cc, g)
+                    for i in range(len(lens)):
+                        assert np.array_equal(got[i], base[i]), (name, rows, cc, depth, g, i, lens[i])
         z = np.zeros((rows, 1, 64), dtype=np.uint64)
         zr = np.zeros((1, 64), dtype=np.uint64)
         got = _batch(ctx, key, ws, lens, 4, 64)
@@ -75,6 +79,9 @@ def test_ring_tuning_values_checked(ctx):
         with pytest.raises(la.GridHipError):
             ctx.set_tuning(batch_mac_ring=bad)
     assert ctx.get_tuning()["batch_mac_ring"] in (0, 2, 3, 4)
+    for bad in (2, 6, 16):
+        with pytest.raises(la.GridHipError):
+            ctx.set_tuning(q32_ring_group=bad)
 
 
 @pytest.mark.parametrize("name", ["q32", "goldilocks"])
@@ -97,9 +104,10 @@ def test_ring_mac_chunk_runs(oracle, ctx, ring_q32, ring_gold, name):
             assert -(-8 * 2 * ctx.num_cus // groups) < -(-cols // cc)  # runs of several chunks
             base = _batch(ctx, key, ws, lens, 0, cc)
             for depth in (2, 4):
-                got = _batch(ctx, key, ws, lens, depth, cc)
-                for i in range(len(lens)):
-                    assert np.array_equal(got[i], base[i]), (name, cc, depth, i, lens[i])
+                for g in ((4, 8) if name == "q32" else (4,)):
+                    got = _batch(ctx, key, ws, lens, depth, cc, g)
+                    for i in range(len(lens)):
+                        assert np.array_equal(got[i], base[i]), (name, cc, depth, g, i, lens[i])
         z = np.zeros((rows, 1, 64), dtype=np.uint64)
         zr = np.zeros((1, 64), dtype=np.uint64)
         for i in (0, 1, 777):
