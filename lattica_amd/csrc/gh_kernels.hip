@@ -156,6 +156,35 @@ __global__ __launch_bounds__(256) void k_twisted_fwd_quad(F f, const typename F:
 // trade places word by word), so only one operand is ever in registers: about
 // 170 VGPRs for q < 2^32 against 270 with both held, which lets two waves
 // share each SIMD.
+// Twisted inverse NTT, one polynomial per quad of lanes (gh_ntt.h
+// quad_inv64): lane t loads the 16 contiguous slots 16*bitrev2(t) + k2 with
+// 16-byte loads (the layout k_twisted_fwd_quad stores) and stores
+// coefficients t + 4*i2 (stride-4 words).
+template <class F>
+__global__ __launch_bounds__(256) void k_twisted_inv_quad(F f, const typename F::W* __restrict__ tab,
+                                                          const typename F::W* src,
+                                                          typename F::W* dst, size_t count) {
+  using W = typename F::W;
+  constexpr int V = 16 / sizeof(W);
+  const int t = threadIdx.x & 3;
+  const size_t stride = ((size_t)gridDim.x * blockDim.x) >> 2;
+  for (size_t p = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 2; p < count; p += stride) {
+    W x[16];
+    const uint4* sp = reinterpret_cast<const uint4*>(src + p * 64 + 16 * bitrev2(t));
+#pragma unroll
+    for (int v = 0; v < 16 / V; v++) {
+      const uint4 o = sp[v];
+      const W* ow = reinterpret_cast<const W*>(&o);
+#pragma unroll
+      for (int c = 0; c < V; c++) x[v * V + c] = ow[c];
+    }
+    quad_inv64(f, x, tab, kTabUntwist, t);
+    W* dp = dst + p * 64 + t;
+#pragma unroll
+    for (int i = 0; i < 16; i++) dp[4 * i] = x[i];
+  }
+}
+
 template <class F, int TILE>
 __global__ __launch_bounds__(TILE) void k_negacyclic_mul(F f, const typename F::W* __restrict__ tab,
                                                          const typename F::W* a,
@@ -1116,7 +1145,15 @@ static int xform_impl(gh_ring* ring, F f, const void* src, void* dst, size_t cou
   gh_ctx* ctx = ring->ctx;
   const W* tab = (const W*)ring->dev_tab;
   constexpr int BIG = sizeof(W) == 4 ? 256 : 128;
-  if (MODE == kTwistedFwd && count <= (size_t)ctx->tune.quad_max) {
+  // four lanes per polynomial at every size by default (for Goldilocks a
+  // whole polynomial per thread needs 267 VGPRs, one wave per SIMD)
+  const int xq = ctx->tune.xform_quad;
+  const bool quad_all = xq == 1 || (xq < 0 && sizeof(W) == 8);
+  if (MODE == kTwistedInv && quad_all) {
+    unsigned grid = grid_for((count + 63) / 64, ctx->num_cus, 8);
+    hipLaunchKernelGGL((k_twisted_inv_quad<F>), dim3(grid), dim3(256), 0, ctx->stream, f, tab,
+                       (const W*)src, (W*)dst, count);
+  } else if (MODE == kTwistedFwd && (quad_all || count <= (size_t)ctx->tune.quad_max)) {
     // four lanes per polynomial: 64 polynomials per 256-thread block
     unsigned grid = grid_for((count + 63) / 64, ctx->num_cus, 8);
     hipLaunchKernelGGL((k_twisted_fwd_quad<F>), dim3(grid), dim3(256), 0, ctx->stream, f, tab,
