@@ -897,22 +897,27 @@ static int batch_enqueue(gh_key* key, hipStream_t stream, const BatchLayout& L, 
   return enqueue_batch_finalize(key->ring, stream, ddesc, n, key->rows, cc);
 }
 
-// Key columns per block of the grouped MAC: the field's tuned width, halved (not below
-// 32) while the grid would hold fewer blocks than half the device's CUs, so a
-// batch of short witnesses still spreads over the device.
-// The one-launch form uses the same rule on its column ranges, down to 16.
+// Key columns per block of the grouped MAC: the field's tuned width, halved
+// while the grid would hold fewer blocks than a target, so a batch of short
+// witnesses still spreads over the device. The target is half the device's
+// CUs, down to 32 columns; for the LDS-ring MACs (whose blocks run through
+// several chunks when there are many) it is two blocks per CU, down to 16.
+// The one-launch form uses the first rule on its column ranges.
 static uint32_t batch_chunk(gh_ctx* ctx, const gh_commit_item* items, size_t count,
                             uint32_t rows, int ring_kind) {
   uint32_t maxlen = 0;
   for (size_t i = 0; i < count; i++) maxlen = std::max(maxlen, items[i].len);
   const bool fused = batch_fused(ctx, ring_kind);
-  const uint32_t group = fused ? (uint32_t)kBfGroup : (uint32_t)ctx->tune.batch_group;
+  uint32_t group = fused ? (uint32_t)kBfGroup : (uint32_t)ctx->tune.batch_group;
+  const bool ring = !fused && rows % 16 == 0 && ctx->tune.batch_mac_ring >= 2 &&
+                    (ring_kind == GH_RING_Q32 || group == 4);
+  if (ring && ring_kind == GH_RING_Q32) group = ctx->tune.q32_ring_group == 8 ? 8 : 4;
   const uint64_t groups = (count + group - 1) / group;
   const uint64_t rblocks = rows >= 16 ? rows / 16 : 1;
   uint32_t cc = fused ? (uint32_t)ctx->tune.batch_fused_cols : tuned_batch_chunk(ctx->tune, ring_kind);
-  const uint32_t floor = 32u;
-  while (cc > floor &&
-         (uint64_t)((maxlen + cc - 1) / cc) * groups * rblocks < (uint64_t)ctx->num_cus / 2)
+  const uint32_t floor = ring ? 16u : 32u;
+  const uint64_t target = ring ? (uint64_t)ctx->num_cus * 2 : (uint64_t)ctx->num_cus / 2;
+  while (cc > floor && (uint64_t)((maxlen + cc - 1) / cc) * groups * rblocks < target)
     cc = std::max(floor, (cc / 2) & ~(floor - 1));
   return cc;
 }
