@@ -14,7 +14,8 @@ transform+MAC kernel, timed with HIP events on the library stream), the CPU
 baseline (the reference's AVX2 prepared-commit path restated in C, run on this
 host), and the side workloads, each checked against the oracle or the plain
 form: the batched form at the same shape, kappa 8 x n 256 commitments
-(`small`), 2^18 negacyclic products (`negacyclic`), LaBRADOR-scale inner
+(`small`), 2^18 twisted forward and inverse transforms in both fields (`ntt`),
+2^18 negacyclic products (`negacyclic`), LaBRADOR-scale inner
 commitments, Gram matrix, JL projection and the steps around them
 (`labrador`), 2000 Goldilocks commitments batched, graph-replayed and per call
 (`goldilocks`), the serving loop against the plain per-call form on a Poisson
@@ -239,6 +240,68 @@ def cpu_seconds(fn, min_s=0.5):
         n += 1
         if time.perf_counter() - a >= min_s:
             return (time.perf_counter() - a) / n
+
+
+def ntt_batch(ctx, la, log, count=1 << 18, reps=20):
+    """Batched twisted transforms (TwistedNttPlan forward and inverse,
+    poly/ntt.rs:75-104) of 2^18 polynomials in place on resident device
+    buffers, q32 and Goldilocks: one gh_twisted_forward and one
+    gh_twisted_inverse call each; a sample checked against the oracle and the
+    round trip checked whole. CPU: the reference's AVX2 transforms, restated,
+    on a sample."""
+    from oracle import gridref
+    L = la.lib()
+    res = {}
+    for name, q in (("q32", Q), ("goldilocks", la.GOLDILOCKS)):
+        r = la.Ring(ctx, q, D)
+        dt = np.uint32 if q < (1 << 32) else np.uint64
+        rng = np.random.default_rng(33)
+        h = (rng.integers(0, 1 << 62, size=(count, D), dtype=np.uint64) % np.uint64(q)).astype(dt)
+        buf = ctx.to_device(h)
+        L.gh_twisted_forward(r.h, buf.ptr, count, la.MEM_DEVICE)
+        ctx.synchronize()
+        fwd = buf.download(dt, (count, D))
+        idx = rng.choice(count, 64, replace=False)
+        ok = bool(np.array_equal(fwd[idx].astype(np.uint64),
+                                 gridref.twisted_forward(q, D, h[idx].astype(np.uint64))))
+        L.gh_twisted_inverse(r.h, buf.ptr, count, la.MEM_DEVICE)
+        ctx.synchronize()
+        ok = ok and bool(np.array_equal(buf.download(dt, (count, D)), h))
+        us = {}
+        for kind, fn in (("forward", L.gh_twisted_forward), ("inverse", L.gh_twisted_inverse)):
+            for _ in range(3):
+                fn(r.h, buf.ptr, count, la.MEM_DEVICE)
+            ctx.synchronize()
+            a = time.perf_counter()
+            for _ in range(reps):
+                fn(r.h, buf.ptr, count, la.MEM_DEVICE)
+            ctx.synchronize()
+            us[kind] = (time.perf_counter() - a) / reps * 1e6
+        buf.free()
+        r.close()
+        nbytes = 2 * count * D * np.dtype(dt).itemsize  # read + written
+        leg = {"polys": count, "forward_us": round(us["forward"], 1), "inverse_us": round(us["inverse"], 1),
+               "forward_GBps": round(nbytes / us["forward"] / 1e3, 1),
+               "inverse_GBps": round(nbytes / us["inverse"] / 1e3, 1), "oracle_sample_and_round_trip": ok}
+        log(f"twisted NTT {name} x{count}: forward {us['forward']:,.1f} us, inverse {us['inverse']:,.1f} us "
+            f"({nbytes / us['forward'] / 1e3:,.0f} / {nbytes / us['inverse'] / 1e3:,.0f} GB/s); oracle sample "
+            f"and round trip {ok}")
+        if CPU_ON:
+            cpu_n = 1 << 14
+            m = gridref.Avx2Twisted(q, 64)
+            m.load(h[:cpu_n].astype(np.uint64))
+            m.run(False)
+            f1 = cpu_n / once_seconds(lambda: m.run(False))
+            thr = cpu_threads()
+            fa = cpu_n / once_seconds(lambda: m.run(False, thr))
+            m.close()
+            log(f"  CPU AVX2 forward {f1 / 1e6:.3f} M polys/s on 1 core, {fa / 1e6:.3f} M on {thr}")
+            leg["cpu_1core"] = {"polys_per_s": round(f1, 1), "kind": "port",
+                                "path": "TwistedNttPlan forward through the AVX2 NTT, L = u64 (oracle/gridref_avx2.c)",
+                                "sample": f"{cpu_n} polynomials"}
+            leg["cpu_all_cores"] = {"polys_per_s": round(fa, 1), "threads": thr}
+        res[name] = leg
+    return res
 
 
 def negacyclic_batch(ctx, la, ring, log, pairs=1 << 18, reps=20):
@@ -871,8 +934,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-check", action="store_true")
-    ap.add_argument("--only", choices=["small", "negacyclic", "labrador", "goldilocks", "serving",
-                                       "wire", "hbm", "issue", "valu"],
+    ap.add_argument("--only", choices=["small", "ntt", "negacyclic", "labrador", "goldilocks",
+                                       "serving", "wire", "hbm", "issue", "valu"],
                     help="run one side workload alone (for profiling) and print its JSON")
     args = ap.parse_args()
     global CPU_ON
@@ -892,6 +955,7 @@ def main():
     ring = la.Ring(ctx, Q, D)
     if args.only:
         legs = {"small": lambda: commit_small(ctx, la, ring, log),
+                "ntt": lambda: ntt_batch(ctx, la, log),
                 "negacyclic": lambda: negacyclic_batch(ctx, la, ring, log),
                 "labrador": lambda: labrador_calls(ctx, la, ring, log),
                 "goldilocks": lambda: goldilocks_batch(ctx, la, log),
@@ -1047,6 +1111,7 @@ def main():
         if os.path.exists(vpath):
             with open(vpath) as fh:
                 valu = json.load(fh)
+        extra["twisted_ntt_2^18_polys"] = ntt_batch(ctx, la, log)
         extra["negacyclic_q32_2^18_pairs"] = negacyclic_batch(ctx, la, ring, log)
         if "negacyclic_q32_2^18_pairs" in valu:
             vp = valu["negacyclic_q32_2^18_pairs"]

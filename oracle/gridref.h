@@ -151,6 +151,8 @@ void gr_avx2_load_witness_len(void *h, const uint64_t *s, uint32_t n_use);
 /* negacyclic products of count pairs of d-coefficient polynomials (canonical
  * in and out; load converts once, run is the timed part) */
 int gr_avx2_negacyclic_load(void *h, const uint64_t *a, const uint64_t *b, size_t count);
+void gr_avx2_xform_run(void *h, int inverse, int threads);
+void gr_avx2_xform_result(void *h, uint64_t *out);
 void gr_avx2_negacyclic_run(void *h, int threads);
 void gr_avx2_negacyclic_result(void *h, uint64_t *out);
 /* sums over k of coefficient-domain negacyclic products x[tx[t]][k] * y[ty[t]][k]

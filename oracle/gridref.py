@@ -101,6 +101,10 @@ def lib():
         L.gr_avx2_negacyclic_run.argtypes = [ctypes.c_void_p, i32]
         L.gr_avx2_negacyclic_result.restype = None
         L.gr_avx2_negacyclic_result.argtypes = [ctypes.c_void_p, U64P]
+        L.gr_avx2_xform_run.restype = None
+        L.gr_avx2_xform_run.argtypes = [ctypes.c_void_p, i32, i32]
+        L.gr_avx2_xform_result.restype = None
+        L.gr_avx2_xform_result.argtypes = [ctypes.c_void_p, U64P]
         U32P = ctypes.POINTER(ctypes.c_uint32)
         L.gr_avx2_dots_load.restype = i32
         L.gr_avx2_dots_load.argtypes = [ctypes.c_void_p, U64P, ctypes.c_uint32, U64P,
@@ -482,6 +486,29 @@ class Avx2Dots:
 
     def __del__(self):
         self.close()
+
+
+class Avx2Twisted:
+    """The reference's twisted transforms (TwistedNttPlan, poly/ntt.rs:75-104)
+    through its AVX2 NTT kernels, restated, in place over a batch of
+    polynomials (CPU baseline)."""
+
+    def __init__(self, q, limb_bits, d=64):
+        self.inner = Avx2Negacyclic(q, limb_bits, d)
+
+    def load(self, x):
+        self.inner.load(x, x)
+
+    def run(self, inverse=False, threads=1):
+        lib().gr_avx2_xform_run(self.inner.h, 1 if inverse else 0, threads)
+
+    def result(self):
+        out = np.zeros((self.inner.count, self.inner.d), dtype=np.uint64)
+        lib().gr_avx2_xform_result(self.inner.h, _p(out))
+        return out
+
+    def close(self):
+        self.inner.close()
 
 
 class Avx2Negacyclic:

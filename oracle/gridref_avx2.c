@@ -769,6 +769,14 @@ static void *worker(void *arg) {
         }
         free(lhs);
         free(rhs);
+    } else if (j->phase == 5 || j->phase == 6) {
+        /* TwistedNttPlan::forward_in_place_trusted / inverse_in_place_trusted
+         * (poly/ntt.rs:75-104) in place on the first operand */
+        for (uint32_t p = j->lo; p < j->hi; p++) {
+            char *poly = (char *)c->pa + lb * d * p;
+            if (j->phase == 5) twisted_fwd(c, poly);
+            else twisted_inv(c, poly);
+        }
     } else if (j->phase == 0) {
         for (uint32_t p = j->lo; p < j->hi; p++) {
             char *dst = (char *)c->s_work + lb * d * p;
@@ -898,6 +906,17 @@ void gr_avx2_items_result(void *h, uint64_t *out) {
     avx_ctx_t *c = (avx_ctx_t *)h;
     for (size_t i = 0; i < (size_t)c->nitems * c->rows * c->d; i++)
         out[i] = fw_from(&c->f, get_w(c, c->iout, i));
+}
+
+/* Twisted forward (inverse = 0) or inverse transforms, in place, of the first
+ * operand loaded by gr_avx2_negacyclic_load. */
+void gr_avx2_xform_run(void *h, int inverse, int threads) {
+    avx_ctx_t *c = (avx_ctx_t *)h;
+    run_phase(c, inverse ? 6 : 5, (uint32_t)c->pairs, threads);
+}
+void gr_avx2_xform_result(void *h, uint64_t *out) {
+    avx_ctx_t *c = (avx_ctx_t *)h;
+    for (size_t i = 0; i < c->pairs * c->d; i++) out[i] = fw_from(&c->f, get_w(c, c->pa, i));
 }
 
 void gr_avx2_negacyclic_result(void *h, uint64_t *out) {

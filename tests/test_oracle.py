@@ -534,3 +534,21 @@ def test_slice_hooks_random_against_integers(oracle, q, n):
             got = oracle.slice_op(q, op, d, s, c)
             assert [int(v) for v in got] == _closed(q, op, dl, sl, c), (op, c)
     assert oracle.dot_product(q, d, s) == sum(x * y for x, y in zip(dl, sl)) % q
+
+
+def test_avx2_twisted_matches_oracle():
+    # the CPU baseline of the transform leg: the AVX2 restatement of
+    # TwistedNttPlan equals the scalar oracle, both limb widths and fields
+    import numpy as np
+    from oracle import gridref
+    rng = np.random.default_rng(77)
+    for q, limbs in ((4294966657, (64, 32)), (0xffffffff00000001, (64,))):
+        x = rng.integers(0, 1 << 63, size=(40, 64), dtype=np.uint64) % np.uint64(q)
+        for limb in limbs:
+            m = gridref.Avx2Twisted(q, limb)
+            m.load(x)
+            m.run(False)
+            assert np.array_equal(m.result(), gridref.twisted_forward(q, 64, x))
+            m.run(True, 2)
+            assert np.array_equal(m.result(), x)
+            m.close()
