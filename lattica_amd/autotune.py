@@ -175,10 +175,18 @@ def tune_batch(ctx, quick=False, log=print):
     rq.close()
     rg.close()
     best = None
+
+    def narrowest_near_best(rows, name, tol=1.02):
+        # among chunk widths within 2 % of the fastest, the narrowest: it gives
+        # small batches and the server's waves more blocks at the same speed
+        t = min(r["us"][name] for r in rows)
+        near = [r for r in rows if r["us"][name] <= t * tol]
+        return min(near, key=lambda r: r["config"]["batch_chunk_cols"])
+
     for g in groups:
         rows = [r for r in results if r["config"]["batch_group"] == g]
-        rq = min(rows, key=lambda r: r["us"]["q32_16x32768"])
-        rg_ = min(rows, key=lambda r: r["us"]["gold_c4_2000"])
+        rq = narrowest_near_best(rows, "q32_16x32768")
+        rg_ = narrowest_near_best(rows, "gold_c4_2000")
         total = rq["us"]["q32_16x32768"] + rg_["us"]["gold_c4_2000"]
         if best is None or total < best["total_us"]:
             best = {"config": {"batch_chunk_cols": rq["config"]["batch_chunk_cols"],
