@@ -143,12 +143,13 @@ GH_API int gh_ctx_profile_read(gh_ctx *ctx, int phase, double *total_ms, uint64_
  * (4, the default, or 8: witnesses sharing each key column in that ring MAC
  * for q32 keys; Goldilocks keys use 4) and
  * "negacyclic_quad" (-1, the default: gh_negacyclic_mul runs one pair per
- * thread for q < 2^32 and one pair per quad of lanes for Goldilocks; 0 and 1:
- * per thread or per quad for both) and "xform_quad" (1, the default: the
+ * quad of lanes, or for q < 2^32 one pair per thread when "quad_lds" is 0;
+ * 0 and 1: per thread or per quad for both) and "xform_quad" (1, the default: the
  * twisted transforms run four lanes per polynomial at every size; 0: the
  * forward does up to "quad_max" polynomials that way and the inverse a
- * polynomial per thread; -1: 1 for Goldilocks, 0 for q < 2^32). Values are
- * checked;
+ * polynomial per thread; -1: 1 for Goldilocks, 0 for q < 2^32) and "quad_lds"
+ * (1, the default: the four-lane transforms finish their 4-point DFT over the
+ * lanes through LDS; 0: with lane exchanges). Values are checked;
  * an unknown key or value gives GH_E_INVALID_ARGUMENT. Every setting yields
  * identical results; only the speed changes. */
 GH_API int gh_ctx_set_tuning(gh_ctx *ctx, const char *key, int64_t value);

@@ -275,7 +275,7 @@ class Context:
     TUNING_KEYS = ("mac_variant", "mac_waves", "mac_blocks_per_cu", "quad_max", "fused", "fused_depth",
                    "batch_chunk_cols", "batch_chunk_cols_gold", "batch_group", "defer_finalize",
                    "stage1_tasks_per_block", "batch_fused", "batch_fused_cols", "batch_mac_ring",
-                   "negacyclic_quad", "q32_ring_group", "xform_quad")
+                   "negacyclic_quad", "q32_ring_group", "quad_lds", "xform_quad")
 
     def set_tuning(self, **kv):
         for k, v in kv.items():

@@ -32,7 +32,7 @@
 
 namespace gh {
 
-template <class F>
+template <class F, bool LX>
 __global__ __launch_bounds__(256, 4) void k_batch_fwd_quad(F f, const typename F::W* __restrict__ tab,
                                                         const BatchDesc* __restrict__ desc,
                                                         const BatchSeg* __restrict__ seg,
@@ -40,6 +40,7 @@ __global__ __launch_bounds__(256, 4) void k_batch_fwd_quad(F f, const typename F
   using W = typename F::W;
   constexpr int TW = F::kTwWords;
   constexpr int V = 16 / sizeof(W);
+  __shared__ W qlds[LX ? 64 * kQuadStride<W> : 1];
   const int t = threadIdx.x & 3;
   W ltw[16][TW];
 #pragma unroll
@@ -55,6 +56,11 @@ __global__ __launch_bounds__(256, 4) void k_batch_fwd_quad(F f, const typename F
     const W* sp = reinterpret_cast<const W*>(d.src) + (size_t)p * 64 + t;
 #pragma unroll
     for (int i = 0; i < 16; i++) x[i] = sp[4 * i];
+    if constexpr (LX) {
+      quad_fwd64_lds(f, x, tab, ltw, t, qlds + (threadIdx.x >> 2) * kQuadStride<W>);
+      quad_store_k1(reinterpret_cast<W*>(d.s_ntt) + (size_t)p * 64, x, t);
+      continue;
+    }
     quad_fwd64(f, x, tab, ltw, t);
     uint4* dp = reinterpret_cast<uint4*>(reinterpret_cast<W*>(d.s_ntt) + (size_t)p * 64 +
                                          16 * bitrev2(t));
@@ -715,12 +721,19 @@ int enqueue_batch_xform(gh_ring* ring, hipStream_t stream, const BatchDesc* desc
   if (nseg == 0) return GH_OK;
   const int cus = ring->ctx->num_cus;
   const unsigned grid = nseg < (uint32_t)cus * 16 ? nseg : (unsigned)cus * 16;
-  if (ring->kind == GH_RING_Q32)
-    hipLaunchKernelGGL((k_batch_fwd_quad<FQ32>), dim3(grid), dim3(256), 0, stream,
-                       FQ32{(uint32_t)ring->q, ring->qinv32}, (const uint32_t*)ring->dev_tab, desc,
-                       seg, nseg);
+  const FQ32 fq{(uint32_t)ring->q, ring->qinv32};
+  const bool lx = ring->ctx->tune.quad_lds != 0;
+  if (ring->kind == GH_RING_Q32 && lx)
+    hipLaunchKernelGGL((k_batch_fwd_quad<FQ32, true>), dim3(grid), dim3(256), 0, stream, fq,
+                       (const uint32_t*)ring->dev_tab, desc, seg, nseg);
+  else if (ring->kind == GH_RING_Q32)
+    hipLaunchKernelGGL((k_batch_fwd_quad<FQ32, false>), dim3(grid), dim3(256), 0, stream, fq,
+                       (const uint32_t*)ring->dev_tab, desc, seg, nseg);
+  else if (lx)
+    hipLaunchKernelGGL((k_batch_fwd_quad<FGold, true>), dim3(grid), dim3(256), 0, stream, FGold{},
+                       (const uint64_t*)ring->dev_tab, desc, seg, nseg);
   else
-    hipLaunchKernelGGL((k_batch_fwd_quad<FGold>), dim3(grid), dim3(256), 0, stream, FGold{},
+    hipLaunchKernelGGL((k_batch_fwd_quad<FGold, false>), dim3(grid), dim3(256), 0, stream, FGold{},
                        (const uint64_t*)ring->dev_tab, desc, seg, nseg);
   GH_HIP(hipGetLastError());
   return GH_OK;

@@ -330,6 +330,9 @@ GH_API int gh_ctx_set_tuning(gh_ctx* ctx, const char* key, int64_t value) {
   } else if (!std::strcmp(key, "xform_quad")) {
     if (value < -1 || value > 1) return GH_E_INVALID_ARGUMENT;
     t.xform_quad = (int)value;
+  } else if (!std::strcmp(key, "quad_lds")) {
+    if (value < 0 || value > 1) return GH_E_INVALID_ARGUMENT;
+    t.quad_lds = (int)value;
   } else if (!std::strcmp(key, "negacyclic_quad")) {
     if (value < -1 || value > 1) return GH_E_INVALID_ARGUMENT;
     t.negacyclic_quad = (int)value;
@@ -362,6 +365,7 @@ GH_API int gh_ctx_get_tuning(gh_ctx* ctx, const char* key, int64_t* value) {
   else if (!std::strcmp(key, "batch_mac_ring")) *value = t.batch_mac_ring;
   else if (!std::strcmp(key, "negacyclic_quad")) *value = t.negacyclic_quad;
   else if (!std::strcmp(key, "q32_ring_group")) *value = t.q32_ring_group;
+  else if (!std::strcmp(key, "quad_lds")) *value = t.quad_lds;
   else if (!std::strcmp(key, "xform_quad")) *value = t.xform_quad;
   else return GH_E_INVALID_ARGUMENT;
   return GH_OK;

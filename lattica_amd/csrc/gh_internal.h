@@ -104,12 +104,15 @@ struct gh_tuning {
   int q32_ring_group = 4;
   // gh_negacyclic_mul: 0 = one pair per thread (the first operand's transform
   // parked in LDS), 1 = one pair per quad of lanes, -1 = the faster per field
-  // (per thread for q < 2^32, per quad for Goldilocks)
+  // (per quad, except per thread for q < 2^32 when quad_lds is 0)
   int negacyclic_quad = -1;
   // twisted transforms of gh_twisted_forward/inverse: 1 = four lanes per
   // polynomial at every size, 0 = forward four lanes up to quad_max and
   // inverse a polynomial per thread, -1 = 1 for Goldilocks and 0 for q < 2^32
   int xform_quad = 1;
+  // quad transforms finish their 4-point DFT through LDS (1) or with lane
+  // exchanges (0)
+  int quad_lds = 1;
 };
 // most stage-1 tasks one dedicated block runs
 constexpr uint32_t kStage1MaxTpb = 8;

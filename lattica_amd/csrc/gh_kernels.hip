@@ -116,16 +116,18 @@ __global__ __launch_bounds__(TILE) void k_xform(F f, const typename F::W* __rest
   }
 }
 
-// Twisted forward NTT, one polynomial per quad of lanes (gh_ntt.h quad_fwd64).
-// Loads are stride-4 words (the quad together reads whole 64-byte segments);
-// each lane stores its 16 contiguous output slots with 16-byte stores.
-template <class F>
+// Twisted forward NTT, one polynomial per quad of lanes (gh_ntt.h quad_fwd64,
+// or quad_fwd64_lds for LX). Loads are stride-4 words (the quad together reads
+// whole 64-byte segments); each lane stores its 16 output slots with 16-byte
+// stores (LX: slots 16*k1 + 4*t .. +3).
+template <class F, bool LX>
 __global__ __launch_bounds__(256) void k_twisted_fwd_quad(F f, const typename F::W* __restrict__ tab,
                                                           const typename F::W* src,
                                                           typename F::W* dst, size_t count) {
   using W = typename F::W;
   constexpr int TW = F::kTwWords;
   constexpr int V = 16 / sizeof(W);
+  __shared__ W qlds[LX ? 64 * kQuadStride<W> : 1];
   const int t = threadIdx.x & 3;
   W ltw[16][TW];
 #pragma unroll
@@ -138,15 +140,20 @@ __global__ __launch_bounds__(256) void k_twisted_fwd_quad(F f, const typename F:
     const W* sp = src + p * 64 + t;
 #pragma unroll
     for (int i = 0; i < 16; i++) x[i] = sp[4 * i];
-    quad_fwd64(f, x, tab, ltw, t);
-    uint4* dp = reinterpret_cast<uint4*>(dst + p * 64 + 16 * bitrev2(t));
+    if constexpr (LX) {
+      quad_fwd64_lds(f, x, tab, ltw, t, qlds + (threadIdx.x >> 2) * kQuadStride<W>);
+      quad_store_k1(dst + p * 64, x, t);
+    } else {
+      quad_fwd64(f, x, tab, ltw, t);
+      uint4* dp = reinterpret_cast<uint4*>(dst + p * 64 + 16 * bitrev2(t));
 #pragma unroll
-    for (int v = 0; v < 16 / V; v++) {
-      uint4 o;
-      W* ow = reinterpret_cast<W*>(&o);
+      for (int v = 0; v < 16 / V; v++) {
+        uint4 o;
+        W* ow = reinterpret_cast<W*>(&o);
 #pragma unroll
-      for (int c = 0; c < V; c++) ow[c] = x[v * V + c];
-      dp[v] = o;
+        for (int c = 0; c < V; c++) ow[c] = x[v * V + c];
+        dp[v] = o;
+      }
     }
   }
 }
@@ -158,18 +165,27 @@ __global__ __launch_bounds__(256) void k_twisted_fwd_quad(F f, const typename F:
 // share each SIMD.
 // Twisted inverse NTT, one polynomial per quad of lanes (gh_ntt.h
 // quad_inv64): lane t loads the 16 contiguous slots 16*bitrev2(t) + k2 with
-// 16-byte loads (the layout k_twisted_fwd_quad stores) and stores
+// 16-byte loads (LX, quad_inv64_lds: slots 16*k1 + 4*t .. +3) and stores
 // coefficients t + 4*i2 (stride-4 words).
-template <class F>
+template <class F, bool LX>
 __global__ __launch_bounds__(256) void k_twisted_inv_quad(F f, const typename F::W* __restrict__ tab,
                                                           const typename F::W* src,
                                                           typename F::W* dst, size_t count) {
   using W = typename F::W;
   constexpr int V = 16 / sizeof(W);
+  __shared__ W qlds[LX ? 64 * kQuadStride<W> : 1];
   const int t = threadIdx.x & 3;
   const size_t stride = ((size_t)gridDim.x * blockDim.x) >> 2;
   for (size_t p = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 2; p < count; p += stride) {
     W x[16];
+    if constexpr (LX) {
+      quad_load_k1(src + p * 64, x, t);
+      quad_inv64_lds(f, x, tab, kTabUntwist, t, qlds + (threadIdx.x >> 2) * kQuadStride<W>);
+      W* dp = dst + p * 64 + t;
+#pragma unroll
+      for (int i = 0; i < 16; i++) dp[4 * i] = x[i];
+      continue;
+    }
     const uint4* sp = reinterpret_cast<const uint4*>(src + p * 64 + 16 * bitrev2(t));
 #pragma unroll
     for (int v = 0; v < 16 / V; v++) {
@@ -232,13 +248,14 @@ __global__ __launch_bounds__(TILE) void k_negacyclic_mul(F f, const typename F::
 // quad_inv64): no LDS and 16 coefficients of each operand per lane, so more
 // waves fit on a SIMD than with a whole polynomial per thread. Loads and
 // stores are stride-4 words; a quad together covers whole 64-byte segments.
-template <class F>
+template <class F, bool LX>
 __global__ __launch_bounds__(256) void k_negacyclic_quad(F f, const typename F::W* __restrict__ tab,
                                                          const typename F::W* a,
                                                          const typename F::W* b,
                                                          typename F::W* out, size_t count) {
   using W = typename F::W;
   constexpr int TW = F::kTwWords;
+  __shared__ W qlds[LX ? 64 * kQuadStride<W> : 1];
   const int t = threadIdx.x & 3;
   W ltw[16][TW];
 #pragma unroll
@@ -254,11 +271,20 @@ __global__ __launch_bounds__(256) void k_negacyclic_quad(F f, const typename F::
     for (int i = 0; i < 16; i++) x[i] = ap[4 * i];
 #pragma unroll
     for (int i = 0; i < 16; i++) y[i] = bp[4 * i];
-    quad_fwd64(f, x, tab, ltw, t);
-    quad_fwd64(f, y, tab, ltw, t);
+    if constexpr (LX) {
+      W* sc = qlds + (threadIdx.x >> 2) * kQuadStride<W>;
+      quad_fwd64_lds(f, x, tab, ltw, t, sc);
+      quad_fwd64_lds(f, y, tab, ltw, t, sc);
 #pragma unroll
-    for (int k = 0; k < 16; k++) x[k] = f.mont(x[k], y[k]);
-    quad_inv64(f, x, tab, kTabUntwistMul, t);
+      for (int k = 0; k < 16; k++) x[k] = f.mont(x[k], y[k]);
+      quad_inv64_lds(f, x, tab, kTabUntwistMul, t, sc);
+    } else {
+      quad_fwd64(f, x, tab, ltw, t);
+      quad_fwd64(f, y, tab, ltw, t);
+#pragma unroll
+      for (int k = 0; k < 16; k++) x[k] = f.mont(x[k], y[k]);
+      quad_inv64(f, x, tab, kTabUntwistMul, t);
+    }
     W* op = out + p * 64 + t;
 #pragma unroll
     for (int i = 0; i < 16; i++) op[4 * i] = x[i];
@@ -635,8 +661,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_q32(
 // reads the column's transformed witness from LDS; PF = 8 * DEPTH columns are
 // kept in flight per wave. The transform, the held-back stages and the
 // clamped prefetch are those of k_commit_fused_q32, as are the results.
-// NC > 0: every block owns exactly NC columns (fully unrolled sweep).
-template <int DEPTH, int NC>
+// NC > 0: every block owns exactly NC columns (fully unrolled sweep). LX: the
+// transform's 4-point DFT over the quad goes through the polynomial's LDS area
+// (quad_fwd64_lds) instead of lane exchanges.
+template <int DEPTH, int NC, bool LX>
 __global__ __launch_bounds__(256) void k_commit_rows_q32(
     FQ32 f, const uint32_t* __restrict__ tab, const uint32_t* __restrict__ A,
     const uint32_t* __restrict__ s_coeff, uint32_t* __restrict__ partial, uint32_t rows,
@@ -696,21 +724,29 @@ __global__ __launch_bounds__(256) void k_commit_rows_q32(
   u32x4 pa[PF];
 #pragma unroll
   for (int q = 0; q < PF; q++) pa[q] = __builtin_nontemporal_load(akey + (size_t)(ch + col_of(q)) * col4);
-  // (c) witness transform into LDS while (b) is in flight
-  quad_fwd64(f, x, tab, ltw, t);
-  if (p0 < ncol) {
-    u32x4* dp = reinterpret_cast<u32x4*>(lds + p0 * kFuseStride + 16 * bitrev2(t));
+  // (c) witness transform into LDS while (b) is in flight (a quad's lanes
+  // share p, so a clamped quad skips as a whole)
+  auto transform_store = [&](uint32_t p) {
+    uint32_t* sc = lds + p * kFuseStride;
+    if constexpr (LX) {
+      quad_fwd64_lds(f, x, tab, ltw, t, sc);
 #pragma unroll
-    for (int v = 0; v < 4; v++) dp[v] = u32x4{x[4 * v], x[4 * v + 1], x[4 * v + 2], x[4 * v + 3]};
-  }
+      for (int k1 = 0; k1 < 4; k1++)
+        *reinterpret_cast<u32x4*>(sc + 16 * k1 + 4 * t) =
+            u32x4{x[4 * k1], x[4 * k1 + 1], x[4 * k1 + 2], x[4 * k1 + 3]};
+    } else {
+      quad_fwd64(f, x, tab, ltw, t);
+      u32x4* dp = reinterpret_cast<u32x4*>(sc + 16 * bitrev2(t));
+#pragma unroll
+      for (int v = 0; v < 4; v++) dp[v] = u32x4{x[4 * v], x[4 * v + 1], x[4 * v + 2], x[4 * v + 3]};
+    }
+  };
+  if (p0 < ncol) transform_store(p0);
   for (uint32_t p = p0 + 16 * WAVES; p < ncol; p += 16 * WAVES) {
     const uint32_t* sp = s_coeff + (size_t)(ch + p) * 64 + t;
 #pragma unroll
     for (int i = 0; i < 16; i++) x[i] = sp[4 * i];
-    quad_fwd64(f, x, tab, ltw, t);
-    u32x4* dp = reinterpret_cast<u32x4*>(lds + p * kFuseStride + 16 * bitrev2(t));
-#pragma unroll
-    for (int v = 0; v < 4; v++) dp[v] = u32x4{x[4 * v], x[4 * v + 1], x[4 * v + 2], x[4 * v + 3]};
+    transform_store(p);
   }
   uint32_t* l1lds = lds + cols_per_block * kFuseStride;
   stage1_wave_sum(f, l1v, l1lds);
@@ -762,7 +798,7 @@ __global__ __launch_bounds__(256) void k_commit_rows_q32(
 constexpr int kFuseChunkGold = 64;
 constexpr int kFuseStrideGold = 66;  // LDS row stride of the transformed witness, words
 
-template <int RB, int WAVES>
+template <int RB, int WAVES, bool LX>
 __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_gold(
     FGold f, const uint64_t* __restrict__ tab, const uint64_t* __restrict__ A,
     const uint64_t* __restrict__ s_coeff, uint64_t* __restrict__ partial, uint32_t rows,
@@ -802,10 +838,16 @@ __global__ __launch_bounds__(64 * WAVES) void k_commit_fused_gold(
     const uint64_t* sp = s_coeff + (size_t)(ch + p) * 64 + t;
 #pragma unroll
     for (int i = 0; i < 16; i++) x[i] = sp[4 * i];
-    quad_fwd64(f, x, tab, ltw, t);
-    uint64_t* dp = lds64 + p * kFuseStrideGold + 16 * bitrev2(t);
+    uint64_t* sc = lds64 + p * kFuseStrideGold;
+    if constexpr (LX) {
+      quad_fwd64_lds(f, x, tab, ltw, t, sc);
+      quad_store_k1(sc, x, t);
+    } else {
+      quad_fwd64(f, x, tab, ltw, t);
+      uint64_t* dp = sc + 16 * bitrev2(t);
 #pragma unroll
-    for (int v = 0; v < 16; v++) dp[v] = x[v];
+      for (int v = 0; v < 16; v++) dp[v] = x[v];
+    }
   }
   uint64_t* l1lds = lds64 + cols_per_block * kFuseStrideGold;
   stage1_wave_sum(f, l1v, l1lds);
@@ -1151,13 +1193,21 @@ static int xform_impl(gh_ring* ring, F f, const void* src, void* dst, size_t cou
   const bool quad_all = xq == 1 || (xq < 0 && sizeof(W) == 8);
   if (MODE == kTwistedInv && quad_all) {
     unsigned grid = grid_for((count + 63) / 64, ctx->num_cus, 8);
-    hipLaunchKernelGGL((k_twisted_inv_quad<F>), dim3(grid), dim3(256), 0, ctx->stream, f, tab,
-                       (const W*)src, (W*)dst, count);
+    if (ctx->tune.quad_lds)
+      hipLaunchKernelGGL((k_twisted_inv_quad<F, true>), dim3(grid), dim3(256), 0, ctx->stream, f,
+                         tab, (const W*)src, (W*)dst, count);
+    else
+      hipLaunchKernelGGL((k_twisted_inv_quad<F, false>), dim3(grid), dim3(256), 0, ctx->stream, f,
+                         tab, (const W*)src, (W*)dst, count);
   } else if (MODE == kTwistedFwd && (quad_all || count <= (size_t)ctx->tune.quad_max)) {
     // four lanes per polynomial: 64 polynomials per 256-thread block
     unsigned grid = grid_for((count + 63) / 64, ctx->num_cus, 8);
-    hipLaunchKernelGGL((k_twisted_fwd_quad<F>), dim3(grid), dim3(256), 0, ctx->stream, f, tab,
-                       (const W*)src, (W*)dst, count);
+    if (ctx->tune.quad_lds)
+      hipLaunchKernelGGL((k_twisted_fwd_quad<F, true>), dim3(grid), dim3(256), 0, ctx->stream, f,
+                         tab, (const W*)src, (W*)dst, count);
+    else
+      hipLaunchKernelGGL((k_twisted_fwd_quad<F, false>), dim3(grid), dim3(256), 0, ctx->stream, f,
+                         tab, (const W*)src, (W*)dst, count);
   } else if (count >= (size_t)BIG * ctx->num_cus * 2) {
     unsigned grid = grid_for((count + BIG - 1) / BIG, ctx->num_cus, 2);
     hipLaunchKernelGGL((k_xform<F, MODE, BIG>), dim3(grid), dim3(BIG), 0, ctx->stream, f, tab,
@@ -1196,11 +1246,15 @@ static int negacyclic_impl(gh_ring* ring, F f, const void* a, const void* b, voi
   gh_ctx* ctx = ring->ctx;
   const W* tab = (const W*)ring->dev_tab;
   const int quad = ctx->tune.negacyclic_quad;
-  if (quad == 1 || (quad < 0 && sizeof(W) == 8)) {
+  if (quad == 1 || (quad < 0 && (sizeof(W) == 8 || ctx->tune.quad_lds))) {
     // 64 pairs per 256-thread block
     unsigned grid = grid_for((count + 63) / 64, ctx->num_cus, 8);
-    hipLaunchKernelGGL((k_negacyclic_quad<F>), dim3(grid), dim3(256), 0, ctx->stream, f, tab,
-                       (const W*)a, (const W*)b, (W*)out, count);
+    if (ctx->tune.quad_lds)
+      hipLaunchKernelGGL((k_negacyclic_quad<F, true>), dim3(grid), dim3(256), 0, ctx->stream, f,
+                         tab, (const W*)a, (const W*)b, (W*)out, count);
+    else
+      hipLaunchKernelGGL((k_negacyclic_quad<F, false>), dim3(grid), dim3(256), 0, ctx->stream, f,
+                         tab, (const W*)a, (const W*)b, (W*)out, count);
     GH_HIP(hipGetLastError());
     return GH_OK;
   }
@@ -1342,17 +1396,17 @@ static void fused_launch(gh_ctx* ctx, dim3 grid, size_t lds_bytes, hipEvent_t e0
   }
 }
 
-template <int DEPTH>
+template <int DEPTH, bool LX>
 static void rows_launch_nc(gh_ctx* ctx, dim3 grid, size_t lds_bytes, hipEvent_t e0, hipEvent_t e1,
                            FQ32 f, const uint32_t* tab, const uint32_t* A, const uint32_t* S,
                            uint32_t* P, uint32_t rows, uint32_t cols, uint32_t cpb,
                            const FinStage1& s1, const FinStage2& s2) {
   // the unrolled sweep when every block owns 64 columns
   if (cpb == 64 && cols % 64 == 0)
-    hipExtLaunchKernelGGL((k_commit_rows_q32<DEPTH, 64>), grid, dim3(256), lds_bytes, ctx->stream,
+    hipExtLaunchKernelGGL((k_commit_rows_q32<DEPTH, 64, LX>), grid, dim3(256), lds_bytes, ctx->stream,
                           e0, e1, 0, f, tab, A, S, P, rows, cols, cpb, s1, s2);
   else
-    hipExtLaunchKernelGGL((k_commit_rows_q32<DEPTH, 0>), grid, dim3(256), lds_bytes, ctx->stream,
+    hipExtLaunchKernelGGL((k_commit_rows_q32<DEPTH, 0, LX>), grid, dim3(256), lds_bytes, ctx->stream,
                           e0, e1, 0, f, tab, A, S, P, rows, cols, cpb, s1, s2);
 }
 
@@ -1360,12 +1414,15 @@ static void rows_launch(gh_ctx* ctx, dim3 grid, size_t lds_bytes, hipEvent_t e0,
                         FQ32 f, const uint32_t* tab, const uint32_t* A, const uint32_t* S,
                         uint32_t* P, uint32_t rows, uint32_t cols, uint32_t cpb,
                         const FinStage1& s1, const FinStage2& s2) {
+#define GH_ROWS(D, L) rows_launch_nc<D, L>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, s1, s2)
+  const bool lx = ctx->tune.quad_lds != 0;
   switch (ctx->tune.fused_depth) {
-    case 1: rows_launch_nc<1>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, s1, s2); break;
-    case 3: rows_launch_nc<3>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, s1, s2); break;
-    case 4: rows_launch_nc<4>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, s1, s2); break;
-    default: rows_launch_nc<2>(ctx, grid, lds_bytes, e0, e1, f, tab, A, S, P, rows, cols, cpb, s1, s2); break;
+    case 1: if (lx) GH_ROWS(1, true); else GH_ROWS(1, false); break;
+    case 3: if (lx) GH_ROWS(3, true); else GH_ROWS(3, false); break;
+    case 4: if (lx) GH_ROWS(4, true); else GH_ROWS(4, false); break;
+    default: if (lx) GH_ROWS(2, true); else GH_ROWS(2, false); break;
   }
+#undef GH_ROWS
 }
 
 // Standalone finalize of up to two commitments of one ring whose sums are
@@ -1630,12 +1687,15 @@ static int launch_commit_fused_gold(gh_key* key, const void* s_dev, uint32_t col
   const size_t lds_bytes = lds_words * 8;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (!scratch) prof_pair(ctx, 1, &e0, &e1);
-#define GH_FUSED_GOLD(R)                                                                        \
-  hipExtLaunchKernelGGL((k_commit_fused_gold<R, kWaves>), grid, dim3(64 * kWaves), lds_bytes,   \
-                        ctx->stream, e0, e1, 0, FGold{}, tab, A, S, P, rows, cols, cpb, plan.s1, \
+#define GH_FUSED_GOLD(R, L)                                                                     \
+  hipExtLaunchKernelGGL((k_commit_fused_gold<R, kWaves, L>), grid, dim3(64 * kWaves), lds_bytes, \
+                        ctx->stream, e0, e1, 0, FGold{}, tab, A, S, P, rows, cols, cpb, plan.s1,  \
                         plan.s2)
-  if (rb == 8) GH_FUSED_GOLD(8);
-  else GH_FUSED_GOLD(4);
+  const bool lx = ctx->tune.quad_lds != 0;
+  if (rb == 8 && lx) GH_FUSED_GOLD(8, true);
+  else if (rb == 8) GH_FUSED_GOLD(8, false);
+  else if (lx) GH_FUSED_GOLD(4, true);
+  else GH_FUSED_GOLD(4, false);
 #undef GH_FUSED_GOLD
   GH_HIP(hipGetLastError());
   return fused_finish(ctx, ring, plan, nparts, rows, t_dev, defer, scratch);

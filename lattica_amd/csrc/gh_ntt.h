@@ -158,15 +158,17 @@ __device__ __forceinline__ uint64_t quad_dpp(uint64_t v) {
 constexpr int kDppXor1 = 0xB1;  // quad_perm [1, 0, 3, 2]
 constexpr int kDppXor2 = 0x4E;  // quad_perm [2, 3, 0, 1]
 
-// In: x[i2] = a[t + 4*i2]. Out: x[k2] = X[16*bitrev2(t) + k2].
+// The lanes' part of the quad transform. In: x[i2] = a[t + 4*i2]. Out:
+// y[k2] = psi^(t(2k2+1)) * sum_i2 a_(t+4i2) (psi^4)^(i2(2k2+1)), the inner
+// 16-point twisted NTT times the lane twiddle.
 // ltw: this lane's psi^(t(2k2+1)) table entries, k2 = 0..15.
 template <class F>
-__device__ __forceinline__ void quad_fwd64(const F& f, typename F::W (&x)[16],
-                                           const typename F::W* __restrict__ tab,
-                                           const typename F::W (&ltw)[16][F::kTwWords], int t) {
+__device__ __forceinline__ void quad_lane16(const F& f, const typename F::W (&x)[16],
+                                            typename F::W (&y)[16],
+                                            const typename F::W* __restrict__ tab,
+                                            const typename F::W (&ltw)[16][F::kTwWords]) {
   using W = typename F::W;
   constexpr int TW = F::kTwWords;
-  W y[16];
 #pragma unroll
   for (int i = 0; i < 16; i++) {
     W v = x[i];
@@ -200,6 +202,26 @@ __device__ __forceinline__ void quad_fwd64(const F& f, typename F::W (&x)[16],
   }
 #pragma unroll
   for (int k = 0; k < 16; k++) y[k] = f.mul_tw(y[k], ltw[k]);
+}
+
+// v * iota, iota = psi^32
+template <class F>
+__device__ __forceinline__ typename F::W mul_iota(const F& f, typename F::W v,
+                                                  const typename F::W* __restrict__ tab) {
+  if constexpr (F::kPow2Roots)  // psi^32 = omega^16
+    return f.mul_pow2(v, (16 * F::kOmegaLog2) % 192);
+  else
+    return f.mul_tw(v, tab + (kTabTwist + 32) * F::kTwWords);
+}
+
+// In: x[i2] = a[t + 4*i2]. Out: x[k2] = X[16*bitrev2(t) + k2].
+template <class F>
+__device__ __forceinline__ void quad_fwd64(const F& f, typename F::W (&x)[16],
+                                           const typename F::W* __restrict__ tab,
+                                           const typename F::W (&ltw)[16][F::kTwWords], int t) {
+  using W = typename F::W;
+  W y[16];
+  quad_lane16(f, x, y, tab, ltw);
   // 4-point DFT over t: E/O sums across lanes t, t^2 ...
 #pragma unroll
   for (int k = 0; k < 16; k++) {
@@ -209,12 +231,7 @@ __device__ __forceinline__ void quad_fwd64(const F& f, typename F::W (&x)[16],
   // ... lane 3 holds C1 - C3 and takes the factor iota ...
   if (t == 3) {
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-      if constexpr (F::kPow2Roots)  // psi^32 = omega^16
-        y[k] = f.mul_pow2(y[k], (16 * F::kOmegaLog2) % 192);
-      else
-        y[k] = f.mul_tw(y[k], tab + (kTabTwist + 32) * TW);
-    }
+    for (int k = 0; k < 16; k++) y[k] = mul_iota(f, y[k], tab);
   }
   // ... and t, t^1 combine: lanes 0..3 end with Y_0, Y_2, Y_1, Y_3.
 #pragma unroll
@@ -223,6 +240,73 @@ __device__ __forceinline__ void quad_fwd64(const F& f, typename F::W (&x)[16],
     x[k] = (t & 1) ? f.sub(o, y[k]) : f.add(y[k], o);
   }
 }
+
+// 16 bytes of words
+template <class W> struct Vec16;
+template <> struct Vec16<uint32_t> {
+  typedef uint32_t T __attribute__((ext_vector_type(4)));
+  static constexpr int N = 4;
+};
+template <> struct Vec16<uint64_t> {
+  typedef uint64_t T __attribute__((ext_vector_type(2)));
+  static constexpr int N = 2;
+};
+
+// quad_fwd64 with the 4-point DFT over t done through LDS instead of lane
+// exchanges: each lane writes its 16 values y_t[k2] to the polynomial's
+// 64-word area sc ([t][k2], 16-byte aligned), reads back the four lanes'
+// values for k2 = 4t .. 4t+3 and finishes those four outputs of each k1 on its
+// own (8 additions and one multiplication by iota per k2, where the exchange
+// form spends 32 selected additions and 16 multiplications per lane).
+// Out: x[4*k1 + u] = X[16*k1 + 4*t + u]. The caller may store the result over
+// sc: the wave's LDS operations complete in order.
+template <class F>
+__device__ __forceinline__ void quad_fwd64_lds(const F& f, typename F::W (&x)[16],
+                                               const typename F::W* __restrict__ tab,
+                                               const typename F::W (&ltw)[16][F::kTwWords],
+                                               int t, typename F::W* sc) {
+  using W = typename F::W;
+  using V = typename Vec16<W>::T;
+  constexpr int N = Vec16<W>::N;
+  W y[16];
+  quad_lane16(f, x, y, tab, ltw);
+#pragma unroll
+  for (int v = 0; v < 16 / N; v++) {
+    V o;
+#pragma unroll
+    for (int e = 0; e < N; e++) o[e] = y[N * v + e];
+    *reinterpret_cast<V*>(sc + 16 * t + N * v) = o;
+  }
+  // the quad's four lanes are in this wave: the reads below follow the
+  // writes above in the wave's LDS order; keep the compiler from moving them
+  __asm__ volatile("" ::: "memory");
+  W w[4][4];  // w[t'][u] = y_t'[4t + u]
+#pragma unroll
+  for (int tp = 0; tp < 4; tp++) {
+#pragma unroll
+    for (int v = 0; v < 4 / N; v++) {
+      const V o = *reinterpret_cast<const V*>(sc + 16 * tp + 4 * t + N * v);
+#pragma unroll
+      for (int e = 0; e < N; e++) w[tp][N * v + e] = o[e];
+    }
+  }
+  __asm__ volatile("" ::: "memory");
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    // X[k2 + 16 k1] = sum_t iota^(t k1) y_t[k2]
+    const W s0 = f.add(w[0][u], w[2][u]), s1 = f.sub(w[0][u], w[2][u]);
+    const W s2 = f.add(w[1][u], w[3][u]), s3 = mul_iota(f, f.sub(w[1][u], w[3][u]), tab);
+    x[u] = f.add(s0, s2);
+    x[8 + u] = f.sub(s0, s2);
+    x[4 + u] = f.add(s1, s3);
+    x[12 + u] = f.sub(s1, s3);
+  }
+}
+
+template <class F>
+__device__ __forceinline__ void quad_inv_lane16(const F& f, typename F::W (&x)[16],
+                                                const typename F::W* __restrict__ tab, int post,
+                                                int t);
 
 // Inverse of quad_fwd64 with the per-coefficient factor at `post`, for a
 // negacyclic product: in, lane t holds x[k2] = C[16*bitrev2(t) + k2]; out,
@@ -263,6 +347,17 @@ __device__ __forceinline__ void quad_inv64(const F& f, typename F::W (&x)[16],
     W o = quad_dpp<kDppXor2>(x[k]);
     x[k] = (t & 2) ? f.sub(o, x[k]) : f.add(x[k], o);
   }
+  quad_inv_lane16(f, x, tab, post, t);
+}
+
+// The lanes' part of the quad inverse. In: x[k2] = D_t[k2] =
+// sum_k1 iota^(-t*k1) C[k2 + 16*k1]. Out: x[i2] = c[t + 4*i2] * post[4*i2].
+template <class F>
+__device__ __forceinline__ void quad_inv_lane16(const F& f, typename F::W (&x)[16],
+                                                const typename F::W* __restrict__ tab, int post,
+                                                int t) {
+  using W = typename F::W;
+  constexpr int TW = F::kTwWords;
 #pragma unroll
   for (int k = 0; k < 16; k++) x[k] = f.mul_tw(x[k], tab + (kTabQuadInv + 16 * t + k) * TW);
   // 16-point DIF with the 64-point tables' stages of length 16, 8, 4, 2
@@ -291,6 +386,98 @@ __device__ __forceinline__ void quad_inv64(const F& f, typename F::W (&x)[16],
   for (int i = 0; i < 16; i++) y[i] = f.mul_tw(x[bitrev4(i)], tab + (post + 4 * i) * TW);
 #pragma unroll
   for (int i = 0; i < 16; i++) x[i] = y[i];
+}
+
+// v * iota^-1
+template <class F>
+__device__ __forceinline__ typename F::W mul_iota_inv(const F& f, typename F::W v,
+                                                      const typename F::W* __restrict__ tab) {
+  if constexpr (F::kPow2Roots)  // psi^-32 = omega^-16 = 2^(-16 * kOmegaLog2)
+    return f.mul_pow2(v, (192 - (16 * F::kOmegaLog2) % 192) % 192);
+  else
+    return f.mul_tw(v, tab + (kTabInv + 16) * F::kTwWords);  // stage 0 root iomega^16
+}
+
+// quad_inv64 with its 4-point inverse DFT over k1 done in each lane and the
+// transpose through LDS (the inverse of quad_fwd64_lds). In: x[4*k1 + u] =
+// C[16*k1 + 4*t + u], the layout quad_fwd64_lds leaves. Each lane finishes
+// D_t'[k2] for its k2 = 4t .. 4t+3 and all t', writes them to the
+// polynomial's 64-word area sc ([t'][k2], 16-byte aligned) and reads back its
+// own D_t[0..15]. Out: as quad_inv64.
+template <class F>
+__device__ __forceinline__ void quad_inv64_lds(const F& f, typename F::W (&x)[16],
+                                               const typename F::W* __restrict__ tab, int post,
+                                               int t, typename F::W* sc) {
+  using W = typename F::W;
+  using V = typename Vec16<W>::T;
+  constexpr int N = Vec16<W>::N;
+  W d[4][4];  // d[t'][u] = D_t'[4t + u]
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    // D_t'[k2] = sum_k1 iota^(-t' k1) C[k2 + 16 k1]
+    const W s0 = f.add(x[u], x[8 + u]), s1 = f.sub(x[u], x[8 + u]);
+    const W s2 = f.add(x[4 + u], x[12 + u]);
+    const W s3 = mul_iota_inv(f, f.sub(x[4 + u], x[12 + u]), tab);
+    d[0][u] = f.add(s0, s2);
+    d[2][u] = f.sub(s0, s2);
+    d[1][u] = f.add(s1, s3);
+    d[3][u] = f.sub(s1, s3);
+  }
+#pragma unroll
+  for (int tp = 0; tp < 4; tp++) {
+#pragma unroll
+    for (int v = 0; v < 4 / N; v++) {
+      V o;
+#pragma unroll
+      for (int e = 0; e < N; e++) o[e] = d[tp][N * v + e];
+      *reinterpret_cast<V*>(sc + 16 * tp + 4 * t + N * v) = o;
+    }
+  }
+  // the quad's four lanes are in this wave: the reads below follow the
+  // writes above in the wave's LDS order; keep the compiler from moving them
+  __asm__ volatile("" ::: "memory");
+#pragma unroll
+  for (int v = 0; v < 16 / N; v++) {
+    const V o = *reinterpret_cast<const V*>(sc + 16 * t + N * v);
+#pragma unroll
+    for (int e = 0; e < N; e++) x[N * v + e] = o[e];
+  }
+  __asm__ volatile("" ::: "memory");
+  quad_inv_lane16(f, x, tab, post, t);
+}
+
+// LDS area per polynomial of the quad transforms' 4-point stage (64 words,
+// padded to spread a wave's 16 polynomials over the banks; 16-byte aligned)
+template <class W>
+constexpr int kQuadStride = sizeof(W) == 4 ? 68 : 66;
+
+// words 16*k1 + 4*t .. +3 of a polynomial, k1 = 0..3, from or to x[4*k1 ..]
+template <class W>
+__device__ __forceinline__ void quad_store_k1(W* poly, const W (&x)[16], int t) {
+  using V = typename Vec16<W>::T;
+  constexpr int N = Vec16<W>::N;
+#pragma unroll
+  for (int k1 = 0; k1 < 4; k1++)
+#pragma unroll
+    for (int v = 0; v < 4 / N; v++) {
+      V o;
+#pragma unroll
+      for (int e = 0; e < N; e++) o[e] = x[4 * k1 + N * v + e];
+      *reinterpret_cast<V*>(poly + 16 * k1 + 4 * t + N * v) = o;
+    }
+}
+template <class W>
+__device__ __forceinline__ void quad_load_k1(const W* poly, W (&x)[16], int t) {
+  using V = typename Vec16<W>::T;
+  constexpr int N = Vec16<W>::N;
+#pragma unroll
+  for (int k1 = 0; k1 < 4; k1++)
+#pragma unroll
+    for (int v = 0; v < 4 / N; v++) {
+      const V o = *reinterpret_cast<const V*>(poly + 16 * k1 + 4 * t + N * v);
+#pragma unroll
+      for (int e = 0; e < N; e++) x[4 * k1 + N * v + e] = o[e];
+    }
 }
 
 // Wave-cooperative forward twisted transform (lane k holds coefficient k).
