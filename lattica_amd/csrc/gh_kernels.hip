@@ -175,12 +175,21 @@ __global__ __launch_bounds__(256) void k_twisted_inv_quad(F f, const typename F:
   constexpr int V = 16 / sizeof(W);
   __shared__ W qlds[LX ? 64 * kQuadStride<W> : 1];
   const int t = threadIdx.x & 3;
+  // this lane's psi^(-t(2k2+1)), loaded once (LX)
+  constexpr int TW = F::kTwWords;
+  W itw[16][TW];
+  if constexpr (LX) {
+#pragma unroll
+    for (int k = 0; k < 16; k++)
+#pragma unroll
+      for (int w = 0; w < TW; w++) itw[k][w] = tab[(kTabQuadInv + 16 * t + k) * TW + w];
+  }
   const size_t stride = ((size_t)gridDim.x * blockDim.x) >> 2;
   for (size_t p = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 2; p < count; p += stride) {
     W x[16];
     if constexpr (LX) {
       quad_load_k1(src + p * 64, x, t);
-      quad_inv64_lds(f, x, tab, kTabUntwist, t, qlds + (threadIdx.x >> 2) * kQuadStride<W>);
+      quad_inv64_lds(f, x, tab, kTabUntwist, t, qlds + (threadIdx.x >> 2) * kQuadStride<W>, itw);
       W* dp = dst + p * 64 + t;
 #pragma unroll
       for (int i = 0; i < 16; i++) dp[4 * i] = x[i];

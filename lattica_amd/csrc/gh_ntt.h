@@ -358,8 +358,22 @@ __device__ __forceinline__ void quad_inv_lane16(const F& f, typename F::W (&x)[1
                                                 int t) {
   using W = typename F::W;
   constexpr int TW = F::kTwWords;
+  W itw[16][TW];
 #pragma unroll
-  for (int k = 0; k < 16; k++) x[k] = f.mul_tw(x[k], tab + (kTabQuadInv + 16 * t + k) * TW);
+  for (int k = 0; k < 16; k++)
+#pragma unroll
+    for (int w = 0; w < TW; w++) itw[k][w] = tab[(kTabQuadInv + 16 * t + k) * TW + w];
+  quad_inv_lane16(f, x, tab, post, itw);
+}
+// the same with this lane's psi^(-t(2k2+1)) entries preloaded (itw[k2])
+template <class F>
+__device__ __forceinline__ void quad_inv_lane16(const F& f, typename F::W (&x)[16],
+                                                const typename F::W* __restrict__ tab, int post,
+                                                const typename F::W (&itw)[16][F::kTwWords]) {
+  using W = typename F::W;
+  constexpr int TW = F::kTwWords;
+#pragma unroll
+  for (int k = 0; k < 16; k++) x[k] = f.mul_tw(x[k], itw[k]);
   // 16-point DIF with the 64-point tables' stages of length 16, 8, 4, 2
 #pragma unroll
   for (int s = 2; s < 6; s++) {
@@ -404,10 +418,10 @@ __device__ __forceinline__ typename F::W mul_iota_inv(const F& f, typename F::W 
 // D_t'[k2] for its k2 = 4t .. 4t+3 and all t', writes them to the
 // polynomial's 64-word area sc ([t'][k2], 16-byte aligned) and reads back its
 // own D_t[0..15]. Out: as quad_inv64.
-template <class F>
+template <class F, class LaneTw>
 __device__ __forceinline__ void quad_inv64_lds(const F& f, typename F::W (&x)[16],
                                                const typename F::W* __restrict__ tab, int post,
-                                               int t, typename F::W* sc) {
+                                               int t, typename F::W* sc, const LaneTw& itw) {
   using W = typename F::W;
   using V = typename Vec16<W>::T;
   constexpr int N = Vec16<W>::N;
@@ -443,7 +457,14 @@ __device__ __forceinline__ void quad_inv64_lds(const F& f, typename F::W (&x)[16
     for (int e = 0; e < N; e++) x[N * v + e] = o[e];
   }
   __asm__ volatile("" ::: "memory");
-  quad_inv_lane16(f, x, tab, post, t);
+  quad_inv_lane16(f, x, tab, post, itw);
+}
+// with the lane twiddles read from the table
+template <class F>
+__device__ __forceinline__ void quad_inv64_lds(const F& f, typename F::W (&x)[16],
+                                               const typename F::W* __restrict__ tab, int post,
+                                               int t, typename F::W* sc) {
+  quad_inv64_lds(f, x, tab, post, t, sc, t);
 }
 
 // LDS area per polynomial of the quad transforms' 4-point stage (64 words,
