@@ -397,7 +397,19 @@ __device__ __forceinline__ void quad_inv_lane16(const F& f, typename F::W (&x)[1
   }
   W y[16];
 #pragma unroll
-  for (int i = 0; i < 16; i++) y[i] = f.mul_tw(x[bitrev4(i)], tab + (post + 4 * i) * TW);
+  for (int i = 0; i < 16; i++) {
+    if constexpr (F::kPow2Roots) {
+      // Goldilocks (R = 1): psi^(-4i) d^-1 = omega^(-2i) 2^-6 = 2^(-2i*kOmegaLog2 - 6),
+      // one more halving for the half table: a shift, not a product
+      if (post == kTabUntwist || post == kTabUntwistMul || post == kTabUntwistMac ||
+          post == kTabUntwistMacHalf) {
+        const int e = 2 * F::kOmegaLog2 * i + (post == kTabUntwistMacHalf ? 7 : 6);
+        y[i] = f.mul_pow2(x[bitrev4(i)], (192 * 8 - e) % 192);
+        continue;
+      }
+    }
+    y[i] = f.mul_tw(x[bitrev4(i)], tab + (post + 4 * i) * TW);
+  }
 #pragma unroll
   for (int i = 0; i < 16; i++) x[i] = y[i];
 }
